@@ -1,0 +1,251 @@
+/*
+ * vps.h — C ABI of the MI355X (gfx950) Bayesian-clustering engine that stands
+ * behind vcfpop's `-structure` GPU seam.
+ *
+ * The reference exposes its GPU path in two layers:
+ *   - an extern "C" runtime seam, src/structureCUDA.h:6-52 (WaitCUDA, MallocDeviceCUDA,
+ *     CopyStructureMemory, CreateStreamCUDA, ...), and
+ *   - five C++ members of BAYESIAN<REAL> selected when `usecuda` is set
+ *     (src/structure.h:259-306): UpdateZAdmixCUDA, UpdateZNoAdmixCUDA,
+ *     UpdateQNoAdmixCUDA, UpdateQMetroCUDA, RecordCUDA.
+ * This header replaces both with one plain C interface: opaque handles,
+ * plain pointers and sizes, int status codes (no exit() on error, unlike
+ * checkErrors at src/structureCUDA.cpp:183-196).  Every entry point below names
+ * the reference interface it replaces.
+ *
+ * Conventions
+ *   - All functions return VPS_OK (0) or a positive VPS_ERR_* code; the
+ *     message of the last failure on the calling thread is vps_last_error().
+ *   - Host buffers are plain host memory; the library copies them.
+ *   - A chain handle is used by one host thread at a time.  Different chains
+ *     may be driven from different threads.
+ *   - Locus index l everywhere is the index in the reference's sorted order
+ *     (ascending genotype bitsize, ties by original index; QSLocus2,
+ *     src/structure.cpp:2202-2273).  vps_dataset_create performs that sort.
+ */
+#ifndef VPS_H
+#define VPS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VPS_ABI_VERSION 1
+
+/* status codes */
+#define VPS_OK               0
+#define VPS_ERR_INVALID      1   /* bad argument or inconsistent sizes */
+#define VPS_ERR_UNSUPPORTED  2   /* valid input outside what the engine supports */
+#define VPS_ERR_DEVICE       3   /* HIP runtime error */
+#define VPS_ERR_NOMEM        4   /* device or host allocation failed */
+#define VPS_ERR_STATE        5   /* call not valid in the handle's current state */
+#define VPS_ERR_BUDGET       6   /* serving: job does not fit the HBM budget */
+#define VPS_ERR_NOTFOUND     7   /* serving: unknown job id */
+
+/* engine limits */
+#define VPS_MAX_PLOIDY       10  /* N_MAX_PLOIDY, src/global.h:27 */
+#define VPS_MAX_K            16  /* clusters per chain on the device path */
+#define VPS_MAX_ALLELES      64  /* alleles per locus on the device path */
+
+/* ------------------------------------------------------------------------- */
+/* Genotype image (host side).  Mirrors vcfpop's in-memory layout:            */
+/*   per locus, N genotype ids of CeilLog2(ngeno) bits, LSB-first, starting   */
+/*   on a byte boundary (GENO_READER, src/function.cpp:3499-3569;             */
+/*   AddOffsetGT, src/misc.cpp:1624-1636), and a per-locus genotype table     */
+/*   whose entries carry ploidy and the allele copies in stored order          */
+/*   (GENOTYPE, src/function.h:53-149; GetAlleleArray, src/function.cpp:514).  */
+/* ------------------------------------------------------------------------- */
+typedef struct vps_dataset_desc {
+    int32_t         n_ind;             /* N individuals                                   */
+    int64_t         n_loc;             /* L loci, in the caller's (original) order         */
+    int32_t         n_pop;             /* S sampling locations (populations)               */
+    const int32_t*  ind_pop;           /* [N] population of each individual; individuals
+                                          of one population are contiguous (IND::popid)    */
+    const uint8_t*  geno_bits;         /* packed genotype ids                              */
+    const uint64_t* geno_byte_offset;  /* [L] first byte of locus l's ids in geno_bits    */
+    const uint8_t*  bitsize;           /* [L] bits per id = CeilLog2(ngeno) (>= 1)         */
+    const int32_t*  n_allele;          /* [L] number of alleles k2 (SLOCUS::k)             */
+    const int64_t*  gtab_offset;       /* [L+1] genotype-table range of locus l            */
+    const uint8_t*  gt_ploidy;         /* [G] ploidy of each genotype; 0 = missing         */
+    const int64_t*  gt_allele_offset;  /* [G] first allele copy in allele_pool             */
+    const uint16_t* allele_pool;       /* allele copies, ascending, gt_ploidy[g] per entry */
+    const char* const* ind_name;       /* [N] optional (NULL): names for output tables     */
+    const char* const* pop_name;       /* [S] optional (NULL)                               */
+    const char* const* loc_name;       /* [L] optional (NULL)                               */
+} vps_dataset_desc;
+
+/* ------------------------------------------------------------------------- */
+/* Run parameters: the `-structure_*` option surface (src/parameters.cpp:    */
+/* 584-627 defaults, :1230-1315 parser) plus engine options.                  */
+/* Enumerated options keep the reference's 1-based encoding: 1 = yes, 2 = no */
+/* (GetParString, src/string2.cpp:1107-1124), and the same derived booleans  */
+/* (ReadPar, src/structure.cpp:405-464).                                     */
+/* ------------------------------------------------------------------------- */
+typedef struct vps_params {
+    /* model */
+    int32_t admix_val;        /* -structure_admix      (default 2 = no)   */
+    int32_t locpriori_val;    /* -structure_locpriori  (default 2)        */
+    int32_t f_val;            /* -structure_f          (default 2)        */
+    /* MCMC */
+    int32_t nburnin;          /* default 1000  */
+    int32_t nreps;            /* default 10000 */
+    int32_t nthinning;        /* default 10    */
+    int32_t nruns;            /* default 1     */
+    int32_t nadmburnin;       /* default 500   */
+    /* lambda */
+    double  lambda, stdlambda, maxlambda;      /* 1, 0.3, 10 */
+    int32_t inferlambda_val;  /* default 2; the reference derives inferlambda = (val == 0) */
+    int32_t difflambda_val;   /* default 2 */
+    int32_t diversity_val;    /* default 2 */
+    /* alpha */
+    double  alpha, stdalpha, maxalpha, alphapriora, alphapriorb;  /* 1, 0.025, 10, 0.05, 0.001 */
+    int32_t inferalpha_val;   /* default 1 */
+    int32_t diffalpha_val;    /* default 0 (derived diffalpha = (val == 1)) */
+    int32_t uniformalpha_val; /* default 1 */
+    int32_t metrofreq;        /* default 10 */
+    /* locprior */
+    double  r, maxr, stdr, epseta, epsgamma;   /* 1, 20, 0.1, 0.025, 0.025 */
+    /* F model */
+    double  pmeanf, pstdf, stdf;               /* 0.01, 0.05, 0.05 */
+    int32_t singlef_val;      /* default 2 */
+    /* output / run control */
+    int32_t writelnl_val;     /* default 2 */
+    int32_t krange_min, krange_max;            /* 1, 5 */
+    uint64_t g_seed;          /* -g_seed; run seed = HashULong(g_seed + id) */
+    /* engine options (no counterpart in the reference) */
+    int32_t rng_topology;     /* VPS_RNG_REFERENCE or VPS_RNG_PARALLEL           */
+    int32_t precision;        /* VPS_PREC_FP64 or VPS_PREC_MIXED                 */
+} vps_params;
+
+#define VPS_RNG_REFERENCE 0  /* P: 128 streams RNG(seed+m*128+b); Q: one stream RNG(seed+m) —
+                                the reference's draw topology (src/structure.cpp:843-861, 990-1003) */
+#define VPS_RNG_PARALLEL  1  /* P: one stream per (k,l); Q: one stream per individual —
+                                same distributions, different draws; Z pass unchanged */
+
+#define VPS_PREC_FP64  0     /* every quantity in FP64 (reference REAL=double)                     */
+#define VPS_PREC_MIXED 1     /* FP32 per-genotype weights; FP64 lnL, Dirichlet, Metropolis          */
+
+/* fills *p with the reference defaults (src/parameters.cpp:584-627, :432-445) */
+void vps_params_default(vps_params* p);
+
+/* ------------------------------------------------------------------------- */
+/* Library                                                                    */
+/* ------------------------------------------------------------------------- */
+int         vps_abi_version(void);
+const char* vps_last_error(void);
+/* GetDeviceCountCUDA, src/structureCUDA.h:30 */
+int         vps_device_count(int* n);
+
+/* ------------------------------------------------------------------------- */
+/* Dataset: device-resident genotype image shared by every chain on a device. */
+/* Replaces AllocMemoryCUDA + CopyStructureMemory (src/structureCUDA.h:24-32, */
+/* src/structureCUDA.cpp:969-1094) and performs CalcBayesian's prologue:     */
+/* locus sort and allele_freq_offset / KT (src/structure.cpp:2350-2395).      */
+/* ------------------------------------------------------------------------- */
+typedef struct vps_dataset vps_dataset;
+
+int vps_dataset_create(const vps_dataset_desc* desc, int device, vps_dataset** out);
+/* FreeStructureMemory + FreeMemoryCUDA, src/structureCUDA.h:22,34 */
+int vps_dataset_destroy(vps_dataset* ds);
+
+typedef struct vps_dataset_info {
+    int32_t n_ind;
+    int64_t n_loc;
+    int64_t kt;             /* KT = total alleles over loci                      */
+    int32_t max_ploidy;     /* maxploidy, src/global.h:189                        */
+    int32_t min_ploidy;
+    int32_t max_alleles;    /* maxK                                               */
+    int32_t n_pop;
+    int64_t n_tiles;        /* 64-locus tiles (one per wavefront)                 */
+    int64_t device_bytes;   /* HBM held by the genotype image                     */
+} vps_dataset_info;
+int vps_dataset_get_info(const vps_dataset* ds, vps_dataset_info* info);
+/* sorted-order tables: original index of sorted locus l, allele_freq_offset[l], vt[i] */
+int vps_dataset_get_order(const vps_dataset* ds, int64_t* orig_index /*[L]*/,
+                          int64_t* allele_freq_offset /*[L]*/, int64_t* vt /*[N]*/);
+
+/* ------------------------------------------------------------------------- */
+/* Chain: one MCMC run (BAYESIAN<double>, src/structure.h:50-307).           */
+/* ------------------------------------------------------------------------- */
+typedef struct vps_chain vps_chain;
+
+/* ReadPar + InitFix + InitAdmix + InitLocPriori + InitFmodel
+   (src/structure.cpp:405-708).  run_id is the reference's 1-based task id
+   (src/structure.cpp:2402-2413), K the number of clusters. */
+int vps_chain_create(vps_dataset* ds, const vps_params* par, int K, int run_id, vps_chain** out);
+int vps_chain_destroy(vps_chain* ch);
+
+/* MCMC loop body for sweeps m .. m+n-1 (src/structure.cpp:2122-2158):
+   UpdateP, UpdateQ, UpdateLocPriori, UpdateZ, UpdateAlpha, UpdateLambda,
+   UpdateA, UpdateF, Record.  Asynchronous on the chain's stream. */
+int vps_chain_step(vps_chain* ch, int n_sweeps);
+/* Arrange (src/structure.cpp:1862-1869) after the last sweep */
+int vps_chain_finish(vps_chain* ch);
+/* WaitCUDA (src/structureCUDA.h:8) */
+int vps_chain_sync(vps_chain* ch);
+
+/* chain state, copied to / from host buffers */
+enum vps_state {
+    VPS_STATE_FREQ   = 1,   /* double[K*KT]  current allele frequencies P     */
+    VPS_STATE_Q      = 2,   /* double[N*K]   admixture proportions           */
+    VPS_STATE_Z      = 3,   /* uint16[N]     cluster of each individual       */
+    VPS_STATE_NI     = 4,   /* int32[K*KT]   allele copies per cluster        */
+    VPS_STATE_MI     = 5,   /* int64[N*K]    allele copies per individual    */
+    VPS_STATE_ALPHA  = 6,   /* double[K]                                       */
+    VPS_STATE_LAMBDA = 7,   /* double[K]                                       */
+    VPS_STATE_ITER   = 8,   /* int32[1]      next sweep index m              */
+    VPS_STATE_BUFN1  = 9,   /* double[N]     Metropolis log ratio of the last QMetro */
+    VPS_STATE_BUFNK1 = 10,  /* double[N*K]   no-admix log likelihoods of the last QNoAdmix */
+    VPS_STATE_LNL    = 11,  /* double[1]     lnL of the last Record pass     */
+    VPS_STATE_MISUM  = 12,  /* int64[N*K] before finish, double[N*K] after    */
+    VPS_STATE_FREQM  = 13,  /* double[K*KT]                                    */
+    VPS_STATE_ROUT   = 14,  /* double[rlen]  run summary accumulators          */
+    VPS_STATE_FREQA  = 15,  /* double[3*KT]  F model ancestral frequencies    */
+    VPS_STATE_FST    = 16,  /* double[2*K]   F[k] then f[k] = (1-F)/F          */
+    VPS_STATE_LOC    = 17,  /* double[2 + K + 3*S*K]: r, nr-unused, Eta[K], Gamma[S*K], AlphaLocal[S*K], Di[S*K] */
+    VPS_STATE_BINARYQ= 18,  /* int32[1]                                        */
+    VPS_STATE_LNLTRACE=19   /* double[nlnL]  per-record lnL trace (writelnl)   */
+};
+int vps_chain_get_state(vps_chain* ch, int which, void* host, size_t bytes);
+int vps_chain_set_state(vps_chain* ch, int which, const void* host, size_t bytes);
+/* rlen of VPS_STATE_ROUT and count of VPS_STATE_LNLTRACE */
+int vps_chain_get_sizes(vps_chain* ch, int64_t* rlen, int64_t* nlnl);
+
+/* Single passes at the chain's current sweep index m.  These are the
+   drop-in counterparts of the BAYESIAN<REAL>::*CUDA members. */
+int vps_pass_update_z_admix(vps_chain* ch);    /* UpdateZAdmixCUDA,   src/structureCUDA.cpp:2092-2119 */
+int vps_pass_update_z_noadmix(vps_chain* ch);  /* UpdateZNoAdmixCUDA, src/structureCUDA.cpp:1987-2009 */
+int vps_pass_update_q_noadmix(vps_chain* ch);  /* UpdateQNoAdmixCUDA, src/structureCUDA.cpp:2175-2214 */
+int vps_pass_update_q_metro(vps_chain* ch);    /* UpdateQMetroCUDA,   src/structureCUDA.cpp:2269-2312 */
+int vps_pass_record(vps_chain* ch);            /* RecordCUDA,         src/structureCUDA.cpp:2381-2410 */
+int vps_pass_update_p(vps_chain* ch);          /* UpdatePCUDA (dead in the reference), src/structure.cpp:812-862 */
+int vps_pass_update_q_admix(vps_chain* ch);    /* UpdateQAdmix,       src/structure.cpp:986-1004 */
+int vps_pass_update_alpha(vps_chain* ch);      /* UpdateAlpha,        src/structure.cpp:1501-1594 */
+
+/* ------------------------------------------------------------------------- */
+/* Timing hook for measurement: average device time of the last launches of  */
+/* a named kernel family, measured with HIP events on the chain's stream.     */
+/* ------------------------------------------------------------------------- */
+int vps_chain_enable_timing(vps_chain* ch, int on);
+/* kernel: 0 = Z admix pass; returns total ms and number of timed launches */
+int vps_chain_kernel_time(vps_chain* ch, int kernel, double* total_ms, int64_t* launches);
+
+/* ------------------------------------------------------------------------- */
+/* Test hooks (no counterpart in the reference).  A dataset created with     */
+/* device = -1 is built on the host only (no HIP call) and keeps its packed   */
+/* tile layout, so the layout can be checked without a GPU.                   */
+/* ------------------------------------------------------------------------- */
+/* decode genotype (sorted locus l, individual i) from the tile layout */
+int vps_debug_decode(const vps_dataset* ds, int64_t l, int32_t i, int32_t* ploidy, uint16_t* alleles /*[VPS_MAX_PLOIDY]*/);
+/* advance a xorshift128+ state (x, y) by D steps with the GF(2) jump table */
+int vps_debug_jump(uint64_t D, uint64_t* x, uint64_t* y);
+/* individual chunking of the genotype passes: G chunks of Nc individuals */
+int vps_debug_chunking(const vps_dataset* ds, int32_t* G, int32_t* Nc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VPS_H */

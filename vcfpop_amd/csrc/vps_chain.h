@@ -1,0 +1,74 @@
+// vps_chain.h — device view of one MCMC chain and kernel entry points.
+#pragma once
+#include "vps_internal.h"
+
+namespace vps {
+
+// Pointers and scalars of one chain, passed by value to every kernel.
+struct ChainDev {
+    int32_t K, N, S, maxploidy;
+    int64_t L, KT;
+    uint64_t seed;
+    int32_t admix, locpriori, fmodel, topology, precision;
+    int32_t inferalpha, diffalpha, uniformalpha, inferlambda, difflambda, fsame;
+    int32_t nadmburnin, rlen;
+    double stdalpha, maxalpha, alphapriora, alphapriorb;
+    double stdlambda, maxlambda;
+    double maxr, epsr, epseta, epsgamma;
+    double pmeanf, pstdf, stdf;
+    // state
+    double*   Freq;      // [K*KT]
+    double*   Q;         // [N*K]
+    uint16_t* Z;         // [N]
+    int32_t*  Ni;        // [K*KT]
+    int32_t*  Ni_part;   // [G*K*KT]
+    int32_t*  Mi;        // [N*K]
+    double*   Alpha;     // [K]
+    double*   Lambda;    // [K]
+    double*   bufNK1;    // [N*K] Metropolis proposals / no-admix log likelihoods
+    double*   bufN1;     // [N]   Metropolis log ratios
+    double*   part;      // partial sums of the likelihood passes
+    int64_t*  MiSum;     // [N*K]
+    double*   FreqM;     // [K*KT]
+    double*   FreqA;     // [3*KT]
+    double*   Fst;       // [2*K]: F[k], then f[k] = (1-F)/F
+    double*   Eta;       // [K]
+    double*   Gamma;     // [S*K]
+    double*   Di;        // [S*K]
+    double*   AlphaLocal;// [S*K]
+    double*   SumAlpha;  // [S]
+    double*   scal;      // [8]: 0 r, 1 lnL of the last Record pass
+    double*   rout;      // [rlen]
+    double*   lnltrace;  // [nlnl capacity]
+    Rng*      rngbuf;    // [N] streams kept between a draw kernel and its acceptance kernel
+};
+
+// likelihood-pass partials: lanes = individuals, one partial per locus range
+constexpr int kLikLociPerWave = 1024;
+
+// launchers (vps_kernels.hip / vps_updates.hip); all asynchronous on `st`
+int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
+int launch_z_noadmix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+// likelihood passes: mode 0 = Record lnL (admix), 1 = Record lnL (no admix),
+// 2 = Metropolis log ratio per individual, 3 = no-admix per-(i,k) log likelihood
+int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c, int mode, hipStream_t st);
+int lik_part_count(const DeviceDataset& ds, int mode, int K);
+
+int launch_update_p(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
+int launch_update_q_admix(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
+int launch_q_metro_propose(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
+int launch_q_metro_accept(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
+int launch_q_noadmix_draw(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
+int launch_update_di(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+int launch_update_locpriori(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
+int launch_update_alpha(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
+int launch_update_lambda(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
+int launch_update_a(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
+int launch_update_f(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
+int launch_record_accumulate(const DeviceDataset& ds, const ChainDev& c, int post_burnin, int trace_slot,
+                             int accumulate_misum, hipStream_t st);
+int launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
+int launch_init_admix(const DeviceDataset& ds, const ChainDev& c, int singlez, hipStream_t st);
+
+}  // namespace vps

@@ -1,0 +1,131 @@
+// vps_internal.h — device-side data layout and host handles of the engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include <mutex>
+#include "../../include/vps.h"
+#include "vps_numeric.h"
+
+namespace vps {
+
+// ---------------------------------------------------------------------------
+// Genotype image in HBM.
+//
+// Loci are kept in the reference's sorted order and cut into tiles of 64
+// consecutive loci of one bitsize (the same tiles as the AVX path,
+// src/structureAVX.cpp:305-312).  One wavefront owns one tile: lane j holds
+// locus l0 + j.  The genotype ids of a tile are re-packed so that one 32-bit
+// load per lane is one coalesced 256-byte wavefront load:
+//     words[wbase + w*64 + j]  = ids of individuals [w*ipw, (w+1)*ipw) of
+//                                locus l0 + j, id i at bits (i % ipw) * bits
+// with ipw = floor(32 / bits) ids per word (no id straddles two words).
+//
+// Genotype ids decode through one of two tables:
+//   kind BI  (every locus of the tile has <= 2 alleles and <= 8 genotypes):
+//            bitab[l] packs 8 one-byte entries, entry g = v | c << 4 where v
+//            is the typed ploidy (0 = missing) and c the dosage of allele 1;
+//            slot a holds allele (a >= v - c).
+//   kind GEN (any other tile): gdesc[gbase[l] + g] = v | allele(a) << (4+6a)
+//            for a < v (alleles < 64, ploidy <= 10).
+// ---------------------------------------------------------------------------
+constexpr int kTileLoci = 64;
+constexpr int kKindBi = 0;
+constexpr int kKindGen = 1;
+
+struct TileDesc {
+    int64_t l0;       // first sorted locus of the tile
+    int64_t wbase;    // first word of the tile in `words`
+    int32_t nloc;     // loci in the tile (<= 64)
+    int32_t bits;     // genotype-id bitsize of every locus in the tile
+    int32_t ipw;      // ids per 32-bit word
+    int32_t kind;     // kKindBi or kKindGen
+    int32_t maxk2;    // largest allele count in the tile
+    int32_t pad;
+};
+
+// Jump-ahead table of xorshift128+: s' = M s over GF(2), split into 32
+// nibbles of the 128-bit state; entry [p][v] is the 128-bit image of nibble
+// p holding v (4 dwords: x_lo, x_hi, y_lo, y_hi).
+constexpr int kJumpWords = 32 * 16 * 4;
+
+struct DeviceDataset {
+    // sizes
+    int32_t N;
+    int32_t S;
+    int64_t L;
+    int64_t KT;
+    int32_t maxploidy;
+    int32_t minploidy;
+    int32_t maxK;
+    int64_t ntiles;
+    int64_t nbi, ngen;         // tiles of each kind
+    int32_t G;                 // individual chunks per tile in genotype passes
+    int32_t Nc;                // individuals per chunk
+    // device arrays
+    TileDesc* tiles;           // [ntiles]
+    int32_t*  bi_list;         // tile indices of kind BI
+    int32_t*  gen_list;        // tile indices of kind GEN
+    uint32_t* words;           // transposed genotype ids
+    int32_t*  afo;             // [L] allele_freq_offset
+    int32_t*  k2;              // [L]
+    uint64_t* bitab;           // [L]
+    int64_t*  gbase;           // [L]
+    uint64_t* gdesc;           // [sum ngeno of GEN loci]
+    int32_t*  vt;              // [N]
+    int32_t*  popid;           // [N]
+    int32_t*  pop_ind0;        // [S]
+    int32_t*  pop_nind;        // [S]
+    uint32_t* jump;            // [G][kJumpWords] (chunk 0 unused)
+    double*   totfreq;         // [KT] (F model)
+    int32_t*  nhaplo;          // [L]
+    int64_t*  kdis;            // [maxK+1] loci per allele count
+};
+
+}  // namespace vps
+
+struct vps_dataset {
+    int device = 0;
+    vps::DeviceDataset d{};
+    // host copies kept for state conversion, output tables and tests
+    std::vector<int64_t> orig;         // sorted -> original locus index
+    std::vector<int64_t> afo;
+    std::vector<int32_t> k2;
+    std::vector<int64_t> vt;
+    std::vector<int32_t> popid;
+    std::vector<int32_t> pop_ind0, pop_nind;
+    std::vector<vps::TileDesc> tiles;
+    std::vector<uint8_t> bitsize;
+    std::vector<std::string> ind_names, pop_names, loc_names;
+    std::vector<int64_t> kdis;
+    int64_t device_bytes = 0;
+    std::vector<void*> allocations;
+    // host-only build (device < 0): the packed layout stays on the host
+    bool host_only = false;
+    std::vector<uint32_t> h_words;
+    std::vector<uint64_t> h_bitab, h_gdesc;
+    std::vector<int64_t> h_gbase;
+};
+
+namespace vps {
+
+// error reporting (thread local), vps_runtime.cpp
+int fail(int code, const char* fmt, ...);
+#define VPS_HIP(call)                                                           \
+    do {                                                                        \
+        hipError_t e_ = (call);                                                 \
+        if (e_ != hipSuccess)                                                   \
+            return ::vps::fail(VPS_ERR_DEVICE, "%s failed: %s (%s:%d)", #call,  \
+                               hipGetErrorString(e_), __FILE__, __LINE__);      \
+    } while (0)
+
+// GF(2) jump table for D steps of xorshift128+ (vps_jump.cpp)
+void jump_table(uint64_t D, uint32_t* out /* kJumpWords */);
+// apply a jump table to a state on the host (tests)
+void jump_apply_host(const uint32_t* tab, uint64_t& x, uint64_t& y);
+
+// device allocation helper that records the block for release
+int dev_alloc(void** p, size_t bytes, std::vector<void*>& list, int64_t* counter);
+
+}  // namespace vps

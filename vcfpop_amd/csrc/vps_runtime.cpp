@@ -1,0 +1,613 @@
+// vps_runtime.cpp — chain lifecycle and the sweep driver (BAYESIAN<double>::
+// MCMC, src/structure.cpp:2112-2166) over the device kernels, plus the C ABI.
+#include <algorithm>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include "vps_chain.h"
+
+using namespace vps;
+
+namespace vps {
+
+static thread_local char g_err[1024] = "";
+
+int fail(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+// CRC32C (reflected 0x82F63B78) of one 64-bit word, the reference's
+// _mm_crc32_u64 (src/hash.cpp:9-16); HashULong = crc(0xB385FDA9, v)
+uint32_t hash_ulong_host(uint64_t v)
+{
+    static uint32_t tab[256];
+    static bool ready = false;
+    if (!ready) {
+        for (uint32_t n = 0; n < 256; ++n) {
+            uint32_t c = n;
+            for (int k = 0; k < 8; ++k) c = (c & 1) ? (c >> 1) ^ 0x82F63B78u : c >> 1;
+            tab[n] = c;
+        }
+        ready = true;
+    }
+    uint32_t c = 0xB385FDA9u;
+    for (int b = 0; b < 8; ++b) c = (c >> 8) ^ tab[(c ^ (uint32_t)(v >> (8 * b))) & 0xFF];
+    return c;
+}
+
+int launch_set_mi_binary(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+int launch_arrange(const DeviceDataset& ds, const ChainDev& c, int nr, hipStream_t st);
+
+}  // namespace vps
+
+struct vps_chain {
+    vps_dataset* ds = nullptr;
+    vps_params par{};
+    ChainDev c{};
+    int K = 0, id = 0, m = 0, nr = 0, nlnl = 0, binaryq = 0, singlez = 0, finished = 0;
+    int nburnin = 0, nreps = 0, nthinning = 1, metrofreq = 0, writelnl = 0, nadmburnin = 0;
+    int64_t lnl_cap = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    int64_t device_bytes = 0;
+    bool timing = false;
+    std::vector<hipEvent_t> evs;   // pairs bracketing each timed Z pass
+    size_t ev_used = 0;
+};
+
+extern "C" void vps_params_default(vps_params* p)
+{
+    memset(p, 0, sizeof(*p));
+    p->admix_val = 2; p->locpriori_val = 2; p->f_val = 2;
+    p->nburnin = 1000; p->nreps = 10000; p->nthinning = 10; p->nruns = 1; p->nadmburnin = 500;
+    p->lambda = 1; p->stdlambda = 0.3; p->maxlambda = 10;
+    p->inferlambda_val = 2; p->difflambda_val = 2; p->diversity_val = 2;
+    p->alpha = 1; p->stdalpha = 0.025; p->maxalpha = 10; p->alphapriora = 0.05; p->alphapriorb = 0.001;
+    p->inferalpha_val = 1; p->diffalpha_val = 0; p->uniformalpha_val = 1; p->metrofreq = 10;
+    p->r = 1; p->maxr = 20; p->stdr = 0.1; p->epseta = 0.025; p->epsgamma = 0.025;
+    p->pmeanf = 0.01; p->pstdf = 0.05; p->stdf = 0.05; p->singlef_val = 2;
+    p->writelnl_val = 2; p->krange_min = 1; p->krange_max = 5;
+    p->g_seed = 0;
+    p->rng_topology = VPS_RNG_REFERENCE;
+    p->precision = VPS_PREC_FP64;
+}
+
+extern "C" int vps_abi_version(void) { return VPS_ABI_VERSION; }
+extern "C" const char* vps_last_error(void) { return vps::g_err; }
+
+extern "C" int vps_device_count(int* n)
+{
+    if (!n) return fail(VPS_ERR_INVALID, "null argument");
+    *n = 0;
+    hipError_t e = hipGetDeviceCount(n);
+    if (e != hipSuccess) { *n = 0; return fail(VPS_ERR_DEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e)); }
+    return VPS_OK;
+}
+
+namespace {
+
+template <typename T>
+int chain_alloc(vps_chain* ch, T** p, int64_t count)
+{
+    return dev_alloc((void**)p, (size_t)std::max<int64_t>(count, 1) * sizeof(T), ch->allocs, &ch->device_bytes);
+}
+
+int fill_f64(vps_chain* ch, double* p, int64_t n, double v)
+{
+    std::vector<double> h(std::max<int64_t>(n, 1), v);
+    VPS_HIP(hipMemcpyAsync(p, h.data(), n * sizeof(double), hipMemcpyHostToDevice, ch->stream));
+    VPS_HIP(hipStreamSynchronize(ch->stream));
+    return VPS_OK;
+}
+
+void chain_free(vps_chain* ch)
+{
+    if (!ch) return;
+    if (ch->ds) (void)hipSetDevice(ch->ds->device);
+    if (ch->stream) (void)hipStreamSynchronize(ch->stream);
+    for (void* p : ch->allocs) (void)hipFree(p);
+    for (hipEvent_t e : ch->evs) (void)hipEventDestroy(e);
+    if (ch->stream) (void)hipStreamDestroy(ch->stream);
+    delete ch;
+}
+
+}  // namespace
+
+extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int run_id, vps_chain** out)
+{
+    if (!ds || !p || !out) return fail(VPS_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (K < 1 || K > VPS_MAX_K) return fail(VPS_ERR_UNSUPPORTED, "K=%d outside 1..%d", K, VPS_MAX_K);
+    if (p->nthinning < 1) return fail(VPS_ERR_INVALID, "nthinning must be >= 1");
+    if (p->rng_topology != VPS_RNG_REFERENCE && p->rng_topology != VPS_RNG_PARALLEL)
+        return fail(VPS_ERR_INVALID, "rng_topology %d", p->rng_topology);
+    if (p->precision != VPS_PREC_FP64 && p->precision != VPS_PREC_MIXED)
+        return fail(VPS_ERR_INVALID, "precision %d", p->precision);
+    const DeviceDataset& d = ds->d;
+    VPS_HIP(hipSetDevice(ds->device));
+
+    vps_chain* ch = new vps_chain();
+    ch->ds = ds; ch->par = *p; ch->K = K; ch->id = run_id;
+    hipError_t e = hipStreamCreate(&ch->stream);
+    if (e != hipSuccess) { delete ch; return fail(VPS_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e)); }
+
+    // ReadPar, src/structure.cpp:405-464
+    ChainDev& c = ch->c;
+    c.K = K; c.N = d.N; c.S = d.S; c.L = d.L; c.KT = d.KT; c.maxploidy = d.maxploidy;
+    c.admix = p->admix_val == 1; c.locpriori = p->locpriori_val == 1; c.fmodel = p->f_val == 1;
+    c.topology = p->rng_topology; c.precision = p->precision;
+    c.inferalpha = p->inferalpha_val == 1; c.diffalpha = p->diffalpha_val == 1; c.uniformalpha = p->uniformalpha_val == 1;
+    c.inferlambda = p->inferlambda_val == 0; c.difflambda = p->difflambda_val == 1; c.fsame = p->singlef_val == 1;
+    c.stdalpha = p->stdalpha; c.maxalpha = p->maxalpha; c.alphapriora = p->alphapriora; c.alphapriorb = p->alphapriorb;
+    c.stdlambda = p->stdlambda; c.maxlambda = p->maxlambda;
+    c.maxr = p->maxr; c.epsr = p->stdr; c.epseta = p->epseta; c.epsgamma = p->epsgamma;
+    c.pmeanf = p->pmeanf; c.pstdf = p->pstdf; c.stdf = p->stdf;
+    ch->nadmburnin = (!c.locpriori && !c.admix) ? p->nadmburnin : 0;
+    c.nadmburnin = ch->nadmburnin;
+    ch->nburnin = p->nburnin + ch->nadmburnin;
+    ch->nreps = p->nreps; ch->nthinning = p->nthinning; ch->metrofreq = p->metrofreq;
+    ch->writelnl = p->writelnl_val == 1;
+    c.seed = hash_ulong_host(p->g_seed + (uint64_t)run_id);
+    const int nf = c.fmodel ? (c.fsame ? 1 : K) : 0;
+    const int na = c.admix ? (c.diffalpha ? K : 1) : 0;
+    const int nl = c.difflambda ? K : 1;
+    c.rlen = c.locpriori ? 4 + nl + 1 + K + K * d.S + nf : 4 + nl + na + nf;
+    ch->lnl_cap = (ch->nburnin + ch->nthinning - 1) / ch->nthinning + (ch->nreps + ch->nthinning - 1) / ch->nthinning + 1;
+
+    const int64_t N = d.N, KT = d.KT, S = d.S;
+    int rc = 0;
+    const int nparts = std::max({lik_part_count(d, 0, K), lik_part_count(d, 3, K)});
+    if (!rc) rc = chain_alloc(ch, &c.Freq, K * KT);
+    if (!rc) rc = chain_alloc(ch, &c.Q, N * K);
+    if (!rc) rc = chain_alloc(ch, &c.Z, N + 32);
+    if (!rc) rc = chain_alloc(ch, &c.Ni, K * KT);
+    if (!rc) rc = chain_alloc(ch, &c.Ni_part, (int64_t)d.G * K * KT);
+    if (!rc) rc = chain_alloc(ch, &c.Mi, N * K);
+    if (!rc) rc = chain_alloc(ch, &c.Alpha, K);
+    if (!rc) rc = chain_alloc(ch, &c.Lambda, K);
+    if (!rc) rc = chain_alloc(ch, &c.bufNK1, N * K);
+    if (!rc) rc = chain_alloc(ch, &c.bufN1, N);
+    if (!rc) rc = chain_alloc(ch, &c.part, nparts);
+    if (!rc) rc = chain_alloc(ch, &c.MiSum, N * K);
+    if (!rc) rc = chain_alloc(ch, &c.FreqM, K * KT);
+    if (!rc) rc = chain_alloc(ch, &c.FreqA, 3 * KT);
+    if (!rc) rc = chain_alloc(ch, &c.Fst, 2 * K);
+    if (!rc) rc = chain_alloc(ch, &c.Eta, K);
+    if (!rc) rc = chain_alloc(ch, &c.Gamma, S * K);
+    if (!rc) rc = chain_alloc(ch, &c.Di, S * K);
+    if (!rc) rc = chain_alloc(ch, &c.AlphaLocal, S * K);
+    if (!rc) rc = chain_alloc(ch, &c.SumAlpha, S);
+    if (!rc) rc = chain_alloc(ch, &c.scal, 8);
+    if (!rc) rc = chain_alloc(ch, &c.rout, c.rlen);
+    if (!rc) rc = chain_alloc(ch, &c.lnltrace, ch->lnl_cap);
+    if (!rc) rc = chain_alloc(ch, &c.rngbuf, N);
+    if (rc) { chain_free(ch); return rc; }
+
+    // InitFix (src/structure.cpp:468-553): P = 0.5 (1/k2 off biallelic), Q = 1/K,
+    // Alpha = alpha, Lambda = lambda, r = 1, accumulators 0
+    {
+        std::vector<double> fr((size_t)K * KT, 0.5);
+        for (int64_t l = 0; l < d.L; ++l)
+            if (ds->k2[l] != 2)
+                for (int k = 0; k < K; ++k)
+                    for (int a = 0; a < ds->k2[l]; ++a) fr[(size_t)k * KT + ds->afo[l] + a] = 1.0 / ds->k2[l];
+        // InitFmodel (src/structure.cpp:679-708) sets P = Lambda; UpdateP
+        // overwrites P before any use, so the value is immaterial
+        e = hipMemcpy(c.Freq, fr.data(), fr.size() * 8, hipMemcpyHostToDevice);
+        if (e != hipSuccess) { chain_free(ch); return fail(VPS_ERR_DEVICE, "upload P: %s", hipGetErrorString(e)); }
+    }
+    rc = fill_f64(ch, c.Q, N * K, 1.0 / K);
+    if (!rc) rc = fill_f64(ch, c.Alpha, K, p->alpha);
+    if (!rc) rc = fill_f64(ch, c.Lambda, K, p->lambda);
+    if (!rc) rc = fill_f64(ch, c.FreqM, K * KT, 0.0);
+    if (!rc) rc = fill_f64(ch, c.rout, c.rlen, 0.0);
+    if (!rc) rc = fill_f64(ch, c.bufNK1, N * K, 0.0);
+    if (!rc) rc = fill_f64(ch, c.bufN1, N, 0.0);
+    if (!rc) rc = fill_f64(ch, c.scal, 8, 0.0);
+    if (!rc) rc = fill_f64(ch, c.scal, 1, 1.0);   // r = 1 (InitFix)
+    if (!rc) rc = fill_f64(ch, c.FreqA, 3 * KT, 0.0);
+    if (rc) { chain_free(ch); return rc; }
+    if (hipMemset(c.MiSum, 0, N * K * 8) != hipSuccess || hipMemset(c.Z, 0, (N + 32) * 2) != hipSuccess ||
+        hipMemset(c.Ni, 0, K * KT * 4) != hipSuccess) {
+        chain_free(ch);
+        return fail(VPS_ERR_DEVICE, "clear chain state");
+    }
+    // InitLocPriori (src/structure.cpp:653-675)
+    rc = fill_f64(ch, c.AlphaLocal, S * K, p->alpha);
+    if (!rc) rc = fill_f64(ch, c.SumAlpha, S, 0.0);
+    if (!rc) rc = fill_f64(ch, c.Eta, K, 1.0 / K);
+    if (!rc) rc = fill_f64(ch, c.Gamma, S * K, 1.0 / K);
+    if (!rc) rc = fill_f64(ch, c.Di, S * K, 0.0);
+    // InitFmodel: F = pmeanf, f = (1-F)/F, ancestral frequencies from the
+    // total-population frequencies
+    if (!rc) {
+        std::vector<double> fst(2 * K);
+        for (int k = 0; k < K; ++k) { fst[k] = p->pmeanf; fst[K + k] = (1 - fst[k]) / fst[k]; }
+        e = hipMemcpy(c.Fst, fst.data(), fst.size() * 8, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = fail(VPS_ERR_DEVICE, "upload F");
+    }
+    if (!rc && c.fmodel) {
+        std::vector<double> tot(KT), nh(d.L);
+        std::vector<int32_t> nhi(d.L);
+        e = hipMemcpy(tot.data(), d.totfreq, KT * 8, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(nhi.data(), d.nhaplo, d.L * 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(VPS_ERR_DEVICE, "read frequencies");
+        std::vector<double> fa(3 * KT, 0.0);
+        for (int64_t l = 0; !rc && l < d.L; ++l) {
+            const int k2 = ds->k2[l];
+            for (int a = 0; a < k2; ++a)
+                fa[ds->afo[l] + a] = (p->lambda + tot[ds->afo[l] + a] * nhi[l]) / (k2 * p->lambda + nhi[l]);
+        }
+        if (!rc && hipMemcpy(c.FreqA, fa.data(), fa.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(VPS_ERR_DEVICE, "upload ancestral frequencies");
+    }
+    // InitAdmix (src/structure.cpp:557-649)
+    ch->singlez = ch->nadmburnin == 0 && !c.admix;
+    if (!rc) rc = launch_init_admix(d, c, ch->singlez, ch->stream);
+    if (!rc) { e = hipStreamSynchronize(ch->stream); if (e != hipSuccess) rc = fail(VPS_ERR_DEVICE, "init: %s", hipGetErrorString(e)); }
+    if (rc) { chain_free(ch); return rc; }
+    *out = ch;
+    return VPS_OK;
+}
+
+extern "C" int vps_chain_destroy(vps_chain* ch)
+{
+    chain_free(ch);
+    return VPS_OK;
+}
+
+namespace {
+
+int do_update_p(vps_chain* ch) { return launch_update_p(ch->ds->d, ch->c, ch->m, ch->stream); }
+
+int do_q_noadmix(vps_chain* ch)
+{
+    int rc = launch_lik_pass(ch->ds->d, ch->c, 3, ch->stream);
+    if (!rc) rc = launch_q_noadmix_draw(ch->ds->d, ch->c, ch->m, ch->stream);
+    ch->binaryq = 1;
+    return rc;
+}
+
+int do_q_admix(vps_chain* ch)
+{
+    ch->binaryq = 0;
+    return launch_update_q_admix(ch->ds->d, ch->c, ch->m, ch->stream);
+}
+
+int do_q_metro(vps_chain* ch)
+{
+    ch->binaryq = 0;
+    int rc = launch_q_metro_propose(ch->ds->d, ch->c, ch->m, ch->stream);
+    if (!rc) rc = launch_lik_pass(ch->ds->d, ch->c, 2, ch->stream);
+    if (!rc) rc = launch_q_metro_accept(ch->ds->d, ch->c, ch->m, ch->stream);
+    return rc;
+}
+
+// UpdateQ, src/structure.cpp:1140-1166
+int do_update_q(vps_chain* ch)
+{
+    const ChainDev& c = ch->c;
+    int rc;
+    if (!c.admix) {
+        rc = ch->m >= ch->nadmburnin ? do_q_noadmix(ch) : do_q_admix(ch);
+        if (!rc && c.locpriori) rc = launch_update_di(ch->ds->d, c, ch->stream);
+    } else {
+        rc = (ch->metrofreq > 0 && ch->m % ch->metrofreq == 0) ? do_q_metro(ch) : do_q_admix(ch);
+    }
+    return rc;
+}
+
+int do_z_admix(vps_chain* ch)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ch->timing) {
+        if (ch->ev_used + 2 > ch->evs.size()) {
+            for (int x = 0; x < 64; ++x) {
+                hipEvent_t e;
+                VPS_HIP(hipEventCreate(&e));
+                ch->evs.push_back(e);
+            }
+        }
+        e0 = ch->evs[ch->ev_used];
+        e1 = ch->evs[ch->ev_used + 1];
+        ch->ev_used += 2;
+    }
+    return launch_z_admix(ch->ds->d, ch->c, ch->m, ch->stream, e0, e1);
+}
+
+int do_z_noadmix(vps_chain* ch)
+{
+    int rc = launch_set_mi_binary(ch->ds->d, ch->c, ch->stream);
+    if (!rc) rc = launch_z_noadmix(ch->ds->d, ch->c, ch->stream);
+    return rc;
+}
+
+int do_update_z(vps_chain* ch) { return ch->binaryq ? do_z_noadmix(ch) : do_z_admix(ch); }
+
+// Record, src/structure.cpp:1943-2045
+int do_record(vps_chain* ch)
+{
+    const int m = ch->m;
+    if (!((ch->writelnl && m < ch->nburnin && m % ch->nthinning == 0) ||
+          (m >= ch->nburnin && (m - ch->nburnin) % ch->nthinning == 0)))
+        return VPS_OK;
+    int rc = launch_lik_pass(ch->ds->d, ch->c, ch->binaryq ? 1 : 0, ch->stream);
+    const int post = m >= ch->nburnin;
+    int slot = -1;
+    if (ch->writelnl && ch->nlnl < ch->lnl_cap) slot = ch->nlnl++;
+    if (!rc) rc = launch_record_accumulate(ch->ds->d, ch->c, post, slot, 1, ch->stream);
+    if (post) ch->nr++;
+    return rc;
+}
+
+int do_sweep(vps_chain* ch)
+{
+    const DeviceDataset& d = ch->ds->d;
+    int rc = do_update_p(ch);
+    if (!rc) rc = do_update_q(ch);
+    if (!rc) rc = launch_update_locpriori(d, ch->c, ch->m, ch->stream);
+    if (!rc) rc = do_update_z(ch);
+    if (!rc) rc = launch_update_alpha(d, ch->c, ch->m, ch->stream);
+    if (!rc) rc = launch_update_lambda(d, ch->c, ch->m, ch->stream);
+    if (!rc) rc = launch_update_a(d, ch->c, ch->m, ch->stream);
+    if (!rc) rc = launch_update_f(d, ch->c, ch->m, ch->stream);
+    if (!rc) rc = do_record(ch);
+    if (!rc) ch->m++;
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int vps_chain_step(vps_chain* ch, int n)
+{
+    if (!ch) return fail(VPS_ERR_INVALID, "null chain");
+    if (ch->finished) return fail(VPS_ERR_STATE, "chain already finished");
+    VPS_HIP(hipSetDevice(ch->ds->device));
+    for (int s = 0; s < n; ++s) {
+        int rc = do_sweep(ch);
+        if (rc) return rc;
+    }
+    return VPS_OK;
+}
+
+extern "C" int vps_chain_sync(vps_chain* ch)
+{
+    if (!ch) return fail(VPS_ERR_INVALID, "null chain");
+    VPS_HIP(hipStreamSynchronize(ch->stream));
+    return VPS_OK;
+}
+
+extern "C" int vps_chain_finish(vps_chain* ch)
+{
+    if (!ch) return fail(VPS_ERR_INVALID, "null chain");
+    if (ch->finished) return VPS_OK;
+    int rc = launch_arrange(ch->ds->d, ch->c, ch->nr, ch->stream);
+    if (rc) return rc;
+    VPS_HIP(hipStreamSynchronize(ch->stream));
+    ch->finished = 1;
+    return VPS_OK;
+}
+
+extern "C" int vps_chain_get_sizes(vps_chain* ch, int64_t* rlen, int64_t* nlnl)
+{
+    if (!ch) return fail(VPS_ERR_INVALID, "null chain");
+    if (rlen) *rlen = ch->c.rlen;
+    if (nlnl) *nlnl = ch->nlnl;
+    return VPS_OK;
+}
+
+// single passes at the current sweep index (the BAYESIAN::*CUDA seam)
+#define VPS_PASS_PROLOGUE(ch)                                                   \
+    if (!(ch)) return fail(VPS_ERR_INVALID, "null chain");                       \
+    if ((ch)->finished) return fail(VPS_ERR_STATE, "chain already finished");    \
+    VPS_HIP(hipSetDevice((ch)->ds->device));
+
+extern "C" int vps_pass_update_z_admix(vps_chain* ch)
+{
+    VPS_PASS_PROLOGUE(ch);
+    int rc = do_z_admix(ch);
+    if (!rc) VPS_HIP(hipStreamSynchronize(ch->stream));
+    return rc;
+}
+
+extern "C" int vps_pass_update_z_noadmix(vps_chain* ch)
+{
+    VPS_PASS_PROLOGUE(ch);
+    int rc = do_z_noadmix(ch);
+    if (!rc) VPS_HIP(hipStreamSynchronize(ch->stream));
+    return rc;
+}
+
+extern "C" int vps_pass_update_q_noadmix(vps_chain* ch)
+{
+    VPS_PASS_PROLOGUE(ch);
+    int rc = do_q_noadmix(ch);
+    if (!rc) VPS_HIP(hipStreamSynchronize(ch->stream));
+    return rc;
+}
+
+extern "C" int vps_pass_update_q_metro(vps_chain* ch)
+{
+    VPS_PASS_PROLOGUE(ch);
+    int rc = do_q_metro(ch);
+    if (!rc) VPS_HIP(hipStreamSynchronize(ch->stream));
+    return rc;
+}
+
+extern "C" int vps_pass_update_q_admix(vps_chain* ch)
+{
+    VPS_PASS_PROLOGUE(ch);
+    int rc = do_q_admix(ch);
+    if (!rc) VPS_HIP(hipStreamSynchronize(ch->stream));
+    return rc;
+}
+
+extern "C" int vps_pass_update_p(vps_chain* ch)
+{
+    VPS_PASS_PROLOGUE(ch);
+    int rc = do_update_p(ch);
+    if (!rc) VPS_HIP(hipStreamSynchronize(ch->stream));
+    return rc;
+}
+
+extern "C" int vps_pass_update_alpha(vps_chain* ch)
+{
+    VPS_PASS_PROLOGUE(ch);
+    int rc = launch_update_alpha(ch->ds->d, ch->c, ch->m, ch->stream);
+    if (!rc) VPS_HIP(hipStreamSynchronize(ch->stream));
+    return rc;
+}
+
+// the genotype pass of Record alone (lnL into VPS_STATE_LNL; MiSum += Mi)
+extern "C" int vps_pass_record(vps_chain* ch)
+{
+    VPS_PASS_PROLOGUE(ch);
+    int rc = launch_lik_pass(ch->ds->d, ch->c, ch->binaryq ? 1 : 0, ch->stream);
+    if (!rc) rc = launch_record_accumulate(ch->ds->d, ch->c, 0, -1, 1, ch->stream);
+    if (!rc) VPS_HIP(hipStreamSynchronize(ch->stream));
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// state transfer
+// ---------------------------------------------------------------------------
+namespace {
+
+int copy_dev(vps_chain* ch, void* dst, const void* src, size_t bytes, hipMemcpyKind kind)
+{
+    VPS_HIP(hipStreamSynchronize(ch->stream));
+    VPS_HIP(hipMemcpy(dst, src, bytes, kind));
+    return VPS_OK;
+}
+
+}  // namespace
+
+extern "C" int vps_chain_get_state(vps_chain* ch, int which, void* host, size_t bytes)
+{
+    if (!ch || !host) return fail(VPS_ERR_INVALID, "null argument");
+    VPS_HIP(hipSetDevice(ch->ds->device));
+    const ChainDev& c = ch->c;
+    const int64_t N = c.N, K = c.K, KT = c.KT, S = c.S;
+    auto need = [&](size_t b) { return bytes == b ? VPS_OK : fail(VPS_ERR_INVALID, "state %d: %zu bytes given, %zu needed", which, bytes, b); };
+    int rc;
+    switch (which) {
+    case VPS_STATE_FREQ: if ((rc = need(K * KT * 8))) return rc; return copy_dev(ch, host, c.Freq, bytes, hipMemcpyDeviceToHost);
+    case VPS_STATE_Q: if ((rc = need(N * K * 8))) return rc; return copy_dev(ch, host, c.Q, bytes, hipMemcpyDeviceToHost);
+    case VPS_STATE_Z: if ((rc = need(N * 2))) return rc; return copy_dev(ch, host, c.Z, bytes, hipMemcpyDeviceToHost);
+    case VPS_STATE_NI: if ((rc = need(K * KT * 4))) return rc; return copy_dev(ch, host, c.Ni, bytes, hipMemcpyDeviceToHost);
+    case VPS_STATE_MI: {
+        if ((rc = need(N * K * 8))) return rc;
+        std::vector<int32_t> t(N * K);
+        if ((rc = copy_dev(ch, t.data(), c.Mi, N * K * 4, hipMemcpyDeviceToHost))) return rc;
+        for (int64_t x = 0; x < N * K; ++x) ((int64_t*)host)[x] = t[x];
+        return VPS_OK;
+    }
+    case VPS_STATE_ALPHA: if ((rc = need(K * 8))) return rc; return copy_dev(ch, host, c.Alpha, bytes, hipMemcpyDeviceToHost);
+    case VPS_STATE_LAMBDA: if ((rc = need(K * 8))) return rc; return copy_dev(ch, host, c.Lambda, bytes, hipMemcpyDeviceToHost);
+    case VPS_STATE_ITER: if ((rc = need(4))) return rc; *(int32_t*)host = ch->m; return VPS_OK;
+    case VPS_STATE_BUFN1: if ((rc = need(N * 8))) return rc; return copy_dev(ch, host, c.bufN1, bytes, hipMemcpyDeviceToHost);
+    case VPS_STATE_BUFNK1: if ((rc = need(N * K * 8))) return rc; return copy_dev(ch, host, c.bufNK1, bytes, hipMemcpyDeviceToHost);
+    case VPS_STATE_LNL: if ((rc = need(8))) return rc; return copy_dev(ch, host, c.scal + 1, 8, hipMemcpyDeviceToHost);
+    case VPS_STATE_MISUM: if ((rc = need(N * K * 8))) return rc; return copy_dev(ch, host, c.MiSum, bytes, hipMemcpyDeviceToHost);
+    case VPS_STATE_FREQM: if ((rc = need(K * KT * 8))) return rc; return copy_dev(ch, host, c.FreqM, bytes, hipMemcpyDeviceToHost);
+    case VPS_STATE_ROUT: if ((rc = need((size_t)c.rlen * 8))) return rc; return copy_dev(ch, host, c.rout, bytes, hipMemcpyDeviceToHost);
+    case VPS_STATE_FREQA: if ((rc = need(3 * KT * 8))) return rc; return copy_dev(ch, host, c.FreqA, bytes, hipMemcpyDeviceToHost);
+    case VPS_STATE_FST: if ((rc = need(2 * K * 8))) return rc; return copy_dev(ch, host, c.Fst, bytes, hipMemcpyDeviceToHost);
+    case VPS_STATE_BINARYQ: if ((rc = need(4))) return rc; *(int32_t*)host = ch->binaryq; return VPS_OK;
+    case VPS_STATE_LNLTRACE: if ((rc = need((size_t)ch->nlnl * 8))) return rc;
+        return ch->nlnl ? copy_dev(ch, host, c.lnltrace, bytes, hipMemcpyDeviceToHost) : VPS_OK;
+    case VPS_STATE_LOC: {
+        if ((rc = need((size_t)(1 + K + 3 * S * K + S) * 8))) return rc;
+        double* h = (double*)host;
+        if ((rc = copy_dev(ch, h, c.scal, 8, hipMemcpyDeviceToHost))) return rc;
+        if ((rc = copy_dev(ch, h + 1, c.Eta, K * 8, hipMemcpyDeviceToHost))) return rc;
+        if ((rc = copy_dev(ch, h + 1 + K, c.Gamma, S * K * 8, hipMemcpyDeviceToHost))) return rc;
+        if ((rc = copy_dev(ch, h + 1 + K + S * K, c.AlphaLocal, S * K * 8, hipMemcpyDeviceToHost))) return rc;
+        if ((rc = copy_dev(ch, h + 1 + K + 2 * S * K, c.Di, S * K * 8, hipMemcpyDeviceToHost))) return rc;
+        return copy_dev(ch, h + 1 + K + 3 * S * K, c.SumAlpha, S * 8, hipMemcpyDeviceToHost);
+    }
+    default: return fail(VPS_ERR_INVALID, "unknown state %d", which);
+    }
+}
+
+extern "C" int vps_chain_set_state(vps_chain* ch, int which, const void* host, size_t bytes)
+{
+    if (!ch || !host) return fail(VPS_ERR_INVALID, "null argument");
+    VPS_HIP(hipSetDevice(ch->ds->device));
+    ChainDev& c = ch->c;
+    const int64_t N = c.N, K = c.K, KT = c.KT, S = c.S;
+    auto need = [&](size_t b) { return bytes == b ? VPS_OK : fail(VPS_ERR_INVALID, "state %d: %zu bytes given, %zu needed", which, bytes, b); };
+    int rc;
+    switch (which) {
+    case VPS_STATE_FREQ: if ((rc = need(K * KT * 8))) return rc; return copy_dev(ch, c.Freq, host, bytes, hipMemcpyHostToDevice);
+    case VPS_STATE_Q: if ((rc = need(N * K * 8))) return rc; return copy_dev(ch, c.Q, host, bytes, hipMemcpyHostToDevice);
+    case VPS_STATE_Z: {
+        if ((rc = need(N * 2))) return rc;
+        for (int64_t i = 0; i < N; ++i)
+            if (((const uint16_t*)host)[i] >= K) return fail(VPS_ERR_INVALID, "Z[%lld] >= K", (long long)i);
+        return copy_dev(ch, c.Z, host, bytes, hipMemcpyHostToDevice);
+    }
+    case VPS_STATE_NI: if ((rc = need(K * KT * 4))) return rc; return copy_dev(ch, c.Ni, host, bytes, hipMemcpyHostToDevice);
+    case VPS_STATE_MI: {
+        if ((rc = need(N * K * 8))) return rc;
+        std::vector<int32_t> t(N * K);
+        for (int64_t x = 0; x < N * K; ++x) t[x] = (int32_t)((const int64_t*)host)[x];
+        return copy_dev(ch, c.Mi, t.data(), N * K * 4, hipMemcpyHostToDevice);
+    }
+    case VPS_STATE_ALPHA: if ((rc = need(K * 8))) return rc; return copy_dev(ch, c.Alpha, host, bytes, hipMemcpyHostToDevice);
+    case VPS_STATE_LAMBDA: if ((rc = need(K * 8))) return rc; return copy_dev(ch, c.Lambda, host, bytes, hipMemcpyHostToDevice);
+    case VPS_STATE_ITER: if ((rc = need(4))) return rc; ch->m = *(const int32_t*)host; return VPS_OK;
+    case VPS_STATE_BUFN1: if ((rc = need(N * 8))) return rc; return copy_dev(ch, c.bufN1, host, bytes, hipMemcpyHostToDevice);
+    case VPS_STATE_BUFNK1: if ((rc = need(N * K * 8))) return rc; return copy_dev(ch, c.bufNK1, host, bytes, hipMemcpyHostToDevice);
+    case VPS_STATE_MISUM: if ((rc = need(N * K * 8))) return rc; return copy_dev(ch, c.MiSum, host, bytes, hipMemcpyHostToDevice);
+    case VPS_STATE_FREQM: if ((rc = need(K * KT * 8))) return rc; return copy_dev(ch, c.FreqM, host, bytes, hipMemcpyHostToDevice);
+    case VPS_STATE_ROUT: if ((rc = need((size_t)c.rlen * 8))) return rc; return copy_dev(ch, c.rout, host, bytes, hipMemcpyHostToDevice);
+    case VPS_STATE_FREQA: if ((rc = need(3 * KT * 8))) return rc; return copy_dev(ch, c.FreqA, host, bytes, hipMemcpyHostToDevice);
+    case VPS_STATE_FST: if ((rc = need(2 * K * 8))) return rc; return copy_dev(ch, c.Fst, host, bytes, hipMemcpyHostToDevice);
+    case VPS_STATE_BINARYQ: if ((rc = need(4))) return rc; ch->binaryq = *(const int32_t*)host ? 1 : 0; return VPS_OK;
+    case VPS_STATE_LOC: {
+        if ((rc = need((size_t)(1 + K + 3 * S * K + S) * 8))) return rc;
+        const double* h = (const double*)host;
+        if ((rc = copy_dev(ch, c.scal, h, 8, hipMemcpyHostToDevice))) return rc;
+        if ((rc = copy_dev(ch, c.Eta, h + 1, K * 8, hipMemcpyHostToDevice))) return rc;
+        if ((rc = copy_dev(ch, c.Gamma, h + 1 + K, S * K * 8, hipMemcpyHostToDevice))) return rc;
+        if ((rc = copy_dev(ch, c.AlphaLocal, h + 1 + K + S * K, S * K * 8, hipMemcpyHostToDevice))) return rc;
+        if ((rc = copy_dev(ch, c.Di, h + 1 + K + 2 * S * K, S * K * 8, hipMemcpyHostToDevice))) return rc;
+        return copy_dev(ch, c.SumAlpha, h + 1 + K + 3 * S * K, S * 8, hipMemcpyHostToDevice);
+    }
+    default: return fail(VPS_ERR_INVALID, "state %d is read-only or unknown", which);
+    }
+}
+
+// Z-pass timing: every timed pass is bracketed by its own pair of events on
+// the chain's stream, so timing adds no host synchronisation to the sweep.
+extern "C" int vps_chain_enable_timing(vps_chain* ch, int on)
+{
+    if (!ch) return fail(VPS_ERR_INVALID, "null chain");
+    VPS_HIP(hipSetDevice(ch->ds->device));
+    VPS_HIP(hipStreamSynchronize(ch->stream));
+    ch->timing = on != 0;
+    ch->ev_used = 0;
+    return VPS_OK;
+}
+
+extern "C" int vps_chain_kernel_time(vps_chain* ch, int kernel, double* total_ms, int64_t* launches)
+{
+    if (!ch || !total_ms || !launches) return fail(VPS_ERR_INVALID, "null argument");
+    if (kernel != 0) return fail(VPS_ERR_INVALID, "kernel id %d", kernel);
+    VPS_HIP(hipStreamSynchronize(ch->stream));
+    double tot = 0;
+    for (size_t x = 0; x + 1 < ch->ev_used; x += 2) {
+        float ms = 0;
+        VPS_HIP(hipEventElapsedTime(&ms, ch->evs[x], ch->evs[x + 1]));
+        tot += ms;
+    }
+    *total_ms = tot;
+    *launches = (int64_t)(ch->ev_used / 2);
+    return VPS_OK;
+}

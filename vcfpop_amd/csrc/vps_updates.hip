@@ -1,0 +1,831 @@
+// vps_updates.hip — the non-genotype steps of a sweep on the device:
+// Dirichlet draws for P (UpdateP) and Q (UpdateQAdmix, Metropolis
+// proposals/acceptance, PolyLog for binary Q), the scalar Metropolis updates
+// (alpha, lambda, location prior, F model), Record's accumulators and the
+// initial cluster assignment (InitAdmix).
+//
+// Two draw topologies (vps_params::rng_topology):
+//   VPS_RNG_REFERENCE  the reference's streams: P in 128 sequential blocks
+//                      RNG(seed + m*128 + b, UPDATEP) over kl = k*L + l
+//                      (src/structure.cpp:843-861); Q in one stream
+//                      RNG(seed + m, UPDATEQ) over individuals
+//                      (src/structure.cpp:990-1003); A in 32 blocks.
+//                      Parity mode: 128 (resp. 1, 32) device lanes.
+//   VPS_RNG_PARALLEL   one stream per (k, l): RNG(seed + (m*K + k)*L + l,
+//                      UPDATEP); per individual RNG(seed + m*N + i, UPDATEQ);
+//                      per locus RNG(seed + m*L + l, UPDATEA).  Same
+//                      distributions with full device parallelism.
+// Every other stream (Z, alpha, lambda, location prior, F) is the
+// reference's in both modes.
+#include <algorithm>
+#include "vps_chain.h"
+
+namespace vps {
+
+// ---------------------------------------------------------------------------
+// UpdateP, src/structure.cpp:812-862
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void draw_p_one(const DeviceDataset& ds, const ChainDev& c, Rng& r, int64_t k, int64_t l)
+{
+    const int64_t KT = ds.KT;
+    const int afo = ds.afo[l];
+    const int k2 = ds.k2[l];
+    double* p = c.Freq + k * KT + afo;
+    const int32_t* ni = c.Ni + k * KT + afo;
+    const double fk = c.fmodel ? c.Fst[c.K + (c.fsame ? 0 : k)] : 0.0;
+    const double lam = c.Lambda[k];
+    double s = 0;
+    for (int a = 0; a < k2; ++a) {
+        const double prior = c.fmodel ? c.FreqA[afo + a] * fk : lam;
+        const double v = r.gamma(prior + (double)ni[a]);
+        p[a] = v;
+        s += v;
+    }
+    const double inv = 1.0 / s;
+    for (int a = 0; a < k2; ++a) p[a] *= inv;
+}
+
+__global__ void k_update_p_par(const DeviceDataset ds, const ChainDev c, int m)
+{
+    const int64_t kl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (kl >= (int64_t)c.K * ds.L) return;
+    const int64_t k = kl / ds.L, l = kl % ds.L;
+    Rng r;
+    r.init(c.seed + ((uint64_t)m * c.K + k) * (uint64_t)ds.L + l, kSaltUpdateP);
+    draw_p_one(ds, c, r, k, l);
+}
+
+__global__ void k_update_p_ref(const DeviceDataset ds, const ChainDev c, int m)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= 128) return;
+    const int64_t KL = (int64_t)c.K * ds.L;
+    const int64_t kl0 = (int64_t)b * KL / 128, kl1 = (int64_t)(b + 1) * KL / 128;
+    Rng r;
+    r.init(c.seed + (uint64_t)m * 128 + b, kSaltUpdateP);
+    for (int64_t kl = kl0; kl < kl1; ++kl) draw_p_one(ds, c, r, kl / ds.L, kl % ds.L);
+}
+
+int launch_update_p(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+{
+    if (c.topology == VPS_RNG_REFERENCE) {
+        hipLaunchKernelGGL(k_update_p_ref, dim3(2), dim3(64), 0, st, ds, c, m);
+    } else {
+        const int64_t n = (int64_t)c.K * ds.L;
+        hipLaunchKernelGGL(k_update_p_par, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ds, c, m);
+    }
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// UpdateQAdmix, src/structure.cpp:986-1004
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ const double* alpha_row(const DeviceDataset& ds, const ChainDev& c, int i)
+{
+    return c.locpriori ? c.AlphaLocal + (int64_t)ds.popid[i] * c.K : c.Alpha;
+}
+
+__global__ void k_q_admix(const DeviceDataset ds, const ChainDev c, int m)
+{
+    if (c.topology == VPS_RNG_REFERENCE) {
+        if (blockIdx.x != 0 || threadIdx.x != 0) return;
+        Rng r;
+        r.init(c.seed + (uint64_t)m, kSaltUpdateQ);
+        for (int i = 0; i < ds.N; ++i)
+            if (ds.vt[i]) dirichlet_ab(r, c.Q + (int64_t)i * c.K, alpha_row(ds, c, i), c.Mi + (int64_t)i * c.K, c.K);
+        return;
+    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ds.N || !ds.vt[i]) return;
+    Rng r;
+    r.init(c.seed + (uint64_t)m * ds.N + i, kSaltUpdateQ);
+    dirichlet_ab(r, c.Q + (int64_t)i * c.K, alpha_row(ds, c, i), c.Mi + (int64_t)i * c.K, c.K);
+}
+
+int launch_update_q_admix(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+{
+    const unsigned nb = c.topology == VPS_RNG_REFERENCE ? 1 : (unsigned)((ds.N + 255) / 256);
+    hipLaunchKernelGGL(k_q_admix, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c, m);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// UpdateQMetro, src/structure.cpp:1090-1136: proposals q'_i ~ Dir(alpha),
+// the log-ratio pass, then acceptance with conditional uniform draws from
+// the same stream(s).
+// ---------------------------------------------------------------------------
+__global__ void k_q_metro_propose(const DeviceDataset ds, const ChainDev c, int m)
+{
+    if (c.topology == VPS_RNG_REFERENCE) {
+        if (blockIdx.x != 0 || threadIdx.x != 0) return;
+        Rng r;
+        r.init(c.seed + (uint64_t)m, kSaltUpdateQ);
+        for (int i = 0; i < ds.N; ++i)
+            if (ds.vt[i]) dirichlet_a(r, c.bufNK1 + (int64_t)i * c.K, alpha_row(ds, c, i), c.K);
+        c.rngbuf[0] = r;
+        return;
+    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ds.N) return;
+    Rng r;
+    r.init(c.seed + (uint64_t)m * ds.N + i, kSaltUpdateQ);
+    if (ds.vt[i]) dirichlet_a(r, c.bufNK1 + (int64_t)i * c.K, alpha_row(ds, c, i), c.K);
+    c.rngbuf[i] = r;
+}
+
+__global__ void k_q_metro_accept(const DeviceDataset ds, const ChainDev c)
+{
+    const int K = c.K;
+    if (c.topology == VPS_RNG_REFERENCE) {
+        if (blockIdx.x != 0 || threadIdx.x != 0) return;
+        Rng r = c.rngbuf[0];
+        for (int i = 0; i < ds.N; ++i) {
+            if (!ds.vt[i]) continue;
+            if (c.bufN1[i] >= kNZero || r.uniform() < exp(c.bufN1[i]))
+                for (int k = 0; k < K; ++k) c.Q[(int64_t)i * K + k] = c.bufNK1[(int64_t)i * K + k];
+        }
+        return;
+    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ds.N || !ds.vt[i]) return;
+    Rng r = c.rngbuf[i];
+    if (c.bufN1[i] >= kNZero || r.uniform() < exp(c.bufN1[i]))
+        for (int k = 0; k < K; ++k) c.Q[(int64_t)i * K + k] = c.bufNK1[(int64_t)i * K + k];
+}
+
+int launch_q_metro_propose(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+{
+    const unsigned nb = c.topology == VPS_RNG_REFERENCE ? 1 : (unsigned)((ds.N + 255) / 256);
+    hipLaunchKernelGGL(k_q_metro_propose, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c, m);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+int launch_q_metro_accept(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+{
+    (void)m;
+    const unsigned nb = c.topology == VPS_RNG_REFERENCE ? 1 : (unsigned)((ds.N + 255) / 256);
+    hipLaunchKernelGGL(k_q_metro_accept, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// UpdateQNoAdmix's draw: Q = 0, then per typed individual PolyLog over the
+// K log likelihoods picks Z_i and sets Q[i][Z_i] = 1 (src/structureAVX.cpp:
+// 279-287).  bufNK1 rows are overwritten by PolyLog, as in the reference.
+// ---------------------------------------------------------------------------
+__global__ void k_q_noadmix_draw(const DeviceDataset ds, const ChainDev c, int m)
+{
+    const int K = c.K;
+    if (c.topology == VPS_RNG_REFERENCE) {
+        if (blockIdx.x != 0 || threadIdx.x != 0) return;
+        Rng r;
+        r.init(c.seed + (uint64_t)m, kSaltUpdateQ);
+        for (int i = 0; i < ds.N; ++i) {
+            for (int k = 0; k < K; ++k) c.Q[(int64_t)i * K + k] = 0;
+            if (!ds.vt[i]) continue;
+            const int k2 = polylog(r, c.bufNK1 + (int64_t)i * K, K);
+            c.Q[(int64_t)i * K + k2] = 1;
+            c.Z[i] = (uint16_t)k2;
+        }
+        return;
+    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ds.N) return;
+    for (int k = 0; k < K; ++k) c.Q[(int64_t)i * K + k] = 0;
+    if (!ds.vt[i]) return;
+    Rng r;
+    r.init(c.seed + (uint64_t)m * ds.N + i, kSaltUpdateQ);
+    const int k2 = polylog(r, c.bufNK1 + (int64_t)i * K, K);
+    c.Q[(int64_t)i * K + k2] = 1;
+    c.Z[i] = (uint16_t)k2;
+}
+
+int launch_q_noadmix_draw(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+{
+    const unsigned nb = c.topology == VPS_RNG_REFERENCE ? 1 : (unsigned)((ds.N + 255) / 256);
+    hipLaunchKernelGGL(k_q_noadmix_draw, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c, m);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+// Di[s][k] = sum of Q rows of typed individuals of location s
+// (UpdateQ, src/structure.cpp:1148-1155); Mi[i][Z_i] = vt_i for binary Q
+// (UpdateZNoAdmix, src/structure.cpp:1268-1269)
+__global__ void k_update_di(const DeviceDataset ds, const ChainDev c)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= c.S * c.K) return;
+    const int s = x / c.K, k = x % c.K;
+    double acc = 0;
+    for (int i = ds.pop_ind0[s]; i < ds.pop_ind0[s] + ds.pop_nind[s]; ++i)
+        if (ds.vt[i]) acc += c.Q[(int64_t)i * c.K + k];
+    c.Di[x] = acc;
+}
+
+int launch_update_di(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_update_di, dim3((unsigned)((c.S * c.K + 63) / 64)), dim3(64), 0, st, ds, c);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// block-cooperative LogProd (any order; agrees with the sequential one to
+// rounding).  Every thread of the block gets the result.
+// ---------------------------------------------------------------------------
+__device__ double block_logprod(const double* a, int64_t n, int64_t sep, int64_t* sh_s, double* sh_p)
+{
+    LogAcc acc;
+    acc.open();
+    for (int64_t x = threadIdx.x; x < n; x += blockDim.x) acc.charge(a[x * sep]);
+    LogAcc::strip(acc.slog, acc.prod);
+    sh_s[threadIdx.x] = acc.slog;
+    sh_p[threadIdx.x] = acc.prod;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            int64_t s = sh_s[threadIdx.x] + sh_s[threadIdx.x + w];
+            double p = sh_p[threadIdx.x] * sh_p[threadIdx.x + w];
+            LogAcc::strip(s, p);
+            sh_s[threadIdx.x] = s;
+            sh_p[threadIdx.x] = p;
+        }
+        __syncthreads();
+    }
+    const double r = ((double)sh_s[0] + log2(sh_p[0])) * kLn2Trunc;
+    __syncthreads();
+    return r;
+}
+
+__device__ double block_sum(const double* a, int64_t n, int64_t sep, double* sh)
+{
+    double s = 0;
+    for (int64_t x = threadIdx.x; x < n; x += blockDim.x) s += a[x * sep];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+constexpr int kScalarThreads = 256;
+
+// ---------------------------------------------------------------------------
+// UpdateAlpha, src/structure.cpp:1501-1594 (one block; the likelihood sums
+// are block reductions, the Metropolis chain runs on thread 0)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kScalarThreads) void k_update_alpha(const DeviceDataset ds, const ChainDev c, int m)
+{
+    __shared__ int64_t sh_s[kScalarThreads];
+    __shared__ double sh_p[kScalarThreads];
+    __shared__ double lp[VPS_MAX_K * 64 + 1];
+    const int K = c.K, N = ds.N, S = c.S;
+    if (c.locpriori && c.admix) {
+        // log prod of Q column k over the individuals of each location
+        for (int s = 0; s < S && s < 64; ++s)
+            for (int k = 0; k < K; ++k)
+                lp[s * K + k] = block_logprod(c.Q + (int64_t)ds.pop_ind0[s] * K + k, ds.pop_nind[s], K, sh_s, sh_p);
+        if (threadIdx.x != 0) return;
+        Rng r;
+        r.init(c.seed + (uint64_t)m, kSaltUpdateAlpha);
+        const double rr = c.scal[0];
+        for (int k = 0; k < K; ++k) {
+            const double am = r.normal(c.Alpha[k], c.stdalpha);
+            if (am > 0 && am < c.maxalpha) {
+                const double d = am - c.Alpha[k];
+                double dlnL = S * (d * rr * my_log(rr) + lgamma(rr * c.Alpha[k]) - lgamma(rr * am));
+                dlnL += rr * d * logprod_sep(c.AlphaLocal + k, S, K);
+                if (dlnL >= kNZero || r.uniform() < exp(dlnL)) c.Alpha[k] = am;
+            }
+        }
+        if (m % 10 == 0)
+            for (int s = 0; s < S; ++s) {
+                double sm = 0;
+                for (int k = 0; k < K; ++k) sm += c.AlphaLocal[s * K + k];
+                c.SumAlpha[s] = sm;
+            }
+        for (int s = 0; s < S; ++s) {
+            for (int k = 0; k < K; ++k) {
+                const double ao = c.AlphaLocal[s * K + k], am = r.normal(ao, c.stdalpha);
+                if (am > 0 && am < c.maxalpha) {
+                    const double d = am - ao;
+                    double dlnL = (rr * c.Alpha[k] - 1) * my_log(am / ao) - d * rr +
+                                  ds.pop_nind[s] * (lgamma(c.SumAlpha[s] + d) - lgamma(am) - lgamma(c.SumAlpha[s]) + lgamma(ao));
+                    const double lq = s < 64 ? lp[s * K + k] : logprod_sep(c.Q + (int64_t)ds.pop_ind0[s] * K + k, ds.pop_nind[s], K);
+                    dlnL += d * lq;
+                    if (dlnL >= kNZero || r.uniform() < exp(dlnL)) {
+                        c.AlphaLocal[s * K + k] = am;
+                        c.SumAlpha[s] += d;
+                    }
+                }
+            }
+        }
+        return;
+    }
+    if (!c.inferalpha) return;
+    if (!c.admix && m >= c.nadmburnin) return;
+    if (c.diffalpha) {
+        for (int k = 0; k < K; ++k) lp[k] = block_logprod(c.Q + k, N, K, sh_s, sh_p);
+        if (threadIdx.x != 0) return;
+        Rng r;
+        r.init(c.seed + (uint64_t)m, kSaltUpdateAlpha);
+        double sumalpha = 0;
+        for (int k = 0; k < K; ++k) sumalpha += c.Alpha[k];
+        for (int k = 0; k < K; ++k) {
+            const double am = r.normal(c.Alpha[k], c.stdalpha);
+            if (am <= 0 || (am >= c.maxalpha && c.uniformalpha)) continue;
+            const double ao = c.Alpha[k];
+            const double d = am - ao;
+            double dlnL = c.uniformalpha ? 0 : ((c.alphapriora - 1) * my_log(am / ao) + (ao - am) / c.alphapriorb);
+            dlnL += (am - ao) * lp[k] + (lgamma(sumalpha + d) - lgamma(sumalpha) - lgamma(am) + lgamma(ao)) * N;
+            if (dlnL >= kNZero || r.uniform() < exp(dlnL)) c.Alpha[k] = am;
+        }
+        return;
+    }
+    const double lq = block_logprod(c.Q, (int64_t)N * K, 1, sh_s, sh_p);
+    if (threadIdx.x != 0) return;
+    Rng r;
+    r.init(c.seed + (uint64_t)m, kSaltUpdateAlpha);
+    const double am = r.normal(c.Alpha[0], c.stdalpha);
+    if (am <= 0 || (am >= c.maxalpha && c.uniformalpha)) return;
+    const double ao = c.Alpha[0];
+    double dlnL = c.uniformalpha ? 0 : ((c.alphapriora - 1) * my_log(am / ao) + (ao - am) / c.alphapriorb);
+    dlnL += (am - ao) * lq + (lgamma(K * am) - lgamma(K * ao) - K * lgamma(am) + K * lgamma(ao)) * N;
+    if (dlnL >= kNZero || r.uniform() < exp(dlnL))
+        for (int k = 0; k < K; ++k) c.Alpha[k] = am;
+}
+
+int launch_update_alpha(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+{
+    if (c.locpriori && c.admix && c.S > 64) {
+        // per-location sums beyond 64 locations run on thread 0 (correct, slower)
+    }
+    hipLaunchKernelGGL(k_update_alpha, dim3(1), dim3(kScalarThreads), 0, st, ds, c, m);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// UpdateLambda, src/structure.cpp:1598-1635 (runs only when inferlambda)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kScalarThreads) void k_update_lambda(const DeviceDataset ds, const ChainDev c, int m)
+{
+    __shared__ int64_t sh_s[kScalarThreads];
+    __shared__ double sh_p[kScalarThreads];
+    __shared__ double lp[VPS_MAX_K + 1];
+    const int K = c.K;
+    const int64_t KT = ds.KT;
+    if (c.difflambda) {
+        for (int k = 0; k < K; ++k) lp[k] = block_logprod(c.Freq + k * KT, KT, 1, sh_s, sh_p);
+    } else {
+        lp[0] = c.fmodel ? block_logprod(c.FreqA, KT, 1, sh_s, sh_p) : block_logprod(c.Freq, K * KT, 1, sh_s, sh_p);
+    }
+    if (threadIdx.x != 0) return;
+    Rng r;
+    r.init(c.seed + (uint64_t)m, kSaltUpdateLambda);
+    if (c.difflambda) {
+        for (int k = 0; k < K; ++k) {
+            const double lo = c.Lambda[k], lm = r.normal(lo, c.stdlambda);
+            if (lm <= 0 || lm >= c.maxlambda) continue;
+            double dlnL = 0, gd = lgamma(lo) - lgamma(lm);
+            for (int i = 2; i <= ds.maxK; ++i) dlnL += ds.kdis[i] * (lgamma(i * lm) - lgamma(i * lo) + i * gd);
+            dlnL += (lm - lo) * lp[k];
+            if (dlnL >= kNZero || r.uniform() < exp(dlnL)) c.Lambda[k] = lm;
+        }
+    } else {
+        const int np = c.fmodel ? 1 : K;
+        const double lo = c.Lambda[0], lm = r.normal(lo, c.stdlambda);
+        if (lm <= 0 || lm >= c.maxlambda) return;
+        double dlnL = 0, gd = lgamma(lo) - lgamma(lm);
+        for (int i = 2; i <= ds.maxK; ++i) dlnL += ds.kdis[i] * np * (lgamma(i * lm) - lgamma(i * lo) + i * gd);
+        dlnL += (lm - lo) * lp[0];
+        if (dlnL >= kNZero || r.uniform() < exp(dlnL))
+            for (int k = 0; k < K; ++k) c.Lambda[k] = lm;
+    }
+}
+
+int launch_update_lambda(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+{
+    if (!c.inferlambda) return VPS_OK;
+    hipLaunchKernelGGL(k_update_lambda, dim3(1), dim3(kScalarThreads), 0, st, ds, c, m);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// UpdateLocPriori, src/structure.cpp:1170-1255 (S x K scalars, thread 0)
+// ---------------------------------------------------------------------------
+__global__ void k_update_locpriori(const DeviceDataset ds, const ChainDev c, int m)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int K = c.K, S = c.S;
+    Rng rng;
+    rng.init(c.seed + (uint64_t)m, kSaltUpdateLoc);
+    double r = c.scal[0];
+    if (c.admix) {
+        const double rm = rng.uniform(r - c.epsr, r + c.epsr);
+        if (rm > 0 && rm < c.maxr) {
+            double dlnL = 0, d = rm - r, dt1 = rm * my_log(rm) - r * my_log(r);
+            for (int k = 0; k < K; ++k) {
+                dlnL += S * (c.Alpha[k] * dt1 - lgamma(rm * c.Alpha[k]) + lgamma(r * c.Alpha[k]));
+                double sa = 0;
+                for (int s = 0; s < S; ++s) sa += c.AlphaLocal[s * K + k];
+                dlnL += c.Alpha[k] * d * logprod_sep(c.AlphaLocal + k, S, K) - sa * d;
+            }
+            if (dlnL >= kNZero || rng.uniform() < exp(dlnL)) c.scal[0] = rm;
+        }
+        return;
+    }
+    {
+        const double rm = rng.uniform(r - c.epsr, r + c.epsr);
+        if (rm > 0 && rm < c.maxr) {
+            double dlnL = lgamma(rm) - lgamma(r);
+            for (int k = 0; k < K; ++k) dlnL += lgamma(r * c.Eta[k]) - lgamma(rm * c.Eta[k]);
+            dlnL *= S;
+            for (int k = 0; k < K; ++k) dlnL += (rm - r) * c.Eta[k] * logprod_sep(c.Gamma + k, S, K);
+            if (dlnL >= kNZero || rng.uniform() < exp(dlnL)) c.scal[0] = rm;
+        }
+    }
+    r = c.scal[0];
+    if (K >= 2) {
+        const int64_t i1 = (int64_t)rng.next(K), j1 = (int64_t)rng.next_avoid(K, i1);
+        const double delta = rng.uniform(0, c.epseta);
+        const double e1 = c.Eta[i1] + delta, e2 = c.Eta[j1] - delta;
+        if (e1 < 1 && e2 > 0) {
+            double dlnL = S * (lgamma(r * c.Eta[i1]) + lgamma(r * c.Eta[j1]) - lgamma(r * e1) - lgamma(r * e2));
+            LogAcc a1, a2;
+            a1.open(); a2.open();
+            for (int s = 0; s < S; ++s) { a1.charge(c.Gamma[s * K + i1]); a2.charge(c.Gamma[s * K + j1]); }
+            dlnL += r * delta * (a1.close() - a2.close());
+            if (dlnL >= kNZero || rng.uniform() < exp(dlnL)) {
+                c.Eta[i1] += delta;
+                c.Eta[j1] -= delta;
+            }
+        }
+        for (int s = 0; s < S; ++s) {
+            const int64_t a = (int64_t)rng.next(K), b = (int64_t)rng.next_avoid(K, a);
+            const double dl = rng.uniform(0, c.epsgamma);
+            double* G = c.Gamma + (int64_t)s * K;
+            if (G[a] + dl < 1 && G[b] - dl > 0) {
+                const double dlnL = (r * c.Eta[a] - 1.0 + c.Di[s * K + a]) * my_log(1 + dl / G[a]) +
+                                    (r * c.Eta[b] - 1.0 + c.Di[s * K + b]) * my_log(1 - dl / G[b]);
+                if (dlnL >= kNZero || rng.uniform() < exp(dlnL)) {
+                    G[a] += dl;
+                    G[b] -= dl;
+                }
+            }
+        }
+    }
+    if (m % 10 == 0) {
+        unify_row(c.Eta, K);
+        for (int s = 0; s < S; ++s) unify_row(c.Gamma + (int64_t)s * K, K);
+    }
+}
+
+int launch_update_locpriori(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+{
+    if (!c.locpriori) return VPS_OK;
+    hipLaunchKernelGGL(k_update_locpriori, dim3(1), dim3(64), 0, st, ds, c, m);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// UpdateA (A1: Dirichlet proposal / A2: pairwise shift), src/structure.cpp:
+// 1639-1804, and UpdateF, src/structure.cpp:1808-1858
+// ---------------------------------------------------------------------------
+__device__ void update_a1_locus(const DeviceDataset& ds, const ChainDev& c, Rng& r, int64_t l)
+{
+    const int K = c.K;
+    const int64_t KT = ds.KT;
+    const double lam = c.Lambda[0];
+    const int k2 = ds.k2[l];
+    const int afo = ds.afo[l];
+    double* pa = c.FreqA + afo;
+    double pb[VPS_MAX_ALLELES], pc[VPS_MAX_ALLELES];
+    for (int a = 0; a < k2; ++a) pb[a] = lam;
+    for (int k = 0; k < K; ++k)
+        for (int a = 0; a < k2; ++a) pb[a] += c.Freq[k * KT + afo + a] * c.Fst[K + k];
+    dirichlet_a(r, pc, pb, k2);
+    double dlnL = 0;
+    for (int a = 0; a < k2; ++a) dlnL += (pb[a] - lam) * (my_log(pa[a] / pc[a]));
+    for (int k = 0; k < K; ++k) {
+        const double* pk = c.Freq + k * KT + afo;
+        const double fr = c.Fst[K + k];
+        for (int a = 0; a < k2; ++a)
+            dlnL += lgamma(fr * pa[a]) - lgamma(fr * pc[a]) + fr * (pc[a] - pa[a]) * my_log(pk[a]);
+    }
+    if (dlnL >= kNZero || r.uniform() < exp(dlnL))
+        for (int a = 0; a < k2; ++a) pa[a] = pc[a];
+}
+
+__device__ void update_a2_locus(const DeviceDataset& ds, const ChainDev& c, Rng& r, int64_t l)
+{
+    const int K = c.K;
+    const int64_t KT = ds.KT;
+    const double lam = c.Lambda[0];
+    const int k2 = ds.k2[l];
+    const int afo = ds.afo[l];
+    double* pa = c.FreqA + afo;
+    const double* p = c.Freq + afo;
+    const int m1 = (int)r.next(k2), n1 = (int)r.next_avoid(k2, m1);
+    const double delta = r.uniform(0, pow((double)ds.N, -0.5));
+    const double pm0 = pa[m1], pm1 = pm0 + delta, pn0 = pa[n1], pn1 = pn0 - delta;
+    if (pm1 >= 1 || pn1 <= 0) return;
+    double dlnL = (lam - 1) * my_log(pm1 * pn1 / pm0 / pn0);
+    for (int k = 0; k < K; ++k) {
+        const double fr = c.Fst[K + k];
+        dlnL += lgamma(fr * pm0) + lgamma(fr * pn0) - lgamma(fr * pm1) - lgamma(fr * pn1) +
+                (fr * delta) * my_log(p[k * KT + m1] / p[k * KT + n1]);
+    }
+    if (dlnL >= kNZero || r.uniform() < exp(dlnL)) { pa[m1] = pm1; pa[n1] = pn1; }
+}
+
+__global__ void k_update_a(const DeviceDataset ds, const ChainDev c, int m, int use_a1)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c.topology == VPS_RNG_REFERENCE) {
+        if (x >= 32) return;
+        const int64_t l0 = x * ds.L / 32, l1 = (x + 1) * ds.L / 32;
+        Rng r;
+        r.init(c.seed + (uint64_t)m * 32 + x, kSaltUpdateA);
+        for (int64_t l = l0; l < l1; ++l) use_a1 ? update_a1_locus(ds, c, r, l) : update_a2_locus(ds, c, r, l);
+        return;
+    }
+    if (x >= ds.L) return;
+    Rng r;
+    r.init(c.seed + (uint64_t)m * ds.L + x, kSaltUpdateA);
+    use_a1 ? update_a1_locus(ds, c, r, x) : update_a2_locus(ds, c, r, x);
+}
+
+// HashULong (CRC32C, init 0xB385FDA9) on the host decides A1 vs A2
+uint32_t hash_ulong_host(uint64_t v);
+
+int launch_update_a(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+{
+    if (!c.fmodel) return VPS_OK;
+    const int use_a1 = hash_ulong_host(c.seed + (uint64_t)m) & 1;
+    const unsigned nb = c.topology == VPS_RNG_REFERENCE ? 1 : (unsigned)((ds.L + 255) / 256);
+    hipLaunchKernelGGL(k_update_a, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c, m, use_a1);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+__global__ __launch_bounds__(kScalarThreads) void k_update_f(const DeviceDataset ds, const ChainDev c, int m)
+{
+    __shared__ double sh[kScalarThreads];
+    __shared__ double bc[4];
+    const int K = c.K;
+    const int64_t KT = ds.KT;
+    Rng r;
+    if (threadIdx.x == 0) r.init(c.seed + (uint64_t)m, kSaltUpdateF);
+    for (int k = 0; k < K; ++k) {
+        if (threadIdx.x == 0) {
+            const double Fo = c.Fst[k];
+            const double Fm = r.normal(Fo, c.stdf);
+            bc[0] = Fo; bc[1] = Fm;
+            bc[2] = (Fm < 0 || Fm > 1) ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        if (bc[2] != 0.0) {
+            if (c.fsame) break;
+            __syncthreads();
+            continue;
+        }
+        const double Fo = bc[0], Fm = bc[1];
+        const double fo = c.Fst[K + k], fm = (1 - Fm) / Fm;
+        double part = 0;
+        for (int k2 = k; k2 < K; ++k2) {
+            const double* p = c.Freq + k2 * KT;
+            double s = 0;
+            for (int64_t a = threadIdx.x; a < KT; a += blockDim.x) {
+                const double pa2 = c.FreqA[a];
+                s += pa2 * (fm - fo) * my_log(p[a]) + lgamma(fo * pa2) - lgamma(fm * pa2);
+            }
+            part += s;
+            if (!c.fsame) break;
+        }
+        sh[threadIdx.x] = part;
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            double dlnL = c.pmeanf * (Fo - Fm) / (c.pstdf * c.pstdf) +
+                          (c.pmeanf * c.pmeanf / (c.pstdf * c.pstdf) - 1) * my_log(Fm / Fo) +
+                          (double)((int64_t)(c.fsame ? K : 1) * ds.L) * (lgamma(fm) - lgamma(fo));
+            dlnL += sh[0];
+            if (dlnL >= kNZero || r.uniform() < exp(dlnL)) {
+                if (c.fsame) for (int j = 0; j < K; ++j) { c.Fst[j] = Fm; c.Fst[K + j] = fm; }
+                else { c.Fst[k] = Fm; c.Fst[K + k] = fm; }
+            }
+        }
+        __syncthreads();
+        if (c.fsame) break;
+    }
+}
+
+int launch_update_f(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+{
+    if (!c.fmodel) return VPS_OK;
+    hipLaunchKernelGGL(k_update_f, dim3(1), dim3(kScalarThreads), 0, st, ds, c, m);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Record's accumulators, src/structure.cpp:1877-1878 and 2017-2041
+// ---------------------------------------------------------------------------
+__global__ void k_record_acc(const DeviceDataset ds, const ChainDev c, int post, int trace_slot, int acc_misum)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int K = c.K;
+    const int64_t nmi = (int64_t)ds.N * K, nfr = (int64_t)K * ds.KT;
+    if (acc_misum && x < nmi) c.MiSum[x] += c.Mi[x];
+    if (post && x < nfr) c.FreqM[x] += c.Freq[x];
+    if (x != 0) return;
+    const double lik = c.scal[1];
+    if (trace_slot >= 0) c.lnltrace[trace_slot] = lik;
+    if (!post) return;
+    c.rout[0] += lik;
+    c.rout[1] += lik * lik;
+    const int nf = c.fmodel ? (c.fsame ? 1 : K) : 0;
+    const int na = c.admix ? (c.diffalpha ? K : 1) : 0;
+    const int nl = c.difflambda ? K : 1;
+    for (int j = 0; j < nl; ++j) c.rout[4 + j] += c.Lambda[j];
+    if (c.locpriori) {
+        c.rout[4 + nl] += c.scal[0];
+        for (int k = 0; k < K; ++k) c.rout[4 + nl + 1 + k] += c.admix ? c.Alpha[k] : c.Eta[k];
+        for (int j = 0; j < c.S * K; ++j) c.rout[4 + nl + 1 + K + j] += c.admix ? c.AlphaLocal[j] : c.Gamma[j];
+    } else if (c.admix) {
+        for (int j = 0; j < na; ++j) c.rout[4 + nl + j] += c.Alpha[j];
+    }
+    for (int j = 0; j < nf; ++j) c.rout[c.rlen - nf + j] += c.Fst[j];
+}
+
+int launch_record_accumulate(const DeviceDataset& ds, const ChainDev& c, int post, int trace_slot, int acc_misum, hipStream_t st)
+{
+    const int64_t n = std::max<int64_t>((int64_t)ds.N * c.K, (int64_t)c.K * ds.KT);
+    hipLaunchKernelGGL(k_record_acc, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ds, c, post, trace_slot, acc_misum);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// InitAdmix, src/structure.cpp:557-649.  Non-single-Z start: per locus one
+// stream RNG(seed + l, INITADM), Next(K) per typed allele copy (no draws for
+// missing copies).  Lanes = loci; allele counts in lane-owned LDS columns;
+// per individual the wavefront ballots its K cluster counts into Mi.
+// Single-Z start: Z_i = Next(K) from RNG(seed - 1, INITADM), Mi[i][Z_i] = vt_i.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_init_admix(const DeviceDataset ds, const ChainDev c, int A, int list_kind)
+{
+    extern __shared__ int32_t ni_lds[];
+    const int lane = threadIdx.x;
+    const int K = c.K;
+    for (int x = lane; x < K * A * kTileLoci; x += 64) ni_lds[x] = 0;
+    __syncthreads();
+    const int64_t tix = blockIdx.x;
+    const int64_t ntl = list_kind == kKindBi ? ds.nbi : ds.ngen;
+    if (tix >= ntl) return;
+    const TileDesc t = ds.tiles[list_kind == kKindBi ? ds.bi_list[tix] : ds.gen_list[tix]];
+    const bool valid = lane < t.nloc;
+    const int64_t l = t.l0 + (valid ? lane : 0);
+    const int afo = ds.afo[l];
+    const int k2 = ds.k2[l];
+    const uint64_t tab = valid ? ds.bitab[l] : 0ull;
+    const uint64_t* gd = ds.gdesc + ds.gbase[l];
+    Rng r;
+    r.init(c.seed + (uint64_t)l, kSaltInitAdm);
+    const int bits = t.bits, ipw = t.ipw;
+    const uint32_t mask = (1u << bits) - 1u;
+    const uint32_t* wp = ds.words + t.wbase + lane;
+    uint32_t word = wp[0];
+    int j = 0, w = 0;
+    for (int i = 0; i < ds.N; ++i) {
+        const uint32_t gid = word & mask;
+        word >>= bits;
+        if (++j == ipw) {
+            j = 0; ++w;
+            if (i + 1 < ds.N) word = wp[(int64_t)w * kTileLoci];
+        }
+        int v, h = 0;
+        uint64_t d64 = 0;
+        if (t.kind == kKindBi) {
+            const uint32_t desc = (uint32_t)(tab >> (gid * 8)) & 0xFFu;
+            v = (int)(desc & 15u);
+            h = v - (int)(desc >> 4);
+        } else {
+            d64 = valid ? gd[gid] : 0ull;
+            v = (int)(d64 & 15u);
+        }
+        const int MP = ds.maxploidy;
+        for (int a = 0; a < MP; ++a) {
+            const bool typed = a < v;
+            int kk = 0;
+            if (typed) {
+                kk = (int)r.next((uint64_t)K);
+                const int x = t.kind == kKindBi ? (a >= h ? 1 : 0) : (int)((d64 >> (4 + 6 * a)) & 63u);
+                ni_lds[(kk * A + x) * kTileLoci + lane] += 1;
+            }
+            for (int k = 0; k < K; ++k) {
+                const int cnt = __builtin_popcountll(__builtin_amdgcn_ballot_w64(typed && kk == k));
+                if (lane == 0 && cnt) atomicAdd(c.Mi + (int64_t)i * K + k, cnt);
+            }
+        }
+    }
+    if (valid)
+        for (int k = 0; k < K; ++k)
+            for (int a = 0; a < k2; ++a) c.Ni[(int64_t)k * ds.KT + afo + a] = ni_lds[(k * A + a) * kTileLoci + lane];
+}
+
+__global__ void k_init_singlez(const DeviceDataset ds, const ChainDev c)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    Rng r;
+    r.init(c.seed - 1, kSaltInitAdm);
+    for (int i = 0; i < ds.N; ++i) {
+        const int z = (int)r.next((uint64_t)c.K);
+        c.Z[i] = (uint16_t)z;
+        c.Mi[(int64_t)i * c.K + z] = ds.vt[i];
+    }
+}
+
+__global__ void k_set_mi_binary(const DeviceDataset ds, const ChainDev c)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= (int64_t)ds.N * c.K) return;
+    const int i = (int)(x / c.K), k = (int)(x % c.K);
+    c.Mi[x] = c.Z[i] == k ? ds.vt[i] : 0;
+}
+
+int launch_set_mi_binary(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
+{
+    const int64_t n = (int64_t)ds.N * c.K;
+    hipLaunchKernelGGL(k_set_mi_binary, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ds, c);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Arrange, src/structure.cpp:1862-1869: MiSum rows -> normalised doubles with
+// MIN_FREQ (UnifyInt64ToDouble), rout and FreqM scaled by 1/nr, variance and
+// lnP(D) = mean - var/2.
+// ---------------------------------------------------------------------------
+__global__ void k_arrange(const DeviceDataset ds, const ChainDev c, int nr)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int K = c.K;
+    if (x < ds.N) {
+        int64_t* row = c.MiSum + x * K;
+        double* rd = reinterpret_cast<double*>(row);
+        double tmp[VPS_MAX_K];
+        for (int k = 0; k < K; ++k) tmp[k] = (double)row[k];
+        unify_row(tmp, K);
+        for (int k = 0; k < K; ++k) rd[k] = tmp[k];
+    }
+    const double inv = 1.0 / nr;
+    if (x < (int64_t)K * ds.KT) c.FreqM[x] *= inv;
+    if (x == 0) {
+        for (int j = 0; j < c.rlen; ++j) c.rout[j] *= inv;
+        c.rout[2] = c.rout[1] - c.rout[0] * c.rout[0];
+        c.rout[3] = c.rout[0] - c.rout[2] / 2;
+    }
+}
+
+int launch_arrange(const DeviceDataset& ds, const ChainDev& c, int nr, hipStream_t st)
+{
+    const int64_t n = std::max<int64_t>((int64_t)ds.N, (int64_t)c.K * ds.KT);
+    hipLaunchKernelGGL(k_arrange, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ds, c, nr);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+int launch_init_admix(const DeviceDataset& ds, const ChainDev& c, int singlez, hipStream_t st)
+{
+    int rc = launch_fill_i32(c.Mi, (int64_t)ds.N * c.K, 0, st);
+    if (rc) return rc;
+    if (singlez) {
+        hipLaunchKernelGGL(k_init_singlez, dim3(1), dim3(64), 0, st, ds, c);
+        VPS_HIP(hipGetLastError());
+        return launch_z_noadmix(ds, c, st);
+    }
+    const int A = ds.maxK > 2 ? ds.maxK : 2;
+    const size_t shm = (size_t)c.K * A * kTileLoci * sizeof(int32_t);
+    if (shm > 64 * 1024) return fail(VPS_ERR_UNSUPPORTED, "initial count table K=%d x %d alleles exceeds LDS", c.K, A);
+    if (ds.nbi > 0) hipLaunchKernelGGL(k_init_admix, dim3((unsigned)ds.nbi), dim3(64), shm, st, ds, c, A, kKindBi);
+    if (ds.ngen > 0) hipLaunchKernelGGL(k_init_admix, dim3((unsigned)ds.ngen), dim3(64), shm, st, ds, c, A, kKindGen);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+}  // namespace vps

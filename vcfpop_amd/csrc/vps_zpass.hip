@@ -1,0 +1,458 @@
+// vps_zpass.hip — allele-copy origin sampling (UpdateZAdmix) and the
+// binary-Q count pass (UpdateZNoAdmix) on gfx950.
+//
+// Reference semantics (src/structureAVX.cpp:788-981, the default x86 path):
+// for every sorted locus l one stream RNG(seed + m*L + l, UPDATEZ) draws
+// exactly one number per (individual i ascending, slot a < maxploidy),
+// including slots that are missing or beyond the genotype's ploidy.  For a
+// typed slot with allele x the weights are w_k = P[k][l][x] * Q[i][k] +
+// MIN_FREQ, s = w_0 + ... + w_{K-1}, t = (U - 1) * s with U in [1,2) built
+// from the draw's 52 low bits, and the cluster is the first k with t < w_k
+// after subtracting w_0..w_{k-1} (K-1 if none), src/mathAVX.cpp:73-122.
+//
+// Mapping: one wavefront = one 64-locus tile x one chunk of individuals.
+// Lane j owns locus l0 + j and its stream; a wavefront that starts at
+// individual i0 jumps its stream ahead by i0 * maxploidy steps with the
+// GF(2) table of its chunk, so a locus is split over G wavefronts without
+// changing any draw.  Per slot:
+//   * "first k with t_k < w_k" is evaluated as sign tests of the running
+//     remainder: t_k < w_k  <=>  fl(t_k - w_k) < 0 for finite doubles, and
+//     once negative the remainder stays negative, so
+//     C_k = [t_{k+1} < 0] = [k* <= k]  (cumulative indicator, k < K-1).
+//   * Ni: each lane accumulates C_k per allele in registers;
+//     Ni[k] = C_k - C_{k-1} after the chunk (linear in the counts).
+//   * Mi: per k the wavefront ballots C_k over typed lanes and popcounts
+//     it on the scalar unit; per individual the K cluster counts go to an
+//     LDS row shared by the block's wavefronts (same individuals, other
+//     tiles), and the block adds its rows to Mi once at the end.
+#include "vps_chain.h"
+
+namespace vps {
+
+// apply a jump table (32 nibbles x 16 entries x 128 bits) to a stream
+__device__ __forceinline__ void jump_apply(const uint32_t* __restrict__ tab, Xs128& r)
+{
+    const uint32_t st[4] = {(uint32_t)r.x, (uint32_t)(r.x >> 32), (uint32_t)r.y, (uint32_t)(r.y >> 32)};
+    uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    const uint4* t4 = reinterpret_cast<const uint4*>(tab);
+#pragma unroll
+    for (int p = 0; p < 32; ++p) {
+        const uint32_t nib = (st[p >> 3] >> ((p & 7) * 4)) & 15u;
+        const uint4 e = t4[p * 16 + nib];
+        a0 ^= e.x; a1 ^= e.y; a2 ^= e.z; a3 ^= e.w;
+    }
+    r.x = (uint64_t)a0 | ((uint64_t)a1 << 32);
+    r.y = (uint64_t)a2 | ((uint64_t)a3 << 32);
+}
+
+template <typename Real> __device__ __forceinline__ Real unit_minus1(uint64_t r);
+template <> __device__ __forceinline__ double unit_minus1<double>(uint64_t r) { return unit12(r) - 1.0; }
+// FP32 weights: U keeps the 23 high bits of the 52-bit mantissa
+template <> __device__ __forceinline__ float unit_minus1<float>(uint64_t r)
+{
+    const uint32_t m = (uint32_t)(r >> 29) & 0x007FFFFFu;
+    return __builtin_bit_cast(float, m | 0x3F800000u) - 1.0f;
+}
+
+__device__ __forceinline__ uint64_t ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
+
+// Flush the K per-individual cluster counts of one wavefront (cumulative
+// form, wave-uniform) into the block's LDS row.
+template <int K>
+__device__ __forceinline__ void flush_mi(int32_t* row, const int (&cum)[K], int lane)
+{
+    int val = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int cnt = cum[k] - (k ? cum[k - 1] : 0);
+        val = lane == k ? cnt : val;
+    }
+    if (lane < K && val) atomicAdd(row + lane, val);
+}
+
+// ---------------------------------------------------------------------------
+// BI tiles: <= 2 alleles, <= 8 genotypes per locus.  P of the lane's locus
+// lives in registers; a genotype decodes from one 64-bit table per lane.
+// ---------------------------------------------------------------------------
+template <int K, typename Real>
+__global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, const double* __restrict__ Freq,
+                                                   const double* __restrict__ Q, uint64_t seed, int m,
+                                                   int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi)
+{
+    extern __shared__ int32_t mi_lds[];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int nwv = blockDim.x >> 6;
+    const int c = blockIdx.y;
+    const int i0 = c * ds.Nc;
+    const int i1 = min(ds.N, i0 + ds.Nc);
+    const int nci = i1 - i0;
+    for (int x = threadIdx.x; x < nci * K; x += blockDim.x) mi_lds[x] = 0;
+    __syncthreads();
+
+    const int64_t tix = (int64_t)blockIdx.x * nwv + wv;
+    if (tix < ds.nbi) {
+        const TileDesc t = ds.tiles[ds.bi_list[tix]];
+        const bool valid = lane < t.nloc;
+        const int64_t l = t.l0 + (valid ? lane : 0);
+        const int64_t KT = ds.KT;
+        const int afo = ds.afo[l];
+        const int k2 = ds.k2[l];
+        const uint64_t tab = valid ? ds.bitab[l] : 0ull;   // invalid lane: every id is untyped
+        Real p0[K], p1[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            p0[k] = (Real)Freq[k * KT + afo];
+            p1[k] = k2 > 1 ? (Real)Freq[k * KT + afo + 1] : p0[k];
+        }
+        Xs128 rng;
+        rng.seed(seed + (uint64_t)m * (uint64_t)ds.L + (uint64_t)l, kSaltUpdateZ);
+        if (c > 0) jump_apply(ds.jump + (size_t)c * kJumpWords, rng);
+
+        int nc_all[K], nc_1[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) { nc_all[k] = 0; nc_1[k] = 0; }
+        const Real eps = (Real)kMinFreq;
+        const int bits = t.bits, ipw = t.ipw;
+        const uint32_t mask = (1u << bits) - 1u;
+        int w = i0 / ipw, j = i0 - w * ipw;
+        const uint32_t* wp = ds.words + t.wbase + lane;
+        uint32_t word = wp[(int64_t)w * kTileLoci] >> (j * bits);
+        const int MP = ds.maxploidy;
+
+        for (int i = i0; i < i1; ++i) {
+            const uint32_t gid = word & mask;
+            word >>= bits;
+            if (++j == ipw) {
+                j = 0; ++w;
+                if (i + 1 < i1) word = wp[(int64_t)w * kTileLoci];
+            }
+            const uint32_t desc = (uint32_t)(tab >> (gid * 8)) & 0xFFu;
+            const int v = (int)(desc & 15u);
+            const int h = v - (int)(desc >> 4);
+            Real q[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) q[k] = (Real)Q[(int64_t)i * K + k];
+            int cum[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) cum[k] = 0;
+
+            for (int a = 0; a < MP; ++a) {
+                const uint64_t r = rng.next();
+                const bool typed = a < v;
+                const bool al1 = a >= h;
+                const uint64_t tmask = ballot(typed);
+                Real wk[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) wk[k] = (al1 ? p1[k] : p0[k]) * q[k] + eps;
+                Real s = wk[0];
+#pragma unroll
+                for (int k = 1; k < K; ++k) s = s + wk[k];
+                Real tt = unit_minus1<Real>(r) * s;
+#pragma unroll
+                for (int k = 0; k < K - 1; ++k) {
+                    tt = tt - wk[k];
+                    const bool neg = tt < (Real)0;
+                    cum[k] += __builtin_popcountll(ballot(neg) & tmask);
+                    nc_all[k] += (neg && typed) ? 1 : 0;
+                    nc_1[k] += (neg && typed && al1) ? 1 : 0;
+                }
+                cum[K - 1] += __builtin_popcountll(tmask);
+                nc_all[K - 1] += typed ? 1 : 0;
+                nc_1[K - 1] += (typed && al1) ? 1 : 0;
+            }
+            flush_mi<K>(mi_lds + (i - i0) * K, cum, lane);
+        }
+
+        if (valid) {
+            int32_t* dst = Ni_part + (int64_t)c * K * KT + afo;
+            int pa = 0, p1c = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int all = nc_all[k] - pa, one = nc_1[k] - p1c;
+                pa = nc_all[k]; p1c = nc_1[k];
+                dst[k * KT] = all - one;
+                if (k2 > 1) dst[k * KT + 1] = one;
+            }
+        }
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < nci * K; x += blockDim.x) {
+        const int v = mi_lds[x];
+        if (v) atomicAdd(Mi + (int64_t)i0 * K + x, v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// GEN tiles: any allele count (< 64) and ploidy mix.  A genotype decodes
+// from a 64-bit descriptor; P is gathered per slot; the lane's allele counts
+// live in an LDS column owned by the lane (or in global atomics when the
+// K x alleles table does not fit).
+// ---------------------------------------------------------------------------
+template <int K, typename Real>
+__global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, const double* __restrict__ Freq,
+                                                    const double* __restrict__ Q, uint64_t seed, int m,
+                                                    int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
+                                                    int A, int ni_in_lds)
+{
+    extern __shared__ int32_t lds[];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int nwv = blockDim.x >> 6;
+    const int c = blockIdx.y;
+    const int i0 = c * ds.Nc;
+    const int i1 = min(ds.N, i0 + ds.Nc);
+    const int nci = i1 - i0;
+    int32_t* mi_lds = lds;
+    int32_t* ni_lds = lds + ds.Nc * K + wv * (K * A * kTileLoci);
+    const int ntot = ds.Nc * K + (ni_in_lds ? nwv * K * A * kTileLoci : 0);
+    for (int x = threadIdx.x; x < ntot; x += blockDim.x) lds[x] = 0;
+    __syncthreads();
+
+    const int64_t tix = (int64_t)blockIdx.x * nwv + wv;
+    if (tix < ds.ngen) {
+        const TileDesc t = ds.tiles[ds.gen_list[tix]];
+        const bool valid = lane < t.nloc;
+        const int64_t l = t.l0 + (valid ? lane : 0);
+        const int64_t KT = ds.KT;
+        const int afo = ds.afo[l];
+        const int k2 = ds.k2[l];
+        const uint64_t* gd = ds.gdesc + ds.gbase[l];
+        Xs128 rng;
+        rng.seed(seed + (uint64_t)m * (uint64_t)ds.L + (uint64_t)l, kSaltUpdateZ);
+        if (c > 0) jump_apply(ds.jump + (size_t)c * kJumpWords, rng);
+
+        const Real eps = (Real)kMinFreq;
+        const int bits = t.bits, ipw = t.ipw;
+        const uint32_t mask = (1u << bits) - 1u;
+        int w = i0 / ipw, j = i0 - w * ipw;
+        const uint32_t* wp = ds.words + t.wbase + lane;
+        uint32_t word = wp[(int64_t)w * kTileLoci] >> (j * bits);
+        const int MP = ds.maxploidy;
+        const double* pl = Freq + afo;
+        int32_t* nig = Ni_part + (int64_t)c * K * KT + afo;
+
+        for (int i = i0; i < i1; ++i) {
+            const uint32_t gid = word & mask;
+            word >>= bits;
+            if (++j == ipw) {
+                j = 0; ++w;
+                if (i + 1 < i1) word = wp[(int64_t)w * kTileLoci];
+            }
+            const uint64_t desc = valid ? gd[gid] : 0ull;
+            const int v = (int)(desc & 15u);
+            Real q[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) q[k] = (Real)Q[(int64_t)i * K + k];
+            int cum[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) cum[k] = 0;
+
+            for (int a = 0; a < MP; ++a) {
+                const uint64_t r = rng.next();
+                const bool typed = a < v;
+                const int al = a < VPS_MAX_PLOIDY ? (int)((desc >> (4 + 6 * a)) & 63u) : 0;
+                const int x = typed ? al : 0;
+                const uint64_t tmask = ballot(typed);
+                Real wk[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) wk[k] = (Real)pl[k * KT + x] * q[k] + eps;
+                Real s = wk[0];
+#pragma unroll
+                for (int k = 1; k < K; ++k) s = s + wk[k];
+                Real tt = unit_minus1<Real>(r) * s;
+                int nneg = 0;
+#pragma unroll
+                for (int k = 0; k < K - 1; ++k) {
+                    tt = tt - wk[k];
+                    const bool neg = tt < (Real)0;
+                    cum[k] += __builtin_popcountll(ballot(neg) & tmask);
+                    nneg += neg ? 1 : 0;
+                }
+                cum[K - 1] += __builtin_popcountll(tmask);
+                if (typed) {
+                    const int ks = K - 1 - nneg;
+                    if (ni_in_lds) ni_lds[(ks * A + x) * kTileLoci + lane] += 1;
+                    else atomicAdd(nig + (int64_t)ks * KT + x, 1);
+                }
+            }
+            flush_mi<K>(mi_lds + (i - i0) * K, cum, lane);
+        }
+
+        if (valid && ni_in_lds) {
+            for (int k = 0; k < K; ++k)
+                for (int a = 0; a < k2; ++a) nig[(int64_t)k * KT + a] = ni_lds[(k * A + a) * kTileLoci + lane];
+        }
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < nci * K; x += blockDim.x) {
+        const int v = mi_lds[x];
+        if (v) atomicAdd(Mi + (int64_t)i0 * K + x, v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// UpdateZNoAdmix (src/structureAVX.cpp:432-552): Ni[Z_i][l][x] += 1 per
+// typed copy; Mi[i][Z_i] = vt_i.  Lanes = loci, counts in lane-owned LDS
+// columns, each chunk writes its own Ni partial.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_z_noadmix(const DeviceDataset ds, const uint16_t* __restrict__ Z, int K,
+                                                 int32_t* __restrict__ Ni_part, int A, int list_kind)
+{
+    extern __shared__ int32_t ni_lds[];   // [K][A][64]
+    const int lane = threadIdx.x;
+    const int c = blockIdx.y;
+    const int i0 = c * ds.Nc;
+    const int i1 = min(ds.N, i0 + ds.Nc);
+    for (int x = lane; x < K * A * kTileLoci; x += 64) ni_lds[x] = 0;
+    __syncthreads();
+    const int64_t tix = blockIdx.x;
+    const int64_t ntl = list_kind == kKindBi ? ds.nbi : ds.ngen;
+    if (tix >= ntl) return;
+    const TileDesc t = ds.tiles[list_kind == kKindBi ? ds.bi_list[tix] : ds.gen_list[tix]];
+    const bool valid = lane < t.nloc;
+    const int64_t l = t.l0 + (valid ? lane : 0);
+    const int afo = ds.afo[l];
+    const int k2 = ds.k2[l];
+    const uint64_t tab = valid ? ds.bitab[l] : 0ull;
+    const uint64_t* gd = ds.gdesc + ds.gbase[l];
+    const int bits = t.bits, ipw = t.ipw;
+    const uint32_t mask = (1u << bits) - 1u;
+    int w = i0 / ipw, j = i0 - w * ipw;
+    const uint32_t* wp = ds.words + t.wbase + lane;
+    uint32_t word = wp[(int64_t)w * kTileLoci] >> (j * bits);
+    for (int i = i0; i < i1; ++i) {
+        const uint32_t gid = word & mask;
+        word >>= bits;
+        if (++j == ipw) {
+            j = 0; ++w;
+            if (i + 1 < i1) word = wp[(int64_t)w * kTileLoci];
+        }
+        const int zk = Z[i];
+        int32_t* col = ni_lds + (zk * A) * kTileLoci + lane;
+        if (t.kind == kKindBi) {
+            const uint32_t desc = (uint32_t)(tab >> (gid * 8)) & 0xFFu;
+            const int v = (int)(desc & 15u), cc = (int)(desc >> 4);
+            col[0] += v - cc;
+            col[kTileLoci] += cc;
+        } else {
+            const uint64_t desc = valid ? gd[gid] : 0ull;
+            const int v = (int)(desc & 15u);
+            for (int a = 0; a < v; ++a) col[(int)((desc >> (4 + 6 * a)) & 63u) * kTileLoci] += 1;
+        }
+    }
+    if (valid) {
+        int32_t* dst = Ni_part + (int64_t)c * K * ds.KT + afo;
+        for (int k = 0; k < K; ++k)
+            for (int a = 0; a < k2; ++a) dst[(int64_t)k * ds.KT + a] = ni_lds[(k * A + a) * kTileLoci + lane];
+    }
+}
+
+// Ni = sum over chunks of Ni_part
+__global__ void k_reduce_ni(const int32_t* __restrict__ Ni_part, int32_t* __restrict__ Ni, int64_t n, int G)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    int32_t s = 0;
+    for (int c = 0; c < G; ++c) s += Ni_part[(int64_t)c * n + x];
+    Ni[x] = s;
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < n) p[x] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kWavesPerBlock = 4;
+
+template <int K, typename Real>
+int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+{
+    const int nthr = 64 * kWavesPerBlock;
+    // GEN tiles keep their per-lane allele counts in LDS when the K x alleles
+    // table fits; otherwise they add into the chunk partials, which must then
+    // be cleared before any tile writes its counts.
+    const int A = ds.maxK;
+    const size_t mi_bytes = (size_t)ds.Nc * K * sizeof(int32_t);
+    size_t ni_bytes = (size_t)kWavesPerBlock * K * A * kTileLoci * sizeof(int32_t);
+    const int in_lds = mi_bytes + ni_bytes <= 64 * 1024;
+    if (ds.ngen > 0 && !in_lds) {
+        int rc = launch_fill_i32(c.Ni_part, (int64_t)ds.G * K * ds.KT, 0, st);
+        if (rc) return rc;
+        ni_bytes = 0;
+    }
+    if (ds.nbi > 0) {
+        dim3 grid((unsigned)((ds.nbi + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)ds.G);
+        hipLaunchKernelGGL((k_z_admix_bi<K, Real>), grid, dim3(nthr), mi_bytes, st, ds, c.Freq, c.Q, c.seed, m, c.Ni_part, c.Mi);
+        VPS_HIP(hipGetLastError());
+    }
+    if (ds.ngen > 0) {
+        dim3 grid((unsigned)((ds.ngen + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)ds.G);
+        hipLaunchKernelGGL((k_z_admix_gen<K, Real>), grid, dim3(nthr), mi_bytes + ni_bytes, st, ds, c.Freq, c.Q, c.seed,
+                           m, c.Ni_part, c.Mi, A, in_lds);
+        VPS_HIP(hipGetLastError());
+    }
+    return VPS_OK;
+}
+
+template <typename Real>
+int dispatch_k(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+{
+    switch (c.K) {
+#define VPS_CASE(k) case k: return launch_z_admix_k<k, Real>(ds, c, m, st);
+        VPS_CASE(1) VPS_CASE(2) VPS_CASE(3) VPS_CASE(4) VPS_CASE(5) VPS_CASE(6) VPS_CASE(7) VPS_CASE(8)
+        VPS_CASE(9) VPS_CASE(10) VPS_CASE(11) VPS_CASE(12) VPS_CASE(13) VPS_CASE(14) VPS_CASE(15) VPS_CASE(16)
+#undef VPS_CASE
+    default: return fail(VPS_ERR_UNSUPPORTED, "K=%d outside 1..%d", c.K, VPS_MAX_K);
+    }
+}
+
+}  // namespace
+
+int launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st)
+{
+    if (n <= 0) return VPS_OK;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
+{
+    int rc = launch_fill_i32(c.Mi, (int64_t)ds.N * c.K, 0, st);
+    if (rc) return rc;
+    if (ev0) VPS_HIP(hipEventRecord(ev0, st));
+    rc = c.precision == VPS_PREC_MIXED ? dispatch_k<float>(ds, c, m, st) : dispatch_k<double>(ds, c, m, st);
+    if (ev1) VPS_HIP(hipEventRecord(ev1, st));
+    if (rc) return rc;
+    return launch_reduce_ni(ds, c, st);
+}
+
+int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
+{
+    const int64_t n = (int64_t)c.K * ds.KT;
+    hipLaunchKernelGGL(k_reduce_ni, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c.Ni_part, c.Ni, n, ds.G);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+int launch_z_noadmix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
+{
+    const int A = ds.maxK > 2 ? ds.maxK : 2;
+    const size_t shm = (size_t)c.K * A * kTileLoci * sizeof(int32_t);
+    if (shm > 64 * 1024) return fail(VPS_ERR_UNSUPPORTED, "no-admix count table K=%d x %d alleles exceeds LDS", c.K, A);
+    if (ds.nbi > 0)
+        hipLaunchKernelGGL(k_z_noadmix, dim3((unsigned)ds.nbi, (unsigned)ds.G), dim3(64), shm, st, ds, c.Z, c.K, c.Ni_part, A, kKindBi);
+    if (ds.ngen > 0)
+        hipLaunchKernelGGL(k_z_noadmix, dim3((unsigned)ds.ngen, (unsigned)ds.G), dim3(64), shm, st, ds, c.Z, c.K, c.Ni_part, A, kKindGen);
+    VPS_HIP(hipGetLastError());
+    return launch_reduce_ni(ds, c, st);
+}
+
+}  // namespace vps
