@@ -44,7 +44,16 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run a plain device copy of MB megabytes (scales traffic counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import vcfpop_amd as V
+        nbytes = args.copy_calibration * 1000 * 1000
+        ms = V.debug_copy(nbytes, iters=5)
+        print(json.dumps({"copy_bytes": nbytes, "launches": 6, "ms_per_copy": ms,
+                          "GB/s (read+write)": 2 * nbytes / ms / 1e6}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (chains shard across GPUs "
               "with no communication, see DESIGN.md)", file=sys.stderr)

@@ -244,6 +244,9 @@ int vps_debug_decode(const vps_dataset* ds, int64_t l, int32_t i, int32_t* ploid
 int vps_debug_jump(uint64_t D, uint64_t* x, uint64_t* y);
 /* individual chunking of the genotype passes: G chunks of Nc individuals */
 int vps_debug_chunking(const vps_dataset* ds, int32_t* G, int32_t* Nc);
+/* one warm-up plus `iters` launches of a plain device copy of `bytes` bytes
+   (scales hardware traffic counters); average ms per copy */
+int vps_debug_copy(int device, size_t bytes, int iters, double* ms_per_copy);
 
 #ifdef __cplusplus
 }

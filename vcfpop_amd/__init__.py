@@ -28,5 +28,6 @@ from ._lib import (  # noqa: F401
     PREC_MIXED,
     device_count,
     debug_jump,
+    debug_copy,
 )
 from .workloads import WORKLOADS, make_synth  # noqa: F401

@@ -123,6 +123,7 @@ _SIGS = {
     "vps_debug_decode": ([_P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), _P], C.c_int),
     "vps_debug_jump": ([C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
     "vps_debug_chunking": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+    "vps_debug_copy": ([C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
     "vps_synth_default": ([C.POINTER(VpsSynthConfig)], None),
     "vps_synth_create": ([C.POINTER(VpsSynthConfig), C.POINTER(_P)], C.c_int),
     "vps_synth_desc": ([_P, C.POINTER(VpsDatasetDesc)], C.c_int),
@@ -171,6 +172,12 @@ def debug_jump(D, x, y):
     xx, yy = C.c_uint64(x), C.c_uint64(y)
     _check(load().vps_debug_jump(C.c_uint64(D), C.byref(xx), C.byref(yy)))
     return xx.value, yy.value
+
+
+def debug_copy(nbytes, iters=10, device=0):
+    ms = C.c_double()
+    _check(load().vps_debug_copy(int(device), int(nbytes), int(iters), C.byref(ms)))
+    return ms.value
 
 
 def _ptr(a, ctype):

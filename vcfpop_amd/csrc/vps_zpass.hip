@@ -456,3 +456,50 @@ int launch_z_noadmix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 }
 
 }  // namespace vps
+
+namespace vps {
+
+// ---------------------------------------------------------------------------
+// Test hook: a plain device copy of known size, used to scale hardware
+// traffic counters (bytes moved per counted unit) for the roofline report.
+// ---------------------------------------------------------------------------
+__global__ void k_copy_u4(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n)
+{
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x)
+        dst[x] = src[x];
+}
+
+}  // namespace vps
+
+extern "C" int vps_debug_copy(int device, size_t bytes, int iters, double* ms_per_copy)
+{
+    using namespace vps;
+    if (bytes < 16 || iters < 1 || !ms_per_copy) return fail(VPS_ERR_INVALID, "copy arguments");
+    VPS_HIP(hipSetDevice(device));
+    void *a = nullptr, *b = nullptr;
+    VPS_HIP(hipMalloc(&a, bytes));
+    hipError_t e = hipMalloc(&b, bytes);
+    if (e != hipSuccess) { (void)hipFree(a); return fail(VPS_ERR_NOMEM, "hipMalloc copy buffer"); }
+    const int64_t n = (int64_t)(bytes / 16);
+    hipEvent_t e0, e1;
+    int rc = VPS_OK;
+    if (hipMemset(a, 1, bytes) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = fail(VPS_ERR_DEVICE, "copy setup");
+    if (!rc) {
+        hipLaunchKernelGGL(k_copy_u4, dim3(4096), dim3(256), 0, 0, (const uint4*)a, (uint4*)b, n);
+        (void)hipEventRecord(e0, 0);
+        for (int it = 0; it < iters; ++it)
+            hipLaunchKernelGGL(k_copy_u4, dim3(4096), dim3(256), 0, 0, (const uint4*)a, (uint4*)b, n);
+        (void)hipEventRecord(e1, 0);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = fail(VPS_ERR_DEVICE, "copy run");
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        *ms_per_copy = ms / iters;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}
+
