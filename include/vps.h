@@ -118,6 +118,9 @@ typedef struct vps_params {
     /* engine options (no counterpart in the reference) */
     int32_t rng_topology;     /* VPS_RNG_REFERENCE or VPS_RNG_PARALLEL           */
     int32_t precision;        /* VPS_PREC_FP64 or VPS_PREC_MIXED                 */
+    int32_t z_chunk;          /* individuals per work chunk of the genotype passes;
+                                 0 = chosen from the kernels' occupancy. Results do
+                                 not depend on it.                              */
 } vps_params;
 
 #define VPS_RNG_REFERENCE 0  /* P: 128 streams RNG(seed+m*128+b); Q: one stream RNG(seed+m) —
@@ -242,8 +245,8 @@ int vps_chain_kernel_time(vps_chain* ch, int kernel, double* total_ms, int64_t* 
 int vps_debug_decode(const vps_dataset* ds, int64_t l, int32_t i, int32_t* ploidy, uint16_t* alleles /*[VPS_MAX_PLOIDY]*/);
 /* advance a xorshift128+ state (x, y) by D steps with the GF(2) jump table */
 int vps_debug_jump(uint64_t D, uint64_t* x, uint64_t* y);
-/* individual chunking of the genotype passes: G chunks of Nc individuals */
-int vps_debug_chunking(const vps_dataset* ds, int32_t* G, int32_t* Nc);
+/* a chain's individual chunking of the genotype passes: G chunks of Nc individuals */
+int vps_debug_chunking(const vps_chain* ch, int32_t* G, int32_t* Nc);
 /* one warm-up plus `iters` launches of a plain device copy of `bytes` bytes
    (scales hardware traffic counters); average ms per copy */
 int vps_debug_copy(int device, size_t bytes, int iters, double* ms_per_copy);

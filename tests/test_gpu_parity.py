@@ -93,6 +93,36 @@ def test_z_admix_bit_exact(vps, orc, pairs, name, K):
         assert (g.get("Mi") == o.get("Mi")).all(), m
 
 
+@pytest.mark.parametrize("name", ["bi_diploid", "mixed"])
+@pytest.mark.parametrize("z_chunk", [1, 7, 32, 512])
+def test_z_chunking_invariance(vps, orc, pairs, name, z_chunk):
+    """every individual chunking of the genotype passes gives the oracle's
+    Z pass (admixture and no-admixture) bit for bit"""
+    say("z chunking", name, z_chunk)
+    ds, od, g, o = make(vps, orc, pairs, name, 3, admix_val=1, g_seed=9, z_chunk=z_chunk)
+    G, Nc = g.chunking()
+    # z_chunk fixes the chunk count; chunks are then balanced
+    assert G == -(-ds.N // z_chunk) and Nc == -(-ds.N // G)
+    orig, afo, vt = ds.order()
+    k2 = pairs[name][0].arrays()["n_allele"][orig]
+    rng = np.random.default_rng(z_chunk)
+    Freq, Q = random_state(rng, ds.N, 3, ds.KT, afo, k2)
+    for c in (g, o):
+        c.set("Freq", Freq)
+        c.set("Q", Q)
+        c.set("m", [11])
+    g.update_z_admix()
+    o.call("update_z_admix")
+    assert (g.get("Ni") == o.get("Ni")).all()
+    assert (g.get("Mi") == o.get("Mi")).all()
+    Z = rng.integers(0, 3, ds.N).astype(np.uint16)
+    for c in (g, o):
+        c.set("Z", Z)
+    g.update_z_noadmix()
+    o.call("update_z_noadmix")
+    assert (g.get("Ni") == o.get("Ni")).all()
+
+
 def test_z_admix_full_size_admix_n2000_l100k_k5(vps, orc):
     """one Z pass at the benchmark size (N=2,000, L=100,000, K=5), bit-exact"""
     say("z_admix full size")

@@ -60,6 +60,7 @@ class VpsParams(C.Structure):
         ("g_seed", C.c_uint64),
         ("rng_topology", C.c_int32),
         ("precision", C.c_int32),
+        ("z_chunk", C.c_int32),
     ]
 
 
@@ -294,11 +295,6 @@ class Dataset:
         _check(load().vps_debug_decode(self.h, int(l), int(i), C.byref(v), als.ctypes.data))
         return als[: v.value].copy()
 
-    def chunking(self):
-        g, nc = C.c_int32(), C.c_int32()
-        _check(load().vps_debug_chunking(self.h, C.byref(g), C.byref(nc)))
-        return g.value, nc.value
-
     def close(self):
         if getattr(self, "h", None):
             load().vps_dataset_destroy(self.h)
@@ -410,6 +406,12 @@ class Chain:
         ms, n = C.c_double(), C.c_int64()
         _check(load().vps_chain_kernel_time(self.h, kernel, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def chunking(self):
+        """(G, Nc): individual chunks of the genotype passes"""
+        g, nc = C.c_int32(), C.c_int32()
+        _check(load().vps_debug_chunking(self.h, C.byref(g), C.byref(nc)))
+        return g.value, nc.value
 
     def close(self):
         if getattr(self, "h", None):

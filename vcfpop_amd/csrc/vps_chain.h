@@ -41,6 +41,10 @@ struct ChainDev {
     double*   rout;      // [rlen]
     double*   lnltrace;  // [nlnl capacity]
     Rng*      rngbuf;    // [N] streams kept between a draw kernel and its acceptance kernel
+    // individual chunking of the genotype passes: G chunks of Nc individuals;
+    // jump[c] advances a locus stream to individual c*Nc
+    int32_t   G, Nc;
+    const uint32_t* jump;
 };
 
 // likelihood-pass partials: lanes = individuals, one partial per locus range
@@ -69,6 +73,8 @@ int launch_update_f(const DeviceDataset& ds, const ChainDev& c, int m, hipStream
 int launch_record_accumulate(const DeviceDataset& ds, const ChainDev& c, int post_burnin, int trace_slot,
                              int accumulate_misum, hipStream_t st);
 int launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
+// chunk count minimising (grid rounds x per-block work) at the kernel's occupancy
+int choose_chunks(const DeviceDataset& ds, int K, int precision, int device, int* G);
 int launch_init_admix(const DeviceDataset& ds, const ChainDev& c, int singlez, hipStream_t st);
 
 }  // namespace vps

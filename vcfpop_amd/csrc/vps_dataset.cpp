@@ -284,28 +284,6 @@ extern "C" int vps_dataset_create(const vps_dataset_desc* in, int device, vps_da
         }
     }
 
-    // individual chunks: enough wavefronts to fill the device
-    {
-        int cus = 256;
-        hipDeviceProp_t prop;
-        if (!host_only && hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
-            cus = prop.multiProcessorCount;
-        // at least 8 wavefronts per SIMD, at least 32 individuals per chunk,
-        // and at most kMaxChunk individuals so a block's per-individual
-        // cluster counts (kMaxChunk x VPS_MAX_K ints) fit in 32 KB of LDS
-        constexpr int64_t kMaxChunk = 512;
-        int64_t target = (int64_t)cus * 4 * 8;
-        int64_t G = (target + d.ntiles - 1) / d.ntiles;
-        G = std::min<int64_t>(G, std::max<int64_t>(1, N / 32));
-        G = std::max<int64_t>(G, (N + kMaxChunk - 1) / kMaxChunk);
-        G = std::max<int64_t>(1, G);
-        d.Nc = (int32_t)((N + G - 1) / G);
-        d.G = (int32_t)((N + d.Nc - 1) / d.Nc);
-    }
-    std::vector<uint32_t> jump((size_t)d.G * kJumpWords, 0);
-    for (int c = 1; c < d.G; ++c)
-        jump_table((uint64_t)c * d.Nc * maxploidy, jump.data() + (size_t)c * kJumpWords);
-
     if (in->ind_name) for (int i = 0; i < N; ++i) ds->ind_names.emplace_back(in->ind_name[i] ? in->ind_name[i] : "");
     if (in->pop_name) for (int s = 0; s < S; ++s) ds->pop_names.emplace_back(in->pop_name[s] ? in->pop_name[s] : "");
     if (in->loc_name) for (int64_t l = 0; l < L; ++l) ds->loc_names.emplace_back(in->loc_name[ds->orig[l]] ? in->loc_name[ds->orig[l]] : "");
@@ -337,7 +315,6 @@ extern "C" int vps_dataset_create(const vps_dataset_desc* in, int device, vps_da
     if (!rc) rc = upload(&d.popid, ds->popid, ds);
     if (!rc) rc = upload(&d.pop_ind0, ds->pop_ind0, ds);
     if (!rc) rc = upload(&d.pop_nind, ds->pop_nind, ds);
-    if (!rc) rc = upload(&d.jump, jump, ds);
     if (!rc) rc = upload(&d.totfreq, totfreq, ds);
     if (!rc) rc = upload(&d.nhaplo, nhaplo, ds);
     if (!rc) rc = upload(&d.kdis, ds->kdis, ds);
@@ -417,10 +394,22 @@ extern "C" int vps_debug_jump(uint64_t D, uint64_t* x, uint64_t* y)
     return VPS_OK;
 }
 
-extern "C" int vps_debug_chunking(const vps_dataset* ds, int32_t* G, int32_t* Nc)
+namespace vps {
+
+int dataset_jump(vps_dataset* ds, int Nc, const uint32_t** out)
 {
-    if (!ds || !G || !Nc) return fail(VPS_ERR_INVALID, "null argument");
-    *G = ds->d.G;
-    *Nc = ds->d.Nc;
+    std::lock_guard<std::mutex> lock(ds->jump_mu);
+    auto it = ds->jump_cache.find(Nc);
+    if (it != ds->jump_cache.end()) { *out = it->second; return VPS_OK; }
+    const int G = (ds->d.N + Nc - 1) / Nc;
+    std::vector<uint32_t> tab((size_t)G * kJumpWords);
+    jump_tables((uint64_t)Nc * ds->d.maxploidy, G, tab.data());
+    uint32_t* dev = nullptr;
+    int rc = upload(&dev, tab, ds);
+    if (rc) return rc;
+    ds->jump_cache[Nc] = dev;
+    *out = dev;
     return VPS_OK;
 }
+
+}  // namespace vps

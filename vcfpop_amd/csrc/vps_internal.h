@@ -3,8 +3,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
-#include <vector>
+#include <map>
 #include <mutex>
+#include <vector>
 #include "../../include/vps.h"
 #include "vps_numeric.h"
 
@@ -61,8 +62,6 @@ struct DeviceDataset {
     int32_t maxK;
     int64_t ntiles;
     int64_t nbi, ngen;         // tiles of each kind
-    int32_t G;                 // individual chunks per tile in genotype passes
-    int32_t Nc;                // individuals per chunk
     // device arrays
     TileDesc* tiles;           // [ntiles]
     int32_t*  bi_list;         // tile indices of kind BI
@@ -77,7 +76,6 @@ struct DeviceDataset {
     int32_t*  popid;           // [N]
     int32_t*  pop_ind0;        // [S]
     int32_t*  pop_nind;        // [S]
-    uint32_t* jump;            // [G][kJumpWords] (chunk 0 unused)
     double*   totfreq;         // [KT] (F model)
     int32_t*  nhaplo;          // [L]
     int64_t*  kdis;            // [maxK+1] loci per allele count
@@ -106,6 +104,10 @@ struct vps_dataset {
     std::vector<uint32_t> h_words;
     std::vector<uint64_t> h_bitab, h_gdesc;
     std::vector<int64_t> h_gbase;
+    // device jump tables per chunk size Nc: [ceil(N/Nc)][kJumpWords], entry c
+    // advances a locus stream by c*Nc*maxploidy draws
+    std::mutex jump_mu;
+    std::map<int, uint32_t*> jump_cache;
 };
 
 namespace vps {
@@ -122,6 +124,13 @@ int fail(int code, const char* fmt, ...);
 
 // GF(2) jump table for D steps of xorshift128+ (vps_jump.cpp)
 void jump_table(uint64_t D, uint32_t* out /* kJumpWords */);
+// tables for 0, D, 2D, ..., (count-1)D steps
+void jump_tables(uint64_t D, int count, uint32_t* out /* count * kJumpWords */);
+// largest individual chunk of the genotype passes (a block keeps
+// kMaxChunk x VPS_MAX_K cluster counts in 32 KB of LDS)
+constexpr int kMaxChunk = 512;
+// device jump tables of a chunk size, built once per dataset (vps_dataset.cpp)
+int dataset_jump(vps_dataset* ds, int Nc, const uint32_t** out);
 // apply a jump table to a state on the host (tests)
 void jump_apply_host(const uint32_t* tab, uint64_t& x, uint64_t& y);
 

@@ -78,6 +78,29 @@ void jump_table(uint64_t D, uint32_t* out)
         }
 }
 
+void jump_tables(uint64_t D, int count, uint32_t* out)
+{
+    M128 P = step_matrix(), M = identity();
+    for (uint64_t d = D; d; d >>= 1) {
+        if (d & 1) M = mulm(P, M);
+        P = mulm(P, P);
+    }
+    M128 R = identity();
+    for (int c = 0; c < count; ++c) {
+        uint32_t* o = out + (size_t)c * kJumpWords;
+        for (int p = 0; p < 32; ++p)
+            for (int v = 0; v < 16; ++v) {
+                V128 acc{0, 0};
+                for (int b = 0; b < 4; ++b)
+                    if ((v >> b) & 1) { acc.lo ^= R.col[4 * p + b].lo; acc.hi ^= R.col[4 * p + b].hi; }
+                uint32_t* e = o + (p * 16 + v) * 4;
+                e[0] = (uint32_t)acc.lo; e[1] = (uint32_t)(acc.lo >> 32);
+                e[2] = (uint32_t)acc.hi; e[3] = (uint32_t)(acc.hi >> 32);
+            }
+        R = mulm(M, R);
+    }
+}
+
 void jump_apply_host(const uint32_t* tab, uint64_t& x, uint64_t& y)
 {
     uint32_t st[4] = {(uint32_t)x, (uint32_t)(x >> 32), (uint32_t)y, (uint32_t)(y >> 32)};

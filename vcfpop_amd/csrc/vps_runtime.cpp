@@ -75,6 +75,7 @@ extern "C" void vps_params_default(vps_params* p)
     p->g_seed = 0;
     p->rng_topology = VPS_RNG_REFERENCE;
     p->precision = VPS_PREC_FP64;
+    p->z_chunk = 0;
 }
 
 extern "C" int vps_abi_version(void) { return VPS_ABI_VERSION; }
@@ -166,7 +167,20 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     if (!rc) rc = chain_alloc(ch, &c.Q, N * K);
     if (!rc) rc = chain_alloc(ch, &c.Z, N + 32);
     if (!rc) rc = chain_alloc(ch, &c.Ni, K * KT);
-    if (!rc) rc = chain_alloc(ch, &c.Ni_part, (int64_t)d.G * K * KT);
+    // individual chunking of the genotype passes
+    if (p->z_chunk < 0 || p->z_chunk > kMaxChunk)
+        rc = fail(VPS_ERR_INVALID, "z_chunk %d outside 0..%d", p->z_chunk, (int)kMaxChunk);
+    if (!rc) {
+        int G = 0;
+        if (p->z_chunk > 0) G = (int)((N + p->z_chunk - 1) / p->z_chunk);
+        else rc = choose_chunks(d, K, p->precision, ds->device, &G);
+        if (!rc) {
+            c.Nc = (int32_t)((N + G - 1) / G);
+            c.G = (int32_t)((N + c.Nc - 1) / c.Nc);
+            rc = dataset_jump(ds, c.Nc, &c.jump);
+        }
+    }
+    if (!rc) rc = chain_alloc(ch, &c.Ni_part, (int64_t)c.G * K * KT);
     if (!rc) rc = chain_alloc(ch, &c.Mi, N * K);
     if (!rc) rc = chain_alloc(ch, &c.Alpha, K);
     if (!rc) rc = chain_alloc(ch, &c.Lambda, K);
@@ -609,5 +623,13 @@ extern "C" int vps_chain_kernel_time(vps_chain* ch, int kernel, double* total_ms
     }
     *total_ms = tot;
     *launches = (int64_t)(ch->ev_used / 2);
+    return VPS_OK;
+}
+
+extern "C" int vps_debug_chunking(const vps_chain* ch, int32_t* G, int32_t* Nc)
+{
+    if (!ch || !G || !Nc) return fail(VPS_ERR_INVALID, "null argument");
+    *G = ch->c.G;
+    *Nc = ch->c.Nc;
     return VPS_OK;
 }

@@ -25,6 +25,7 @@
 //     it on the scalar unit; per individual the K cluster counts go to an
 //     LDS row shared by the block's wavefronts (same individuals, other
 //     tiles), and the block adds its rows to Mi once at the end.
+#include <algorithm>
 #include "vps_chain.h"
 
 namespace vps {
@@ -61,64 +62,86 @@ __device__ __forceinline__ uint64_t ballot(bool b) { return __builtin_amdgcn_bal
 template <int K>
 __device__ __forceinline__ void flush_mi(int32_t* row, const int (&cum)[K], int lane)
 {
+    // lane k receives the (wave-uniform) count of cluster k
     int val = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int cnt = cum[k] - (k ? cum[k - 1] : 0);
-        val = lane == k ? cnt : val;
+        asm volatile("v_writelane_b32 %0, %1, %2" : "+v"(val) : "s"(cnt), "i"(k));
     }
     if (lane < K && val) atomicAdd(row + lane, val);
 }
 
 // ---------------------------------------------------------------------------
-// BI tiles: <= 2 alleles, <= 8 genotypes per locus.  P of the lane's locus
-// lives in registers; a genotype decodes from one 64-bit table per lane.
+// BI tiles: <= 2 alleles, <= 8 genotypes per locus.  A genotype decodes from
+// one 64-bit table per lane (entry = typed ploidy | dosage of allele 1 << 4;
+// slot a holds allele (a >= ploidy - dosage)).  The tile's P values sit in
+// LDS as [k][allele][lane] doubles, so selecting a slot's allele is one LDS
+// read per cluster instead of two register selects.  Per-lane cumulative
+// cluster counts are packed: low 16 bits all typed copies, high 16 bits the
+// copies of allele 1 (a chunk has <= 512 individuals x 10 slots < 65536).
+// MPT > 0 fixes maxploidy at compile time (unrolled slot loop).
 // ---------------------------------------------------------------------------
-template <int K, typename Real>
+template <int K, typename Real, int MPT>
 __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, const double* __restrict__ Freq,
                                                    const double* __restrict__ Q, uint64_t seed, int m,
-                                                   int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi)
+                                                   int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
+                                                   int Nc, const uint32_t* __restrict__ jump)
 {
-    extern __shared__ int32_t mi_lds[];
+    extern __shared__ int32_t lds_i32[];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const int nwv = blockDim.x >> 6;
     const int c = blockIdx.y;
-    const int i0 = c * ds.Nc;
-    const int i1 = min(ds.N, i0 + ds.Nc);
+    const int i0 = c * Nc;
+    const int i1 = min(ds.N, i0 + Nc);
     const int nci = i1 - i0;
+    // LDS: [wave][k][allele][lane] P values, then the block's Mi rows
+    Real* p_lds = reinterpret_cast<Real*>(lds_i32) + (size_t)wv * (K * 2 * kTileLoci);
+    int32_t* mi_lds = lds_i32 + (size_t)nwv * K * 2 * kTileLoci * (sizeof(Real) / 4);
     for (int x = threadIdx.x; x < nci * K; x += blockDim.x) mi_lds[x] = 0;
-    __syncthreads();
 
     const int64_t tix = (int64_t)blockIdx.x * nwv + wv;
-    if (tix < ds.nbi) {
-        const TileDesc t = ds.tiles[ds.bi_list[tix]];
-        const bool valid = lane < t.nloc;
-        const int64_t l = t.l0 + (valid ? lane : 0);
+    const bool has_tile = tix < ds.nbi;
+    TileDesc t{};
+    int afo = 0, k2 = 1;
+    bool valid = false;
+    int64_t l = 0;
+    if (has_tile) {
+        t = ds.tiles[ds.bi_list[tix]];
+        valid = lane < t.nloc;
+        l = t.l0 + (valid ? lane : 0);
+        afo = ds.afo[l];
+        k2 = ds.k2[l];
         const int64_t KT = ds.KT;
-        const int afo = ds.afo[l];
-        const int k2 = ds.k2[l];
-        const uint64_t tab = valid ? ds.bitab[l] : 0ull;   // invalid lane: every id is untyped
-        Real p0[K], p1[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            p0[k] = (Real)Freq[k * KT + afo];
-            p1[k] = k2 > 1 ? (Real)Freq[k * KT + afo + 1] : p0[k];
+            const double a0 = Freq[k * KT + afo];
+            p_lds[(k * 2 + 0) * kTileLoci + lane] = (Real)a0;
+            p_lds[(k * 2 + 1) * kTileLoci + lane] = (Real)(k2 > 1 ? Freq[k * KT + afo + 1] : a0);
         }
+    }
+    __syncthreads();
+
+    if (has_tile) {
+        const int64_t KT = ds.KT;
+        const uint64_t tab = valid ? ds.bitab[l] : 0ull;   // invalid lane: every id is untyped
         Xs128 rng;
         rng.seed(seed + (uint64_t)m * (uint64_t)ds.L + (uint64_t)l, kSaltUpdateZ);
-        if (c > 0) jump_apply(ds.jump + (size_t)c * kJumpWords, rng);
+        if (c > 0) jump_apply(jump + (size_t)c * kJumpWords, rng);
 
-        int nc_all[K], nc_1[K];
+        uint32_t nc[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) { nc_all[k] = 0; nc_1[k] = 0; }
+        for (int k = 0; k < K; ++k) nc[k] = 0;
         const Real eps = (Real)kMinFreq;
         const int bits = t.bits, ipw = t.ipw;
         const uint32_t mask = (1u << bits) - 1u;
         int w = i0 / ipw, j = i0 - w * ipw;
         const uint32_t* wp = ds.words + t.wbase + lane;
         uint32_t word = wp[(int64_t)w * kTileLoci] >> (j * bits);
-        const int MP = ds.maxploidy;
+        const int MP = MPT > 0 ? MPT : ds.maxploidy;
+        const Real* pl0 = p_lds + lane;
+        const Real* pl1 = pl0 + kTileLoci;
 
         for (int i = i0; i < i1; ++i) {
             const uint32_t gid = word & mask;
@@ -137,40 +160,43 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
 #pragma unroll
             for (int k = 0; k < K; ++k) cum[k] = 0;
 
+            constexpr int kSlotUnroll = MPT > 0 ? MPT : 1;
+#pragma unroll kSlotUnroll
             for (int a = 0; a < MP; ++a) {
                 const uint64_t r = rng.next();
                 const bool typed = a < v;
                 const bool al1 = a >= h;
                 const uint64_t tmask = ballot(typed);
+                const Real* pa = al1 ? pl1 : pl0;
                 Real wk[K];
 #pragma unroll
-                for (int k = 0; k < K; ++k) wk[k] = (al1 ? p1[k] : p0[k]) * q[k] + eps;
+                for (int k = 0; k < K; ++k) wk[k] = pa[k * 2 * kTileLoci] * q[k] + eps;
                 Real s = wk[0];
 #pragma unroll
                 for (int k = 1; k < K; ++k) s = s + wk[k];
                 Real tt = unit_minus1<Real>(r) * s;
+                const uint32_t inc = al1 ? 0x10001u : 1u;
 #pragma unroll
                 for (int k = 0; k < K - 1; ++k) {
                     tt = tt - wk[k];
                     const bool neg = tt < (Real)0;
                     cum[k] += __builtin_popcountll(ballot(neg) & tmask);
-                    nc_all[k] += (neg && typed) ? 1 : 0;
-                    nc_1[k] += (neg && typed && al1) ? 1 : 0;
+                    nc[k] += (neg && typed) ? inc : 0u;
                 }
                 cum[K - 1] += __builtin_popcountll(tmask);
-                nc_all[K - 1] += typed ? 1 : 0;
-                nc_1[K - 1] += (typed && al1) ? 1 : 0;
+                nc[K - 1] += typed ? inc : 0u;
             }
             flush_mi<K>(mi_lds + (i - i0) * K, cum, lane);
         }
 
         if (valid) {
             int32_t* dst = Ni_part + (int64_t)c * K * KT + afo;
-            int pa = 0, p1c = 0;
+            uint32_t prev = 0;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                const int all = nc_all[k] - pa, one = nc_1[k] - p1c;
-                pa = nc_all[k]; p1c = nc_1[k];
+                const uint32_t d = nc[k] - prev;   // per-half differences never borrow (cumulative counts)
+                prev = nc[k];
+                const int all = (int)(d & 0xFFFFu), one = (int)(d >> 16);
                 dst[k * KT] = all - one;
                 if (k2 > 1) dst[k * KT + 1] = one;
             }
@@ -193,19 +219,19 @@ template <int K, typename Real>
 __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, const double* __restrict__ Freq,
                                                     const double* __restrict__ Q, uint64_t seed, int m,
                                                     int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
-                                                    int A, int ni_in_lds)
+                                                    int A, int ni_in_lds, int Nc, const uint32_t* __restrict__ jump)
 {
     extern __shared__ int32_t lds[];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const int nwv = blockDim.x >> 6;
     const int c = blockIdx.y;
-    const int i0 = c * ds.Nc;
-    const int i1 = min(ds.N, i0 + ds.Nc);
+    const int i0 = c * Nc;
+    const int i1 = min(ds.N, i0 + Nc);
     const int nci = i1 - i0;
     int32_t* mi_lds = lds;
-    int32_t* ni_lds = lds + ds.Nc * K + wv * (K * A * kTileLoci);
-    const int ntot = ds.Nc * K + (ni_in_lds ? nwv * K * A * kTileLoci : 0);
+    int32_t* ni_lds = lds + Nc * K + wv * (K * A * kTileLoci);
+    const int ntot = Nc * K + (ni_in_lds ? nwv * K * A * kTileLoci : 0);
     for (int x = threadIdx.x; x < ntot; x += blockDim.x) lds[x] = 0;
     __syncthreads();
 
@@ -220,7 +246,7 @@ __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, con
         const uint64_t* gd = ds.gdesc + ds.gbase[l];
         Xs128 rng;
         rng.seed(seed + (uint64_t)m * (uint64_t)ds.L + (uint64_t)l, kSaltUpdateZ);
-        if (c > 0) jump_apply(ds.jump + (size_t)c * kJumpWords, rng);
+        if (c > 0) jump_apply(jump + (size_t)c * kJumpWords, rng);
 
         const Real eps = (Real)kMinFreq;
         const int bits = t.bits, ipw = t.ipw;
@@ -297,13 +323,13 @@ __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, con
 // columns, each chunk writes its own Ni partial.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_z_noadmix(const DeviceDataset ds, const uint16_t* __restrict__ Z, int K,
-                                                 int32_t* __restrict__ Ni_part, int A, int list_kind)
+                                                 int32_t* __restrict__ Ni_part, int A, int list_kind, int Nc)
 {
     extern __shared__ int32_t ni_lds[];   // [K][A][64]
     const int lane = threadIdx.x;
     const int c = blockIdx.y;
-    const int i0 = c * ds.Nc;
-    const int i1 = min(ds.N, i0 + ds.Nc);
+    const int i0 = c * Nc;
+    const int i1 = min(ds.N, i0 + Nc);
     for (int x = lane; x < K * A * kTileLoci; x += 64) ni_lds[x] = 0;
     __syncthreads();
     const int64_t tix = blockIdx.x;
@@ -379,26 +405,68 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, int m, hipStrea
     // table fits; otherwise they add into the chunk partials, which must then
     // be cleared before any tile writes its counts.
     const int A = ds.maxK;
-    const size_t mi_bytes = (size_t)ds.Nc * K * sizeof(int32_t);
+    const size_t mi_bytes = (size_t)c.Nc * K * sizeof(int32_t);
     size_t ni_bytes = (size_t)kWavesPerBlock * K * A * kTileLoci * sizeof(int32_t);
     const int in_lds = mi_bytes + ni_bytes <= 64 * 1024;
     if (ds.ngen > 0 && !in_lds) {
-        int rc = launch_fill_i32(c.Ni_part, (int64_t)ds.G * K * ds.KT, 0, st);
+        int rc = launch_fill_i32(c.Ni_part, (int64_t)c.G * K * ds.KT, 0, st);
         if (rc) return rc;
         ni_bytes = 0;
     }
     if (ds.nbi > 0) {
-        dim3 grid((unsigned)((ds.nbi + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)ds.G);
-        hipLaunchKernelGGL((k_z_admix_bi<K, Real>), grid, dim3(nthr), mi_bytes, st, ds, c.Freq, c.Q, c.seed, m, c.Ni_part, c.Mi);
+        dim3 grid((unsigned)((ds.nbi + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)c.G);
+        const size_t p_bytes = (size_t)kWavesPerBlock * K * 2 * kTileLoci * sizeof(Real);
+        if (ds.maxploidy == 2)
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 2>), grid, dim3(nthr), p_bytes + mi_bytes, st, ds, c.Freq, c.Q, c.seed, m, c.Ni_part, c.Mi, c.Nc, c.jump);
+        else if (ds.maxploidy == 4)
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 4>), grid, dim3(nthr), p_bytes + mi_bytes, st, ds, c.Freq, c.Q, c.seed, m, c.Ni_part, c.Mi, c.Nc, c.jump);
+        else
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 0>), grid, dim3(nthr), p_bytes + mi_bytes, st, ds, c.Freq, c.Q, c.seed, m, c.Ni_part, c.Mi, c.Nc, c.jump);
         VPS_HIP(hipGetLastError());
     }
     if (ds.ngen > 0) {
-        dim3 grid((unsigned)((ds.ngen + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)ds.G);
+        dim3 grid((unsigned)((ds.ngen + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)c.G);
         hipLaunchKernelGGL((k_z_admix_gen<K, Real>), grid, dim3(nthr), mi_bytes + ni_bytes, st, ds, c.Freq, c.Q, c.seed,
-                           m, c.Ni_part, c.Mi, A, in_lds);
+                           m, c.Ni_part, c.Mi, A, in_lds, c.Nc, c.jump);
         VPS_HIP(hipGetLastError());
     }
     return VPS_OK;
+}
+
+// resident blocks per CU of the Z kernels for a chunk of Nc individuals
+template <int K, typename Real>
+int z_occupancy(const DeviceDataset& ds, int Nc, int* bi_blocks, int* gen_blocks)
+{
+    const size_t mi_bytes = (size_t)Nc * K * sizeof(int32_t);
+    const size_t p_bytes = (size_t)kWavesPerBlock * K * 2 * kTileLoci * sizeof(Real);
+    *bi_blocks = *gen_blocks = 1;
+    hipError_t e = hipSuccess;
+    if (ds.maxploidy == 2)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 2>, 256, p_bytes + mi_bytes);
+    else if (ds.maxploidy == 4)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 4>, 256, p_bytes + mi_bytes);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 0>, 256, p_bytes + mi_bytes);
+    if (e != hipSuccess) return fail(VPS_ERR_DEVICE, "occupancy query: %s", hipGetErrorString(e));
+    const size_t ni_bytes = (size_t)kWavesPerBlock * K * ds.maxK * kTileLoci * sizeof(int32_t);
+    const size_t gshm = mi_bytes + (mi_bytes + ni_bytes <= 64 * 1024 ? ni_bytes : 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(gen_blocks, (const void*)k_z_admix_gen<K, Real>, 256, gshm);
+    if (e != hipSuccess) return fail(VPS_ERR_DEVICE, "occupancy query: %s", hipGetErrorString(e));
+    *bi_blocks = std::max(*bi_blocks, 1);
+    *gen_blocks = std::max(*gen_blocks, 1);
+    return VPS_OK;
+}
+
+template <typename Real>
+int z_occupancy_k(const DeviceDataset& ds, int K, int Nc, int* b, int* g)
+{
+    switch (K) {
+#define VPS_CASE(k) case k: return z_occupancy<k, Real>(ds, Nc, b, g);
+        VPS_CASE(1) VPS_CASE(2) VPS_CASE(3) VPS_CASE(4) VPS_CASE(5) VPS_CASE(6) VPS_CASE(7) VPS_CASE(8)
+        VPS_CASE(9) VPS_CASE(10) VPS_CASE(11) VPS_CASE(12) VPS_CASE(13) VPS_CASE(14) VPS_CASE(15) VPS_CASE(16)
+#undef VPS_CASE
+    default: return fail(VPS_ERR_UNSUPPORTED, "K=%d outside 1..%d", K, VPS_MAX_K);
+    }
 }
 
 template <typename Real>
@@ -414,6 +482,52 @@ int dispatch_k(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st
 }
 
 }  // namespace
+
+// Chunking model.  A block (4 wavefronts, one per SIMD) costs
+// (Nc * maxploidy + 8) slot-steps (jump-ahead and set-up ~ 8 slots); a CU
+// streams its blocks back to back, so the pass takes ceil(blocks / CUs)
+// block-times, stretched when fewer than 3 wavefronts per SIMD are resident
+// to hide FP64 latency.  Chunk partial counts add 8 bytes per (chunk, cluster,
+// allele) of HBM traffic (written, then reduced).  A slot-step is ~(12 + 10K)
+// VALU instructions of 4 cycles at 2.4 GHz.  The chunk count with the
+// smallest modelled time wins, among chunks of 16..kMaxChunk individuals.
+int choose_chunks(const DeviceDataset& ds, int K, int precision, int device, int* Gout)
+{
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
+    const int N = ds.N;
+    const double slot_us = (12.0 + 10.0 * K) * 4.0 / 2400.0;
+    const double hbm_bytes_per_us = 5.0e6;
+    double best = 1e300;
+    int bestG = (N + kMaxChunk - 1) / kMaxChunk;
+    const int gmin = std::max(1, (N + kMaxChunk - 1) / kMaxChunk);
+    const int gmax = std::max(gmin, std::min(256, (N + 15) / 16));
+    int lastNc = -1;
+    auto eff = [](int64_t waves_per_simd) { return waves_per_simd >= 3 ? 1.0 : waves_per_simd == 2 ? 0.8 : 0.55; };
+    for (int G = gmin; G <= gmax; ++G) {
+        const int Nc = (N + G - 1) / G;
+        if (Nc == lastNc) continue;
+        lastNc = Nc;
+        const int Gr = (N + Nc - 1) / Nc;
+        int bb = 1, gb = 1;
+        int rc = precision == VPS_PREC_MIXED ? z_occupancy_k<float>(ds, K, Nc, &bb, &gb) : z_occupancy_k<double>(ds, K, Nc, &bb, &gb);
+        if (rc) return rc;
+        const double work = ((double)Nc * ds.maxploidy + 8.0) * slot_us;
+        double t = (double)Gr * K * ds.KT * 8.0 / hbm_bytes_per_us;
+        const int64_t nb[2] = {(ds.nbi + kWavesPerBlock - 1) / kWavesPerBlock * (int64_t)Gr,
+                               (ds.ngen + kWavesPerBlock - 1) / kWavesPerBlock * (int64_t)Gr};
+        const int occ[2] = {bb, gb};
+        for (int x = 0; x < 2; ++x) {
+            if (!nb[x]) continue;
+            const int64_t per_cu = (nb[x] + cus - 1) / cus;
+            t += (double)per_cu * work / eff(std::min<int64_t>(per_cu, occ[x]));
+        }
+        if (t < best * 0.999) { best = t; bestG = Gr; }
+    }
+    *Gout = bestG;
+    return VPS_OK;
+}
 
 int launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st)
 {
@@ -437,7 +551,7 @@ int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_
 int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     const int64_t n = (int64_t)c.K * ds.KT;
-    hipLaunchKernelGGL(k_reduce_ni, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c.Ni_part, c.Ni, n, ds.G);
+    hipLaunchKernelGGL(k_reduce_ni, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c.Ni_part, c.Ni, n, c.G);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
@@ -448,9 +562,9 @@ int launch_z_noadmix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
     const size_t shm = (size_t)c.K * A * kTileLoci * sizeof(int32_t);
     if (shm > 64 * 1024) return fail(VPS_ERR_UNSUPPORTED, "no-admix count table K=%d x %d alleles exceeds LDS", c.K, A);
     if (ds.nbi > 0)
-        hipLaunchKernelGGL(k_z_noadmix, dim3((unsigned)ds.nbi, (unsigned)ds.G), dim3(64), shm, st, ds, c.Z, c.K, c.Ni_part, A, kKindBi);
+        hipLaunchKernelGGL(k_z_noadmix, dim3((unsigned)ds.nbi, (unsigned)c.G), dim3(64), shm, st, ds, c.Z, c.K, c.Ni_part, A, kKindBi, c.Nc);
     if (ds.ngen > 0)
-        hipLaunchKernelGGL(k_z_noadmix, dim3((unsigned)ds.ngen, (unsigned)ds.G), dim3(64), shm, st, ds, c.Z, c.K, c.Ni_part, A, kKindGen);
+        hipLaunchKernelGGL(k_z_noadmix, dim3((unsigned)ds.ngen, (unsigned)c.G), dim3(64), shm, st, ds, c.Z, c.K, c.Ni_part, A, kKindGen, c.Nc);
     VPS_HIP(hipGetLastError());
     return launch_reduce_ni(ds, c, st);
 }
