@@ -9,12 +9,21 @@
 // sums run over typed allele copies x of individual i at locus l.
 //
 // These passes draw no random numbers, so lanes map to individuals: a lane
-// keeps its individual's Q row in registers and its own exponent-tracking
-// log accumulator (src/math2.h:716-902) across a range of loci, with no
-// cross-lane reduction inside the loop.  Each (locus range, individual)
-// writes one closed partial; a fixed-order reduction adds the partials.
-// The reference multiplies 64-lane products into one accumulator instead;
-// the two orders agree to rounding (a few ulp per partial).
+// keeps its individual's Q row in registers and its own log accumulator
+// across a range of tiles, with no cross-lane reduction inside the loop.
+// Within one tile every lane reads the same locus, so on biallelic tiles the
+// 2K frequencies of a locus are wave-uniform scalar loads and a lane forms
+// both allele mixtures f_a = sum_k q_k P[k][l][a] once per locus, then
+// charges one of them per allele copy.  Multiallelic tiles gather P per copy.
+//
+// The accumulator keeps the product as mantissa in [0.5, 1) times 2^e
+// (frexp after each factor), which is the reference's exponent-stripping
+// log accumulator (src/math2.h:716-902) with the strip done every time:
+// scaling by powers of two is exact, so only the product order differs from
+// the reference's 64-lane products, and the sums agree to rounding.
+// Each (locus range, individual) closes one partial; record passes sum a
+// block's partials in LDS in a fixed tree, and a fixed-order pass adds the
+// block sums (or, per individual, the range partials).
 #include "vps_chain.h"
 
 namespace vps {
@@ -22,29 +31,87 @@ namespace vps {
 constexpr int kLikThreads = 256;
 constexpr int kLikTilesPerRange = 8;
 
+struct FrexpAcc {
+    double m;
+    int e;
+    __device__ void open() { m = 1.0; e = 0; }
+    __device__ void charge(double x)
+    {
+        const double y = m * x;
+        e += __builtin_amdgcn_frexp_exp(y);
+        m = __builtin_amdgcn_frexp_mant(y);
+    }
+    __device__ double close() const { return ((double)e + log2(m)) * kLn2Trunc; }
+};
+
+// modes 0/1: sum the block's 256 partials in a fixed tree
+__device__ void block_store_sum(double v, double* dst)
+{
+    __shared__ double sh[kLikThreads];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = kLikThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *dst = sh[0];
+}
+
 template <int K, int MODE>
-__global__ __launch_bounds__(kLikThreads) void k_lik(const DeviceDataset ds, const ChainDev c, int tiles_per_range)
+__device__ void lik_load_q(const ChainDev& c, int i, double* q, double* qp)
+{
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (MODE == 1) q[k] = c.Z[i] == k ? 1.0 : 0.0;   // one-hot: sum_k q_k p_k == p_Z exactly
+        else if (MODE != 3) q[k] = c.Q[(int64_t)i * K + k];
+        else q[k] = 0.0;
+        qp[k] = MODE == 2 ? c.bufNK1[(int64_t)i * K + k] : 0.0;
+    }
+}
+
+template <int K, int MODE>
+__device__ void lik_write(const DeviceDataset& ds, const ChainDev& c, FrexpAcc* acc, bool ok, int i, int64_t range,
+                          int64_t block_slot)
+{
+    if (MODE <= 1) {
+        block_store_sum(ok ? acc[0].close() : 0.0, c.part + block_slot);
+    } else if (MODE == 2) {
+        if (ok) c.part[range * ds.N + i] = acc[0].close() - acc[1].close();
+    } else {
+        if (ok) {
+            double* out = c.part + range * ds.N * K + (int64_t)i * K;
+#pragma unroll
+            for (int k = 0; k < K; ++k) out[k] = acc[k].close();
+        }
+    }
+}
+
+// number of accumulators per lane
+template <int K, int MODE>
+struct LikNA { static constexpr int value = MODE == 3 ? K : (MODE == 2 ? 2 : 1); };
+
+// biallelic tiles (at most two alleles, genotype table in ds.bitab).  MPT is
+// the maximum ploidy when it is 2 or 4 (copies loop fully unrolled), else 0.
+template <int K, int MODE, int MPT>
+__global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, const ChainDev c, int tiles_per_range)
 {
     const int i = blockIdx.y * kLikThreads + threadIdx.x;
     const bool ok = i < ds.N;
     const int ii = ok ? i : 0;
     const int64_t KT = ds.KT;
     double q[K], qp[K];
+    lik_load_q<K, MODE>(c, ii, q, qp);
+    constexpr int NA = LikNA<K, MODE>::value;
+    FrexpAcc acc[NA];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        q[k] = c.Q[(int64_t)ii * K + k];
-        qp[k] = MODE == 2 ? c.bufNK1[(int64_t)ii * K + k] : 0.0;
-    }
-    const int zi = MODE == 1 ? c.Z[ii] : 0;
-    constexpr int NA = MODE == 3 ? K : 1;
-    LogAcc acc[NA];
-#pragma unroll
-    for (int k = 0; k < NA; ++k) acc[k].open();
+    for (int a = 0; a < NA; ++a) acc[a].open();
+    constexpr int kUnroll = MPT > 0 ? MPT : 1;
+    const int ncopies = MPT > 0 ? MPT : ds.maxploidy;
 
-    const int64_t t0 = (int64_t)blockIdx.x * tiles_per_range;
-    const int64_t t1 = min(ds.ntiles, t0 + tiles_per_range);
-    for (int64_t t = t0; t < t1; ++t) {
-        const TileDesc td = ds.tiles[t];
+    const int64_t r0 = (int64_t)blockIdx.x * tiles_per_range;
+    const int64_t r1 = min(ds.nbi, r0 + tiles_per_range);
+    for (int64_t r = r0; r < r1; ++r) {
+        const TileDesc td = ds.tiles[ds.bi_list[r]];
         const int w = ii / td.ipw;
         const int sh = (ii - w * td.ipw) * td.bits;
         const uint32_t mask = (1u << td.bits) - 1u;
@@ -52,52 +119,112 @@ __global__ __launch_bounds__(kLikThreads) void k_lik(const DeviceDataset ds, con
         for (int j = 0; j < td.nloc; ++j) {
             const int64_t l = td.l0 + j;
             const uint32_t gid = (wp[j] >> sh) & mask;
+            const uint32_t desc = (uint32_t)(ds.bitab[l] >> (gid * 8)) & 0xFFu;
+            const int v = ok ? (int)(desc & 15u) : 0;
+            const int h = v - (int)(desc >> 4);    // copies a >= h carry allele 1
             const double* pl = c.Freq + ds.afo[l];
-            int v;
-            uint64_t d64 = 0;
-            int h = 0;
-            if (td.kind == kKindBi) {
-                const uint32_t desc = (uint32_t)(ds.bitab[l] >> (gid * 8)) & 0xFFu;
-                v = (int)(desc & 15u);
-                h = v - (int)(desc >> 4);
-            } else {
-                d64 = ds.gdesc[ds.gbase[l] + gid];
-                v = (int)(d64 & 15u);
+            const int a1 = ds.k2[l] > 1 ? 1 : 0;   // monomorphic loci have no allele 1
+            double p0[K], p1[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                p0[k] = pl[k * KT];
+                p1[k] = pl[k * KT + a1];
             }
-            if (!ok) v = 0;
-            for (int a = 0; a < v; ++a) {
-                const int x = td.kind == kKindBi ? (a >= h ? 1 : 0) : (int)((d64 >> (4 + 6 * a)) & 63u);
-                const double* px = pl + x;
-                if (MODE == 0) {
-                    double f = 0;
+            if (MODE == 3) {
+#pragma unroll kUnroll
+                for (int a = 0; a < ncopies; ++a) {
+                    const bool on = a < v, hi = a >= h;
 #pragma unroll
-                    for (int k = 0; k < K; ++k) f = f + px[k * KT] * q[k];
-                    acc[0].charge(f);
-                } else if (MODE == 1) {
-                    acc[0].charge(px[(int64_t)zi * KT]);
-                } else if (MODE == 2) {
-                    double f1 = 0, f2 = 0;
+                    for (int k = 0; k < K; ++k) acc[k].charge(on ? (hi ? p1[k] : p0[k]) : 1.0);
+                }
+            } else {
+                // sums start from the k = 0 term (0 + x == x for the
+                // non-negative terms here)
+                double f0 = p0[0] * q[0], f1 = p1[0] * q[0];
+                double g0 = MODE == 2 ? p0[0] * qp[0] : 0.0, g1 = MODE == 2 ? p1[0] * qp[0] : 0.0;
 #pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        const double p = px[k * KT];
-                        f1 = f1 + p * qp[k];
-                        f2 = f2 + p * q[k];
+                for (int k = 1; k < K; ++k) {
+                    f0 = f0 + p0[k] * q[k];
+                    f1 = f1 + p1[k] * q[k];
+                    if (MODE == 2) {
+                        g0 = g0 + p0[k] * qp[k];
+                        g1 = g1 + p1[k] * qp[k];
                     }
-                    acc[0].charge(f1 / f2);
-                } else {
-#pragma unroll
-                    for (int k = 0; k < K; ++k) acc[k < NA ? k : 0].charge(px[k * KT]);
+                }
+#pragma unroll kUnroll
+                for (int a = 0; a < ncopies; ++a) {
+                    const bool on = a < v, hi = a >= h;
+                    if (MODE == 2) {
+                        acc[0].charge(on ? (hi ? g1 : g0) : 1.0);
+                        acc[1].charge(on ? (hi ? f1 : f0) : 1.0);
+                    } else {
+                        acc[0].charge(on ? (hi ? f1 : f0) : 1.0);
+                    }
                 }
             }
         }
     }
-    if (!ok) return;
-    double* out = c.part + (int64_t)blockIdx.x * ds.N * NA + (int64_t)i * NA;
-#pragma unroll
-    for (int k = 0; k < NA; ++k) out[k] = acc[k].close();
+    lik_write<K, MODE>(ds, c, acc, ok, i, (int64_t)blockIdx.x, (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
-// lnL = sum of all partials, one block, fixed order
+// multiallelic tiles: genotype descriptors in ds.gdesc, P gathered per copy
+template <int K, int MODE>
+__global__ __launch_bounds__(kLikThreads) void k_lik_gen(const DeviceDataset ds, const ChainDev c, int tiles_per_range,
+                                                        int range_base, int64_t slot_base)
+{
+    const int i = blockIdx.y * kLikThreads + threadIdx.x;
+    const bool ok = i < ds.N;
+    const int ii = ok ? i : 0;
+    const int64_t KT = ds.KT;
+    double q[K], qp[K];
+    lik_load_q<K, MODE>(c, ii, q, qp);
+    constexpr int NA = LikNA<K, MODE>::value;
+    FrexpAcc acc[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) acc[a].open();
+
+    const int64_t r0 = (int64_t)blockIdx.x * tiles_per_range;
+    const int64_t r1 = min(ds.ngen, r0 + tiles_per_range);
+    for (int64_t r = r0; r < r1; ++r) {
+        const TileDesc td = ds.tiles[ds.gen_list[r]];
+        const int w = ii / td.ipw;
+        const int sh = (ii - w * td.ipw) * td.bits;
+        const uint32_t mask = (1u << td.bits) - 1u;
+        const uint32_t* wp = ds.words + td.wbase + (int64_t)w * kTileLoci;
+        for (int j = 0; j < td.nloc; ++j) {
+            const int64_t l = td.l0 + j;
+            const uint32_t gid = (wp[j] >> sh) & mask;
+            const uint64_t d64 = ds.gdesc[ds.gbase[l] + gid];
+            const int v = ok ? (int)(d64 & 15u) : 0;
+            const double* pl = c.Freq + ds.afo[l];
+            for (int a = 0; a < v; ++a) {
+                const double* px = pl + (int)((d64 >> (4 + 6 * a)) & 63u);
+                if (MODE == 3) {
+#pragma unroll
+                    for (int k = 0; k < K; ++k) acc[k].charge(px[k * KT]);
+                } else {
+                    double f = 0, g = 0;
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const double p = px[k * KT];
+                        f = f + p * q[k];
+                        if (MODE == 2) g = g + p * qp[k];
+                    }
+                    if (MODE == 2) {
+                        acc[0].charge(g);
+                        acc[1].charge(f);
+                    } else {
+                        acc[0].charge(f);
+                    }
+                }
+            }
+        }
+    }
+    lik_write<K, MODE>(ds, c, acc, ok, i, range_base + (int64_t)blockIdx.x,
+                       slot_base + (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+// lnL = sum of the block partials, one block, fixed order
 __global__ __launch_bounds__(256) void k_reduce_lnl(const double* __restrict__ part, int64_t n, double* __restrict__ out)
 {
     __shared__ double sh[256];
@@ -106,7 +233,7 @@ __global__ __launch_bounds__(256) void k_reduce_lnl(const double* __restrict__ p
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) *out = sh[0];
@@ -114,70 +241,126 @@ __global__ __launch_bounds__(256) void k_reduce_lnl(const double* __restrict__ p
 
 // per-individual (x NA) sums over locus ranges; mode 3 adds the location
 // prior log Gamma[pop][k] for typed individuals (charged first in the
-// reference, src/structureAVX.cpp:260-266)
-__global__ void k_reduce_rows(const DeviceDataset ds, const ChainDev c, int nranges, int NA, int add_prior, double* __restrict__ dst)
+// reference, src/structureAVX.cpp:260-266).  A block covers 64 outputs with
+// 4 threads each; thread s sums ranges s, s+4, ... and the four partials are
+// added in a fixed order.
+constexpr int kRowSplit = 4;
+__global__ __launch_bounds__(256) void k_reduce_rows(const DeviceDataset ds, const ChainDev c, int nranges, int NA,
+                                                     int add_prior, double* __restrict__ dst)
 {
-    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double sh[kRowSplit][64];
+    const int lane = threadIdx.x & 63, split = threadIdx.x >> 6;
+    const int64_t x = (int64_t)blockIdx.x * 64 + lane;
     const int64_t n = (int64_t)ds.N * NA;
-    if (x >= n) return;
     double s = 0;
+    if (x < n) {
+        const double* p = c.part + x;
+        int r = split;
+        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        for (; r + 3 * kRowSplit < nranges; r += 4 * kRowSplit) {
+            a0 += p[(int64_t)r * n];
+            a1 += p[(int64_t)(r + kRowSplit) * n];
+            a2 += p[(int64_t)(r + 2 * kRowSplit) * n];
+            a3 += p[(int64_t)(r + 3 * kRowSplit) * n];
+        }
+        for (; r < nranges; r += kRowSplit) a0 += p[(int64_t)r * n];
+        s = (a0 + a1) + (a2 + a3);
+    }
+    sh[split][lane] = s;
+    __syncthreads();
+    if (split != 0 || x >= n) return;
+    double t = 0;
     if (add_prior) {
         const int i = (int)(x / NA), k = (int)(x % NA);
         if (ds.vt[i]) {
             LogAcc a;
             a.open();
             a.charge(c.Gamma[(int64_t)ds.popid[i] * c.K + k]);
-            s = a.close();
+            t = a.close();
         }
     }
-    for (int r = 0; r < nranges; ++r) s += c.part[(int64_t)r * n + x];
-    dst[x] = s;
+    dst[x] = t + ((sh[0][lane] + sh[1][lane]) + (sh[2][lane] + sh[3][lane]));
 }
 
 namespace {
 
+struct LikGrid {
+    int nr_bi, nr_gen, gy;
+};
+
+LikGrid lik_grid(const DeviceDataset& ds)
+{
+    LikGrid g;
+    g.nr_bi = (int)((ds.nbi + kLikTilesPerRange - 1) / kLikTilesPerRange);
+    g.nr_gen = (int)((ds.ngen + kLikTilesPerRange - 1) / kLikTilesPerRange);
+    g.gy = (ds.N + kLikThreads - 1) / kLikThreads;
+    return g;
+}
+
+template <int K, int MODE>
+int launch_lik_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, const LikGrid& g)
+{
+    if (g.nr_bi > 0) {
+        const dim3 grid((unsigned)g.nr_bi, (unsigned)g.gy);
+        if (ds.maxploidy == 2)
+            hipLaunchKernelGGL((k_lik_bi<K, MODE, 2>), grid, dim3(kLikThreads), 0, st, ds, c, kLikTilesPerRange);
+        else if (ds.maxploidy == 4)
+            hipLaunchKernelGGL((k_lik_bi<K, MODE, 4>), grid, dim3(kLikThreads), 0, st, ds, c, kLikTilesPerRange);
+        else
+            hipLaunchKernelGGL((k_lik_bi<K, MODE, 0>), grid, dim3(kLikThreads), 0, st, ds, c, kLikTilesPerRange);
+        VPS_HIP(hipGetLastError());
+    }
+    if (g.nr_gen > 0) {
+        const dim3 grid((unsigned)g.nr_gen, (unsigned)g.gy);
+        hipLaunchKernelGGL((k_lik_gen<K, MODE>), grid, dim3(kLikThreads), 0, st, ds, c, kLikTilesPerRange, g.nr_bi,
+                           (int64_t)g.nr_bi * g.gy);
+        VPS_HIP(hipGetLastError());
+    }
+    return VPS_OK;
+}
+
 template <int MODE>
-int launch_lik_mode(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, dim3 grid)
+int launch_lik_mode(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, const LikGrid& g)
 {
     switch (c.K) {
-#define VPS_CASE(k) case k: hipLaunchKernelGGL((k_lik<k, MODE>), grid, dim3(kLikThreads), 0, st, ds, c, kLikTilesPerRange); break;
+#define VPS_CASE(k) case k: return launch_lik_k<k, MODE>(ds, c, st, g);
         VPS_CASE(1) VPS_CASE(2) VPS_CASE(3) VPS_CASE(4) VPS_CASE(5) VPS_CASE(6) VPS_CASE(7) VPS_CASE(8)
         VPS_CASE(9) VPS_CASE(10) VPS_CASE(11) VPS_CASE(12) VPS_CASE(13) VPS_CASE(14) VPS_CASE(15) VPS_CASE(16)
 #undef VPS_CASE
     default: return fail(VPS_ERR_UNSUPPORTED, "K=%d outside 1..%d", c.K, VPS_MAX_K);
     }
-    VPS_HIP(hipGetLastError());
-    return VPS_OK;
 }
 
 }  // namespace
 
 int lik_part_count(const DeviceDataset& ds, int mode, int K)
 {
-    const int64_t nranges = (ds.ntiles + kLikTilesPerRange - 1) / kLikTilesPerRange;
-    return (int)(nranges * ds.N * (mode == 3 ? K : 1));
+    const LikGrid g = lik_grid(ds);
+    const int64_t nr = (int64_t)g.nr_bi + g.nr_gen;
+    if (mode <= 1) return (int)(nr * g.gy);
+    return (int)(nr * ds.N * (mode == 3 ? K : 1));
 }
 
 int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c, int mode, hipStream_t st)
 {
-    const int nranges = (int)((ds.ntiles + kLikTilesPerRange - 1) / kLikTilesPerRange);
-    dim3 grid((unsigned)nranges, (unsigned)((ds.N + kLikThreads - 1) / kLikThreads));
+    const LikGrid g = lik_grid(ds);
     int rc;
     switch (mode) {
-    case 0: rc = launch_lik_mode<0>(ds, c, st, grid); break;
-    case 1: rc = launch_lik_mode<1>(ds, c, st, grid); break;
-    case 2: rc = launch_lik_mode<2>(ds, c, st, grid); break;
-    case 3: rc = launch_lik_mode<3>(ds, c, st, grid); break;
+    case 0: rc = launch_lik_mode<0>(ds, c, st, g); break;
+    case 1: rc = launch_lik_mode<1>(ds, c, st, g); break;
+    case 2: rc = launch_lik_mode<2>(ds, c, st, g); break;
+    case 3: rc = launch_lik_mode<3>(ds, c, st, g); break;
     default: return fail(VPS_ERR_INVALID, "likelihood mode %d", mode);
     }
     if (rc) return rc;
+    const int nranges = g.nr_bi + g.nr_gen;
     if (mode <= 1) {
-        hipLaunchKernelGGL(k_reduce_lnl, dim3(1), dim3(256), 0, st, c.part, (int64_t)nranges * ds.N, c.scal + 1);
+        hipLaunchKernelGGL(k_reduce_lnl, dim3(1), dim3(256), 0, st, c.part, (int64_t)nranges * g.gy, c.scal + 1);
     } else {
         const int NA = mode == 3 ? c.K : 1;
         const int64_t n = (int64_t)ds.N * NA;
         double* dst = mode == 2 ? c.bufN1 : c.bufNK1;
-        hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ds, c, nranges, NA,
+        hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)((n + 63) / 64)), dim3(64 * kRowSplit), 0, st, ds, c, nranges, NA,
                            (mode == 3 && c.locpriori) ? 1 : 0, dst);
     }
     VPS_HIP(hipGetLastError());
