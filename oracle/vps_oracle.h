@@ -15,7 +15,10 @@
  * subthread (structure_nsubthread = 1); the Z pass and integer counts are
  * thread-count invariant in the reference and here.
  *
- * Parity status of each function is recorded in oracle/README.md.
+ * Each function is pinned by tests/test_oracle_*.py (closed forms, invariants,
+ * distribution tests); functions without an independent check: the locprior
+ * r/eta/gamma and F-model Metropolis updates, per-cluster and local alpha,
+ * lambda and Arrange -- parity unpinned.
  */
 #ifndef VPS_ORACLE_H
 #define VPS_ORACLE_H
