@@ -229,6 +229,25 @@ int vps_pass_update_q_admix(vps_chain* ch);    /* UpdateQAdmix,       src/struct
 int vps_pass_update_alpha(vps_chain* ch);      /* UpdateAlpha,        src/structure.cpp:1501-1594 */
 
 /* ------------------------------------------------------------------------- */
+/* Batched chains.  The reference runs K values x replicates as a task pool of */
+/* host threads, each leasing a device slot for one chain and driving it with */
+/* synchronous per-pass calls (CalcBayesian, BayesianThread, ManageCUDA;      */
+/* src/structure.cpp:2397-2501, 711-756).  A batch advances any number of      */
+/* chains on one device together: one HIP-graph launch per sweep, with every   */
+/* chain a parallel branch.  Results equal those of vps_chain_step on each     */
+/* chain alone, bit for bit.  Chains stay owned by the caller and must outlive */
+/* the batch; chain calls may be mixed with batch steps.                       */
+/* ------------------------------------------------------------------------- */
+typedef struct vps_batch vps_batch;
+int vps_batch_create(vps_chain* const* chains, int n, vps_batch** out);
+/* n_sweeps sweeps of every chain; asynchronous */
+int vps_batch_step(vps_batch* b, int n_sweeps);
+int vps_batch_sync(vps_batch* b);
+int vps_batch_destroy(vps_batch* b);
+/* graph launches, graphs captured, sweeps launched without a graph */
+int vps_batch_stats(const vps_batch* b, int64_t* graph_launches, int64_t* captures, int64_t* eager_sweeps);
+
+/* ------------------------------------------------------------------------- */
 /* Timing hook for measurement: average device time of the last launches of  */
 /* a named kernel family, measured with HIP events on the chain's stream.     */
 /* ------------------------------------------------------------------------- */

@@ -21,6 +21,7 @@ from ._lib import (  # noqa: F401
     Synth,
     Dataset,
     Chain,
+    Batch,
     STATE,
     RNG_REFERENCE,
     RNG_PARALLEL,

@@ -121,6 +121,11 @@ _SIGS = {
     "vps_pass_update_alpha": ([_P], C.c_int),
     "vps_chain_enable_timing": ([_P, C.c_int], C.c_int),
     "vps_chain_kernel_time": ([_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)], C.c_int),
+    "vps_batch_create": ([C.POINTER(_P), C.c_int, C.POINTER(_P)], C.c_int),
+    "vps_batch_step": ([_P, C.c_int], C.c_int),
+    "vps_batch_sync": ([_P], C.c_int),
+    "vps_batch_destroy": ([_P], C.c_int),
+    "vps_batch_stats": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "vps_debug_decode": ([_P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), _P], C.c_int),
     "vps_debug_jump": ([C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
     "vps_debug_chunking": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
@@ -416,6 +421,40 @@ class Chain:
     def close(self):
         if getattr(self, "h", None):
             load().vps_chain_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Chains on one device advanced together, one graph launch per sweep
+    (vps_batch_*).  Keeps references to its chains."""
+
+    def __init__(self, chains):
+        self.chains = list(chains)
+        arr = (_P * len(self.chains))(*[c.h for c in self.chains])
+        h = _P()
+        _check(load().vps_batch_create(arr, len(self.chains), C.byref(h)))
+        self.h = h
+
+    def step(self, n=1):
+        _check(load().vps_batch_step(self.h, int(n)))
+
+    def sync(self):
+        _check(load().vps_batch_sync(self.h))
+
+    def stats(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(load().vps_batch_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"graph_launches": a.value, "captures": b.value, "eager_sweeps": c.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().vps_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -12,6 +12,7 @@ struct ChainDev {
     int32_t admix, locpriori, fmodel, topology, precision;
     int32_t inferalpha, diffalpha, uniformalpha, inferlambda, difflambda, fsame;
     int32_t nadmburnin, rlen;
+    int32_t nburnin, nthinning, writelnl, lnl_cap;   // Record schedule (nburnin includes nadmburnin)
     double stdalpha, maxalpha, alphapriora, alphapriorb;
     double stdlambda, maxlambda;
     double maxr, epsr, epseta, epsgamma;
@@ -41,6 +42,8 @@ struct ChainDev {
     double*   rout;      // [rlen]
     double*   lnltrace;  // [nlnl capacity]
     Rng*      rngbuf;    // [N] streams kept between a draw kernel and its acceptance kernel
+    int32_t*  mdev;      // [1] sweep index m; kernels read it, the sweep's last kernel advances it,
+                         // so one captured sweep replays at any m
     // individual chunking of the genotype passes: G chunks of Nc individuals;
     // jump[c] advances a locus stream to individual c*Nc
     int32_t   G, Nc;
@@ -51,7 +54,7 @@ struct ChainDev {
 constexpr int kLikLociPerWave = 1024;
 
 // launchers (vps_kernels.hip / vps_updates.hip); all asynchronous on `st`
-int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
+int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
 int launch_z_noadmix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 // likelihood passes: mode 0 = Record lnL (admix), 1 = Record lnL (no admix),
@@ -59,19 +62,22 @@ int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c, int mode, hipStream_t st);
 int lik_part_count(const DeviceDataset& ds, int mode, int K);
 
-int launch_update_p(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
-int launch_update_q_admix(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
-int launch_q_metro_propose(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
-int launch_q_metro_accept(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
-int launch_q_noadmix_draw(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
+int launch_update_p(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+int launch_update_q_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+int launch_q_metro_propose(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+int launch_q_metro_accept(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+int launch_q_noadmix_draw(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 int launch_update_di(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
-int launch_update_locpriori(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
-int launch_update_alpha(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
-int launch_update_lambda(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
-int launch_update_a(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
-int launch_update_f(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st);
-int launch_record_accumulate(const DeviceDataset& ds, const ChainDev& c, int post_burnin, int trace_slot,
+int launch_update_locpriori(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+int launch_update_alpha(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+int launch_update_lambda(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+int launch_update_a(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+int launch_update_f(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+// trace: 1 = store lnL in the writelnl trace at the slot that sweep m fills
+int launch_record_accumulate(const DeviceDataset& ds, const ChainDev& c, int post_burnin, int trace,
                              int accumulate_misum, hipStream_t st);
+// m += 1 on the device (the sweep's last launch)
+int launch_advance_m(const ChainDev& c, hipStream_t st);
 int launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
 // chunk count minimising (grid rounds x per-block work) at the kernel's occupancy
 int choose_chunks(const DeviceDataset& ds, int K, int precision, int device, int* G);

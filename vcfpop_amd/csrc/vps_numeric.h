@@ -61,6 +61,19 @@ VPS_HD uint64_t murmur64(uint64_t data, uint64_t seed)
     return ((uint64_t)murmur2(d1, s1) << 32) | murmur2(d0, s0);
 }
 
+// HashULong: CRC32C (reflected polynomial 0x82F63B78) of the 8 bytes of v,
+// little-endian, from the initial value 0xB385FDA9 (src/hash.cpp:219-223).
+// Bitwise form: eight shift/xor steps per byte.
+VPS_HD uint32_t hash_ulong(uint64_t v)
+{
+    uint32_t c = 0xB385FDA9u;
+    for (int b = 0; b < 8; ++b) {
+        c ^= (uint32_t)(v >> (8 * b)) & 0xFFu;
+        for (int k = 0; k < 8; ++k) c = (c >> 1) ^ (0x82F63B78u & (0u - (c & 1u)));
+    }
+    return c;
+}
+
 // xorshift128+ state of one stream, seeded as RNG<double>(s, salt),
 // src/statistics.cpp:62-70
 struct Xs128 {

@@ -4,7 +4,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
-#include "vps_chain.h"
+#include "vps_runtime.h"
 
 using namespace vps;
 
@@ -40,25 +40,7 @@ uint32_t hash_ulong_host(uint64_t v)
     return c;
 }
 
-int launch_set_mi_binary(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
-int launch_arrange(const DeviceDataset& ds, const ChainDev& c, int nr, hipStream_t st);
-
 }  // namespace vps
-
-struct vps_chain {
-    vps_dataset* ds = nullptr;
-    vps_params par{};
-    ChainDev c{};
-    int K = 0, id = 0, m = 0, nr = 0, nlnl = 0, binaryq = 0, singlez = 0, finished = 0;
-    int nburnin = 0, nreps = 0, nthinning = 1, metrofreq = 0, writelnl = 0, nadmburnin = 0;
-    int64_t lnl_cap = 0;
-    hipStream_t stream = nullptr;
-    std::vector<void*> allocs;
-    int64_t device_bytes = 0;
-    bool timing = false;
-    std::vector<hipEvent_t> evs;   // pairs bracketing each timed Z pass
-    size_t ev_used = 0;
-};
 
 extern "C" void vps_params_default(vps_params* p)
 {
@@ -159,6 +141,7 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     const int nl = c.difflambda ? K : 1;
     c.rlen = c.locpriori ? 4 + nl + 1 + K + K * d.S + nf : 4 + nl + na + nf;
     ch->lnl_cap = (ch->nburnin + ch->nthinning - 1) / ch->nthinning + (ch->nreps + ch->nthinning - 1) / ch->nthinning + 1;
+    c.nburnin = ch->nburnin; c.nthinning = ch->nthinning; c.writelnl = ch->writelnl; c.lnl_cap = (int32_t)ch->lnl_cap;
 
     const int64_t N = d.N, KT = d.KT, S = d.S;
     int rc = 0;
@@ -200,6 +183,8 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     if (!rc) rc = chain_alloc(ch, &c.rout, c.rlen);
     if (!rc) rc = chain_alloc(ch, &c.lnltrace, ch->lnl_cap);
     if (!rc) rc = chain_alloc(ch, &c.rngbuf, N);
+    if (!rc) rc = chain_alloc(ch, &c.mdev, 1);
+    if (!rc && hipMemset(c.mdev, 0, sizeof(int32_t)) != hipSuccess) rc = fail(VPS_ERR_DEVICE, "clear sweep index");
     if (rc) { chain_free(ch); return rc; }
 
     // InitFix (src/structure.cpp:468-553): P = 0.5 (1/k2 off biallelic), Q = 1/K,
@@ -277,42 +262,22 @@ extern "C" int vps_chain_destroy(vps_chain* ch)
 
 namespace {
 
-int do_update_p(vps_chain* ch) { return launch_update_p(ch->ds->d, ch->c, ch->m, ch->stream); }
+int do_update_p(vps_chain* ch) { return launch_update_p(ch->ds->d, ch->c, ch->stream); }
 
 int do_q_noadmix(vps_chain* ch)
 {
     int rc = launch_lik_pass(ch->ds->d, ch->c, 3, ch->stream);
-    if (!rc) rc = launch_q_noadmix_draw(ch->ds->d, ch->c, ch->m, ch->stream);
-    ch->binaryq = 1;
+    if (!rc) rc = launch_q_noadmix_draw(ch->ds->d, ch->c, ch->stream);
     return rc;
 }
 
-int do_q_admix(vps_chain* ch)
-{
-    ch->binaryq = 0;
-    return launch_update_q_admix(ch->ds->d, ch->c, ch->m, ch->stream);
-}
+int do_q_admix(vps_chain* ch) { return launch_update_q_admix(ch->ds->d, ch->c, ch->stream); }
 
 int do_q_metro(vps_chain* ch)
 {
-    ch->binaryq = 0;
-    int rc = launch_q_metro_propose(ch->ds->d, ch->c, ch->m, ch->stream);
+    int rc = launch_q_metro_propose(ch->ds->d, ch->c, ch->stream);
     if (!rc) rc = launch_lik_pass(ch->ds->d, ch->c, 2, ch->stream);
-    if (!rc) rc = launch_q_metro_accept(ch->ds->d, ch->c, ch->m, ch->stream);
-    return rc;
-}
-
-// UpdateQ, src/structure.cpp:1140-1166
-int do_update_q(vps_chain* ch)
-{
-    const ChainDev& c = ch->c;
-    int rc;
-    if (!c.admix) {
-        rc = ch->m >= ch->nadmburnin ? do_q_noadmix(ch) : do_q_admix(ch);
-        if (!rc && c.locpriori) rc = launch_update_di(ch->ds->d, c, ch->stream);
-    } else {
-        rc = (ch->metrofreq > 0 && ch->m % ch->metrofreq == 0) ? do_q_metro(ch) : do_q_admix(ch);
-    }
+    if (!rc) rc = launch_q_metro_accept(ch->ds->d, ch->c, ch->stream);
     return rc;
 }
 
@@ -331,7 +296,7 @@ int do_z_admix(vps_chain* ch)
         e1 = ch->evs[ch->ev_used + 1];
         ch->ev_used += 2;
     }
-    return launch_z_admix(ch->ds->d, ch->c, ch->m, ch->stream, e0, e1);
+    return launch_z_admix(ch->ds->d, ch->c, ch->stream, e0, e1);
 }
 
 int do_z_noadmix(vps_chain* ch)
@@ -341,37 +306,69 @@ int do_z_noadmix(vps_chain* ch)
     return rc;
 }
 
-int do_update_z(vps_chain* ch) { return ch->binaryq ? do_z_noadmix(ch) : do_z_admix(ch); }
+}  // namespace
 
-// Record, src/structure.cpp:1943-2045
-int do_record(vps_chain* ch)
+namespace vps {
+
+// What sweep m does, decided on the host from m and the run schedule
+// (MCMC loop, src/structure.cpp:2122-2158).
+SweepPlan plan_sweep(const vps_chain* ch)
 {
+    const ChainDev& c = ch->c;
     const int m = ch->m;
-    if (!((ch->writelnl && m < ch->nburnin && m % ch->nthinning == 0) ||
-          (m >= ch->nburnin && (m - ch->nburnin) % ch->nthinning == 0)))
-        return VPS_OK;
-    int rc = launch_lik_pass(ch->ds->d, ch->c, ch->binaryq ? 1 : 0, ch->stream);
-    const int post = m >= ch->nburnin;
-    int slot = -1;
-    if (ch->writelnl && ch->nlnl < ch->lnl_cap) slot = ch->nlnl++;
-    if (!rc) rc = launch_record_accumulate(ch->ds->d, ch->c, post, slot, 1, ch->stream);
-    if (post) ch->nr++;
+    SweepPlan p{};
+    // UpdateQ, src/structure.cpp:1140-1166
+    if (!c.admix) p.q = m >= ch->nadmburnin ? kQNoAdmix : kQAdmix;
+    else p.q = (ch->metrofreq > 0 && m % ch->metrofreq == 0) ? kQMetro : kQAdmix;
+    // Record, src/structure.cpp:1943-1950
+    p.record = (ch->writelnl && m < ch->nburnin && m % ch->nthinning == 0) ||
+               (m >= ch->nburnin && (m - ch->nburnin) % ch->nthinning == 0);
+    p.post = m >= ch->nburnin;
+    p.trace = p.record && ch->writelnl && ch->nlnl < ch->lnl_cap;
+    return p;
+}
+
+// Every launch of sweep m, all on the chain's stream.  Kernels read m from
+// the device, and the last launch advances it.
+int launch_sweep(vps_chain* ch, const SweepPlan& p)
+{
+    const DeviceDataset& d = ch->ds->d;
+    const ChainDev& c = ch->c;
+    int rc = do_update_p(ch);
+    if (!rc) rc = p.q == kQNoAdmix ? do_q_noadmix(ch) : p.q == kQMetro ? do_q_metro(ch) : do_q_admix(ch);
+    if (!rc && !c.admix && c.locpriori) rc = launch_update_di(d, c, ch->stream);
+    if (!rc) rc = launch_update_locpriori(d, c, ch->stream);
+    if (!rc) rc = p.q == kQNoAdmix ? do_z_noadmix(ch) : do_z_admix(ch);
+    if (!rc) rc = launch_update_alpha(d, c, ch->stream);
+    if (!rc) rc = launch_update_lambda(d, c, ch->stream);
+    if (!rc) rc = launch_update_a(d, c, ch->stream);
+    if (!rc) rc = launch_update_f(d, c, ch->stream);
+    if (!rc && p.record) {
+        rc = launch_lik_pass(d, c, p.q == kQNoAdmix ? 1 : 0, ch->stream);
+        if (!rc) rc = launch_record_accumulate(d, c, p.post, p.trace, 1, ch->stream);
+    }
+    if (!rc) rc = launch_advance_m(c, ch->stream);
     return rc;
 }
 
+// host bookkeeping after sweep m was launched
+void commit_sweep(vps_chain* ch, const SweepPlan& p)
+{
+    ch->binaryq = p.q == kQNoAdmix;
+    if (p.trace) ch->nlnl++;
+    if (p.record && p.post) ch->nr++;
+    ch->m++;
+}
+
+}  // namespace vps
+
+namespace {
+
 int do_sweep(vps_chain* ch)
 {
-    const DeviceDataset& d = ch->ds->d;
-    int rc = do_update_p(ch);
-    if (!rc) rc = do_update_q(ch);
-    if (!rc) rc = launch_update_locpriori(d, ch->c, ch->m, ch->stream);
-    if (!rc) rc = do_update_z(ch);
-    if (!rc) rc = launch_update_alpha(d, ch->c, ch->m, ch->stream);
-    if (!rc) rc = launch_update_lambda(d, ch->c, ch->m, ch->stream);
-    if (!rc) rc = launch_update_a(d, ch->c, ch->m, ch->stream);
-    if (!rc) rc = launch_update_f(d, ch->c, ch->m, ch->stream);
-    if (!rc) rc = do_record(ch);
-    if (!rc) ch->m++;
+    const SweepPlan p = plan_sweep(ch);
+    int rc = launch_sweep(ch, p);
+    if (!rc) commit_sweep(ch, p);
     return rc;
 }
 
@@ -441,6 +438,7 @@ extern "C" int vps_pass_update_q_noadmix(vps_chain* ch)
 {
     VPS_PASS_PROLOGUE(ch);
     int rc = do_q_noadmix(ch);
+    ch->binaryq = 1;
     if (!rc) VPS_HIP(hipStreamSynchronize(ch->stream));
     return rc;
 }
@@ -449,6 +447,7 @@ extern "C" int vps_pass_update_q_metro(vps_chain* ch)
 {
     VPS_PASS_PROLOGUE(ch);
     int rc = do_q_metro(ch);
+    ch->binaryq = 0;
     if (!rc) VPS_HIP(hipStreamSynchronize(ch->stream));
     return rc;
 }
@@ -457,6 +456,7 @@ extern "C" int vps_pass_update_q_admix(vps_chain* ch)
 {
     VPS_PASS_PROLOGUE(ch);
     int rc = do_q_admix(ch);
+    ch->binaryq = 0;
     if (!rc) VPS_HIP(hipStreamSynchronize(ch->stream));
     return rc;
 }
@@ -472,7 +472,7 @@ extern "C" int vps_pass_update_p(vps_chain* ch)
 extern "C" int vps_pass_update_alpha(vps_chain* ch)
 {
     VPS_PASS_PROLOGUE(ch);
-    int rc = launch_update_alpha(ch->ds->d, ch->c, ch->m, ch->stream);
+    int rc = launch_update_alpha(ch->ds->d, ch->c, ch->stream);
     if (!rc) VPS_HIP(hipStreamSynchronize(ch->stream));
     return rc;
 }
@@ -482,7 +482,7 @@ extern "C" int vps_pass_record(vps_chain* ch)
 {
     VPS_PASS_PROLOGUE(ch);
     int rc = launch_lik_pass(ch->ds->d, ch->c, ch->binaryq ? 1 : 0, ch->stream);
-    if (!rc) rc = launch_record_accumulate(ch->ds->d, ch->c, 0, -1, 1, ch->stream);
+    if (!rc) rc = launch_record_accumulate(ch->ds->d, ch->c, 0, 0, 1, ch->stream);
     if (!rc) VPS_HIP(hipStreamSynchronize(ch->stream));
     return rc;
 }
@@ -575,7 +575,10 @@ extern "C" int vps_chain_set_state(vps_chain* ch, int which, const void* host, s
     }
     case VPS_STATE_ALPHA: if ((rc = need(K * 8))) return rc; return copy_dev(ch, c.Alpha, host, bytes, hipMemcpyHostToDevice);
     case VPS_STATE_LAMBDA: if ((rc = need(K * 8))) return rc; return copy_dev(ch, c.Lambda, host, bytes, hipMemcpyHostToDevice);
-    case VPS_STATE_ITER: if ((rc = need(4))) return rc; ch->m = *(const int32_t*)host; return VPS_OK;
+    case VPS_STATE_ITER:
+        if ((rc = need(4))) return rc;
+        ch->m = *(const int32_t*)host;
+        return copy_dev(ch, ch->c.mdev, host, 4, hipMemcpyHostToDevice);
     case VPS_STATE_BUFN1: if ((rc = need(N * 8))) return rc; return copy_dev(ch, c.bufN1, host, bytes, hipMemcpyHostToDevice);
     case VPS_STATE_BUFNK1: if ((rc = need(N * K * 8))) return rc; return copy_dev(ch, c.bufNK1, host, bytes, hipMemcpyHostToDevice);
     case VPS_STATE_MISUM: if ((rc = need(N * K * 8))) return rc; return copy_dev(ch, c.MiSum, host, bytes, hipMemcpyHostToDevice);

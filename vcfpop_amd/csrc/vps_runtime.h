@@ -1,0 +1,46 @@
+// vps_runtime.h — host-side chain handle and the sweep plan shared by the
+// plain, batched and served drivers.
+#pragma once
+#include "vps_chain.h"
+
+namespace vps {
+
+int launch_set_mi_binary(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+int launch_arrange(const DeviceDataset& ds, const ChainDev& c, int nr, hipStream_t st);
+uint32_t hash_ulong_host(uint64_t v);
+
+// UpdateQ variants of one sweep
+constexpr uint8_t kQAdmix = 0;    // Dirichlet Q (admixture, or no-admixture burn-in)
+constexpr uint8_t kQMetro = 1;    // Metropolis Q (every metrofreq sweeps)
+constexpr uint8_t kQNoAdmix = 2;  // PolyLog Z, binary Q
+
+// what sweep m launches; equal plans launch identical kernel sequences
+struct SweepPlan {
+    uint8_t q, record, post, trace;
+    uint32_t code() const { return (uint32_t)q | (uint32_t)record << 2 | (uint32_t)post << 3 | (uint32_t)trace << 4; }
+};
+
+}  // namespace vps
+
+struct vps_chain {
+    vps_dataset* ds = nullptr;
+    vps_params par{};
+    vps::ChainDev c{};
+    int K = 0, id = 0, m = 0, nr = 0, nlnl = 0, binaryq = 0, singlez = 0, finished = 0;
+    int nburnin = 0, nreps = 0, nthinning = 1, metrofreq = 0, writelnl = 0, nadmburnin = 0;
+    int64_t lnl_cap = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    int64_t device_bytes = 0;
+    bool timing = false;
+    std::vector<hipEvent_t> evs;   // pairs bracketing each timed Z pass
+    size_t ev_used = 0;
+};
+
+namespace vps {
+
+SweepPlan plan_sweep(const vps_chain* ch);
+int launch_sweep(vps_chain* ch, const SweepPlan& p);   // every launch of the sweep, on ch->stream
+void commit_sweep(vps_chain* ch, const SweepPlan& p);  // host bookkeeping after the launch
+
+}  // namespace vps

@@ -45,8 +45,9 @@ __device__ __forceinline__ void draw_p_one(const DeviceDataset& ds, const ChainD
     for (int a = 0; a < k2; ++a) p[a] *= inv;
 }
 
-__global__ void k_update_p_par(const DeviceDataset ds, const ChainDev c, int m)
+__global__ void k_update_p_par(const DeviceDataset ds, const ChainDev c)
 {
+    const int m = *c.mdev;
     const int64_t kl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (kl >= (int64_t)c.K * ds.L) return;
     const int64_t k = kl / ds.L, l = kl % ds.L;
@@ -55,8 +56,9 @@ __global__ void k_update_p_par(const DeviceDataset ds, const ChainDev c, int m)
     draw_p_one(ds, c, r, k, l);
 }
 
-__global__ void k_update_p_ref(const DeviceDataset ds, const ChainDev c, int m)
+__global__ void k_update_p_ref(const DeviceDataset ds, const ChainDev c)
 {
+    const int m = *c.mdev;
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= 128) return;
     const int64_t KL = (int64_t)c.K * ds.L;
@@ -66,13 +68,13 @@ __global__ void k_update_p_ref(const DeviceDataset ds, const ChainDev c, int m)
     for (int64_t kl = kl0; kl < kl1; ++kl) draw_p_one(ds, c, r, kl / ds.L, kl % ds.L);
 }
 
-int launch_update_p(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+int launch_update_p(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     if (c.topology == VPS_RNG_REFERENCE) {
-        hipLaunchKernelGGL(k_update_p_ref, dim3(2), dim3(64), 0, st, ds, c, m);
+        hipLaunchKernelGGL(k_update_p_ref, dim3(2), dim3(64), 0, st, ds, c);
     } else {
         const int64_t n = (int64_t)c.K * ds.L;
-        hipLaunchKernelGGL(k_update_p_par, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ds, c, m);
+        hipLaunchKernelGGL(k_update_p_par, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ds, c);
     }
     VPS_HIP(hipGetLastError());
     return VPS_OK;
@@ -86,8 +88,9 @@ __device__ __forceinline__ const double* alpha_row(const DeviceDataset& ds, cons
     return c.locpriori ? c.AlphaLocal + (int64_t)ds.popid[i] * c.K : c.Alpha;
 }
 
-__global__ void k_q_admix(const DeviceDataset ds, const ChainDev c, int m)
+__global__ void k_q_admix(const DeviceDataset ds, const ChainDev c)
 {
+    const int m = *c.mdev;
     if (c.topology == VPS_RNG_REFERENCE) {
         if (blockIdx.x != 0 || threadIdx.x != 0) return;
         Rng r;
@@ -103,10 +106,10 @@ __global__ void k_q_admix(const DeviceDataset ds, const ChainDev c, int m)
     dirichlet_ab(r, c.Q + (int64_t)i * c.K, alpha_row(ds, c, i), c.Mi + (int64_t)i * c.K, c.K);
 }
 
-int launch_update_q_admix(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+int launch_update_q_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     const unsigned nb = c.topology == VPS_RNG_REFERENCE ? 1 : (unsigned)((ds.N + 255) / 256);
-    hipLaunchKernelGGL(k_q_admix, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c, m);
+    hipLaunchKernelGGL(k_q_admix, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
@@ -116,8 +119,9 @@ int launch_update_q_admix(const DeviceDataset& ds, const ChainDev& c, int m, hip
 // the log-ratio pass, then acceptance with conditional uniform draws from
 // the same stream(s).
 // ---------------------------------------------------------------------------
-__global__ void k_q_metro_propose(const DeviceDataset ds, const ChainDev c, int m)
+__global__ void k_q_metro_propose(const DeviceDataset ds, const ChainDev c)
 {
+    const int m = *c.mdev;
     if (c.topology == VPS_RNG_REFERENCE) {
         if (blockIdx.x != 0 || threadIdx.x != 0) return;
         Rng r;
@@ -155,17 +159,16 @@ __global__ void k_q_metro_accept(const DeviceDataset ds, const ChainDev c)
         for (int k = 0; k < K; ++k) c.Q[(int64_t)i * K + k] = c.bufNK1[(int64_t)i * K + k];
 }
 
-int launch_q_metro_propose(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+int launch_q_metro_propose(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     const unsigned nb = c.topology == VPS_RNG_REFERENCE ? 1 : (unsigned)((ds.N + 255) / 256);
-    hipLaunchKernelGGL(k_q_metro_propose, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c, m);
+    hipLaunchKernelGGL(k_q_metro_propose, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
 
-int launch_q_metro_accept(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+int launch_q_metro_accept(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
-    (void)m;
     const unsigned nb = c.topology == VPS_RNG_REFERENCE ? 1 : (unsigned)((ds.N + 255) / 256);
     hipLaunchKernelGGL(k_q_metro_accept, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c);
     VPS_HIP(hipGetLastError());
@@ -177,8 +180,9 @@ int launch_q_metro_accept(const DeviceDataset& ds, const ChainDev& c, int m, hip
 // K log likelihoods picks Z_i and sets Q[i][Z_i] = 1 (src/structureAVX.cpp:
 // 279-287).  bufNK1 rows are overwritten by PolyLog, as in the reference.
 // ---------------------------------------------------------------------------
-__global__ void k_q_noadmix_draw(const DeviceDataset ds, const ChainDev c, int m)
+__global__ void k_q_noadmix_draw(const DeviceDataset ds, const ChainDev c)
 {
+    const int m = *c.mdev;
     const int K = c.K;
     if (c.topology == VPS_RNG_REFERENCE) {
         if (blockIdx.x != 0 || threadIdx.x != 0) return;
@@ -204,10 +208,10 @@ __global__ void k_q_noadmix_draw(const DeviceDataset ds, const ChainDev c, int m
     c.Z[i] = (uint16_t)k2;
 }
 
-int launch_q_noadmix_draw(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+int launch_q_noadmix_draw(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     const unsigned nb = c.topology == VPS_RNG_REFERENCE ? 1 : (unsigned)((ds.N + 255) / 256);
-    hipLaunchKernelGGL(k_q_noadmix_draw, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c, m);
+    hipLaunchKernelGGL(k_q_noadmix_draw, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
@@ -282,8 +286,9 @@ constexpr int kScalarThreads = 256;
 // UpdateAlpha, src/structure.cpp:1501-1594 (one block; the likelihood sums
 // are block reductions, the Metropolis chain runs on thread 0)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kScalarThreads) void k_update_alpha(const DeviceDataset ds, const ChainDev c, int m)
+__global__ __launch_bounds__(kScalarThreads) void k_update_alpha(const DeviceDataset ds, const ChainDev c)
 {
+    const int m = *c.mdev;
     __shared__ int64_t sh_s[kScalarThreads];
     __shared__ double sh_p[kScalarThreads];
     __shared__ double lp[VPS_MAX_K * 64 + 1];
@@ -363,12 +368,12 @@ __global__ __launch_bounds__(kScalarThreads) void k_update_alpha(const DeviceDat
         for (int k = 0; k < K; ++k) c.Alpha[k] = am;
 }
 
-int launch_update_alpha(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+int launch_update_alpha(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     if (c.locpriori && c.admix && c.S > 64) {
         // per-location sums beyond 64 locations run on thread 0 (correct, slower)
     }
-    hipLaunchKernelGGL(k_update_alpha, dim3(1), dim3(kScalarThreads), 0, st, ds, c, m);
+    hipLaunchKernelGGL(k_update_alpha, dim3(1), dim3(kScalarThreads), 0, st, ds, c);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
@@ -376,8 +381,9 @@ int launch_update_alpha(const DeviceDataset& ds, const ChainDev& c, int m, hipSt
 // ---------------------------------------------------------------------------
 // UpdateLambda, src/structure.cpp:1598-1635 (runs only when inferlambda)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kScalarThreads) void k_update_lambda(const DeviceDataset ds, const ChainDev c, int m)
+__global__ __launch_bounds__(kScalarThreads) void k_update_lambda(const DeviceDataset ds, const ChainDev c)
 {
+    const int m = *c.mdev;
     __shared__ int64_t sh_s[kScalarThreads];
     __shared__ double sh_p[kScalarThreads];
     __shared__ double lp[VPS_MAX_K + 1];
@@ -412,10 +418,10 @@ __global__ __launch_bounds__(kScalarThreads) void k_update_lambda(const DeviceDa
     }
 }
 
-int launch_update_lambda(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+int launch_update_lambda(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     if (!c.inferlambda) return VPS_OK;
-    hipLaunchKernelGGL(k_update_lambda, dim3(1), dim3(kScalarThreads), 0, st, ds, c, m);
+    hipLaunchKernelGGL(k_update_lambda, dim3(1), dim3(kScalarThreads), 0, st, ds, c);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
@@ -423,8 +429,9 @@ int launch_update_lambda(const DeviceDataset& ds, const ChainDev& c, int m, hipS
 // ---------------------------------------------------------------------------
 // UpdateLocPriori, src/structure.cpp:1170-1255 (S x K scalars, thread 0)
 // ---------------------------------------------------------------------------
-__global__ void k_update_locpriori(const DeviceDataset ds, const ChainDev c, int m)
+__global__ void k_update_locpriori(const DeviceDataset ds, const ChainDev c)
 {
+    const int m = *c.mdev;
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int K = c.K, S = c.S;
     Rng rng;
@@ -490,10 +497,10 @@ __global__ void k_update_locpriori(const DeviceDataset ds, const ChainDev c, int
     }
 }
 
-int launch_update_locpriori(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+int launch_update_locpriori(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     if (!c.locpriori) return VPS_OK;
-    hipLaunchKernelGGL(k_update_locpriori, dim3(1), dim3(64), 0, st, ds, c, m);
+    hipLaunchKernelGGL(k_update_locpriori, dim3(1), dim3(64), 0, st, ds, c);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
@@ -549,8 +556,11 @@ __device__ void update_a2_locus(const DeviceDataset& ds, const ChainDev& c, Rng&
     if (dlnL >= kNZero || r.uniform() < exp(dlnL)) { pa[m1] = pm1; pa[n1] = pn1; }
 }
 
-__global__ void k_update_a(const DeviceDataset ds, const ChainDev c, int m, int use_a1)
+__global__ void k_update_a(const DeviceDataset ds, const ChainDev c)
 {
+    const int m = *c.mdev;
+    // UpdateA picks A1 or A2 by HashULong(seed + m) & 1, src/structure.cpp:1639-1647
+    const int use_a1 = hash_ulong(c.seed + (uint64_t)m) & 1;
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c.topology == VPS_RNG_REFERENCE) {
         if (x >= 32) return;
@@ -566,21 +576,18 @@ __global__ void k_update_a(const DeviceDataset ds, const ChainDev c, int m, int 
     use_a1 ? update_a1_locus(ds, c, r, x) : update_a2_locus(ds, c, r, x);
 }
 
-// HashULong (CRC32C, init 0xB385FDA9) on the host decides A1 vs A2
-uint32_t hash_ulong_host(uint64_t v);
-
-int launch_update_a(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+int launch_update_a(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     if (!c.fmodel) return VPS_OK;
-    const int use_a1 = hash_ulong_host(c.seed + (uint64_t)m) & 1;
     const unsigned nb = c.topology == VPS_RNG_REFERENCE ? 1 : (unsigned)((ds.L + 255) / 256);
-    hipLaunchKernelGGL(k_update_a, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c, m, use_a1);
+    hipLaunchKernelGGL(k_update_a, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
 
-__global__ __launch_bounds__(kScalarThreads) void k_update_f(const DeviceDataset ds, const ChainDev c, int m)
+__global__ __launch_bounds__(kScalarThreads) void k_update_f(const DeviceDataset ds, const ChainDev c)
 {
+    const int m = *c.mdev;
     __shared__ double sh[kScalarThreads];
     __shared__ double bc[4];
     const int K = c.K;
@@ -634,10 +641,10 @@ __global__ __launch_bounds__(kScalarThreads) void k_update_f(const DeviceDataset
     }
 }
 
-int launch_update_f(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+int launch_update_f(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     if (!c.fmodel) return VPS_OK;
-    hipLaunchKernelGGL(k_update_f, dim3(1), dim3(kScalarThreads), 0, st, ds, c, m);
+    hipLaunchKernelGGL(k_update_f, dim3(1), dim3(kScalarThreads), 0, st, ds, c);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
@@ -645,8 +652,23 @@ int launch_update_f(const DeviceDataset& ds, const ChainDev& c, int m, hipStream
 // ---------------------------------------------------------------------------
 // Record's accumulators, src/structure.cpp:1877-1878 and 2017-2041
 // ---------------------------------------------------------------------------
-__global__ void k_record_acc(const DeviceDataset ds, const ChainDev c, int post, int trace_slot, int acc_misum)
+// writelnl trace slot of the Record at sweep m: Records happen at burn-in
+// sweeps m % nthinning == 0 (writelnl only) and at post-burn-in sweeps
+// (m - nburnin) % nthinning == 0 (src/structure.cpp:1943-1950); the slot is
+// the number of Records before m.
+__device__ int record_trace_slot(const ChainDev& c, int m)
 {
+    if (!c.writelnl) return -1;
+    const int burn = min(m, c.nburnin);
+    const int pre = (burn + c.nthinning - 1) / c.nthinning;
+    const int post = m > c.nburnin ? (m - c.nburnin + c.nthinning - 1) / c.nthinning : 0;
+    const int slot = pre + post;
+    return slot < c.lnl_cap ? slot : -1;
+}
+
+__global__ void k_record_acc(const DeviceDataset ds, const ChainDev c, int post, int trace, int acc_misum)
+{
+    const int trace_slot = trace ? record_trace_slot(c, *c.mdev) : -1;
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int K = c.K;
     const int64_t nmi = (int64_t)ds.N * K, nfr = (int64_t)K * ds.KT;
@@ -672,10 +694,19 @@ __global__ void k_record_acc(const DeviceDataset ds, const ChainDev c, int post,
     for (int j = 0; j < nf; ++j) c.rout[c.rlen - nf + j] += c.Fst[j];
 }
 
-int launch_record_accumulate(const DeviceDataset& ds, const ChainDev& c, int post, int trace_slot, int acc_misum, hipStream_t st)
+__global__ void k_advance_m(int32_t* mdev) { mdev[0] += 1; }
+
+int launch_advance_m(const ChainDev& c, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_advance_m, dim3(1), dim3(1), 0, st, c.mdev);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+int launch_record_accumulate(const DeviceDataset& ds, const ChainDev& c, int post, int trace, int acc_misum, hipStream_t st)
 {
     const int64_t n = std::max<int64_t>((int64_t)ds.N * c.K, (int64_t)c.K * ds.KT);
-    hipLaunchKernelGGL(k_record_acc, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ds, c, post, trace_slot, acc_misum);
+    hipLaunchKernelGGL(k_record_acc, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ds, c, post, trace, acc_misum);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }

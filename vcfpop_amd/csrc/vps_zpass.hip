@@ -84,10 +84,12 @@ __device__ __forceinline__ void flush_mi(int32_t* row, const int (&cum)[K], int 
 // ---------------------------------------------------------------------------
 template <int K, typename Real, int MPT>
 __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, const double* __restrict__ Freq,
-                                                   const double* __restrict__ Q, uint64_t seed, int m,
+                                                   const double* __restrict__ Q, uint64_t seed,
+                                                   const int32_t* __restrict__ mp,
                                                    int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
                                                    int Nc, const uint32_t* __restrict__ jump)
 {
+    const int m = *mp;
     extern __shared__ int32_t lds_i32[];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -217,10 +219,12 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
 // ---------------------------------------------------------------------------
 template <int K, typename Real>
 __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, const double* __restrict__ Freq,
-                                                    const double* __restrict__ Q, uint64_t seed, int m,
+                                                    const double* __restrict__ Q, uint64_t seed,
+                                                    const int32_t* __restrict__ mp,
                                                     int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
                                                     int A, int ni_in_lds, int Nc, const uint32_t* __restrict__ jump)
 {
+    const int m = *mp;
     extern __shared__ int32_t lds[];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -398,7 +402,7 @@ namespace {
 constexpr int kWavesPerBlock = 4;
 
 template <int K, typename Real>
-int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     const int nthr = 64 * kWavesPerBlock;
     // GEN tiles keep their per-lane allele counts in LDS when the K x alleles
@@ -417,17 +421,17 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, int m, hipStrea
         dim3 grid((unsigned)((ds.nbi + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)c.G);
         const size_t p_bytes = (size_t)kWavesPerBlock * K * 2 * kTileLoci * sizeof(Real);
         if (ds.maxploidy == 2)
-            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 2>), grid, dim3(nthr), p_bytes + mi_bytes, st, ds, c.Freq, c.Q, c.seed, m, c.Ni_part, c.Mi, c.Nc, c.jump);
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 2>), grid, dim3(nthr), p_bytes + mi_bytes, st, ds, c.Freq, c.Q, c.seed, c.mdev, c.Ni_part, c.Mi, c.Nc, c.jump);
         else if (ds.maxploidy == 4)
-            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 4>), grid, dim3(nthr), p_bytes + mi_bytes, st, ds, c.Freq, c.Q, c.seed, m, c.Ni_part, c.Mi, c.Nc, c.jump);
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 4>), grid, dim3(nthr), p_bytes + mi_bytes, st, ds, c.Freq, c.Q, c.seed, c.mdev, c.Ni_part, c.Mi, c.Nc, c.jump);
         else
-            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 0>), grid, dim3(nthr), p_bytes + mi_bytes, st, ds, c.Freq, c.Q, c.seed, m, c.Ni_part, c.Mi, c.Nc, c.jump);
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 0>), grid, dim3(nthr), p_bytes + mi_bytes, st, ds, c.Freq, c.Q, c.seed, c.mdev, c.Ni_part, c.Mi, c.Nc, c.jump);
         VPS_HIP(hipGetLastError());
     }
     if (ds.ngen > 0) {
         dim3 grid((unsigned)((ds.ngen + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)c.G);
         hipLaunchKernelGGL((k_z_admix_gen<K, Real>), grid, dim3(nthr), mi_bytes + ni_bytes, st, ds, c.Freq, c.Q, c.seed,
-                           m, c.Ni_part, c.Mi, A, in_lds, c.Nc, c.jump);
+                           c.mdev, c.Ni_part, c.Mi, A, in_lds, c.Nc, c.jump);
         VPS_HIP(hipGetLastError());
     }
     return VPS_OK;
@@ -470,10 +474,10 @@ int z_occupancy_k(const DeviceDataset& ds, int K, int Nc, int* b, int* g)
 }
 
 template <typename Real>
-int dispatch_k(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st)
+int dispatch_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     switch (c.K) {
-#define VPS_CASE(k) case k: return launch_z_admix_k<k, Real>(ds, c, m, st);
+#define VPS_CASE(k) case k: return launch_z_admix_k<k, Real>(ds, c, st);
         VPS_CASE(1) VPS_CASE(2) VPS_CASE(3) VPS_CASE(4) VPS_CASE(5) VPS_CASE(6) VPS_CASE(7) VPS_CASE(8)
         VPS_CASE(9) VPS_CASE(10) VPS_CASE(11) VPS_CASE(12) VPS_CASE(13) VPS_CASE(14) VPS_CASE(15) VPS_CASE(16)
 #undef VPS_CASE
@@ -537,12 +541,12 @@ int launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st)
     return VPS_OK;
 }
 
-int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, int m, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
+int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
 {
     int rc = launch_fill_i32(c.Mi, (int64_t)ds.N * c.K, 0, st);
     if (rc) return rc;
     if (ev0) VPS_HIP(hipEventRecord(ev0, st));
-    rc = c.precision == VPS_PREC_MIXED ? dispatch_k<float>(ds, c, m, st) : dispatch_k<double>(ds, c, m, st);
+    rc = c.precision == VPS_PREC_MIXED ? dispatch_k<float>(ds, c, st) : dispatch_k<double>(ds, c, st);
     if (ev1) VPS_HIP(hipEventRecord(ev1, st));
     if (rc) return rc;
     return launch_reduce_ni(ds, c, st);
