@@ -248,6 +248,67 @@ int vps_batch_destroy(vps_batch* b);
 int vps_batch_stats(const vps_batch* b, int64_t* graph_launches, int64_t* captures, int64_t* eager_sweeps);
 
 /* ------------------------------------------------------------------------- */
+/* Serving: a job server on one device.  The reference queues runs for a free */
+/* device slot and holds the slot for a whole chain (ManageCUDA, BayesianThread,*/
+/* src/structure.cpp:711-756, 2468-2501).  The server admits jobs while their  */
+/* device memory fits an HBM budget and an active-job limit, builds each job's */
+/* dataset and chain on a loader thread, and runs all admitted jobs at once,   */
+/* each on its own stream in slices of sweeps (continuous batching).  A job's  */
+/* results are bit-identical to the same job run alone.                        */
+/* ------------------------------------------------------------------------- */
+typedef struct vps_server vps_server;
+
+typedef struct vps_server_config {
+    int32_t device;
+    int32_t max_active;         /* jobs loaded or running at once (0 = no limit) */
+    int64_t hbm_budget_bytes;   /* device bytes of all admitted jobs              */
+    int32_t slice_sweeps;       /* sweeps issued per job per scheduling step      */
+    int32_t reserved;
+} vps_server_config;
+
+#define VPS_JOB_PENDING  0
+#define VPS_JOB_LOADING  1
+#define VPS_JOB_RUNNING  2
+#define VPS_JOB_DONE     3
+#define VPS_JOB_FAILED   4
+
+typedef struct vps_job_info {
+    int64_t job_id;
+    int32_t state;              /* VPS_JOB_*                                      */
+    int32_t status;             /* VPS_OK, or the failure code                    */
+    int32_t n_ind, K;
+    int64_t n_loc;
+    int64_t sweeps_done, sweeps_total;
+    int64_t device_bytes;       /* measured once loaded                           */
+    int64_t estimated_bytes;    /* reservation held against the budget            */
+    double  t_submit, t_start, t_end;   /* seconds on the server clock; t_start = admission */
+    int64_t rout_len, lnltrace_len;     /* result lengths once done                */
+} vps_job_info;
+
+typedef struct vps_server_stats_t {
+    int64_t pending, loading, running, completed, failed;
+    int64_t reserved_bytes, peak_reserved_bytes;
+    double  uptime_s;
+} vps_server_stats_t;
+
+void vps_server_config_default(vps_server_config* c);
+int  vps_server_create(const vps_server_config* cfg, vps_server** out);
+/* copies the genotype image; VPS_ERR_BUDGET if the job can never fit */
+int  vps_server_submit(vps_server* s, const vps_dataset_desc* desc, const vps_params* par, int K, int run_id,
+                       int64_t n_sweeps, int64_t* job_id);
+/* blocks until the job is done (timeout_s < 0: no limit); VPS_ERR_STATE on timeout */
+int  vps_server_wait(vps_server* s, int64_t job_id, double timeout_s);
+int  vps_server_job_info(vps_server* s, int64_t job_id, vps_job_info* info);
+/* results of a finished job: VPS_STATE_Q, VPS_STATE_MISUM (normalised ancestry),
+   VPS_STATE_ROUT, VPS_STATE_LNLTRACE, VPS_STATE_LNL */
+int  vps_server_job_result(vps_server* s, int64_t job_id, int which, void* host, size_t bytes);
+/* forget a finished job's results */
+int  vps_server_release(vps_server* s, int64_t job_id);
+int  vps_server_stats(vps_server* s, vps_server_stats_t* st);
+/* stops the threads; running jobs are abandoned */
+int  vps_server_destroy(vps_server* s);
+
+/* ------------------------------------------------------------------------- */
 /* Timing hook for measurement: average device time of the last launches of  */
 /* a named kernel family, measured with HIP events on the chain's stream.     */
 /* ------------------------------------------------------------------------- */

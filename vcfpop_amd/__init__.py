@@ -22,6 +22,7 @@ from ._lib import (  # noqa: F401
     Dataset,
     Chain,
     Batch,
+    Server,
     STATE,
     RNG_REFERENCE,
     RNG_PARALLEL,

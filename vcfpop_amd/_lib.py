@@ -64,6 +64,24 @@ class VpsParams(C.Structure):
     ]
 
 
+class VpsServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("max_active", C.c_int32), ("hbm_budget_bytes", C.c_int64),
+                ("slice_sweeps", C.c_int32), ("reserved", C.c_int32)]
+
+
+class VpsJobInfo(C.Structure):
+    _fields_ = [("job_id", C.c_int64), ("state", C.c_int32), ("status", C.c_int32), ("n_ind", C.c_int32),
+                ("K", C.c_int32), ("n_loc", C.c_int64), ("sweeps_done", C.c_int64), ("sweeps_total", C.c_int64),
+                ("device_bytes", C.c_int64), ("estimated_bytes", C.c_int64), ("t_submit", C.c_double),
+                ("t_start", C.c_double), ("t_end", C.c_double), ("rout_len", C.c_int64), ("lnltrace_len", C.c_int64)]
+
+
+class VpsServerStats(C.Structure):
+    _fields_ = [("pending", C.c_int64), ("loading", C.c_int64), ("running", C.c_int64), ("completed", C.c_int64),
+                ("failed", C.c_int64), ("reserved_bytes", C.c_int64), ("peak_reserved_bytes", C.c_int64),
+                ("uptime_s", C.c_double)]
+
+
 class VpsSynthConfig(C.Structure):
     _fields_ = [
         ("n_ind", C.c_int32),
@@ -121,6 +139,16 @@ _SIGS = {
     "vps_pass_update_alpha": ([_P], C.c_int),
     "vps_chain_enable_timing": ([_P, C.c_int], C.c_int),
     "vps_chain_kernel_time": ([_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)], C.c_int),
+    "vps_server_config_default": ([C.POINTER(VpsServerConfig)], None),
+    "vps_server_create": ([C.POINTER(VpsServerConfig), C.POINTER(_P)], C.c_int),
+    "vps_server_submit": ([_P, C.POINTER(VpsDatasetDesc), C.POINTER(VpsParams), C.c_int, C.c_int, C.c_int64,
+                           C.POINTER(C.c_int64)], C.c_int),
+    "vps_server_wait": ([_P, C.c_int64, C.c_double], C.c_int),
+    "vps_server_job_info": ([_P, C.c_int64, C.POINTER(VpsJobInfo)], C.c_int),
+    "vps_server_job_result": ([_P, C.c_int64, C.c_int, _P, C.c_size_t], C.c_int),
+    "vps_server_release": ([_P, C.c_int64], C.c_int),
+    "vps_server_stats": ([_P, C.POINTER(VpsServerStats)], C.c_int),
+    "vps_server_destroy": ([_P], C.c_int),
     "vps_batch_create": ([C.POINTER(_P), C.c_int, C.POINTER(_P)], C.c_int),
     "vps_batch_step": ([_P, C.c_int], C.c_int),
     "vps_batch_sync": ([_P], C.c_int),
@@ -455,6 +483,78 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             load().vps_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+JOB_STATES = {0: "pending", 1: "loading", 2: "running", 3: "done", 4: "failed"}
+
+
+class Server:
+    """Job server on one device (vps_server_*): HBM-budget admission and
+    concurrent slices of every admitted job."""
+
+    def __init__(self, device=0, hbm_budget_bytes=64 << 30, max_active=64, slice_sweeps=16):
+        cfg = VpsServerConfig()
+        load().vps_server_config_default(C.byref(cfg))
+        cfg.device, cfg.hbm_budget_bytes, cfg.max_active, cfg.slice_sweeps = device, hbm_budget_bytes, max_active, slice_sweeps
+        h = _P()
+        _check(load().vps_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._shapes = {}
+
+    def submit(self, source, params, K, run_id=1, sweeps=None):
+        """source: a Synth (or anything with .desc / n_ind); sweeps defaults to
+        the params' burn-in + sampling length"""
+        if sweeps is None:
+            sweeps = params.nburnin + params.nreps + (params.nadmburnin if params.admix_val != 1 and params.locpriori_val != 1 else 0)
+        jid = C.c_int64()
+        _check(load().vps_server_submit(self.h, C.byref(source.desc), C.byref(params), int(K), int(run_id),
+                                        int(sweeps), C.byref(jid)))
+        self._shapes[jid.value] = (source.desc.n_ind, int(K))
+        return jid.value
+
+    def wait(self, job, timeout=-1.0):
+        _check(load().vps_server_wait(self.h, int(job), float(timeout)))
+
+    def info(self, job):
+        inf = VpsJobInfo()
+        _check(load().vps_server_job_info(self.h, int(job), C.byref(inf)))
+        d = {f: getattr(inf, f) for f, _ in VpsJobInfo._fields_}
+        d["state"] = JOB_STATES[d["state"]]
+        return d
+
+    def result(self, job, name):
+        """final Q, MiSum (normalised ancestry), rout, lnltrace or lnL of a done job"""
+        N, K = self._shapes[int(job)]
+        if name == "lnL":
+            a = np.empty(1)
+        elif name in ("Q", "MiSum"):
+            a = np.empty((N, K))
+        elif name in ("rout", "lnltrace"):
+            a = np.empty(self.info(job)[name + "_len" if name == "rout" else "lnltrace_len"])
+        else:
+            raise VpsError(f"the server keeps no {name}")
+        _check(load().vps_server_job_result(self.h, int(job), STATE[name][0], a.ctypes.data, a.nbytes))
+        return a
+
+    def release(self, job):
+        _check(load().vps_server_release(self.h, int(job)))
+        self._shapes.pop(int(job), None)
+
+    def stats(self):
+        st = VpsServerStats()
+        _check(load().vps_server_stats(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in VpsServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().vps_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -116,7 +116,9 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
 
     vps_chain* ch = new vps_chain();
     ch->ds = ds; ch->par = *p; ch->K = K; ch->id = run_id;
-    hipError_t e = hipStreamCreate(&ch->stream);
+    // non-blocking: other threads' null-stream copies (dataset uploads of a
+    // server) do not serialise with this chain's kernels
+    hipError_t e = hipStreamCreateWithFlags(&ch->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete ch; return fail(VPS_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e)); }
 
     // ReadPar, src/structure.cpp:405-464
@@ -184,7 +186,7 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     if (!rc) rc = chain_alloc(ch, &c.lnltrace, ch->lnl_cap);
     if (!rc) rc = chain_alloc(ch, &c.rngbuf, N);
     if (!rc) rc = chain_alloc(ch, &c.mdev, 1);
-    if (!rc && hipMemset(c.mdev, 0, sizeof(int32_t)) != hipSuccess) rc = fail(VPS_ERR_DEVICE, "clear sweep index");
+    if (!rc && hipMemsetAsync(c.mdev, 0, sizeof(int32_t), ch->stream) != hipSuccess) rc = fail(VPS_ERR_DEVICE, "clear sweep index");
     if (rc) { chain_free(ch); return rc; }
 
     // InitFix (src/structure.cpp:468-553): P = 0.5 (1/k2 off biallelic), Q = 1/K,
@@ -211,8 +213,9 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     if (!rc) rc = fill_f64(ch, c.scal, 1, 1.0);   // r = 1 (InitFix)
     if (!rc) rc = fill_f64(ch, c.FreqA, 3 * KT, 0.0);
     if (rc) { chain_free(ch); return rc; }
-    if (hipMemset(c.MiSum, 0, N * K * 8) != hipSuccess || hipMemset(c.Z, 0, (N + 32) * 2) != hipSuccess ||
-        hipMemset(c.Ni, 0, K * KT * 4) != hipSuccess) {
+    if (hipMemsetAsync(c.MiSum, 0, N * K * 8, ch->stream) != hipSuccess ||
+        hipMemsetAsync(c.Z, 0, (N + 32) * 2, ch->stream) != hipSuccess ||
+        hipMemsetAsync(c.Ni, 0, K * KT * 4, ch->stream) != hipSuccess) {
         chain_free(ch);
         return fail(VPS_ERR_DEVICE, "clear chain state");
     }
