@@ -1,16 +1,27 @@
 #!/usr/bin/env python3
-"""MCMC sweeps/s of vcfpop's Bayesian clustering on one MI355X.
+"""MCMC sweeps/s of vcfpop's Bayesian clustering on one MI355X, plus the
+batched, served and mixed-precision forms.
 
-Workload (default): plain drop-in at N=2,000 individuals, L=100,000 biallelic
-diploid SNPs packed at 2 bits, K=5, admixture model with alpha Metropolis,
-FP64 (synthetic data: Fst 0.05 drift, 2% missing).  A step is one full
-sweep (UpdateP, UpdateQ [Metropolis every 10th], UpdateZ, UpdateAlpha,
+Headline (`value`): the plain drop-in at N=2,000 individuals, L=100,000
+biallelic diploid SNPs packed at 2 bits, K=5, admixture model with alpha
+Metropolis, FP64 (synthetic data: Fst 0.05 drift, 2% missing).  A step is one
+full sweep (UpdateP, UpdateQ [Metropolis every 10th], UpdateZ, UpdateAlpha,
 Record every 10th) with inputs resident in HBM.
 
-Prints one JSON line.  `roofline` is for the dominant kernel (the Z pass),
-timed with HIP events on the chain's stream; `cpu_baseline` times the CPU
-oracle (C, OpenMP, this build's restatement of the reference path) on a
-bounded number of sweeps of the same workload on the host cores.
+Extra objects on the same JSON line:
+  roofline      the Z pass, timed with HIP events on the chain's stream
+                (every 10th pass of the timed region)
+  cpu_baseline  the CPU oracle (C, OpenMP) on a bounded number of sweeps of
+                the same workload on the host cores
+  batched       64 chains (K=2..9 x 8 seeds) on N=1,000 x L=50,000: chains/hr
+                one chain at a time vs one graph launch per sweep of all 64
+  served        a Poisson job stream (N ~ U[100,5000], L ~ logU[1k,200k],
+                ploidy 2 or 4, K ~ U[1,10], admix / no-admix / F model) through
+                the job server vs the same jobs one after another; p50/p99
+                latency and chains/hr
+  precision     N=3,000, L=200,000 mixed ploidy 2/4/6, 20% multiallelic, K=6,
+                locprior: sweeps/s FP64 vs FP32 weights, and the FP32 run's
+                deviation from the FP64 run
 """
 import argparse
 import json
@@ -34,6 +45,10 @@ def zpass_algorithmic(N, L, K, bits_per_geno, maxploidy, KT):
     return bytes_, fp64
 
 
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -44,6 +59,12 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
+    ap.add_argument("--batch-sweeps", type=int, default=40)
+    ap.add_argument("--serve-jobs", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-sweep-scale", type=int, default=20,
+                    help="job sweep counts U[2000,10000] are divided by this")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB megabytes (scales traffic counters) and exit")
     args = ap.parse_args()
@@ -62,7 +83,7 @@ def main():
     import vcfpop_amd as V
     spec = V.WORKLOADS[args.workload]
     K = spec["K"] if isinstance(spec["K"], int) else spec["K"][0]
-    print(f"[bench] generating {args.workload}", file=sys.stderr, flush=True)
+    log("generating", args.workload)
     s = V.make_synth(args.workload)
     ds = V.Dataset(s, 0)
     topo = V.RNG_PARALLEL if args.topology == "parallel" else V.RNG_REFERENCE
@@ -71,10 +92,10 @@ def main():
                          nburnin=spec["nburnin"], nreps=spec["nreps"], g_seed=1,
                          rng_topology=topo, precision=prec)
     ch = V.Chain(ds, p, K, 1)
-    print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
+    log("warmup", args.warmup)
     ch.step(args.warmup)
     ch.sync()
-    ch.enable_timing(True)
+    ch.enable_timing(True, every=10)
     t0 = time.perf_counter()
     ch.step(args.steps)
     ch.sync()
@@ -82,8 +103,7 @@ def main():
     z_ms, z_n = ch.kernel_time(0)
     sweeps_s = args.steps / dt
 
-    bits = 2 if args.workload == "admix_n2000_l100k_k5" else 2
-    b_alg, f_alg = zpass_algorithmic(ds.N, ds.L, K, bits, ds.maxploidy, ds.KT)
+    b_alg, f_alg = zpass_algorithmic(ds.N, ds.L, K, 2, ds.maxploidy, ds.KT)
     z_avg_s = (z_ms / 1e3) / max(z_n, 1)
     gbs = b_alg / z_avg_s / 1e9
     tops = f_alg / z_avg_s / 1e12
@@ -91,9 +111,11 @@ def main():
     tp = os.path.join(ROOT, "profiles", "zpass_admix_n2000_l100k_k5_traffic.json")
     if os.path.exists(tp) and args.workload == "admix_n2000_l100k_k5":
         traffic = json.load(open(tp)).get("bytes_per_launch")
+    ch.close()
 
     cpu = None
     if not args.no_cpu:
+        log("cpu baseline")
         cpu = cpu_baseline(s, p, K, args.cpu_seconds)
 
     out = {
@@ -115,13 +137,23 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
-            "kernel": "z_admix", "kernel_avg_ms": round(z_avg_s * 1e3, 4), "launches": z_n,
+            "kernel": "z_admix", "kernel_avg_ms": round(z_avg_s * 1e3, 4), "launches_timed": z_n,
             "alg_bytes_per_launch": int(b_alg), "alg_fp64_ops_per_launch": int(f_alg),
             "valu": {"bound": "fp64-valu (binding)", "achieved": round(tops, 3), "peak": FP64_VALU_PEAK_TOPS,
                      "unit": "TFLOP/s", "frac": round(tops / FP64_VALU_PEAK_TOPS, 4)},
         },
         "cpu_baseline": cpu,
     }
+    ds.close()
+    if not args.no_extra:
+        for name, fn in (("batched", lambda: bench_batched(V, args.batch_sweeps)),
+                         ("served", lambda: bench_served(V, args.serve_jobs, args.serve_rate, args.serve_sweep_scale)),
+                         ("precision", lambda: bench_precision(V))):
+            log(name)
+            try:
+                out[name] = fn()
+            except Exception as e:   # an extra leg must not hide the headline line
+                out[name] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(out), flush=True)
 
 
@@ -146,6 +178,169 @@ def cpu_baseline(s, p, K, budget_s):
             break
     return {"value": round(n / el, 4), "unit": "sweeps/s", "cores": cores, "kind": "port",
             "sample": f"{n} sweeps from m=0 of the same workload, same RNG topology"}
+
+
+def bench_batched(V, S):
+    """64 chains (K=2..9 x 8 replicate seeds) over one N=1,000 x L=50,000
+    dataset, 5,000 sweeps each: chains/hr when chains run one at a time
+    (the reference's one-chain-per-device leasing) and as one batch"""
+    s = V.make_synth("batch_n1000_l50k_k2to9x8")
+    ds = V.Dataset(s, 0)
+
+    def chains():
+        return [V.Chain(ds, V.params_default(admix_val=1, nburnin=0, nreps=5000, g_seed=1,
+                                             rng_topology=V.RNG_PARALLEL), K, r)
+                for K in range(2, 10) for r in range(1, 9)]
+
+    cs = chains()
+    for c in cs:
+        c.step(2)
+        c.sync()
+    t0 = time.perf_counter()
+    for c in cs:
+        c.step(S)
+        c.sync()
+    plain = (time.perf_counter() - t0) / S
+    for c in cs:
+        c.close()
+    cs = chains()
+    b = V.Batch(cs)
+    b.step(2)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(S)
+    b.sync()
+    batched = (time.perf_counter() - t0) / S
+    st = b.stats()
+    b.close()
+    for c in cs:
+        c.close()
+    ds.close()
+    per_hr = lambda t: 64 * 3600.0 / (t * 5000)
+    return {"workload": "batch_n1000_l50k_k2to9x8", "chains": 64, "sweeps_per_chain": 5000,
+            "measured_sweeps": S, "plain_chains_per_hr": round(per_hr(plain), 1),
+            "batched_chains_per_hr": round(per_hr(batched), 1), "speedup": round(plain / batched, 3),
+            "ms_per_sweep_of_64": {"plain": round(plain * 1e3, 3), "batched": round(batched * 1e3, 3)},
+            "graphs": st}
+
+
+def make_jobs(V, n, rate, scale, seed=7):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    jobs, t = [], 0.0
+    for j in range(n):
+        t += rng.exponential(1.0 / rate)
+        N = int(rng.integers(100, 5001))
+        L = int(np.exp(rng.uniform(np.log(1000), np.log(200000))))
+        pl = [(2, 1.0)] if rng.random() < 0.5 else [(4, 1.0)]
+        K = int(rng.integers(1, 11))
+        model = ("admix", "noadmix", "fmodel")[int(rng.integers(0, 3))]
+        sweeps = int(rng.integers(2000, 10001)) // scale
+        s = V.Synth(n_ind=N, n_loc=L, k_true=3, alpha_q=0.5, missing=0.01, ploidy=pl, seed=1000 + j)
+        kw = (dict(admix_val=1) if model == "admix" else
+              dict(admix_val=2, nadmburnin=sweeps // 10) if model == "noadmix" else dict(admix_val=1, f_val=1))
+        burn = sweeps // 4
+        p = V.params_default(nburnin=burn, nreps=sweeps - burn - kw.get("nadmburnin", 0),
+                             rng_topology=V.RNG_PARALLEL, g_seed=j, **kw)
+        jobs.append(dict(arrival=t, synth=s, params=p, K=K, N=N, L=L, sweeps=sweeps, model=model))
+    return jobs
+
+
+def bench_served(V, n, rate, scale):
+    """The job stream through the server against the same jobs run one after
+    another on plain chains.
+
+    open loop: jobs submitted at their Poisson arrival times; latency =
+      completion - arrival (queueing included).  The plain latencies replay
+      the measured per-job durations through a one-job-at-a-time queue.
+    saturated: every job submitted at once; chains/hr = jobs / makespan."""
+    import numpy as np
+    log(f"served: generating {n} jobs")
+    jobs = make_jobs(V, n, rate, scale)
+    total = lambda j: j["params"].nburnin + j["params"].nreps + (j["params"].nadmburnin if j["params"].admix_val != 1 else 0)
+    # plain, back to back: per-job durations and the saturated makespan
+    dur = []
+    t0 = time.perf_counter()
+    for j in jobs:
+        a = time.perf_counter()
+        d = V.Dataset(j["synth"], 0)
+        c = V.Chain(d, j["params"], j["K"], 1)
+        c.step(total(j))
+        c.finish()
+        c.close()
+        d.close()
+        dur.append(time.perf_counter() - a)
+    plain_sat = time.perf_counter() - t0
+    end, lat_plain = 0.0, []
+    for j, d_ in zip(jobs, dur):
+        end = max(end, j["arrival"]) + d_
+        lat_plain.append(end - j["arrival"])
+
+    def serve(at_arrival):
+        srv = V.Server(hbm_budget_bytes=128 << 30, max_active=32, slice_sweeps=16, loaders=2)
+        ids = []
+        t0 = time.perf_counter()
+        for j in jobs:
+            now = time.perf_counter() - t0
+            if at_arrival and now < j["arrival"]:
+                time.sleep(j["arrival"] - now)
+            ids.append(srv.submit(j["synth"], j["params"], j["K"], 1))
+        for i in ids:
+            srv.wait(i, 900.0)
+        span = time.perf_counter() - t0
+        lat = [x["t_end"] - x["t_submit"] for x in (srv.info(i) for i in ids)]
+        st = srv.stats()
+        srv.close()
+        return span, lat, st
+
+    open_span, lat, st = serve(True)
+    sat_span, _, st_sat = serve(False)
+    pct = lambda a, q: round(float(np.percentile(a, q)), 3)
+    return {"jobs": n, "arrival_rate_per_s": rate,
+            "sweeps_per_job": f"U[2000,10000]/{scale}", "model_mix": "admix/no-admix/F",
+            "open_loop": {"served_p50_s": pct(lat, 50), "served_p99_s": pct(lat, 99),
+                          "plain_p50_s": pct(lat_plain, 50), "plain_p99_s": pct(lat_plain, 99),
+                          "served_makespan_s": round(open_span, 3)},
+            "saturated": {"served_makespan_s": round(sat_span, 3), "plain_makespan_s": round(plain_sat, 3),
+                          "served_chains_per_hr": round(n * 3600.0 / sat_span, 1),
+                          "plain_chains_per_hr": round(n * 3600.0 / plain_sat, 1),
+                          "speedup": round(plain_sat / sat_span, 3),
+                          "peak_reserved_GB": round(st_sat["peak_reserved_bytes"] / 1e9, 3)},
+            "policy": "FIFO admission under a 128 GB budget, at most 32 jobs on the device; "
+                      "latency counts queueing; the stream is drained"}
+
+
+def bench_precision(V, S=20):
+    """FP64 vs FP32 weights on the mixed-ploidy locprior workload; the FP32
+    chain's deviation from the FP64 chain (same seed, same streams)"""
+    import numpy as np
+    spec = V.WORKLOADS["locprior_n3000_l200k_k6_mixed"]
+    s = V.make_synth("locprior_n3000_l200k_k6_mixed")
+    ds = V.Dataset(s, 0)
+    res = {"workload": "locprior_n3000_l200k_k6_mixed", "N": ds.N, "L": ds.L, "K": spec["K"],
+           "maxploidy": ds.maxploidy}
+    states = {}
+    for name, prec in (("fp64", V.PREC_FP64), ("mixed", V.PREC_MIXED)):
+        p = V.params_default(admix_val=1, locpriori_val=1, nburnin=0, nreps=10000, g_seed=1,
+                             rng_topology=V.RNG_PARALLEL, precision=prec)
+        c = V.Chain(ds, p, spec["K"], 1)
+        c.step(2)
+        c.sync()
+        t0 = time.perf_counter()
+        c.step(S)
+        c.sync()
+        res[f"{name}_sweeps_per_s"] = round(S / (time.perf_counter() - t0), 3)
+        c.record()
+        states[name] = {n: c.get(n) for n in ("Q", "Freq", "lnL", "Mi")}
+        c.close()
+    a, b = states["fp64"], states["mixed"]
+    res["after_sweeps"] = S + 2
+    res["max_abs_dQ"] = float(np.abs(a["Q"] - b["Q"]).max())
+    res["max_abs_dP"] = float(np.abs(a["Freq"] - b["Freq"]).max())
+    res["rel_dlnL"] = float(abs(a["lnL"][0] - b["lnL"][0]) / abs(a["lnL"][0]))
+    res["frac_Mi_cells_differing"] = float((a["Mi"] != b["Mi"]).mean())
+    ds.close()
+    return res
 
 
 if __name__ == "__main__":

@@ -121,6 +121,11 @@ typedef struct vps_params {
     int32_t z_chunk;          /* individuals per work chunk of the genotype passes;
                                  0 = chosen from the kernels' occupancy. Results do
                                  not depend on it.                              */
+    int32_t sweep_graphs;     /* 1: vps_chain_step launches each sweep as a cached
+                                 HIP graph (one launch per sweep); 0 (default):
+                                 kernel by kernel.  Results do not depend on it.
+                                 Capture needs no other host thread to make HIP
+                                 calls meanwhile; the server always uses 0.      */
 } vps_params;
 
 #define VPS_RNG_REFERENCE 0  /* P: 128 streams RNG(seed+m*128+b); Q: one stream RNG(seed+m) —
@@ -263,7 +268,7 @@ typedef struct vps_server_config {
     int32_t max_active;         /* jobs loaded or running at once (0 = no limit) */
     int64_t hbm_budget_bytes;   /* device bytes of all admitted jobs              */
     int32_t slice_sweeps;       /* sweeps issued per job per scheduling step      */
-    int32_t reserved;
+    int32_t loaders;            /* threads building admitted jobs' datasets and chains */
 } vps_server_config;
 
 #define VPS_JOB_PENDING  0
@@ -312,7 +317,8 @@ int  vps_server_destroy(vps_server* s);
 /* Timing hook for measurement: average device time of the last launches of  */
 /* a named kernel family, measured with HIP events on the chain's stream.     */
 /* ------------------------------------------------------------------------- */
-int vps_chain_enable_timing(vps_chain* ch, int on);
+/* every = n: time every n-th Z pass (1 = all, 0 = off) */
+int vps_chain_enable_timing(vps_chain* ch, int every);
 /* kernel: 0 = Z admix pass; returns total ms and number of timed launches */
 int vps_chain_kernel_time(vps_chain* ch, int kernel, double* total_ms, int64_t* launches);
 

@@ -118,3 +118,18 @@ def test_batch_then_chain_calls(vps, data):
         c.sync()
         assert_same(a, c, "interleaved")
     b.close()
+
+
+@pytest.mark.parametrize("model", list(MODELS))
+def test_sweep_graphs_equal_eager(vps, data, model):
+    """vps_chain_step with cached per-plan sweep graphs gives the same chain
+    as kernel-by-kernel launches"""
+    s, ds = data
+    g = make_chains(vps, ds, [(model, 3, dict(sweep_graphs=1))])[0]
+    e = make_chains(vps, ds, [(model, 3, dict(sweep_graphs=0))])[0]
+    for n in (4, 9, 7):
+        g.step(n)
+        e.step(n)
+        g.sync()
+        e.sync()
+        assert_same(g, e, (model, n))

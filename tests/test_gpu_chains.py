@@ -1,0 +1,52 @@
+"""Whole chains against the CPU oracle, free-running (no state fed between
+sweeps): the plain admixture workload (N=200, L=2,000 biallelic diploid,
+K=3, 1,000 burn-in + 2,000 sampling sweeps, FP64) across 10 seeds.
+
+Bar: the cluster assignments and count tables stay bit-identical through
+every sweep (checked every 250 sweeps), the reported ancestry (normalised
+MiSum) is identical, and Q, mean lnL and the run accumulators agree to
+1e-12 relative."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def config0(vps, orc):
+    orc.set_threads(16)
+    s = vps.make_synth("admix_n200_l2k_k3")
+    return s, vps.Dataset(s, 0), orc.Data(s.desc)
+
+
+@pytest.mark.parametrize("seed,topology", [(s, 0) for s in range(1, 11)] + [(1, 1), (2, 1)])
+def test_full_chain_matches_oracle(vps, orc, config0, seed, topology):
+    s, ds, od = config0
+    say("full chain seed", seed, "topology", topology)
+    p = vps.params_default(admix_val=1, nburnin=1000, nreps=2000, g_seed=seed, rng_topology=topology)
+    g, o = vps.Chain(ds, p, 3, 1), orc.Chain(od, p, 3, 1)
+    for blk in range(12):
+        g.step(250)
+        o.sweep(250)
+        for n in ("Z", "Mi", "Ni"):
+            assert (g.get(n) == o.get(n)).all(), (n, (blk + 1) * 250)
+    g.finish()
+    o.finish()
+    assert (g.get("MiSum") == o.get("MiSum")).all()
+    qg, qo = g.get("Q"), o.get("Q")
+    assert np.abs(qg - qo).max() <= 1e-12
+    rg, ro = g.get("rout"), o.get("rout")
+    # rout = [mean lnL, mean lnL^2, var = mean lnL^2 - mean^2, lnP(D) = mean - var/2, ...]
+    # (Arrange, src/structure.cpp:1862-1869).  The variance cancels two ~2e11
+    # terms, so its error bound is relative to mean lnL^2, not to itself.
+    tol = 1e-12 * np.abs(ro)
+    tol[2] = tol[3] = 1e-12 * abs(ro[1])
+    assert (np.abs(rg - ro) <= tol).all(), (rg, ro)
+    assert (g.get("Alpha") == o.get("Alpha")).all() or np.allclose(g.get("Alpha"), o.get("Alpha"), rtol=1e-12)

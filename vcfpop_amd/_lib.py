@@ -61,12 +61,13 @@ class VpsParams(C.Structure):
         ("rng_topology", C.c_int32),
         ("precision", C.c_int32),
         ("z_chunk", C.c_int32),
+        ("sweep_graphs", C.c_int32),
     ]
 
 
 class VpsServerConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("max_active", C.c_int32), ("hbm_budget_bytes", C.c_int64),
-                ("slice_sweeps", C.c_int32), ("reserved", C.c_int32)]
+                ("slice_sweeps", C.c_int32), ("loaders", C.c_int32)]
 
 
 class VpsJobInfo(C.Structure):
@@ -432,8 +433,9 @@ class Chain:
     def record(self):
         _check(load().vps_pass_record(self.h))
 
-    def enable_timing(self, on=True):
-        _check(load().vps_chain_enable_timing(self.h, 1 if on else 0))
+    def enable_timing(self, on=True, every=1):
+        """time the Z pass with HIP events: every n-th pass (0 / False = off)"""
+        _check(load().vps_chain_enable_timing(self.h, int(every) if on else 0))
 
     def kernel_time(self, kernel=0):
         ms, n = C.c_double(), C.c_int64()
@@ -499,10 +501,11 @@ class Server:
     """Job server on one device (vps_server_*): HBM-budget admission and
     concurrent slices of every admitted job."""
 
-    def __init__(self, device=0, hbm_budget_bytes=64 << 30, max_active=64, slice_sweeps=16):
+    def __init__(self, device=0, hbm_budget_bytes=64 << 30, max_active=64, slice_sweeps=16, loaders=2):
         cfg = VpsServerConfig()
         load().vps_server_config_default(C.byref(cfg))
         cfg.device, cfg.hbm_budget_bytes, cfg.max_active, cfg.slice_sweeps = device, hbm_budget_bytes, max_active, slice_sweeps
+        cfg.loaders = loaders
         h = _P()
         _check(load().vps_server_create(C.byref(cfg), C.byref(h)))
         self.h = h

@@ -51,7 +51,7 @@ void destroy_batch(vps_batch* b)
 // capture one sweep of every chain with the given plans
 int capture(vps_batch* b, const std::vector<SweepPlan>& plans, hipGraphExec_t* out)
 {
-    VPS_HIP(hipStreamBeginCapture(b->root, hipStreamCaptureModeThreadLocal));
+    VPS_HIP(hipStreamBeginCapture(b->root, hipStreamCaptureModeRelaxed));
     int rc = VPS_OK;
     hipError_t e = hipEventRecord(b->fork, b->root);
     if (e != hipSuccess) rc = fail(VPS_ERR_DEVICE, "capture fork: %s", hipGetErrorString(e));

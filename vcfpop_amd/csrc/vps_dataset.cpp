@@ -14,15 +14,37 @@ using namespace vps;
 
 namespace vps {
 
-int dev_alloc(void** p, size_t bytes, std::vector<void*>& list, int64_t* counter)
+// Stream-ordered allocations from the device's default pool, which keeps
+// freed blocks: releasing a finished chain or dataset then neither returns
+// memory to the driver nor synchronises the device, so jobs that still run
+// beside it are not stalled.
+int dev_alloc(void** p, size_t bytes, std::vector<void*>& list, int64_t* counter, hipStream_t st)
 {
     *p = nullptr;
     if (bytes == 0) bytes = 16;
-    hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) return fail(VPS_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    static std::once_flag pool_once[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 64)
+        std::call_once(pool_once[dev], [dev] {
+            hipMemPool_t pool;
+            if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+                uint64_t keep = UINT64_MAX;
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            }
+        });
+    hipError_t e = hipMallocAsync(p, bytes, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(VPS_ERR_NOMEM, "device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
     list.push_back(*p);
     if (counter) *counter += (int64_t)bytes;
     return VPS_OK;
+}
+
+void dev_free_all(std::vector<void*>& list, hipStream_t st)
+{
+    for (void* p : list) (void)hipFreeAsync(p, st);
+    list.clear();
+    (void)hipStreamSynchronize(st);
 }
 
 }  // namespace vps
@@ -32,7 +54,7 @@ namespace {
 template <typename T>
 int upload(T** dst, const std::vector<T>& src, vps_dataset* ds)
 {
-    int rc = dev_alloc((void**)dst, src.size() * sizeof(T), ds->allocations, &ds->device_bytes);
+    int rc = dev_alloc((void**)dst, src.size() * sizeof(T), ds->allocations, &ds->device_bytes, ds->stream);
     if (rc) return rc;
     if (!src.empty()) VPS_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
     return VPS_OK;
@@ -298,6 +320,14 @@ extern "C" int vps_dataset_create(const vps_dataset_desc* in, int device, vps_da
     }
 
     // upload
+    {
+        hipError_t e = hipSetDevice(device);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&ds->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete ds;
+            return fail(VPS_ERR_DEVICE, "dataset stream on device %d: %s", device, hipGetErrorString(e));
+        }
+    }
     std::vector<int32_t> afo32(L), vt32(N);
     for (int64_t l = 0; l < L; ++l) afo32[l] = (int32_t)ds->afo[l];
     for (int i = 0; i < N; ++i) vt32[i] = (int32_t)vt[i];
@@ -326,8 +356,11 @@ extern "C" int vps_dataset_create(const vps_dataset_desc* in, int device, vps_da
 extern "C" int vps_dataset_destroy(vps_dataset* ds)
 {
     if (!ds) return VPS_OK;
-    if (!ds->host_only) (void)hipSetDevice(ds->device);
-    for (void* p : ds->allocations) (void)hipFree(p);
+    if (!ds->host_only) {
+        (void)hipSetDevice(ds->device);
+        dev_free_all(ds->allocations, ds->stream);
+        if (ds->stream) (void)hipStreamDestroy(ds->stream);
+    }
     delete ds;
     return VPS_OK;
 }

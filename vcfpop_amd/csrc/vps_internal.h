@@ -99,6 +99,7 @@ struct vps_dataset {
     std::vector<int64_t> kdis;
     int64_t device_bytes = 0;
     std::vector<void*> allocations;
+    hipStream_t stream = nullptr;      // allocation and upload stream (non-blocking)
     // host-only build (device < 0): the packed layout stays on the host
     bool host_only = false;
     std::vector<uint32_t> h_words;
@@ -135,6 +136,7 @@ int dataset_jump(vps_dataset* ds, int Nc, const uint32_t** out);
 void jump_apply_host(const uint32_t* tab, uint64_t& x, uint64_t& y);
 
 // device allocation helper that records the block for release
-int dev_alloc(void** p, size_t bytes, std::vector<void*>& list, int64_t* counter);
+int dev_alloc(void** p, size_t bytes, std::vector<void*>& list, int64_t* counter, hipStream_t st);
+void dev_free_all(std::vector<void*>& list, hipStream_t st);
 
 }  // namespace vps

@@ -58,6 +58,7 @@ extern "C" void vps_params_default(vps_params* p)
     p->rng_topology = VPS_RNG_REFERENCE;
     p->precision = VPS_PREC_FP64;
     p->z_chunk = 0;
+    p->sweep_graphs = 0;
 }
 
 extern "C" int vps_abi_version(void) { return VPS_ABI_VERSION; }
@@ -77,7 +78,7 @@ namespace {
 template <typename T>
 int chain_alloc(vps_chain* ch, T** p, int64_t count)
 {
-    return dev_alloc((void**)p, (size_t)std::max<int64_t>(count, 1) * sizeof(T), ch->allocs, &ch->device_bytes);
+    return dev_alloc((void**)p, (size_t)std::max<int64_t>(count, 1) * sizeof(T), ch->allocs, &ch->device_bytes, ch->stream);
 }
 
 int fill_f64(vps_chain* ch, double* p, int64_t n, double v)
@@ -92,9 +93,12 @@ void chain_free(vps_chain* ch)
 {
     if (!ch) return;
     if (ch->ds) (void)hipSetDevice(ch->ds->device);
-    if (ch->stream) (void)hipStreamSynchronize(ch->stream);
-    for (void* p : ch->allocs) (void)hipFree(p);
+    if (ch->stream) {
+        (void)hipStreamSynchronize(ch->stream);
+        dev_free_all(ch->allocs, ch->stream);
+    }
     for (hipEvent_t e : ch->evs) (void)hipEventDestroy(e);
+    for (auto& kv : ch->graphs) (void)hipGraphExecDestroy(kv.second);
     if (ch->stream) (void)hipStreamDestroy(ch->stream);
     delete ch;
 }
@@ -287,7 +291,7 @@ int do_q_metro(vps_chain* ch)
 int do_z_admix(vps_chain* ch)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (ch->timing) {
+    if (ch->timing && (ch->timing_every <= 1 || ch->timed_calls++ % ch->timing_every == 0)) {
         if (ch->ev_used + 2 > ch->evs.size()) {
             for (int x = 0; x < 64; ++x) {
                 hipEvent_t e;
@@ -367,10 +371,48 @@ void commit_sweep(vps_chain* ch, const SweepPlan& p)
 
 namespace {
 
+// One sweep as a cached graph of its plan, captured on first use.  Relaxed
+// capture mode: other host threads (a server's loaders) may allocate, copy
+// and synchronise while this thread captures.  If a capture still fails,
+// the chain falls back to kernel-by-kernel launches for good; a failed
+// capture has run nothing, so the sweep is then simply launched directly.
+int graph_sweep(vps_chain* ch, const SweepPlan& p, bool* launched)
+{
+    *launched = false;
+    auto it = ch->graphs.find(p.code());
+    if (it == ch->graphs.end()) {
+        if (hipStreamBeginCapture(ch->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            (void)hipGetLastError();
+            ch->graph_off = true;
+            return VPS_OK;
+        }
+        int rc = launch_sweep(ch, p);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ch->stream, &g);
+        hipGraphExec_t exec = nullptr;
+        if (!rc && e == hipSuccess && g) e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (rc || e != hipSuccess || !exec) {
+            (void)hipGetLastError();
+            if (exec) (void)hipGraphExecDestroy(exec);
+            ch->graph_off = true;
+            return VPS_OK;
+        }
+        it = ch->graphs.emplace(p.code(), exec).first;
+    }
+    VPS_HIP(hipGraphLaunch(it->second, ch->stream));
+    *launched = true;
+    return VPS_OK;
+}
+
 int do_sweep(vps_chain* ch)
 {
     const SweepPlan p = plan_sweep(ch);
-    int rc = launch_sweep(ch, p);
+    // timed sweeps record events around the Z pass, so they launch kernel by kernel
+    bool launched = false;
+    int rc = VPS_OK;
+    if (ch->par.sweep_graphs && !ch->timing && !ch->graph_off) rc = graph_sweep(ch, p, &launched);
+    if (!rc && !launched) rc = launch_sweep(ch, p);
     if (!rc) commit_sweep(ch, p);
     return rc;
 }
@@ -604,14 +646,19 @@ extern "C" int vps_chain_set_state(vps_chain* ch, int which, const void* host, s
     }
 }
 
-// Z-pass timing: every timed pass is bracketed by its own pair of events on
-// the chain's stream, so timing adds no host synchronisation to the sweep.
-extern "C" int vps_chain_enable_timing(vps_chain* ch, int on)
+// Z-pass timing: a timed pass is bracketed by its own pair of events on the
+// chain's stream, so timing adds no host synchronisation to the sweep.
+// `every` = n times every n-th Z pass (1 = all), which keeps the cost of the
+// event records out of most sweeps.
+extern "C" int vps_chain_enable_timing(vps_chain* ch, int every)
 {
     if (!ch) return fail(VPS_ERR_INVALID, "null chain");
+    if (every < 0) return fail(VPS_ERR_INVALID, "timing period %d", every);
     VPS_HIP(hipSetDevice(ch->ds->device));
     VPS_HIP(hipStreamSynchronize(ch->stream));
-    ch->timing = on != 0;
+    ch->timing = every != 0;
+    ch->timing_every = every;
+    ch->timed_calls = 0;
     ch->ev_used = 0;
     return VPS_OK;
 }

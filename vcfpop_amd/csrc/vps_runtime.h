@@ -1,6 +1,7 @@
 // vps_runtime.h — host-side chain handle and the sweep plan shared by the
 // plain, batched and served drivers.
 #pragma once
+#include <map>
 #include "vps_chain.h"
 
 namespace vps {
@@ -33,8 +34,12 @@ struct vps_chain {
     std::vector<void*> allocs;
     int64_t device_bytes = 0;
     bool timing = false;
+    int timing_every = 1;          // time every n-th Z pass
+    int64_t timed_calls = 0;
     std::vector<hipEvent_t> evs;   // pairs bracketing each timed Z pass
     size_t ev_used = 0;
+    std::map<uint32_t, hipGraphExec_t> graphs;   // one captured sweep per plan
+    bool graph_off = false;                       // a capture failed: launch kernel by kernel
 };
 
 namespace vps {

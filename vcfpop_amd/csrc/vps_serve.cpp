@@ -146,7 +146,8 @@ struct vps_server {
     int64_t n_loading = 0;
     int64_t completed = 0, failed = 0;
     bool stop = false;
-    std::thread scheduler, loader;
+    std::thread scheduler;
+    std::vector<std::thread> loaders;
 };
 
 namespace {
@@ -312,13 +313,14 @@ extern "C" void vps_server_config_default(vps_server_config* c)
     c->hbm_budget_bytes = (int64_t)64 << 30;
     c->max_active = 64;
     c->slice_sweeps = 16;
+    c->loaders = 2;
 }
 
 extern "C" int vps_server_create(const vps_server_config* cfg, vps_server** out)
 {
     if (!cfg || !out) return fail(VPS_ERR_INVALID, "null argument");
     *out = nullptr;
-    if (cfg->hbm_budget_bytes <= 0 || cfg->slice_sweeps < 1 || cfg->max_active < 0)
+    if (cfg->hbm_budget_bytes <= 0 || cfg->slice_sweeps < 1 || cfg->max_active < 0 || cfg->loaders < 0 || cfg->loaders > 16)
         return fail(VPS_ERR_INVALID, "server configuration");
     int n = 0;
     int rc = vps_device_count(&n);
@@ -327,7 +329,7 @@ extern "C" int vps_server_create(const vps_server_config* cfg, vps_server** out)
     vps_server* s = new vps_server();
     s->cfg = *cfg;
     s->t0 = Clock::now();
-    s->loader = std::thread(loader_main, s);
+    for (int t = 0; t < std::max(1, cfg->loaders); ++t) s->loaders.emplace_back(loader_main, s);
     s->scheduler = std::thread(scheduler_main, s);
     *out = s;
     return VPS_OK;
@@ -345,6 +347,7 @@ extern "C" int vps_server_submit(vps_server* s, const vps_dataset_desc* desc, co
     j->K = K;
     j->run_id = run_id;
     j->par = *par;
+    j->par.sweep_graphs = 0;   // loader threads make HIP calls while jobs run
     j->sweeps = n_sweeps;
     j->est = estimate_bytes(j->img, K);
     if (j->est > s->cfg.hbm_budget_bytes)
@@ -465,7 +468,8 @@ extern "C" int vps_server_destroy(vps_server* s)
     s->cv_load.notify_all();
     s->cv_done.notify_all();
     if (s->scheduler.joinable()) s->scheduler.join();
-    if (s->loader.joinable()) s->loader.join();
+    for (auto& t : s->loaders)
+        if (t.joinable()) t.join();
     (void)hipSetDevice(s->cfg.device);
     for (auto& jp : s->active) {
         if (jp->ch) (void)hipStreamSynchronize(jp->ch->stream);
