@@ -102,6 +102,7 @@ def main():
     dt = time.perf_counter() - t0
     z_ms, z_n = ch.kernel_time(0)
     sweeps_s = args.steps / dt
+    G, Nc = ch.chunking()
 
     b_alg, f_alg = zpass_algorithmic(ds.N, ds.L, K, 2, ds.maxploidy, ds.KT)
     z_avg_s = (z_ms / 1e3) / max(z_n, 1)
@@ -133,7 +134,8 @@ def main():
         "data": "synthetic (seeded generator: Fst 0.05 drift, 2% missing)",
         "config": {"workload": args.workload, "N": ds.N, "L": ds.L, "K": K, "model": "admixture+alpha Metropolis",
                    "rng_topology": args.topology, "precision": args.precision,
-                   "record_every": 10, "metro_every": 10},
+                   "record_every": 10, "metro_every": 10,
+                   "z_chunks": {"G": G, "individuals_per_chunk": Nc, "chosen_by": "timed candidates"}},
         "roofline": {
             "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": traffic,

@@ -118,8 +118,9 @@ typedef struct vps_params {
     /* engine options (no counterpart in the reference) */
     int32_t rng_topology;     /* VPS_RNG_REFERENCE or VPS_RNG_PARALLEL           */
     int32_t precision;        /* VPS_PREC_FP64 or VPS_PREC_MIXED                 */
-    int32_t z_chunk;          /* individuals per work chunk of the genotype passes;
-                                 0 = chosen from the kernels' occupancy. Results do
+    int32_t z_chunk;          /* individuals per work chunk of the genotype passes:
+                                 > 0 fixed; 0 (default) = timed candidates at chain
+                                 creation; -1 = occupancy model only.  Results do
                                  not depend on it.                              */
     int32_t sweep_graphs;     /* 1: vps_chain_step launches each sweep as a cached
                                  HIP graph (one launch per sweep); 0 (default):

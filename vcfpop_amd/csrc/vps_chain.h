@@ -56,6 +56,7 @@ constexpr int kLikLociPerWave = 1024;
 // launchers (vps_kernels.hip / vps_updates.hip); all asynchronous on `st`
 int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
 int launch_z_noadmix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 // likelihood passes: mode 0 = Record lnL (admix), 1 = Record lnL (no admix),
 // 2 = Metropolis log ratio per individual, 3 = no-admix per-(i,k) log likelihood

@@ -141,6 +141,7 @@ extern "C" int vps_dataset_create(const vps_dataset_desc* in, int device, vps_da
     std::vector<int64_t> vt(N, 0);
     std::vector<int> vmin(N, 100), vmax(N, 0);
     std::vector<double> totfreq(KT, 0.0);
+    std::vector<int32_t> ncopies(KT, 0);   // typed copies of each allele over all individuals
     std::vector<int32_t> nhaplo(L, 0);
     std::vector<int> bad(L, 0);
     {
@@ -178,6 +179,7 @@ extern "C" int vps_dataset_create(const vps_dataset_desc* in, int device, vps_da
                         for (int a = 0; a < v; ++a) p[als[a]] += 1;
                     }
                     nhaplo[l] = nh;
+                    for (int a = 0; a < k2; ++a) ncopies[ds->afo[l] + a] = (int32_t)p[a];
                     unify_row(p, k2);   // CalcFreqGcount's Unify (k2 < 32 path)
                 }
             });
@@ -347,6 +349,7 @@ extern "C" int vps_dataset_create(const vps_dataset_desc* in, int device, vps_da
     if (!rc) rc = upload(&d.pop_nind, ds->pop_nind, ds);
     if (!rc) rc = upload(&d.totfreq, totfreq, ds);
     if (!rc) rc = upload(&d.nhaplo, nhaplo, ds);
+    if (!rc) rc = upload(&d.ncopies, ncopies, ds);
     if (!rc) rc = upload(&d.kdis, ds->kdis, ds);
     if (rc) { vps_dataset_destroy(ds); return rc; }
     *out = ds;

@@ -78,6 +78,7 @@ struct DeviceDataset {
     int32_t*  pop_nind;        // [S]
     double*   totfreq;         // [KT] (F model)
     int32_t*  nhaplo;          // [L]
+    int32_t*  ncopies;         // [KT] typed copies of each allele (sum over clusters of Ni)
     int64_t*  kdis;            // [maxK+1] loci per allele count
 };
 

@@ -103,6 +103,77 @@ void chain_free(vps_chain* ch)
     delete ch;
 }
 
+// Chunk-size autotuner: time the Z pass's count kernels for the model's
+// chunking and a ladder of chunk sizes on the freshly initialised chain, and
+// keep the fastest.  Chunking changes no draw and no count, so the choice
+// only moves time.  The trial passes write a scratch Mi (the Metropolis
+// buffer, unused until the first sweep) and scratch partials, leaving the
+// initial state untouched.  At most kTuneMaxChunks chunks are tried.
+constexpr int kTuneMaxChunks = 32;
+
+int tune_chunks(vps_chain* ch)
+{
+    ChainDev& c = ch->c;
+    const DeviceDataset& d = ch->ds->d;
+    const int N = c.N;
+    std::vector<int> cand;
+    auto add = [&](int nc) {
+        nc = std::max(1, std::min(nc, std::min(N, (int)kMaxChunk)));
+        const int G = (N + nc - 1) / nc;
+        if (G > kTuneMaxChunks && G != c.G) return;
+        if (std::find(cand.begin(), cand.end(), G) == cand.end()) cand.push_back(G);
+    };
+    add(c.Nc);
+    for (int nc : {32, 48, 64, 96, 128, 192, 256, 384, 512}) add(nc);
+    if (cand.size() < 2) return VPS_OK;
+    const int Gmax = *std::max_element(cand.begin(), cand.end());
+    const size_t part_bytes = (size_t)Gmax * c.K * c.KT * sizeof(int32_t);
+    int32_t* tmp = nullptr;
+    VPS_HIP(hipMallocAsync((void**)&tmp, std::max<size_t>(part_bytes, 16), ch->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = VPS_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(VPS_ERR_DEVICE, "tuner events");
+    float best_ms = 0;
+    int bestG = c.G;
+    for (size_t x = 0; x < cand.size() && !rc; ++x) {
+        ChainDev t = c;
+        t.Nc = (N + cand[x] - 1) / cand[x];
+        t.G = (N + t.Nc - 1) / t.Nc;
+        rc = dataset_jump(ch->ds, t.Nc, &t.jump);
+        if (rc) break;
+        t.Ni_part = tmp;
+        t.Mi = reinterpret_cast<int32_t*>(c.bufNK1);   // N*K doubles >= N*K ints
+        rc = launch_z_admix_counts(d, t, ch->stream);  // warm-up
+        hipError_t e = hipEventRecord(e0, ch->stream);
+        for (int r = 0; r < 2 && !rc && e == hipSuccess; ++r) rc = launch_z_admix_counts(d, t, ch->stream);
+        if (!rc && e == hipSuccess) e = hipEventRecord(e1, ch->stream);
+        if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0;
+        if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (!rc && e != hipSuccess) rc = fail(VPS_ERR_DEVICE, "tuner timing: %s", hipGetErrorString(e));
+        if (!rc && (x == 0 || ms < best_ms * 0.99f)) { best_ms = ms; bestG = t.G; }
+    }
+    (void)hipFreeAsync(tmp, ch->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (!rc) rc = fill_f64(ch, c.bufNK1, (int64_t)N * c.K, 0.0);   // the scratch was the Metropolis buffer
+    if (rc) return rc;
+    VPS_HIP(hipStreamSynchronize(ch->stream));
+    if (bestG == c.G) return VPS_OK;
+    // swap the partial buffer for the chosen chunking
+    auto it = std::find(ch->allocs.begin(), ch->allocs.end(), (void*)c.Ni_part);
+    if (it != ch->allocs.end()) {
+        ch->device_bytes -= (int64_t)c.G * c.K * c.KT * sizeof(int32_t);
+        ch->allocs.erase(it);
+        VPS_HIP(hipFreeAsync(c.Ni_part, ch->stream));
+    }
+    c.Nc = (N + bestG - 1) / bestG;
+    c.G = (N + c.Nc - 1) / c.Nc;
+    rc = dataset_jump(ch->ds, c.Nc, &c.jump);
+    if (!rc) rc = chain_alloc(ch, &c.Ni_part, (int64_t)c.G * c.K * c.KT);
+    return rc;
+}
+
 }  // namespace
 
 extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int run_id, vps_chain** out)
@@ -156,9 +227,10 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     if (!rc) rc = chain_alloc(ch, &c.Q, N * K);
     if (!rc) rc = chain_alloc(ch, &c.Z, N + 32);
     if (!rc) rc = chain_alloc(ch, &c.Ni, K * KT);
-    // individual chunking of the genotype passes
-    if (p->z_chunk < 0 || p->z_chunk > kMaxChunk)
-        rc = fail(VPS_ERR_INVALID, "z_chunk %d outside 0..%d", p->z_chunk, (int)kMaxChunk);
+    // individual chunking of the genotype passes (model choice here; z_chunk
+    // = 0 then measures candidates once the chain is initialised)
+    if (p->z_chunk < -1 || p->z_chunk > kMaxChunk)
+        rc = fail(VPS_ERR_INVALID, "z_chunk %d outside -1..%d", p->z_chunk, (int)kMaxChunk);
     if (!rc) {
         int G = 0;
         if (p->z_chunk > 0) G = (int)((N + p->z_chunk - 1) / p->z_chunk);
@@ -257,6 +329,10 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     if (!rc) rc = launch_init_admix(d, c, ch->singlez, ch->stream);
     if (!rc) { e = hipStreamSynchronize(ch->stream); if (e != hipSuccess) rc = fail(VPS_ERR_DEVICE, "init: %s", hipGetErrorString(e)); }
     if (rc) { chain_free(ch); return rc; }
+    if (p->z_chunk == 0) {
+        rc = tune_chunks(ch);
+        if (rc) { chain_free(ch); return rc; }
+    }
     *out = ch;
     return VPS_OK;
 }

@@ -348,6 +348,7 @@ extern "C" int vps_server_submit(vps_server* s, const vps_dataset_desc* desc, co
     j->run_id = run_id;
     j->par = *par;
     j->par.sweep_graphs = 0;   // loader threads make HIP calls while jobs run
+    if (j->par.z_chunk == 0) j->par.z_chunk = -1;   // no timing trials beside running jobs
     j->sweeps = n_sweeps;
     j->est = estimate_bytes(j->img, K);
     if (j->est > s->cfg.hbm_budget_bytes)

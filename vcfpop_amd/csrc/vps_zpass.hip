@@ -58,18 +58,33 @@ template <> __device__ __forceinline__ float unit_minus1<float>(uint64_t r)
 __device__ __forceinline__ uint64_t ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
 
 // Flush the K per-individual cluster counts of one wavefront (cumulative
-// form, wave-uniform) into the block's LDS row.
+// form, wave-uniform) into the block's LDS row, two counts per 32-bit word
+// (cluster 2j low half, 2j+1 high half; a block adds at most 4 waves x 64
+// loci x 10 slots < 65536 per count).  `val` keeps its other lanes between
+// calls: only lanes < (K+1)/2 are written and added.
 template <int K>
-__device__ __forceinline__ void flush_mi(int32_t* row, const int (&cum)[K], int lane)
+__device__ __forceinline__ void flush_mi(uint32_t* row, const int (&cum)[K], int lane, uint32_t& val)
 {
-    // lane k receives the (wave-uniform) count of cluster k
-    int val = 0;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int cnt = cum[k] - (k ? cum[k - 1] : 0);
-        asm volatile("v_writelane_b32 %0, %1, %2" : "+v"(val) : "s"(cnt), "i"(k));
+    for (int k = 0; k < K; k += 2) {
+        const uint32_t lo = (uint32_t)(cum[k] - (k ? cum[k - 1] : 0));
+        const uint32_t hi = k + 1 < K ? (uint32_t)(cum[k + 1] - cum[k]) : 0u;
+        const uint32_t pk = lo | (hi << 16);
+        asm volatile("v_writelane_b32 %0, %1, %2" : "+v"(val) : "s"(pk), "i"(k / 2));
     }
-    if (lane < K && val) atomicAdd(row + lane, val);
+    if (lane < (K + 1) / 2) atomicAdd(row + lane, val);
+}
+
+// add a block's packed per-individual rows to Mi
+template <int K>
+__device__ __forceinline__ void flush_mi_rows(const uint32_t* mi_lds, int nci, int32_t* __restrict__ Mi_blk)
+{
+    constexpr int KH = (K + 1) / 2;
+    for (int x = threadIdx.x; x < nci * K; x += blockDim.x) {
+        const int i = x / K, k = x - i * K;
+        const int v = (int)((mi_lds[i * KH + (k >> 1)] >> ((k & 1) * 16)) & 0xFFFFu);
+        if (v) atomicAdd(Mi_blk + x, v);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -92,16 +107,19 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
     const int m = *mp;
     extern __shared__ int32_t lds_i32[];
     const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
+    // the wavefront index is uniform: tile fields and loop counters then
+    // live in scalar registers
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwv = blockDim.x >> 6;
     const int c = blockIdx.y;
     const int i0 = c * Nc;
     const int i1 = min(ds.N, i0 + Nc);
     const int nci = i1 - i0;
-    // LDS: [wave][k][allele][lane] P values, then the block's Mi rows
+    // LDS: [wave][k][allele][lane] P values, then the block's packed Mi rows
+    constexpr int KH = (K + 1) / 2;
     Real* p_lds = reinterpret_cast<Real*>(lds_i32) + (size_t)wv * (K * 2 * kTileLoci);
-    int32_t* mi_lds = lds_i32 + (size_t)nwv * K * 2 * kTileLoci * (sizeof(Real) / 4);
-    for (int x = threadIdx.x; x < nci * K; x += blockDim.x) mi_lds[x] = 0;
+    uint32_t* mi_lds = reinterpret_cast<uint32_t*>(lds_i32) + (size_t)nwv * K * 2 * kTileLoci * (sizeof(Real) / 4);
+    for (int x = threadIdx.x; x < nci * KH; x += blockDim.x) mi_lds[x] = 0;
 
     const int64_t tix = (int64_t)blockIdx.x * nwv + wv;
     const bool has_tile = tix < ds.nbi;
@@ -132,9 +150,12 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
         rng.seed(seed + (uint64_t)m * (uint64_t)ds.L + (uint64_t)l, kSaltUpdateZ);
         if (c > 0) jump_apply(jump + (size_t)c * kJumpWords, rng);
 
-        uint32_t nc[K];
+        // cumulative counts C_k, k < K-1 (C_{K-1} = all typed copies is
+        // fixed by the data: the last cluster's counts come from ncopies)
+        uint32_t nc[K > 1 ? K - 1 : 1];
 #pragma unroll
-        for (int k = 0; k < K; ++k) nc[k] = 0;
+        for (int k = 0; k < K - 1; ++k) nc[k] = 0;
+        uint32_t mval = 0;
         const Real eps = (Real)kMinFreq;
         const int bits = t.bits, ipw = t.ipw;
         const uint32_t mask = (1u << bits) - 1u;
@@ -167,8 +188,11 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
             for (int a = 0; a < MP; ++a) {
                 const uint64_t r = rng.next();
                 const bool typed = a < v;
-                const bool al1 = a >= h;
                 const uint64_t tmask = ballot(typed);
+                // a slot no lane types (an individual below maxploidy) only
+                // consumes its draw; diploid-only tiles keep straight-line code
+                if (MPT != 2 && tmask == 0) continue;
+                const bool al1 = a >= h;
                 const Real* pa = al1 ? pl1 : pl0;
                 Real wk[K];
 #pragma unroll
@@ -186,16 +210,15 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
                     nc[k] += (neg && typed) ? inc : 0u;
                 }
                 cum[K - 1] += __builtin_popcountll(tmask);
-                nc[K - 1] += typed ? inc : 0u;
             }
-            flush_mi<K>(mi_lds + (i - i0) * K, cum, lane);
+            flush_mi<K>(mi_lds + (i - i0) * KH, cum, lane, mval);
         }
 
         if (valid) {
             int32_t* dst = Ni_part + (int64_t)c * K * KT + afo;
             uint32_t prev = 0;
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
+            for (int k = 0; k < K - 1; ++k) {
                 const uint32_t d = nc[k] - prev;   // per-half differences never borrow (cumulative counts)
                 prev = nc[k];
                 const int all = (int)(d & 0xFFFFu), one = (int)(d >> 16);
@@ -205,10 +228,7 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
         }
     }
     __syncthreads();
-    for (int x = threadIdx.x; x < nci * K; x += blockDim.x) {
-        const int v = mi_lds[x];
-        if (v) atomicAdd(Mi + (int64_t)i0 * K + x, v);
-    }
+    flush_mi_rows<K>(mi_lds, nci, Mi + (int64_t)i0 * K);
 }
 
 // ---------------------------------------------------------------------------
@@ -227,16 +247,20 @@ __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, con
     const int m = *mp;
     extern __shared__ int32_t lds[];
     const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
+    // the wavefront index is uniform: tile fields and loop counters then
+    // live in scalar registers
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwv = blockDim.x >> 6;
     const int c = blockIdx.y;
     const int i0 = c * Nc;
     const int i1 = min(ds.N, i0 + Nc);
     const int nci = i1 - i0;
-    int32_t* mi_lds = lds;
-    int32_t* ni_lds = lds + Nc * K + wv * (K * A * kTileLoci);
-    const int ntot = Nc * K + (ni_in_lds ? nwv * K * A * kTileLoci : 0);
+    constexpr int KH = (K + 1) / 2;
+    uint32_t* mi_lds = reinterpret_cast<uint32_t*>(lds);   // packed rows, as in the BI kernel
+    int32_t* ni_lds = lds + Nc * KH + wv * (K * A * kTileLoci);
+    const int ntot = Nc * KH + (ni_in_lds ? nwv * K * A * kTileLoci : 0);
     for (int x = threadIdx.x; x < ntot; x += blockDim.x) lds[x] = 0;
+    uint32_t mval = 0;
     __syncthreads();
 
     const int64_t tix = (int64_t)blockIdx.x * nwv + wv;
@@ -281,9 +305,10 @@ __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, con
             for (int a = 0; a < MP; ++a) {
                 const uint64_t r = rng.next();
                 const bool typed = a < v;
+                const uint64_t tmask = ballot(typed);
+                if (tmask == 0) continue;   // no lane types this slot: the draw only
                 const int al = a < VPS_MAX_PLOIDY ? (int)((desc >> (4 + 6 * a)) & 63u) : 0;
                 const int x = typed ? al : 0;
-                const uint64_t tmask = ballot(typed);
                 Real wk[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) wk[k] = (Real)pl[k * KT + x] * q[k] + eps;
@@ -306,19 +331,16 @@ __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, con
                     else atomicAdd(nig + (int64_t)ks * KT + x, 1);
                 }
             }
-            flush_mi<K>(mi_lds + (i - i0) * K, cum, lane);
+            flush_mi<K>(mi_lds + (i - i0) * KH, cum, lane, mval);
         }
 
         if (valid && ni_in_lds) {
-            for (int k = 0; k < K; ++k)
+            for (int k = 0; k < K - 1; ++k)
                 for (int a = 0; a < k2; ++a) nig[(int64_t)k * KT + a] = ni_lds[(k * A + a) * kTileLoci + lane];
         }
     }
     __syncthreads();
-    for (int x = threadIdx.x; x < nci * K; x += blockDim.x) {
-        const int v = mi_lds[x];
-        if (v) atomicAdd(Mi + (int64_t)i0 * K + x, v);
-    }
+    flush_mi_rows<K>(mi_lds, nci, Mi + (int64_t)i0 * K);
 }
 
 // ---------------------------------------------------------------------------
@@ -379,13 +401,23 @@ __global__ __launch_bounds__(64) void k_z_noadmix(const DeviceDataset ds, const 
 }
 
 // Ni = sum over chunks of Ni_part
-__global__ void k_reduce_ni(const int32_t* __restrict__ Ni_part, int32_t* __restrict__ Ni, int64_t n, int G)
+// Ni[k][a] = sum over chunks of the partial counts for k < K-1; every typed
+// copy lands in exactly one cluster, so Ni[K-1][a] = ncopies[a] - the rest
+// (the Z passes leave the last cluster's partials unwritten).
+__global__ void k_reduce_ni(const int32_t* __restrict__ Ni_part, const int32_t* __restrict__ ncopies,
+                            int32_t* __restrict__ Ni, int64_t KT, int K, int G)
 {
-    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= n) return;
-    int32_t s = 0;
-    for (int c = 0; c < G; ++c) s += Ni_part[(int64_t)c * n + x];
-    Ni[x] = s;
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= KT) return;
+    const int64_t n = (int64_t)K * KT;
+    int32_t tot = 0;
+    for (int k = 0; k < K - 1; ++k) {
+        int32_t s = 0;
+        for (int c = 0; c < G; ++c) s += Ni_part[(int64_t)c * n + k * KT + a];
+        Ni[k * KT + a] = s;
+        tot += s;
+    }
+    Ni[(int64_t)(K - 1) * KT + a] = ncopies[a] - tot;
 }
 
 __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v)
@@ -409,9 +441,9 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
     // table fits; otherwise they add into the chunk partials, which must then
     // be cleared before any tile writes its counts.
     const int A = ds.maxK;
-    const size_t mi_bytes = (size_t)c.Nc * K * sizeof(int32_t);
+    const size_t mi_packed = (size_t)c.Nc * ((K + 1) / 2) * sizeof(uint32_t);
     size_t ni_bytes = (size_t)kWavesPerBlock * K * A * kTileLoci * sizeof(int32_t);
-    const int in_lds = mi_bytes + ni_bytes <= 64 * 1024;
+    const int in_lds = mi_packed + ni_bytes <= 64 * 1024;
     if (ds.ngen > 0 && !in_lds) {
         int rc = launch_fill_i32(c.Ni_part, (int64_t)c.G * K * ds.KT, 0, st);
         if (rc) return rc;
@@ -421,16 +453,16 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
         dim3 grid((unsigned)((ds.nbi + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)c.G);
         const size_t p_bytes = (size_t)kWavesPerBlock * K * 2 * kTileLoci * sizeof(Real);
         if (ds.maxploidy == 2)
-            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 2>), grid, dim3(nthr), p_bytes + mi_bytes, st, ds, c.Freq, c.Q, c.seed, c.mdev, c.Ni_part, c.Mi, c.Nc, c.jump);
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 2>), grid, dim3(nthr), p_bytes + mi_packed, st, ds, c.Freq, c.Q, c.seed, c.mdev, c.Ni_part, c.Mi, c.Nc, c.jump);
         else if (ds.maxploidy == 4)
-            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 4>), grid, dim3(nthr), p_bytes + mi_bytes, st, ds, c.Freq, c.Q, c.seed, c.mdev, c.Ni_part, c.Mi, c.Nc, c.jump);
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 4>), grid, dim3(nthr), p_bytes + mi_packed, st, ds, c.Freq, c.Q, c.seed, c.mdev, c.Ni_part, c.Mi, c.Nc, c.jump);
         else
-            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 0>), grid, dim3(nthr), p_bytes + mi_bytes, st, ds, c.Freq, c.Q, c.seed, c.mdev, c.Ni_part, c.Mi, c.Nc, c.jump);
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 0>), grid, dim3(nthr), p_bytes + mi_packed, st, ds, c.Freq, c.Q, c.seed, c.mdev, c.Ni_part, c.Mi, c.Nc, c.jump);
         VPS_HIP(hipGetLastError());
     }
     if (ds.ngen > 0) {
         dim3 grid((unsigned)((ds.ngen + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)c.G);
-        hipLaunchKernelGGL((k_z_admix_gen<K, Real>), grid, dim3(nthr), mi_bytes + ni_bytes, st, ds, c.Freq, c.Q, c.seed,
+        hipLaunchKernelGGL((k_z_admix_gen<K, Real>), grid, dim3(nthr), mi_packed + ni_bytes, st, ds, c.Freq, c.Q, c.seed,
                            c.mdev, c.Ni_part, c.Mi, A, in_lds, c.Nc, c.jump);
         VPS_HIP(hipGetLastError());
     }
@@ -441,19 +473,19 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 template <int K, typename Real>
 int z_occupancy(const DeviceDataset& ds, int Nc, int* bi_blocks, int* gen_blocks)
 {
-    const size_t mi_bytes = (size_t)Nc * K * sizeof(int32_t);
+    const size_t mi_packed = (size_t)Nc * ((K + 1) / 2) * sizeof(uint32_t);
     const size_t p_bytes = (size_t)kWavesPerBlock * K * 2 * kTileLoci * sizeof(Real);
     *bi_blocks = *gen_blocks = 1;
     hipError_t e = hipSuccess;
     if (ds.maxploidy == 2)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 2>, 256, p_bytes + mi_bytes);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 2>, 256, p_bytes + mi_packed);
     else if (ds.maxploidy == 4)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 4>, 256, p_bytes + mi_bytes);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 4>, 256, p_bytes + mi_packed);
     else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 0>, 256, p_bytes + mi_bytes);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 0>, 256, p_bytes + mi_packed);
     if (e != hipSuccess) return fail(VPS_ERR_DEVICE, "occupancy query: %s", hipGetErrorString(e));
     const size_t ni_bytes = (size_t)kWavesPerBlock * K * ds.maxK * kTileLoci * sizeof(int32_t);
-    const size_t gshm = mi_bytes + (mi_bytes + ni_bytes <= 64 * 1024 ? ni_bytes : 0);
+    const size_t gshm = mi_packed + (mi_packed + ni_bytes <= 64 * 1024 ? ni_bytes : 0);
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(gen_blocks, (const void*)k_z_admix_gen<K, Real>, 256, gshm);
     if (e != hipSuccess) return fail(VPS_ERR_DEVICE, "occupancy query: %s", hipGetErrorString(e));
     *bi_blocks = std::max(*bi_blocks, 1);
@@ -541,6 +573,14 @@ int launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st)
     return VPS_OK;
 }
 
+// the Z pass's count kernels alone: Mi and the chunk partials, no reduction
+int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
+{
+    int rc = launch_fill_i32(c.Mi, (int64_t)ds.N * c.K, 0, st);
+    if (rc) return rc;
+    return c.precision == VPS_PREC_MIXED ? dispatch_k<float>(ds, c, st) : dispatch_k<double>(ds, c, st);
+}
+
 int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
 {
     int rc = launch_fill_i32(c.Mi, (int64_t)ds.N * c.K, 0, st);
@@ -554,8 +594,8 @@ int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, h
 
 int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
-    const int64_t n = (int64_t)c.K * ds.KT;
-    hipLaunchKernelGGL(k_reduce_ni, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c.Ni_part, c.Ni, n, c.G);
+    hipLaunchKernelGGL(k_reduce_ni, dim3((unsigned)((ds.KT + 255) / 256)), dim3(256), 0, st, c.Ni_part, ds.ncopies, c.Ni,
+                       ds.KT, c.K, c.G);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
