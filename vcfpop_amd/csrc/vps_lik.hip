@@ -116,6 +116,7 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
         const int sh = (ii - w * td.ipw) * td.bits;
         const uint32_t mask = (1u << td.bits) - 1u;
         const uint32_t* wp = ds.words + td.wbase + (int64_t)w * kTileLoci;
+#pragma unroll 2
         for (int j = 0; j < td.nloc; ++j) {
             const int64_t l = td.l0 + j;
             const uint32_t gid = (wp[j] >> sh) & mask;
@@ -123,12 +124,14 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
             const int v = ok ? (int)(desc & 15u) : 0;
             const int h = v - (int)(desc >> 4);    // copies a >= h carry allele 1
             const double* pl = c.Freq + ds.afo[l];
-            const int a1 = ds.k2[l] > 1 ? 1 : 0;   // monomorphic loci have no allele 1
+            const bool mono = ds.k2[l] < 2;        // monomorphic loci have no allele 1
             double p0[K], p1[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                p0[k] = pl[k * KT];
-                p1[k] = pl[k * KT + a1];
+                // both alleles in one 16-byte scalar load (P is padded past KT)
+                const double x0 = pl[k * KT], x1 = pl[k * KT + 1];
+                p0[k] = x0;
+                p1[k] = mono ? x0 : x1;
             }
             if (MODE == 3) {
 #pragma unroll kUnroll

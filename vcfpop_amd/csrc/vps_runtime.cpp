@@ -103,12 +103,13 @@ void chain_free(vps_chain* ch)
     delete ch;
 }
 
-// Chunk-size autotuner: time the Z pass's count kernels for the model's
-// chunking and a ladder of chunk sizes on the freshly initialised chain, and
-// keep the fastest.  Chunking changes no draw and no count, so the choice
-// only moves time.  The trial passes write a scratch Mi (the Metropolis
-// buffer, unused until the first sweep) and scratch partials, leaving the
-// initial state untouched.  At most kTuneMaxChunks chunks are tried.
+// Chunk-size autotuner: time the whole Z pass (count kernels and chunk
+// reduction) for the model's chunking and a ladder of chunk sizes on the
+// freshly initialised chain, and keep the fastest.  Chunking changes no draw
+// and no count, so the choice only moves time.  The trial passes write a
+// scratch Mi (the Metropolis buffer, unused until the first sweep), scratch
+// partials and a scratch Ni, leaving the initial state untouched.  At most
+// kTuneMaxChunks chunks are tried.
 constexpr int kTuneMaxChunks = 32;
 
 int tune_chunks(vps_chain* ch)
@@ -127,7 +128,7 @@ int tune_chunks(vps_chain* ch)
     for (int nc : {32, 48, 64, 96, 128, 192, 256, 384, 512}) add(nc);
     if (cand.size() < 2) return VPS_OK;
     const int Gmax = *std::max_element(cand.begin(), cand.end());
-    const size_t part_bytes = (size_t)Gmax * c.K * c.KT * sizeof(int32_t);
+    const size_t part_bytes = (size_t)(Gmax + 1) * c.K * c.KT * sizeof(int32_t);   // partials + a scratch Ni
     int32_t* tmp = nullptr;
     VPS_HIP(hipMallocAsync((void**)&tmp, std::max<size_t>(part_bytes, 16), ch->stream));
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -142,6 +143,7 @@ int tune_chunks(vps_chain* ch)
         rc = dataset_jump(ch->ds, t.Nc, &t.jump);
         if (rc) break;
         t.Ni_part = tmp;
+        t.Ni = tmp + (size_t)Gmax * c.K * c.KT;
         t.Mi = reinterpret_cast<int32_t*>(c.bufNK1);   // N*K doubles >= N*K ints
         rc = launch_z_admix_counts(d, t, ch->stream);  // warm-up
         hipError_t e = hipEventRecord(e0, ch->stream);
@@ -223,7 +225,7 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     const int64_t N = d.N, KT = d.KT, S = d.S;
     int rc = 0;
     const int nparts = std::max({lik_part_count(d, 0, K), lik_part_count(d, 3, K)});
-    if (!rc) rc = chain_alloc(ch, &c.Freq, K * KT);
+    if (!rc) rc = chain_alloc(ch, &c.Freq, K * KT + 2);   // +2: paired allele loads past the last locus
     if (!rc) rc = chain_alloc(ch, &c.Q, N * K);
     if (!rc) rc = chain_alloc(ch, &c.Z, N + 32);
     if (!rc) rc = chain_alloc(ch, &c.Ni, K * KT);

@@ -403,21 +403,42 @@ __global__ __launch_bounds__(64) void k_z_noadmix(const DeviceDataset ds, const 
 // Ni = sum over chunks of Ni_part
 // Ni[k][a] = sum over chunks of the partial counts for k < K-1; every typed
 // copy lands in exactly one cluster, so Ni[K-1][a] = ncopies[a] - the rest
-// (the Z passes leave the last cluster's partials unwritten).
-__global__ void k_reduce_ni(const int32_t* __restrict__ Ni_part, const int32_t* __restrict__ ncopies,
-                            int32_t* __restrict__ Ni, int64_t KT, int K, int G)
+// (the Z passes leave the last cluster's partials unwritten).  A block covers
+// 64 alleles x 4 cluster rows; the row totals meet in LDS in a fixed order.
+constexpr int kReduceRows = 4;
+__global__ __launch_bounds__(64 * kReduceRows) void k_reduce_ni(const int32_t* __restrict__ Ni_part,
+                                                               const int32_t* __restrict__ ncopies,
+                                                               int32_t* __restrict__ Ni, int64_t KT, int K, int G)
 {
-    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= KT) return;
+    __shared__ int32_t tot[kReduceRows][64];
+    const int al = threadIdx.x & 63, kk = threadIdx.x >> 6;
+    const int64_t a = (int64_t)blockIdx.x * 64 + al;
     const int64_t n = (int64_t)K * KT;
-    int32_t tot = 0;
-    for (int k = 0; k < K - 1; ++k) {
-        int32_t s = 0;
-        for (int c = 0; c < G; ++c) s += Ni_part[(int64_t)c * n + k * KT + a];
-        Ni[k * KT + a] = s;
-        tot += s;
+    int32_t mine = 0;
+    if (a < KT) {
+        for (int k = kk; k < K - 1; k += kReduceRows) {
+            const int32_t* p = Ni_part + k * KT + a;
+            int32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+            int c = 0;
+            for (; c + 4 <= G; c += 4) {
+                s0 += p[(int64_t)c * n];
+                s1 += p[(int64_t)(c + 1) * n];
+                s2 += p[(int64_t)(c + 2) * n];
+                s3 += p[(int64_t)(c + 3) * n];
+            }
+            for (; c < G; ++c) s0 += p[(int64_t)c * n];
+            const int32_t s = (s0 + s1) + (s2 + s3);
+            Ni[k * KT + a] = s;
+            mine += s;
+        }
     }
-    Ni[(int64_t)(K - 1) * KT + a] = ncopies[a] - tot;
+    tot[kk][al] = mine;
+    __syncthreads();
+    if (kk == 0 && a < KT) {
+        int32_t t = 0;
+        for (int r = 0; r < kReduceRows; ++r) t += tot[r][al];
+        Ni[(int64_t)(K - 1) * KT + a] = ncopies[a] - t;
+    }
 }
 
 __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v)
@@ -573,12 +594,14 @@ int launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st)
     return VPS_OK;
 }
 
-// the Z pass's count kernels alone: Mi and the chunk partials, no reduction
+// the Z pass into the buffers `c` names (the chunk-size tuner passes
+// scratch Mi, partials and Ni)
 int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     int rc = launch_fill_i32(c.Mi, (int64_t)ds.N * c.K, 0, st);
-    if (rc) return rc;
-    return c.precision == VPS_PREC_MIXED ? dispatch_k<float>(ds, c, st) : dispatch_k<double>(ds, c, st);
+    if (!rc) rc = c.precision == VPS_PREC_MIXED ? dispatch_k<float>(ds, c, st) : dispatch_k<double>(ds, c, st);
+    if (!rc) rc = launch_reduce_ni(ds, c, st);
+    return rc;
 }
 
 int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
@@ -594,8 +617,8 @@ int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, h
 
 int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_reduce_ni, dim3((unsigned)((ds.KT + 255) / 256)), dim3(256), 0, st, c.Ni_part, ds.ncopies, c.Ni,
-                       ds.KT, c.K, c.G);
+    hipLaunchKernelGGL(k_reduce_ni, dim3((unsigned)((ds.KT + 63) / 64)), dim3(64 * kReduceRows), 0, st, c.Ni_part,
+                       ds.ncopies, c.Ni, ds.KT, c.K, c.G);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
