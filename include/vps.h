@@ -235,6 +235,49 @@ int vps_pass_update_q_admix(vps_chain* ch);    /* UpdateQAdmix,       src/struct
 int vps_pass_update_alpha(vps_chain* ch);      /* UpdateAlpha,        src/structure.cpp:1501-1594 */
 
 /* ------------------------------------------------------------------------- */
+/* Result tables (WriteStructure / WriteStructureSummary,                     */
+/* src/structure.cpp:88-401): the run file "<prefix>.structure.k=K_rep=R_id=I */
+/* .txt" (and ".lnl.txt" with writelnl) with the reference's sections and     */
+/* number format.  Regions are not part of the genotype image: the RegL1      */
+/* column reads "Total" (the reference's value for a population without one). */
+/* ------------------------------------------------------------------------- */
+typedef struct vps_output_options {
+    int32_t     decimal;      /* digits after the point (-decimal, default 5)          */
+    int32_t     scientific;   /* 1 = %e (-scientific, default 2)                       */
+    char        delimiter;    /* '\t' (default) or ',' (-delimiter)                    */
+    char        pad[7];
+    const char* linebreak;    /* "\r\n" (default) or "\n" (-linebreak)                 */
+    int32_t     diversity;    /* 1 = per-locus diversity table (-structure_diversity)  */
+    int32_t     rep;          /* replicate number of the run                           */
+} vps_output_options;
+
+typedef struct vps_run_summary {
+    int32_t id, k, rep, pad;
+    double  mean_lnl, var_lnl, lnpd;
+} vps_run_summary;
+
+/* a finished run's host-side state, for writing tables without a device */
+typedef struct vps_run_state {
+    const double* Q;          /* [N*K] final admixture proportions              */
+    const double* ancestry;   /* [N*K] normalised MiSum (after finish)          */
+    const double* FreqM;      /* [K*KT] mean allele frequencies                 */
+    const double* rout;       /* [rout_len] arranged run accumulators           */
+    int64_t       rout_len;
+    const double* lnltrace;   /* [lnltrace_len] or NULL                         */
+    int64_t       lnltrace_len;
+    uint64_t      seed;       /* run seed, HashULong(g_seed + id)               */
+} vps_run_state;
+
+void vps_output_options_default(vps_output_options* o);
+/* after vps_chain_finish; fills *summary (may be NULL) for the summary table */
+int  vps_chain_write_tables(vps_chain* ch, const char* prefix, const vps_output_options* opt, vps_run_summary* summary);
+/* the same tables from host arrays (any dataset, including device = -1) */
+int  vps_write_tables_host(const vps_dataset* ds, const vps_params* par, int K, int run_id, const vps_run_state* st,
+                           const char* prefix, const vps_output_options* opt, vps_run_summary* summary);
+/* WriteStructureSummary: one row per run (id, k, rep, mean lnL, var lnL, lnP(D)) */
+int  vps_write_summary(const char* path, const vps_run_summary* runs, int n, const vps_output_options* opt);
+
+/* ------------------------------------------------------------------------- */
 /* Batched chains.  The reference runs K values x replicates as a task pool of */
 /* host threads, each leasing a device slot for one chain and driving it with */
 /* synchronous per-pass calls (CalcBayesian, BayesianThread, ManageCUDA;      */

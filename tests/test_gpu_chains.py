@@ -50,3 +50,23 @@ def test_full_chain_matches_oracle(vps, orc, config0, seed, topology):
     tol[2] = tol[3] = 1e-12 * abs(ro[1])
     assert (np.abs(rg - ro) <= tol).all(), (rg, ro)
     assert (g.get("Alpha") == o.get("Alpha")).all() or np.allclose(g.get("Alpha"), o.get("Alpha"), rtol=1e-12)
+
+
+def test_tables_match_oracle_run(vps, orc, config0, tmp_path):
+    """the result tables of a device run equal those written from the oracle
+    run's state (same seed, same options)"""
+    s, ds, od = config0
+    p = vps.params_default(admix_val=1, nburnin=200, nreps=400, writelnl_val=1, g_seed=3)
+    g, o = vps.Chain(ds, p, 3, 1), orc.Chain(od, p, 3, 1)
+    g.step(600)
+    o.sweep(600)
+    g.finish()
+    o.finish()
+    g.write_tables(str(tmp_path / "gpu"), diversity=1)
+    st = dict(Q=o.get("Q"), ancestry=o.get("MiSum"), FreqM=o.get("FreqM"), rout=o.get("rout"),
+              lnltrace=o.get("lnltrace"), seed=o.seed())
+    vps.write_tables_host(ds, p, 3, 1, st, str(tmp_path / "cpu"), diversity=1)
+    for suffix in (".structure.k=3_rep=1_id=1.txt", ".structure.k=3_rep=1_id=1.lnl.txt"):
+        a = open(str(tmp_path / "gpu") + suffix, newline="").read()
+        b = open(str(tmp_path / "cpu") + suffix, newline="").read()
+        assert a == b, suffix

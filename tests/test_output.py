@@ -1,0 +1,103 @@
+"""Result tables (WriteStructure, src/structure.cpp:88-377) written from a
+finished run's host state: sections, layout and number format, and that each
+table carries the right quantity (final-Q membership per location, MiSum
+ancestry, net nucleotide distance from FreqM).  Runs on the CPU: the run is
+the oracle's, the genotype image a host-only dataset."""
+import numpy as np
+
+FMT = "%0.5f"
+
+
+def run(vps, orc, tmp_path, **par):
+    s = vps.Synth(n_ind=30, n_loc=60, k_true=3, alpha_q=0.5, missing=0.05, frac_multi=0.3,
+                  ploidy=[(2, 0.6), (4, 0.4)], n_pop=3, seed=12)
+    ds = vps.Dataset(s, -1)
+    kw = dict(admix_val=1, nburnin=4, nreps=8, nthinning=2, writelnl_val=1, g_seed=2)
+    kw.update(par)
+    p = vps.params_default(**kw)
+    o = orc.Chain(orc.Data(s.desc), p, 3, 7)
+    o.sweep(kw["nburnin"] + kw["nreps"])
+    o.finish()
+    st = dict(Q=o.get("Q"), ancestry=o.get("MiSum"), FreqM=o.get("FreqM"), rout=o.get("rout"),
+              lnltrace=o.get("lnltrace"), seed=o.seed())
+    prefix = str(tmp_path / "out")
+    summ = vps.write_tables_host(ds, p, 3, 7, st, prefix, rep=2, diversity=1)
+    text = open(prefix + ".structure.k=3_rep=2_id=7.txt", newline="").read()
+    return s, ds, p, st, text, summ, prefix
+
+
+def section(lines, title):
+    i = lines.index(title)
+    out = []
+    for ln in lines[i + 1:]:
+        if ln == "":
+            break
+        out.append(ln)
+    return out
+
+
+def test_run_table_sections_and_values(vps, orc, tmp_path):
+    s, ds, p, st, text, summ, prefix = run(vps, orc, tmp_path)
+    assert "\r\n" in text and "\n\n" not in text.replace("\r\n", "\x00")
+    lines = text.split("\r\n")
+    assert f"Seed={st['seed']}" in lines
+    assert "Model=ADM,NOLOC,NOF" in lines
+    assert "Number of cluster=3" in lines and "Replicate=2" in lines and "Run id=7" in lines
+    r = st["rout"]
+    assert f"Mean value of ln likelihood={FMT % r[0]}" in lines
+    assert f"Variance of ln likelihood={FMT % r[2]}" in lines
+    assert f"Estimated Ln Prob of Data={FMT % r[3]}" in lines
+    assert summ == {"id": 7, "k": 3, "rep": 2, "mean_lnl": r[0], "var_lnl": r[2], "lnpd": r[3]}
+    # membership of each location from the final Q (sum, then Unify with MIN_FREQ)
+    ind_pop = s.arrays()["ind_pop"]
+    mem = section(lines, "Proportion of membership of each pre-defined population in each of the 3 clusters")
+    assert mem[0] == "\tCluster" and mem[1] == "Pop\t1\t2\t3"
+    for sidx, row in enumerate(mem[2:]):
+        o = st["Q"][ind_pop == sidx].sum(0)
+        o = (o + 1e-10) / (o.sum() + 3e-10)
+        assert row == "\t".join([f"Pop{sidx + 1}"] + [FMT % v for v in o])
+    # ancestry = normalised MiSum
+    anc = section(lines, "Inferred ancestry of individuals")
+    assert anc[0] == "\t\tCluster" and anc[1] == "Ind\tPop\tRegL1\t1\t2\t3"
+    for i, row in enumerate(anc[2:]):
+        assert row == "\t".join([f"Ind{i + 1}", f"Pop{ind_pop[i] + 1}", "Total"] + [FMT % v for v in st["ancestry"][i]])
+    # net nucleotide distance from FreqM
+    orig, afo, vt = ds.order()
+    k2 = np.diff(np.append(afo, ds.KT))
+    F = st["FreqM"]
+    H = np.array([np.mean([1 - (F[k, afo[l]:afo[l] + k2[l]] ** 2).sum() for l in range(ds.L)]) for k in range(3)])
+    D = np.array([[np.mean([1 - (F[i, afo[l]:afo[l] + k2[l]] * F[j, afo[l]:afo[l] + k2[l]]).sum() for l in range(ds.L)])
+                   - (H[i] + H[j]) / 2 for j in range(3)] for i in range(3)])
+    dist = section(lines, "Allele-frequency divergence among pops (Net nucleotide distance)")
+    assert dist[0] == "\tCluster" and dist[1] == "Cluster\t1\t2\t3"
+    for i, row in enumerate(dist[2:]):
+        got = [float(x) for x in row.split("\t")[1:]]
+        assert np.allclose(got, D[i], atol=6e-6)
+    # diversity table: one Average row per cluster, then K rows per locus
+    div = section(lines, "Estimated genetic diversity in each cluster")
+    assert div[0].startswith("Locus\tCluster\tHeterozygosity\tAllele0")
+    assert len(div) == 1 + 3 + 3 * ds.L
+    assert div[1].startswith("Average\t1")
+
+
+def test_lnl_trace_file(vps, orc, tmp_path):
+    s, ds, p, st, text, summ, prefix = run(vps, orc, tmp_path)
+    rows = open(prefix + ".structure.k=3_rep=2_id=7.lnl.txt", newline="").read().split("\r\n")
+    assert rows[0] == "4" and rows[1] == "Iteration\tlnL"
+    body = [r for r in rows[2:] if r]
+    assert [int(r.split("\t")[0]) for r in body] == [0, 2, 4, 6, 8, 10]
+    assert [r.split("\t")[1] for r in body] == [FMT % v for v in st["lnltrace"]]
+
+
+def test_options_and_summary(vps, orc, tmp_path):
+    s, ds, p, st, text, summ, prefix = run(vps, orc, tmp_path, locpriori_val=1)
+    assert "Model=ADM,LOC,NOF" in text and "Mean value of r=" in text
+    out = vps.write_tables_host(ds, p, 3, 7, st, str(tmp_path / "csv"), delimiter=",", linebreak="\n",
+                                decimal=3, rep=1)
+    t2 = open(str(tmp_path / "csv") + ".structure.k=3_rep=1_id=7.txt", newline="").read()
+    assert "\r\n" not in t2 and "Ind,Pop,RegL1,1,2,3" in t2
+    assert f"Mean value of ln likelihood={'%0.3f' % st['rout'][0]}" in t2
+    vps.write_summary(tmp_path / "sum.txt", [summ, out])
+    rows = open(tmp_path / "sum.txt", newline="").read().split("\r\n")
+    assert rows[2] == "ID\tk\trep\tMean lnL\tVar lnL\tlnP(D)"
+    assert rows[3].split("\t")[:3] == ["7", "3", "2"]

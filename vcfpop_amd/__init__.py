@@ -23,6 +23,9 @@ from ._lib import (  # noqa: F401
     Chain,
     Batch,
     Server,
+    output_options,
+    write_tables_host,
+    write_summary,
     STATE,
     RNG_REFERENCE,
     RNG_PARALLEL,

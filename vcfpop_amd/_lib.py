@@ -65,6 +65,21 @@ class VpsParams(C.Structure):
     ]
 
 
+class VpsOutputOptions(C.Structure):
+    _fields_ = [("decimal", C.c_int32), ("scientific", C.c_int32), ("delimiter", C.c_char), ("pad", C.c_char * 7),
+                ("linebreak", C.c_char_p), ("diversity", C.c_int32), ("rep", C.c_int32)]
+
+
+class VpsRunSummary(C.Structure):
+    _fields_ = [("id", C.c_int32), ("k", C.c_int32), ("rep", C.c_int32), ("pad", C.c_int32),
+                ("mean_lnl", C.c_double), ("var_lnl", C.c_double), ("lnpd", C.c_double)]
+
+
+class VpsRunState(C.Structure):
+    _fields_ = [("Q", C.c_void_p), ("ancestry", C.c_void_p), ("FreqM", C.c_void_p), ("rout", C.c_void_p),
+                ("rout_len", C.c_int64), ("lnltrace", C.c_void_p), ("lnltrace_len", C.c_int64), ("seed", C.c_uint64)]
+
+
 class VpsServerConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("max_active", C.c_int32), ("hbm_budget_bytes", C.c_int64),
                 ("slice_sweeps", C.c_int32), ("loaders", C.c_int32)]
@@ -150,6 +165,11 @@ _SIGS = {
     "vps_server_release": ([_P, C.c_int64], C.c_int),
     "vps_server_stats": ([_P, C.POINTER(VpsServerStats)], C.c_int),
     "vps_server_destroy": ([_P], C.c_int),
+    "vps_output_options_default": ([C.POINTER(VpsOutputOptions)], None),
+    "vps_chain_write_tables": ([_P, C.c_char_p, C.POINTER(VpsOutputOptions), C.POINTER(VpsRunSummary)], C.c_int),
+    "vps_write_tables_host": ([_P, C.POINTER(VpsParams), C.c_int, C.c_int, C.POINTER(VpsRunState), C.c_char_p,
+                               C.POINTER(VpsOutputOptions), C.POINTER(VpsRunSummary)], C.c_int),
+    "vps_write_summary": ([C.c_char_p, C.POINTER(VpsRunSummary), C.c_int, C.POINTER(VpsOutputOptions)], C.c_int),
     "vps_batch_create": ([C.POINTER(_P), C.c_int, C.POINTER(_P)], C.c_int),
     "vps_batch_step": ([_P, C.c_int], C.c_int),
     "vps_batch_sync": ([_P], C.c_int),
@@ -442,6 +462,13 @@ class Chain:
         _check(load().vps_chain_kernel_time(self.h, kernel, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def write_tables(self, prefix, **opts):
+        """WriteStructure after finish(); returns the run's summary row"""
+        o = output_options(**opts)
+        sm = VpsRunSummary()
+        _check(load().vps_chain_write_tables(self.h, str(prefix).encode(), C.byref(o), C.byref(sm)))
+        return _summary_dict(sm)
+
     def chunking(self):
         """(G, Nc): individual chunks of the genotype passes"""
         g, nc = C.c_int32(), C.c_int32()
@@ -565,3 +592,50 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def output_options(**kw):
+    """vps_output_options with the reference defaults, fields overridden by kw
+    (linebreak as str, delimiter as a one-character str)"""
+    o = VpsOutputOptions()
+    load().vps_output_options_default(C.byref(o))
+    for k, v in kw.items():
+        if k == "linebreak":
+            o._lb = v.encode()
+            o.linebreak = o._lb
+        elif k == "delimiter":
+            o.delimiter = v.encode()
+        else:
+            setattr(o, k, v)
+    return o
+
+
+def _summary_dict(sm):
+    return {"id": sm.id, "k": sm.k, "rep": sm.rep, "mean_lnl": sm.mean_lnl, "var_lnl": sm.var_lnl, "lnpd": sm.lnpd}
+
+
+def write_tables_host(ds, params, K, run_id, state, prefix, **opts):
+    """WriteStructure from host arrays: state = dict(Q, ancestry, FreqM, rout,
+    lnltrace (optional), seed)"""
+    keep = {n: np.ascontiguousarray(state[n], dtype=np.float64) for n in ("Q", "ancestry", "FreqM", "rout")}
+    tr = state.get("lnltrace")
+    if tr is not None:
+        keep["lnltrace"] = np.ascontiguousarray(tr, dtype=np.float64)
+    st = VpsRunState(keep["Q"].ctypes.data, keep["ancestry"].ctypes.data, keep["FreqM"].ctypes.data,
+                     keep["rout"].ctypes.data, keep["rout"].size,
+                     keep["lnltrace"].ctypes.data if tr is not None else None,
+                     keep["lnltrace"].size if tr is not None else 0, int(state["seed"]))
+    o = output_options(**opts)
+    sm = VpsRunSummary()
+    _check(load().vps_write_tables_host(ds.h, C.byref(params), int(K), int(run_id), C.byref(st),
+                                        str(prefix).encode(), C.byref(o), C.byref(sm)))
+    return _summary_dict(sm)
+
+
+def write_summary(path, summaries, **opts):
+    """WriteStructureSummary over the runs' summary dicts"""
+    arr = (VpsRunSummary * max(1, len(summaries)))()
+    for i, d in enumerate(summaries):
+        arr[i] = VpsRunSummary(d["id"], d["k"], d["rep"], 0, d["mean_lnl"], d["var_lnl"], d["lnpd"])
+    o = output_options(**opts)
+    _check(load().vps_write_summary(str(path).encode(), arr, len(summaries), C.byref(o)))
