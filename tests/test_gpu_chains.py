@@ -66,7 +66,15 @@ def test_tables_match_oracle_run(vps, orc, config0, tmp_path):
     st = dict(Q=o.get("Q"), ancestry=o.get("MiSum"), FreqM=o.get("FreqM"), rout=o.get("rout"),
               lnltrace=o.get("lnltrace"), seed=o.seed())
     vps.write_tables_host(ds, p, 3, 1, st, str(tmp_path / "cpu"), diversity=1)
+    # the variance of lnL (and ln P(D) through it) cancels two ~2e11 terms:
+    # those two lines agree to 1e-12 of mean lnL^2; every other line is identical
+    scale = 1e-12 * abs(st["rout"][1])
     for suffix in (".structure.k=3_rep=1_id=1.txt", ".structure.k=3_rep=1_id=1.lnl.txt"):
-        a = open(str(tmp_path / "gpu") + suffix, newline="").read()
-        b = open(str(tmp_path / "cpu") + suffix, newline="").read()
-        assert a == b, suffix
+        a = open(str(tmp_path / "gpu") + suffix, newline="").read().split("\r\n")
+        b = open(str(tmp_path / "cpu") + suffix, newline="").read().split("\r\n")
+        assert len(a) == len(b), suffix
+        for x, y in zip(a, b):
+            if x.startswith(("Variance of ln likelihood=", "Estimated Ln Prob of Data=")):
+                assert abs(float(x.split("=")[1]) - float(y.split("=")[1])) <= max(scale, 1e-5), (x, y)
+            else:
+                assert x == y, (suffix, x, y)

@@ -61,7 +61,9 @@ int capture(vps_batch* b, const std::vector<SweepPlan>& plans, hipGraphExec_t* o
         if (e != hipSuccess) { rc = fail(VPS_ERR_DEVICE, "capture wait: %s", hipGetErrorString(e)); break; }
         const bool timing = ch->timing;
         ch->timing = false;   // no event records inside a graph
+        ch->capturing = true;
         rc = launch_sweep(ch, plans[i]);
+        ch->capturing = false;
         ch->timing = timing;
         if (rc) break;
         e = hipEventRecord(b->join[i], ch->stream);

@@ -93,12 +93,16 @@ void chain_free(vps_chain* ch)
 {
     if (!ch) return;
     if (ch->ds) (void)hipSetDevice(ch->ds->device);
+    if (ch->aux) (void)hipStreamSynchronize(ch->aux);
     if (ch->stream) {
         (void)hipStreamSynchronize(ch->stream);
         dev_free_all(ch->allocs, ch->stream);
     }
     for (hipEvent_t e : ch->evs) (void)hipEventDestroy(e);
     for (auto& kv : ch->graphs) (void)hipGraphExecDestroy(kv.second);
+    if (ch->aux) (void)hipStreamDestroy(ch->aux);
+    if (ch->ev_fork) (void)hipEventDestroy(ch->ev_fork);
+    if (ch->ev_join) (void)hipEventDestroy(ch->ev_join);
     if (ch->stream) (void)hipStreamDestroy(ch->stream);
     delete ch;
 }
@@ -196,7 +200,10 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     // non-blocking: other threads' null-stream copies (dataset uploads of a
     // server) do not serialise with this chain's kernels
     hipError_t e = hipStreamCreateWithFlags(&ch->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) { delete ch; return fail(VPS_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e)); }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ch->aux, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ch->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ch->ev_join, hipEventDisableTiming);
+    if (e != hipSuccess) { chain_free(ch); return fail(VPS_ERR_DEVICE, "chain streams: %s", hipGetErrorString(e)); }
 
     // ReadPar, src/structure.cpp:405-464
     ChainDev& c = ch->c;
@@ -423,11 +430,24 @@ int launch_sweep(vps_chain* ch, const SweepPlan& p)
     if (!rc) rc = p.q == kQNoAdmix ? do_q_noadmix(ch) : p.q == kQMetro ? do_q_metro(ch) : do_q_admix(ch);
     if (!rc && !c.admix && c.locpriori) rc = launch_update_di(d, c, ch->stream);
     if (!rc) rc = launch_update_locpriori(d, c, ch->stream);
+    // UpdateAlpha, UpdateLambda, UpdateA and UpdateF read Q and P and write
+    // only their own parameters, none of which the Z pass reads or writes:
+    // they run beside it on the chain's second stream and join before Record
+    // (captured sweeps keep one stream: graph branches cost more than they hide)
+    hipStream_t side = ch->capturing ? ch->stream : ch->aux;
+    if (!rc && side != ch->stream) {
+        VPS_HIP(hipEventRecord(ch->ev_fork, ch->stream));
+        VPS_HIP(hipStreamWaitEvent(side, ch->ev_fork, 0));
+    }
+    if (!rc) rc = launch_update_alpha(d, c, side);
+    if (!rc) rc = launch_update_lambda(d, c, side);
+    if (!rc) rc = launch_update_a(d, c, side);
+    if (!rc) rc = launch_update_f(d, c, side);
     if (!rc) rc = p.q == kQNoAdmix ? do_z_noadmix(ch) : do_z_admix(ch);
-    if (!rc) rc = launch_update_alpha(d, c, ch->stream);
-    if (!rc) rc = launch_update_lambda(d, c, ch->stream);
-    if (!rc) rc = launch_update_a(d, c, ch->stream);
-    if (!rc) rc = launch_update_f(d, c, ch->stream);
+    if (!rc && side != ch->stream) {
+        VPS_HIP(hipEventRecord(ch->ev_join, side));
+        VPS_HIP(hipStreamWaitEvent(ch->stream, ch->ev_join, 0));
+    }
     if (!rc && p.record) {
         rc = launch_lik_pass(d, c, p.q == kQNoAdmix ? 1 : 0, ch->stream);
         if (!rc) rc = launch_record_accumulate(d, c, p.post, p.trace, 1, ch->stream);
@@ -464,7 +484,9 @@ int graph_sweep(vps_chain* ch, const SweepPlan& p, bool* launched)
             ch->graph_off = true;
             return VPS_OK;
         }
+        ch->capturing = true;
         int rc = launch_sweep(ch, p);
+        ch->capturing = false;
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(ch->stream, &g);
         hipGraphExec_t exec = nullptr;

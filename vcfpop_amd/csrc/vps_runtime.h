@@ -31,6 +31,9 @@ struct vps_chain {
     int nburnin = 0, nreps = 0, nthinning = 1, metrofreq = 0, writelnl = 0, nadmburnin = 0;
     int64_t lnl_cap = 0;
     hipStream_t stream = nullptr;
+    hipStream_t aux = nullptr;                  // parameter updates beside the Z pass
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool capturing = false;                     // launch_sweep is being captured into a graph
     std::vector<void*> allocs;
     int64_t device_bytes = 0;
     bool timing = false;
