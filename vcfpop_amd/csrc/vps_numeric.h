@@ -85,15 +85,19 @@ struct Xs128 {
         y = 0x054913331F123BB5ull ^ (h << 6);
     }
     // one step with shifts (23, 18, 5); returns new_x + new_y,
-    // src/statistics.cpp:73-86.  On gfx950 the four-way XOR
-    // a1 ^ (a1 >> 18) ^ b ^ (b >> 5) uses the 3-input v_bitop3_b32.
+    // src/statistics.cpp:73-86.  On gfx950 the two right shifts are single
+    // 64-bit v_lshrrev_b64 (the compiler splits them into two 32-bit
+    // funnel shifts each) and the four-way XOR a1 ^ (a1 >> 18) ^ b ^ (b >> 5)
+    // uses the 3-input v_bitop3_b32.
     VPS_HD uint64_t next()
     {
         uint64_t a = x, b = y;
         x = b;
         a ^= a << 23;
 #if defined(__HIP_DEVICE_COMPILE__) && defined(__gfx950__)
-        const uint64_t s = a >> 18, u = b >> 5;
+        uint64_t s, u;
+        asm("v_lshrrev_b64 %0, 18, %1" : "=v"(s) : "v"(a));
+        asm("v_lshrrev_b64 %0, 5, %1" : "=v"(u) : "v"(b));
         const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)s, (uint32_t)b, 0x96) ^ (uint32_t)u;
         const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), (uint32_t)(s >> 32), (uint32_t)(b >> 32), 0x96) ^
                             (uint32_t)(u >> 32);

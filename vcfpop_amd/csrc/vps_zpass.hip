@@ -145,7 +145,22 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
 
     if (has_tile) {
         const int64_t KT = ds.KT;
-        const uint64_t tab = valid ? ds.bitab[l] : 0ull;   // invalid lane: every id is untyped
+        uint64_t tab = valid ? ds.bitab[l] : 0ull;   // invalid lane: every id is untyped
+        if (MPT == 2 || MPT == 4) {
+            // recode each genotype's byte as ploidy | (allele-1 flag of slot a) << (4 + a),
+            // so a slot's allele is one bit extract instead of a compare and select
+            uint64_t st = 0;
+#pragma unroll
+            for (int g = 0; g < 8; ++g) {
+                const uint32_t e = (uint32_t)(tab >> (8 * g)) & 0xFFu;
+                const int v = (int)(e & 15u), h = v - (int)(e >> 4);
+                uint32_t f = 0;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) f |= (a >= h ? 1u : 0u) << a;
+                st |= (uint64_t)((uint32_t)v | (f << 4)) << (8 * g);
+            }
+            tab = st;
+        }
         Xs128 rng;
         rng.seed(seed + (uint64_t)m * (uint64_t)ds.L + (uint64_t)l, kSaltUpdateZ);
         if (c > 0) jump_apply(jump + (size_t)c * kJumpWords, rng);
@@ -164,7 +179,6 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
         uint32_t word = wp[(int64_t)w * kTileLoci] >> (j * bits);
         const int MP = MPT > 0 ? MPT : ds.maxploidy;
         const Real* pl0 = p_lds + lane;
-        const Real* pl1 = pl0 + kTileLoci;
 
         for (int i = i0; i < i1; ++i) {
             const uint32_t gid = word & mask;
@@ -173,9 +187,9 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
                 j = 0; ++w;
                 if (i + 1 < i1) word = wp[(int64_t)w * kTileLoci];
             }
-            const uint32_t desc = (uint32_t)(tab >> (gid * 8)) & 0xFFu;
+            const uint32_t desc = (uint32_t)(tab >> (gid * 8));
             const int v = (int)(desc & 15u);
-            const int h = v - (int)(desc >> 4);
+            const int h = v - (int)((desc >> 4) & 15u);   // the generic path's dosage form
             Real q[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) q[k] = (Real)Q[(int64_t)i * K + k];
@@ -192,8 +206,10 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
                 // a slot no lane types (an individual below maxploidy) only
                 // consumes its draw; diploid-only tiles keep straight-line code
                 if (MPT != 2 && tmask == 0) continue;
-                const bool al1 = a >= h;
-                const Real* pa = al1 ? pl1 : pl0;
+                // allele of slot a: a flag bit on the recoded tables, a >= h otherwise
+                const uint32_t al1b = (MPT == 2 || MPT == 4) ? ((desc >> (4 + a)) & 1u) : (a >= h ? 1u : 0u);
+                const bool al1 = al1b != 0;
+                const Real* pa = pl0 + al1b * kTileLoci;
                 Real wk[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) wk[k] = pa[k * 2 * kTileLoci] * q[k] + eps;
@@ -201,7 +217,7 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
 #pragma unroll
                 for (int k = 1; k < K; ++k) s = s + wk[k];
                 Real tt = unit_minus1<Real>(r) * s;
-                const uint32_t inc = al1 ? 0x10001u : 1u;
+                const uint32_t inc = (al1b << 16) | 1u;
 #pragma unroll
                 for (int k = 0; k < K - 1; ++k) {
                     tt = tt - wk[k];
