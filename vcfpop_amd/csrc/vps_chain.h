@@ -54,7 +54,9 @@ struct ChainDev {
 constexpr int kLikLociPerWave = 1024;
 
 // launchers (vps_kernels.hip / vps_updates.hip); all asynchronous on `st`
-int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
+// clear_mi = false: the caller has already zeroed Mi on an ordered stream
+int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1,
+                   bool clear_mi = true);
 int launch_z_noadmix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);

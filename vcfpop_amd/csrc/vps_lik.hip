@@ -105,7 +105,7 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
     FrexpAcc acc[NA];
 #pragma unroll
     for (int a = 0; a < NA; ++a) acc[a].open();
-    constexpr int kUnroll = MPT > 0 ? MPT : 1;
+    constexpr int kUnroll = MPT > 0 ? (MPT + 1) / 2 : 1;
     const int ncopies = MPT > 0 ? MPT : ds.maxploidy;
 
     const int64_t r0 = (int64_t)blockIdx.x * tiles_per_range;
@@ -133,12 +133,18 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
                 p0[k] = x0;
                 p1[k] = mono ? x0 : x1;
             }
+            // Copies are charged in pairs: the product of two factors is
+            // formed first.  Every factor is a frequency or a mixture of
+            // frequencies, above 1e-154 in any chain, so a pair cannot
+            // underflow; only the product order differs from per-copy
+            // charging.
             if (MODE == 3) {
 #pragma unroll kUnroll
-                for (int a = 0; a < ncopies; ++a) {
-                    const bool on = a < v, hi = a >= h;
+                for (int a = 0; a < ncopies; a += 2) {
+                    const bool on0 = a < v, hi0 = a >= h, on1 = a + 1 < v, hi1 = a + 1 >= h;
 #pragma unroll
-                    for (int k = 0; k < K; ++k) acc[k].charge(on ? (hi ? p1[k] : p0[k]) : 1.0);
+                    for (int k = 0; k < K; ++k)
+                        acc[k].charge((on0 ? (hi0 ? p1[k] : p0[k]) : 1.0) * (on1 ? (hi1 ? p1[k] : p0[k]) : 1.0));
                 }
             } else {
                 // sums start from the k = 0 term (0 + x == x for the
@@ -155,14 +161,11 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
                     }
                 }
 #pragma unroll kUnroll
-                for (int a = 0; a < ncopies; ++a) {
-                    const bool on = a < v, hi = a >= h;
-                    if (MODE == 2) {
-                        acc[0].charge(on ? (hi ? g1 : g0) : 1.0);
-                        acc[1].charge(on ? (hi ? f1 : f0) : 1.0);
-                    } else {
-                        acc[0].charge(on ? (hi ? f1 : f0) : 1.0);
-                    }
+                for (int a = 0; a < ncopies; a += 2) {
+                    const bool on0 = a < v, hi0 = a >= h, on1 = a + 1 < v, hi1 = a + 1 >= h;
+                    if (MODE == 2)
+                        acc[0].charge((on0 ? (hi0 ? g1 : g0) : 1.0) * (on1 ? (hi1 ? g1 : g0) : 1.0));
+                    acc[MODE == 2 ? 1 : 0].charge((on0 ? (hi0 ? f1 : f0) : 1.0) * (on1 ? (hi1 ? f1 : f0) : 1.0));
                 }
             }
         }

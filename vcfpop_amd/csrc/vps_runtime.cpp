@@ -103,6 +103,8 @@ void chain_free(vps_chain* ch)
     if (ch->aux) (void)hipStreamDestroy(ch->aux);
     if (ch->ev_fork) (void)hipEventDestroy(ch->ev_fork);
     if (ch->ev_join) (void)hipEventDestroy(ch->ev_join);
+    if (ch->ev_fork2) (void)hipEventDestroy(ch->ev_fork2);
+    if (ch->ev_join2) (void)hipEventDestroy(ch->ev_join2);
     if (ch->stream) (void)hipStreamDestroy(ch->stream);
     delete ch;
 }
@@ -203,6 +205,8 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ch->aux, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ch->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ch->ev_join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ch->ev_fork2, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ch->ev_join2, hipEventDisableTiming);
     if (e != hipSuccess) { chain_free(ch); return fail(VPS_ERR_DEVICE, "chain streams: %s", hipGetErrorString(e)); }
 
     // ReadPar, src/structure.cpp:405-464
@@ -373,7 +377,7 @@ int do_q_metro(vps_chain* ch)
     return rc;
 }
 
-int do_z_admix(vps_chain* ch)
+int do_z_admix(vps_chain* ch, bool clear_mi = true)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ch->timing && (ch->timing_every <= 1 || ch->timed_calls++ % ch->timing_every == 0)) {
@@ -388,7 +392,7 @@ int do_z_admix(vps_chain* ch)
         e1 = ch->evs[ch->ev_used + 1];
         ch->ev_used += 2;
     }
-    return launch_z_admix(ch->ds->d, ch->c, ch->stream, e0, e1);
+    return launch_z_admix(ch->ds->d, ch->c, ch->stream, e0, e1, clear_mi);
 }
 
 int do_z_noadmix(vps_chain* ch)
@@ -426,8 +430,20 @@ int launch_sweep(vps_chain* ch, const SweepPlan& p)
 {
     const DeviceDataset& d = ch->ds->d;
     const ChainDev& c = ch->c;
-    int rc = do_update_p(ch);
-    if (!rc) rc = p.q == kQNoAdmix ? do_q_noadmix(ch) : p.q == kQMetro ? do_q_metro(ch) : do_q_admix(ch);
+    int rc = VPS_OK;
+    // A Dirichlet UpdateQ reads Mi and Alpha, not P: it runs beside UpdateP on
+    // the second stream and then clears Mi for the Z pass
+    const bool q_beside = p.q == kQAdmix && !ch->capturing;
+    if (q_beside) {
+        VPS_HIP(hipEventRecord(ch->ev_fork2, ch->stream));
+        VPS_HIP(hipStreamWaitEvent(ch->aux, ch->ev_fork2, 0));
+        rc = launch_update_q_admix(d, c, ch->aux);
+        if (!rc) rc = launch_fill_i32(c.Mi, (int64_t)d.N * c.K, 0, ch->aux);
+        if (!rc) VPS_HIP(hipEventRecord(ch->ev_join2, ch->aux));
+    }
+    if (!rc) rc = do_update_p(ch);
+    if (!rc && q_beside) VPS_HIP(hipStreamWaitEvent(ch->stream, ch->ev_join2, 0));
+    if (!rc && !q_beside) rc = p.q == kQNoAdmix ? do_q_noadmix(ch) : p.q == kQMetro ? do_q_metro(ch) : do_q_admix(ch);
     if (!rc && !c.admix && c.locpriori) rc = launch_update_di(d, c, ch->stream);
     if (!rc) rc = launch_update_locpriori(d, c, ch->stream);
     // UpdateAlpha, UpdateLambda, UpdateA and UpdateF read Q and P and write
@@ -443,7 +459,7 @@ int launch_sweep(vps_chain* ch, const SweepPlan& p)
     if (!rc) rc = launch_update_lambda(d, c, side);
     if (!rc) rc = launch_update_a(d, c, side);
     if (!rc) rc = launch_update_f(d, c, side);
-    if (!rc) rc = p.q == kQNoAdmix ? do_z_noadmix(ch) : do_z_admix(ch);
+    if (!rc) rc = p.q == kQNoAdmix ? do_z_noadmix(ch) : do_z_admix(ch, !q_beside);
     if (!rc && side != ch->stream) {
         VPS_HIP(hipEventRecord(ch->ev_join, side));
         VPS_HIP(hipStreamWaitEvent(ch->stream, ch->ev_join, 0));

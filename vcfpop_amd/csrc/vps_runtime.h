@@ -32,7 +32,7 @@ struct vps_chain {
     int64_t lnl_cap = 0;
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;                  // parameter updates beside the Z pass
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr;
     bool capturing = false;                     // launch_sweep is being captured into a graph
     std::vector<void*> allocs;
     int64_t device_bytes = 0;

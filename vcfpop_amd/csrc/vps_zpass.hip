@@ -620,9 +620,9 @@ int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_
     return rc;
 }
 
-int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
+int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, bool clear_mi)
 {
-    int rc = launch_fill_i32(c.Mi, (int64_t)ds.N * c.K, 0, st);
+    int rc = clear_mi ? launch_fill_i32(c.Mi, (int64_t)ds.N * c.K, 0, st) : VPS_OK;
     if (rc) return rc;
     if (ev0) VPS_HIP(hipEventRecord(ev0, st));
     rc = c.precision == VPS_PREC_MIXED ? dispatch_k<float>(ds, c, st) : dispatch_k<double>(ds, c, st);
