@@ -26,6 +26,9 @@ DATASETS = {
                   seed=22),
     # ragged: N not a multiple of anything, a single short tile
     "ragged": dict(n_ind=13, n_loc=37, k_true=2, alpha_q=1.0, missing=0.2, ploidy=[(1, 0.3), (3, 0.7)], seed=23),
+    # many alleles: K x alleles tables beyond one block's LDS at large K
+    "many_alleles": dict(n_ind=40, n_loc=160, k_true=3, alpha_q=0.5, missing=0.05, frac_multi=0.6, min_alleles=12,
+                         max_alleles=40, multi_dir=0.5, ploidy=[(2, 0.5), (4, 0.5)], n_pop=2, seed=24),
 }
 
 
@@ -91,6 +94,25 @@ def test_z_admix_bit_exact(vps, orc, pairs, name, K):
         o.call("update_z_admix")
         assert (g.get("Ni") == o.get("Ni")).all(), m
         assert (g.get("Mi") == o.get("Mi")).all(), m
+
+
+@pytest.mark.parametrize("K", [12, 16])
+def test_count_tables_beyond_lds(vps, orc, pairs, K):
+    """initial counts and the no-admixture count pass when the K x alleles
+    table does not fit in LDS (counts by global atomics)"""
+    say("counts beyond LDS", K)
+    ds, od, g, o = make(vps, orc, pairs, "many_alleles", K, admix_val=1, g_seed=15)
+    assert ds.maxK * K * 64 * 4 > 64 * 1024
+    for n in ("Z", "Mi", "Ni"):
+        assert (g.get(n) == o.get(n)).all(), n
+    Z = np.random.default_rng(K).integers(0, K, ds.N).astype(np.uint16)
+    for c in (g, o):
+        c.set("Z", Z)
+    g.set("binaryq", [1])
+    g.update_z_noadmix()
+    o.call("update_z_noadmix")
+    assert (g.get("Ni") == o.get("Ni")).all()
+    assert (g.get("Mi") == o.get("Mi")).all()
 
 
 @pytest.mark.parametrize("name", ["bi_diploid", "mixed"])

@@ -222,33 +222,48 @@ extern "C" int vps_dataset_create(const vps_dataset_desc* in, int device, vps_da
     }
     for (int s = 0; s < S; ++s) if (ds->pop_ind0[s] < 0) ds->pop_ind0[s] = 0;
 
-    // tiles: 64 consecutive sorted loci of equal bitsize
+    // tiles: each run of equal bitsize split into BI and GEN loci, each part
+    // cut in sorted order into tiles of up to 64 loci
     std::vector<TileDesc>& tiles = ds->tiles;
+    std::vector<int32_t>& tloc = ds->tloc;
+    ds->lpos.assign(L, 0);
+    auto bi_locus = [&](int64_t l) {
+        if (ds->k2[l] > 2 || ds->bitsize[l] > kBiMaxBits) return false;
+        const int64_t o = ds->orig[l];
+        // a BI entry needs the allele copies in ascending order
+        for (int64_t g = in->gtab_offset[o]; g < in->gtab_offset[o + 1]; ++g)
+            for (int a = 1; a < in->gt_ploidy[g]; ++a)
+                if (in->allele_pool[in->gt_allele_offset[g] + a] < in->allele_pool[in->gt_allele_offset[g] + a - 1]) return false;
+        return true;
+    };
     {
-        int64_t l = 0, wbase = 0;
+        int64_t l = 0, wbase = 0, dbase = 0;
+        std::vector<int64_t> part[2];
         while (l < L) {
-            int b = ds->bitsize[l];
+            const int b = ds->bitsize[l];
             int64_t e = l;
-            while (e < L && e - l < kTileLoci && ds->bitsize[e] == b) ++e;
-            TileDesc t{};
-            t.l0 = l; t.nloc = (int32_t)(e - l); t.bits = b; t.ipw = 32 / b;
-            t.wbase = wbase;
-            int64_t nwords = (N + t.ipw - 1) / t.ipw;
-            wbase += nwords * kTileLoci;
-            int mk = 0; bool bi = true;
-            for (int64_t j = l; j < e; ++j) {
-                int64_t o = ds->orig[j];
-                mk = std::max(mk, ds->k2[j]);
-                int64_t ng = in->gtab_offset[o + 1] - in->gtab_offset[o];
-                if (ds->k2[j] > 2 || ng > 8) bi = false;
-                // a BI entry needs the allele copies in ascending order
-                for (int64_t g = in->gtab_offset[o]; bi && g < in->gtab_offset[o + 1]; ++g)
-                    for (int a = 1; a < in->gt_ploidy[g]; ++a)
-                        if (in->allele_pool[in->gt_allele_offset[g] + a] < in->allele_pool[in->gt_allele_offset[g] + a - 1]) bi = false;
-            }
-            t.maxk2 = mk;
-            t.kind = bi ? kKindBi : kKindGen;
-            tiles.push_back(t);
+            part[0].clear(); part[1].clear();
+            for (; e < L && ds->bitsize[e] == b; ++e) part[bi_locus(e) ? 0 : 1].push_back(e);
+            for (int kind = 0; kind < 2; ++kind)
+                for (size_t s0 = 0; s0 < part[kind].size(); s0 += kTileLoci) {
+                    const int n = (int)std::min<size_t>(kTileLoci, part[kind].size() - s0);
+                    TileDesc t{};
+                    t.lmap = (int64_t)tiles.size() * kTileLoci;
+                    t.nloc = n; t.bits = b; t.ipw = 32 / b;
+                    t.wbase = wbase;
+                    wbase += (int64_t)((N + t.ipw - 1) / t.ipw) * kTileLoci;
+                    t.kind = kind == 0 ? kKindBi : kKindGen;
+                    t.dbase = -1;
+                    if (t.kind == kKindBi) { t.dbase = dbase; dbase += (int64_t)kTileLoci << b; }
+                    int mk = 0;
+                    for (int j = 0; j < kTileLoci; ++j) {
+                        const int64_t lj = part[kind][s0 + (j < n ? j : 0)];
+                        tloc.push_back((int32_t)lj);
+                        if (j < n) { ds->lpos[lj] = t.lmap + j; mk = std::max(mk, ds->k2[lj]); }
+                    }
+                    t.maxk2 = mk;
+                    tiles.push_back(t);
+                }
             l = e;
         }
     }
@@ -265,7 +280,7 @@ extern "C" int vps_dataset_create(const vps_dataset_desc* in, int device, vps_da
         const TileDesc& t = tiles[ti];
         int64_t nwords = (N + t.ipw - 1) / t.ipw;
         for (int j = 0; j < t.nloc; ++j) {
-            int64_t o = ds->orig[t.l0 + j];
+            int64_t o = ds->orig[tloc[t.lmap + j]];
             for (int64_t w = 0; w < nwords; ++w) {
                 uint32_t word = 0;
                 for (int s = 0; s < t.ipw; ++s) {
@@ -282,9 +297,18 @@ extern "C" int vps_dataset_create(const vps_dataset_desc* in, int device, vps_da
     std::vector<uint64_t> bitab(L, 0);
     std::vector<int64_t> gbase(L, 0);
     std::vector<uint64_t> gdesc;
+    int64_t btab_bytes = 0;
+    int bi_rows = 0;
+    for (auto& t : tiles)
+        if (t.kind == kKindBi) {
+            btab_bytes = std::max<int64_t>(btab_bytes, t.dbase + ((int64_t)kTileLoci << t.bits));
+            if (t.bits > kBiRegBits) bi_rows = std::max(bi_rows, 1 << t.bits);
+        }
+    std::vector<uint8_t> btab(std::max<int64_t>(btab_bytes, 1), 0);
+    d.bi_tab_rows = bi_rows;
     for (auto& t : tiles) {
         for (int j = 0; j < t.nloc; ++j) {
-            int64_t l = t.l0 + j, o = ds->orig[l];
+            int64_t l = tloc[t.lmap + j], o = ds->orig[l];
             int64_t g0 = in->gtab_offset[o], ng = in->gtab_offset[o + 1] - g0;
             if (t.kind == kKindBi) {
                 uint64_t tab = 0;
@@ -292,7 +316,9 @@ extern "C" int vps_dataset_create(const vps_dataset_desc* in, int device, vps_da
                     int v = in->gt_ploidy[g0 + g];
                     int c = 0;
                     for (int a = 0; a < v; ++a) c += in->allele_pool[in->gt_allele_offset[g0 + g] + a] == 1;
-                    tab |= (uint64_t)(v | (c << 4)) << (8 * g);
+                    const uint8_t e = (uint8_t)(v | (c << 4));
+                    btab[t.dbase + g * kTileLoci + j] = e;
+                    if (g < 8) tab |= (uint64_t)e << (8 * g);
                 }
                 bitab[l] = tab;
             } else {
@@ -315,6 +341,7 @@ extern "C" int vps_dataset_create(const vps_dataset_desc* in, int device, vps_da
     if (host_only) {
         ds->h_words.swap(words);
         ds->h_bitab.swap(bitab);
+        ds->h_btab.swap(btab);
         ds->h_gdesc.swap(gdesc);
         ds->h_gbase.swap(gbase);
         *out = ds;
@@ -337,6 +364,8 @@ extern "C" int vps_dataset_create(const vps_dataset_desc* in, int device, vps_da
     if (!rc) rc = upload(&d.tiles, tiles, ds);
     if (!rc) rc = upload(&d.bi_list, bi_list, ds);
     if (!rc) rc = upload(&d.gen_list, gen_list, ds);
+    if (!rc) rc = upload(&d.tloc, tloc, ds);
+    if (!rc) rc = upload(&d.btab, btab, ds);
     if (!rc) rc = upload(&d.words, words, ds);
     if (!rc) rc = upload(&d.afo, afo32, ds);
     if (!rc) rc = upload(&d.k2, ds->k2, ds);
@@ -401,14 +430,14 @@ extern "C" int vps_debug_decode(const vps_dataset* ds, int64_t l, int32_t i, int
     if (!ds->host_only) return fail(VPS_ERR_STATE, "decode needs a host-only dataset (device = -1)");
     if (l < 0 || l >= ds->d.L || i < 0 || i >= ds->d.N) return fail(VPS_ERR_INVALID, "index out of range");
     // the tile holding sorted locus l
-    auto it = std::upper_bound(ds->tiles.begin(), ds->tiles.end(), l,
-                               [](int64_t v, const TileDesc& t) { return v < t.l0; });
-    const TileDesc& t = *(it - 1);
-    const int j = (int)(l - t.l0);
+    const int64_t pos = ds->lpos[l];
+    const TileDesc& t = ds->tiles[pos / kTileLoci];
+    const int j = (int)(pos % kTileLoci);
     const uint32_t word = ds->h_words[t.wbase + (int64_t)(i / t.ipw) * kTileLoci + j];
     const uint32_t gid = (word >> ((i % t.ipw) * t.bits)) & ((1u << t.bits) - 1u);
     if (t.kind == kKindBi) {
-        const uint32_t desc = (uint32_t)(ds->h_bitab[l] >> (gid * 8)) & 0xFFu;
+        const uint32_t desc = t.bits <= kBiRegBits ? (uint32_t)(ds->h_bitab[l] >> (gid * 8)) & 0xFFu
+                                                   : ds->h_btab[t.dbase + (int64_t)gid * kTileLoci + j];
         const int v = desc & 15, c = desc >> 4;
         *ploidy = v;
         for (int a = 0; a < v; ++a) alleles[a] = a >= v - c ? 1 : 0;

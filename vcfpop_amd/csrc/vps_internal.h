@@ -14,30 +14,40 @@ namespace vps {
 // ---------------------------------------------------------------------------
 // Genotype image in HBM.
 //
-// Loci are kept in the reference's sorted order and cut into tiles of 64
-// consecutive loci of one bitsize (the same tiles as the AVX path,
-// src/structureAVX.cpp:305-312).  One wavefront owns one tile: lane j holds
-// locus l0 + j.  The genotype ids of a tile are re-packed so that one 32-bit
-// load per lane is one coalesced 256-byte wavefront load:
+// Loci are kept in the reference's sorted order (bitsize, then original
+// index; src/structure.cpp:2202-2273).  Each run of equal bitsize is split
+// into the loci the biallelic kernels decode (kind BI) and the rest (kind
+// GEN); each part is cut, in sorted order, into tiles of up to 64 loci.  One
+// wavefront owns one tile: lane j holds sorted locus tloc[lmap + j] (lanes
+// past nloc repeat the tile's first locus and decode every id as untyped).
+// The genotype ids of a tile are re-packed so that one 32-bit load per lane
+// is one coalesced 256-byte wavefront load:
 //     words[wbase + w*64 + j]  = ids of individuals [w*ipw, (w+1)*ipw) of
-//                                locus l0 + j, id i at bits (i % ipw) * bits
+//                                lane j's locus, id i at bits (i % ipw) * bits
 // with ipw = floor(32 / bits) ids per word (no id straddles two words).
 //
-// Genotype ids decode through one of two tables:
-//   kind BI  (every locus of the tile has <= 2 alleles and <= 8 genotypes):
-//            bitab[l] packs 8 one-byte entries, entry g = v | c << 4 where v
-//            is the typed ploidy (0 = missing) and c the dosage of allele 1;
-//            slot a holds allele (a >= v - c).
+// Genotype ids decode through one of these tables:
+//   kind BI  (every locus of the tile has <= 2 alleles, <= 64 genotypes and
+//            its allele copies in ascending order): one byte per genotype,
+//            g -> v | c << 4 where v is the typed ploidy (0 = missing) and c
+//            the dosage of allele 1; slot a holds allele (a >= v - c).
+//            btab[dbase + g*64 + j] holds the byte of lane j for every
+//            g < 2^bits (zero past the locus' table and on idle lanes); for
+//            bits <= 3 the 8 bytes of locus l are also packed in bitab[l],
+//            one 64-bit register per lane.
 //   kind GEN (any other tile): gdesc[gbase[l] + g] = v | allele(a) << (4+6a)
 //            for a < v (alleles < 64, ploidy <= 10).
 // ---------------------------------------------------------------------------
 constexpr int kTileLoci = 64;
 constexpr int kKindBi = 0;
 constexpr int kKindGen = 1;
+constexpr int kBiRegBits = 3;    // BI tiles up to this bitsize decode from bitab
+constexpr int kBiMaxBits = 6;    // BI tiles up to this bitsize (64 genotypes)
 
 struct TileDesc {
-    int64_t l0;       // first sorted locus of the tile
+    int64_t lmap;     // first entry of the tile in `tloc` (tile index * 64)
     int64_t wbase;    // first word of the tile in `words`
+    int64_t dbase;    // first byte of the tile in `btab` (BI tiles)
     int32_t nloc;     // loci in the tile (<= 64)
     int32_t bits;     // genotype-id bitsize of every locus in the tile
     int32_t ipw;      // ids per 32-bit word
@@ -66,6 +76,9 @@ struct DeviceDataset {
     TileDesc* tiles;           // [ntiles]
     int32_t*  bi_list;         // tile indices of kind BI
     int32_t*  gen_list;        // tile indices of kind GEN
+    int32_t*  tloc;            // [ntiles * 64] sorted locus of each tile lane
+    uint8_t*  btab;            // BI genotype bytes, tile-major (see above)
+    int32_t   bi_tab_rows;     // largest 2^bits over BI tiles above kBiRegBits (0: none)
     uint32_t* words;           // transposed genotype ids
     int32_t*  afo;             // [L] allele_freq_offset
     int32_t*  k2;              // [L]
@@ -95,6 +108,8 @@ struct vps_dataset {
     std::vector<int32_t> popid;
     std::vector<int32_t> pop_ind0, pop_nind;
     std::vector<vps::TileDesc> tiles;
+    std::vector<int32_t> tloc;         // as DeviceDataset::tloc
+    std::vector<int64_t> lpos;         // sorted locus -> its entry in tloc
     std::vector<uint8_t> bitsize;
     std::vector<std::string> ind_names, pop_names, loc_names;
     std::vector<int64_t> kdis;
@@ -105,6 +120,7 @@ struct vps_dataset {
     bool host_only = false;
     std::vector<uint32_t> h_words;
     std::vector<uint64_t> h_bitab, h_gdesc;
+    std::vector<uint8_t> h_btab;
     std::vector<int64_t> h_gbase;
     // device jump tables per chunk size Nc: [ceil(N/Nc)][kJumpWords], entry c
     // advances a locus stream by c*Nc*maxploidy draws

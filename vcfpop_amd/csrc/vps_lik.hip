@@ -90,7 +90,8 @@ __device__ void lik_write(const DeviceDataset& ds, const ChainDev& c, FrexpAcc* 
 template <int K, int MODE>
 struct LikNA { static constexpr int value = MODE == 3 ? K : (MODE == 2 ? 2 : 1); };
 
-// biallelic tiles (at most two alleles, genotype table in ds.bitab).  MPT is
+// biallelic tiles (at most two alleles; genotype bytes in ds.bitab, or in
+// the tile's ds.btab columns above 3 bits per id).  MPT is
 // the maximum ploidy when it is 2 or 4 (copies loop fully unrolled), else 0.
 template <int K, int MODE, int MPT>
 __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, const ChainDev c, int tiles_per_range)
@@ -116,11 +117,13 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
         const int sh = (ii - w * td.ipw) * td.bits;
         const uint32_t mask = (1u << td.bits) - 1u;
         const uint32_t* wp = ds.words + td.wbase + (int64_t)w * kTileLoci;
+        const bool big = td.bits > kBiRegBits;
+        const uint8_t* btab = ds.btab + td.dbase;
 #pragma unroll 2
         for (int j = 0; j < td.nloc; ++j) {
-            const int64_t l = td.l0 + j;
+            const int64_t l = ds.tloc[td.lmap + j];
             const uint32_t gid = (wp[j] >> sh) & mask;
-            const uint32_t desc = (uint32_t)(ds.bitab[l] >> (gid * 8)) & 0xFFu;
+            const uint32_t desc = big ? (uint32_t)btab[gid * kTileLoci + j] : (uint32_t)(ds.bitab[l] >> (gid * 8)) & 0xFFu;
             const int v = ok ? (int)(desc & 15u) : 0;
             const int h = v - (int)(desc >> 4);    // copies a >= h carry allele 1
             const double* pl = c.Freq + ds.afo[l];
@@ -198,7 +201,7 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_gen(const DeviceDataset ds,
         const uint32_t mask = (1u << td.bits) - 1u;
         const uint32_t* wp = ds.words + td.wbase + (int64_t)w * kTileLoci;
         for (int j = 0; j < td.nloc; ++j) {
-            const int64_t l = td.l0 + j;
+            const int64_t l = ds.tloc[td.lmap + j];
             const uint32_t gid = (wp[j] >> sh) & mask;
             const uint64_t d64 = ds.gdesc[ds.gbase[l] + gid];
             const int v = ok ? (int)(d64 & 15u) : 0;

@@ -104,9 +104,12 @@ int64_t estimate_bytes(const HostImage& h, int K)
         const int64_t ipw = std::max(1, 32 / std::min(bits, 32));
         words += ((N + ipw - 1) / ipw) * 64;
     }
-    const int64_t ntiles = (L + 63) / 64;
+    // tiles split by bitsize and kind: at most two partial tiles per bitsize
+    const int64_t ntiles = (L + 63) / 64 + 2 * 24;
+    int64_t btab = 0;   // biallelic genotype bytes, 2^bits per locus
+    for (int64_t l = 0; l < L; ++l) btab += (int64_t)1 << std::min<int>(std::max<int>(1, h.bitsize[l]), 6);
     const int64_t dataset = words * 4 + L * (4 + 4 + 8 + 8 + 4) + (int64_t)h.gt_ploidy.size() * 8 + N * 8 +
-                            ntiles * 48 + ((N + 15) / 16) * 8192 + (1 << 20);
+                            ntiles * (56 + 64 * 4) + btab + ((N + 15) / 16) * 8192 + (1 << 20);
     const int64_t parts = std::min<int64_t>(kEstChunkParts, (N + 15) / 16);
     const int64_t chain = K * KT * (8 + 4 + 8) + parts * K * KT * 4 + 3 * KT * 8 + N * K * (8 + 4 + 8 + 8 + 8) +
                           N * 64 + (ntiles / 8 + 2) * N * K * 8 + (1 << 20);

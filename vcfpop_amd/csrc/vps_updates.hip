@@ -718,22 +718,27 @@ int launch_record_accumulate(const DeviceDataset& ds, const ChainDev& c, int pos
 // per individual the wavefront ballots its K cluster counts into Mi.
 // Single-Z start: Z_i = Next(K) from RNG(seed - 1, INITADM), Mi[i][Z_i] = vt_i.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_init_admix(const DeviceDataset ds, const ChainDev c, int A, int list_kind)
+// (in_lds = 0: the K x alleles table does not fit in LDS; counts go to Ni
+// by global atomics, Ni cleared first)
+__global__ __launch_bounds__(64) void k_init_admix(const DeviceDataset ds, const ChainDev c, int A, int list_kind,
+                                                  int in_lds)
 {
     extern __shared__ int32_t ni_lds[];
     const int lane = threadIdx.x;
     const int K = c.K;
-    for (int x = lane; x < K * A * kTileLoci; x += 64) ni_lds[x] = 0;
+    if (in_lds)
+        for (int x = lane; x < K * A * kTileLoci; x += 64) ni_lds[x] = 0;
     __syncthreads();
     const int64_t tix = blockIdx.x;
     const int64_t ntl = list_kind == kKindBi ? ds.nbi : ds.ngen;
     if (tix >= ntl) return;
     const TileDesc t = ds.tiles[list_kind == kKindBi ? ds.bi_list[tix] : ds.gen_list[tix]];
     const bool valid = lane < t.nloc;
-    const int64_t l = t.l0 + (valid ? lane : 0);
+    const int64_t l = ds.tloc[t.lmap + lane];
     const int afo = ds.afo[l];
     const int k2 = ds.k2[l];
     const uint64_t tab = valid ? ds.bitab[l] : 0ull;
+    const uint8_t* btab = ds.btab + t.dbase + lane;   // BI tiles above 3 bits
     const uint64_t* gd = ds.gdesc + ds.gbase[l];
     Rng r;
     r.init(c.seed + (uint64_t)l, kSaltInitAdm);
@@ -752,7 +757,7 @@ __global__ __launch_bounds__(64) void k_init_admix(const DeviceDataset ds, const
         int v, h = 0;
         uint64_t d64 = 0;
         if (t.kind == kKindBi) {
-            const uint32_t desc = (uint32_t)(tab >> (gid * 8)) & 0xFFu;
+            const uint32_t desc = bits > kBiRegBits ? (uint32_t)btab[gid * kTileLoci] : (uint32_t)(tab >> (gid * 8)) & 0xFFu;
             v = (int)(desc & 15u);
             h = v - (int)(desc >> 4);
         } else {
@@ -766,7 +771,8 @@ __global__ __launch_bounds__(64) void k_init_admix(const DeviceDataset ds, const
             if (typed) {
                 kk = (int)r.next((uint64_t)K);
                 const int x = t.kind == kKindBi ? (a >= h ? 1 : 0) : (int)((d64 >> (4 + 6 * a)) & 63u);
-                ni_lds[(kk * A + x) * kTileLoci + lane] += 1;
+                if (in_lds) ni_lds[(kk * A + x) * kTileLoci + lane] += 1;
+                else atomicAdd(c.Ni + (int64_t)kk * ds.KT + afo + x, 1);
             }
             for (int k = 0; k < K; ++k) {
                 const int cnt = __builtin_popcountll(__builtin_amdgcn_ballot_w64(typed && kk == k));
@@ -774,7 +780,7 @@ __global__ __launch_bounds__(64) void k_init_admix(const DeviceDataset ds, const
             }
         }
     }
-    if (valid)
+    if (valid && in_lds)
         for (int k = 0; k < K; ++k)
             for (int a = 0; a < k2; ++a) c.Ni[(int64_t)k * ds.KT + afo + a] = ni_lds[(k * A + a) * kTileLoci + lane];
 }
@@ -851,10 +857,15 @@ int launch_init_admix(const DeviceDataset& ds, const ChainDev& c, int singlez, h
         return launch_z_noadmix(ds, c, st);
     }
     const int A = ds.maxK > 2 ? ds.maxK : 2;
-    const size_t shm = (size_t)c.K * A * kTileLoci * sizeof(int32_t);
-    if (shm > 64 * 1024) return fail(VPS_ERR_UNSUPPORTED, "initial count table K=%d x %d alleles exceeds LDS", c.K, A);
-    if (ds.nbi > 0) hipLaunchKernelGGL(k_init_admix, dim3((unsigned)ds.nbi), dim3(64), shm, st, ds, c, A, kKindBi);
-    if (ds.ngen > 0) hipLaunchKernelGGL(k_init_admix, dim3((unsigned)ds.ngen), dim3(64), shm, st, ds, c, A, kKindGen);
+    size_t shm = (size_t)c.K * A * kTileLoci * sizeof(int32_t);
+    const int in_lds = shm <= 64 * 1024;
+    if (!in_lds) {
+        rc = launch_fill_i32(c.Ni, (int64_t)c.K * ds.KT, 0, st);
+        if (rc) return rc;
+        shm = 0;
+    }
+    if (ds.nbi > 0) hipLaunchKernelGGL(k_init_admix, dim3((unsigned)ds.nbi), dim3(64), shm, st, ds, c, A, kKindBi, in_lds);
+    if (ds.ngen > 0) hipLaunchKernelGGL(k_init_admix, dim3((unsigned)ds.ngen), dim3(64), shm, st, ds, c, A, kKindGen, in_lds);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }

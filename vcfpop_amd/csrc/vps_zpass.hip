@@ -11,7 +11,7 @@
 // after subtracting w_0..w_{k-1} (K-1 if none), src/mathAVX.cpp:73-122.
 //
 // Mapping: one wavefront = one 64-locus tile x one chunk of individuals.
-// Lane j owns locus l0 + j and its stream; a wavefront that starts at
+// Lane j owns the tile's j-th locus and its stream; a wavefront that starts at
 // individual i0 jumps its stream ahead by i0 * maxploidy steps with the
 // GF(2) table of its chunk, so a locus is split over G wavefronts without
 // changing any draw.  Per slot:
@@ -88,15 +88,148 @@ __device__ __forceinline__ void flush_mi_rows(const uint32_t* mi_lds, int nci, i
 }
 
 // ---------------------------------------------------------------------------
-// BI tiles: <= 2 alleles, <= 8 genotypes per locus.  A genotype decodes from
-// one 64-bit table per lane (entry = typed ploidy | dosage of allele 1 << 4;
-// slot a holds allele (a >= ploidy - dosage)).  The tile's P values sit in
-// LDS as [k][allele][lane] doubles, so selecting a slot's allele is one LDS
-// read per cluster instead of two register selects.  Per-lane cumulative
-// cluster counts are packed: low 16 bits all typed copies, high 16 bits the
-// copies of allele 1 (a chunk has <= 512 individuals x 10 slots < 65536).
+// BI tiles: <= 2 alleles, <= 64 genotypes per locus.  A genotype decodes to
+// one byte (typed ploidy | dosage of allele 1 << 4; slot a holds allele
+// (a >= ploidy - dosage)): from one 64-bit register table per lane when the
+// tile's ids have <= 3 bits, else from the lane's column of the tile's byte
+// table, copied into LDS.  The tile's P values sit in LDS as
+// [k][allele][lane] values, so selecting a slot's allele is one LDS read
+// per cluster instead of two register selects.  Per-lane cumulative cluster
+// counts are packed: low 16 bits all typed copies, high 16 bits the copies
+// of allele 1 (a chunk has <= 512 individuals x 10 slots < 65536).
 // MPT > 0 fixes maxploidy at compile time (unrolled slot loop).
 // ---------------------------------------------------------------------------
+
+// a genotype byte recoded as ploidy | (allele-1 flag of slot a) << (4 + a),
+// a < 4, so a slot's allele is one bit extract instead of a compare and
+// select (tables of maximum ploidy 2 and 4)
+__device__ __forceinline__ uint32_t slot_flags(uint32_t e)
+{
+    const int v = (int)(e & 15u), h = v - (int)((e >> 4) & 15u);
+    uint32_t f = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) f |= (a >= h ? 1u : 0u) << a;
+    return (uint32_t)v | (f << 4);
+}
+
+// one wavefront's chunk of one BI tile; BIG: ids of more than 3 bits, the
+// genotype bytes come from the lane's LDS column `tab_lds`
+template <int K, typename Real, int MPT, bool BIG>
+__device__ __forceinline__ void z_bi_tile(const DeviceDataset& ds, const TileDesc& t, const double* __restrict__ Q,
+                                          uint64_t seed, int m, int64_t l, bool valid, int afo, int k2, int lane,
+                                          int c, int i0, int i1, const Real* pl0, uint8_t* tab_lds,
+                                          uint32_t* mi_lds, int32_t* __restrict__ Ni_part,
+                                          const uint32_t* __restrict__ jump)
+{
+    constexpr int KH = (K + 1) / 2;
+    const int64_t KT = ds.KT;
+    uint64_t tab = 0;
+    if (BIG) {
+        // each lane copies (and reads back) only its own column
+        const int rows = 1 << t.bits;
+        const uint8_t* src = ds.btab + t.dbase + lane;
+        for (int g = 0; g < rows; ++g) {
+            uint32_t e = src[(int64_t)g * kTileLoci];
+            if (MPT == 2 || MPT == 4) e = slot_flags(e);
+            tab_lds[g * kTileLoci + lane] = (uint8_t)e;
+        }
+    } else {
+        tab = valid ? ds.bitab[l] : 0ull;   // invalid lane: every id is untyped
+        if (MPT == 2 || MPT == 4) {
+            uint64_t st = 0;
+#pragma unroll
+            for (int g = 0; g < 8; ++g) st |= (uint64_t)slot_flags((uint32_t)(tab >> (8 * g)) & 0xFFu) << (8 * g);
+            tab = st;
+        }
+    }
+    Xs128 rng;
+    rng.seed(seed + (uint64_t)m * (uint64_t)ds.L + (uint64_t)l, kSaltUpdateZ);
+    if (c > 0) jump_apply(jump + (size_t)c * kJumpWords, rng);
+
+    // cumulative counts C_k, k < K-1 (C_{K-1} = all typed copies is
+    // fixed by the data: the last cluster's counts come from ncopies)
+    uint32_t nc[K > 1 ? K - 1 : 1];
+#pragma unroll
+    for (int k = 0; k < K - 1; ++k) nc[k] = 0;
+    uint32_t mval = 0;
+    const Real eps = (Real)kMinFreq;
+    const int bits = t.bits, ipw = t.ipw;
+    const uint32_t mask = (1u << bits) - 1u;
+    int w = i0 / ipw, j = i0 - w * ipw;
+    const uint32_t* wp = ds.words + t.wbase + lane;
+    uint32_t word = wp[(int64_t)w * kTileLoci] >> (j * bits);
+    const int MP = MPT > 0 ? MPT : ds.maxploidy;
+
+    for (int i = i0; i < i1; ++i) {
+        const uint32_t gid = word & mask;
+        word >>= bits;
+        if (++j == ipw) {
+            j = 0; ++w;
+            if (i + 1 < i1) word = wp[(int64_t)w * kTileLoci];
+        }
+        const uint32_t desc = BIG ? (uint32_t)tab_lds[gid * kTileLoci + lane] : (uint32_t)(tab >> (gid * 8));
+        const int v = (int)(desc & 15u);
+        const int h = v - (int)((desc >> 4) & 15u);   // the generic path's dosage form
+        Real q[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) q[k] = (Real)Q[(int64_t)i * K + k];
+        int cum[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) cum[k] = 0;
+
+        constexpr int kSlotUnroll = MPT > 0 ? MPT : 1;
+#pragma unroll kSlotUnroll
+        for (int a = 0; a < MP; ++a) {
+            const uint64_t r = rng.next();
+            const bool typed = a < v;
+            const uint64_t tmask = ballot(typed);
+            // a slot no lane types (an individual below maxploidy) only
+            // consumes its draw; diploid-only tiles keep straight-line code
+            if (MPT != 2 && tmask == 0) continue;
+            // allele of slot a: a flag bit on the recoded tables, a >= h otherwise
+            const uint32_t al1b = (MPT == 2 || MPT == 4) ? ((desc >> (4 + a)) & 1u) : (a >= h ? 1u : 0u);
+            const Real* pa = pl0 + al1b * kTileLoci;
+            Real wk[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) wk[k] = pa[k * 2 * kTileLoci] * q[k] + eps;
+            Real s = wk[0];
+#pragma unroll
+            for (int k = 1; k < K; ++k) s = s + wk[k];
+            Real tt = unit_minus1<Real>(r) * s;
+            const uint32_t inc = (al1b << 16) | 1u;
+#pragma unroll
+            for (int k = 0; k < K - 1; ++k) {
+                tt = tt - wk[k];
+                const bool neg = tt < (Real)0;
+                cum[k] += __builtin_popcountll(ballot(neg) & tmask);
+                nc[k] += (neg && typed) ? inc : 0u;
+            }
+            cum[K - 1] += __builtin_popcountll(tmask);
+        }
+        flush_mi<K>(mi_lds + (i - i0) * KH, cum, lane, mval);
+    }
+
+    if (valid) {
+        int32_t* dst = Ni_part + (int64_t)c * K * KT + afo;
+        uint32_t prev = 0;
+#pragma unroll
+        for (int k = 0; k < K - 1; ++k) {
+            const uint32_t d = nc[k] - prev;   // per-half differences never borrow (cumulative counts)
+            prev = nc[k];
+            const int all = (int)(d & 0xFFFFu), one = (int)(d >> 16);
+            dst[k * KT] = all - one;
+            if (k2 > 1) dst[k * KT + 1] = one;
+        }
+    }
+}
+
+// LDS of the BI kernel: [wave][k][allele][lane] P values, then
+// [wave][genotype][lane] table bytes (bi_tab_rows rows; none when every BI
+// tile decodes from registers), then the block's packed Mi rows
+template <int K, typename Real>
+__host__ __device__ constexpr size_t bi_p_bytes(int nwv) { return (size_t)nwv * K * 2 * kTileLoci * sizeof(Real); }
+__host__ __device__ inline size_t bi_tab_bytes(int nwv, int rows) { return (size_t)nwv * rows * kTileLoci; }
+
 template <int K, typename Real, int MPT>
 __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, const double* __restrict__ Freq,
                                                    const double* __restrict__ Q, uint64_t seed,
@@ -115,10 +248,11 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
     const int i0 = c * Nc;
     const int i1 = min(ds.N, i0 + Nc);
     const int nci = i1 - i0;
-    // LDS: [wave][k][allele][lane] P values, then the block's packed Mi rows
     constexpr int KH = (K + 1) / 2;
-    Real* p_lds = reinterpret_cast<Real*>(lds_i32) + (size_t)wv * (K * 2 * kTileLoci);
-    uint32_t* mi_lds = reinterpret_cast<uint32_t*>(lds_i32) + (size_t)nwv * K * 2 * kTileLoci * (sizeof(Real) / 4);
+    uint8_t* lds_u8 = reinterpret_cast<uint8_t*>(lds_i32);
+    Real* p_lds = reinterpret_cast<Real*>(lds_u8) + (size_t)wv * (K * 2 * kTileLoci);
+    uint8_t* tab_lds = lds_u8 + bi_p_bytes<K, Real>(nwv) + (size_t)wv * ds.bi_tab_rows * kTileLoci;
+    uint32_t* mi_lds = reinterpret_cast<uint32_t*>(lds_u8 + bi_p_bytes<K, Real>(nwv) + bi_tab_bytes(nwv, ds.bi_tab_rows));
     for (int x = threadIdx.x; x < nci * KH; x += blockDim.x) mi_lds[x] = 0;
 
     const int64_t tix = (int64_t)blockIdx.x * nwv + wv;
@@ -130,7 +264,7 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
     if (has_tile) {
         t = ds.tiles[ds.bi_list[tix]];
         valid = lane < t.nloc;
-        l = t.l0 + (valid ? lane : 0);
+        l = ds.tloc[t.lmap + lane];
         afo = ds.afo[l];
         k2 = ds.k2[l];
         const int64_t KT = ds.KT;
@@ -144,121 +278,40 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
     __syncthreads();
 
     if (has_tile) {
-        const int64_t KT = ds.KT;
-        uint64_t tab = valid ? ds.bitab[l] : 0ull;   // invalid lane: every id is untyped
-        if (MPT == 2 || MPT == 4) {
-            // recode each genotype's byte as ploidy | (allele-1 flag of slot a) << (4 + a),
-            // so a slot's allele is one bit extract instead of a compare and select
-            uint64_t st = 0;
-#pragma unroll
-            for (int g = 0; g < 8; ++g) {
-                const uint32_t e = (uint32_t)(tab >> (8 * g)) & 0xFFu;
-                const int v = (int)(e & 15u), h = v - (int)(e >> 4);
-                uint32_t f = 0;
-#pragma unroll
-                for (int a = 0; a < 4; ++a) f |= (a >= h ? 1u : 0u) << a;
-                st |= (uint64_t)((uint32_t)v | (f << 4)) << (8 * g);
-            }
-            tab = st;
-        }
-        Xs128 rng;
-        rng.seed(seed + (uint64_t)m * (uint64_t)ds.L + (uint64_t)l, kSaltUpdateZ);
-        if (c > 0) jump_apply(jump + (size_t)c * kJumpWords, rng);
-
-        // cumulative counts C_k, k < K-1 (C_{K-1} = all typed copies is
-        // fixed by the data: the last cluster's counts come from ncopies)
-        uint32_t nc[K > 1 ? K - 1 : 1];
-#pragma unroll
-        for (int k = 0; k < K - 1; ++k) nc[k] = 0;
-        uint32_t mval = 0;
-        const Real eps = (Real)kMinFreq;
-        const int bits = t.bits, ipw = t.ipw;
-        const uint32_t mask = (1u << bits) - 1u;
-        int w = i0 / ipw, j = i0 - w * ipw;
-        const uint32_t* wp = ds.words + t.wbase + lane;
-        uint32_t word = wp[(int64_t)w * kTileLoci] >> (j * bits);
-        const int MP = MPT > 0 ? MPT : ds.maxploidy;
-        const Real* pl0 = p_lds + lane;
-
-        for (int i = i0; i < i1; ++i) {
-            const uint32_t gid = word & mask;
-            word >>= bits;
-            if (++j == ipw) {
-                j = 0; ++w;
-                if (i + 1 < i1) word = wp[(int64_t)w * kTileLoci];
-            }
-            const uint32_t desc = (uint32_t)(tab >> (gid * 8));
-            const int v = (int)(desc & 15u);
-            const int h = v - (int)((desc >> 4) & 15u);   // the generic path's dosage form
-            Real q[K];
-#pragma unroll
-            for (int k = 0; k < K; ++k) q[k] = (Real)Q[(int64_t)i * K + k];
-            int cum[K];
-#pragma unroll
-            for (int k = 0; k < K; ++k) cum[k] = 0;
-
-            constexpr int kSlotUnroll = MPT > 0 ? MPT : 1;
-#pragma unroll kSlotUnroll
-            for (int a = 0; a < MP; ++a) {
-                const uint64_t r = rng.next();
-                const bool typed = a < v;
-                const uint64_t tmask = ballot(typed);
-                // a slot no lane types (an individual below maxploidy) only
-                // consumes its draw; diploid-only tiles keep straight-line code
-                if (MPT != 2 && tmask == 0) continue;
-                // allele of slot a: a flag bit on the recoded tables, a >= h otherwise
-                const uint32_t al1b = (MPT == 2 || MPT == 4) ? ((desc >> (4 + a)) & 1u) : (a >= h ? 1u : 0u);
-                const bool al1 = al1b != 0;
-                const Real* pa = pl0 + al1b * kTileLoci;
-                Real wk[K];
-#pragma unroll
-                for (int k = 0; k < K; ++k) wk[k] = pa[k * 2 * kTileLoci] * q[k] + eps;
-                Real s = wk[0];
-#pragma unroll
-                for (int k = 1; k < K; ++k) s = s + wk[k];
-                Real tt = unit_minus1<Real>(r) * s;
-                const uint32_t inc = (al1b << 16) | 1u;
-#pragma unroll
-                for (int k = 0; k < K - 1; ++k) {
-                    tt = tt - wk[k];
-                    const bool neg = tt < (Real)0;
-                    cum[k] += __builtin_popcountll(ballot(neg) & tmask);
-                    nc[k] += (neg && typed) ? inc : 0u;
-                }
-                cum[K - 1] += __builtin_popcountll(tmask);
-            }
-            flush_mi<K>(mi_lds + (i - i0) * KH, cum, lane, mval);
-        }
-
-        if (valid) {
-            int32_t* dst = Ni_part + (int64_t)c * K * KT + afo;
-            uint32_t prev = 0;
-#pragma unroll
-            for (int k = 0; k < K - 1; ++k) {
-                const uint32_t d = nc[k] - prev;   // per-half differences never borrow (cumulative counts)
-                prev = nc[k];
-                const int all = (int)(d & 0xFFFFu), one = (int)(d >> 16);
-                dst[k * KT] = all - one;
-                if (k2 > 1) dst[k * KT + 1] = one;
-            }
-        }
+        if (t.bits > kBiRegBits)
+            z_bi_tile<K, Real, MPT, true>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane, tab_lds,
+                                          mi_lds, Ni_part, jump);
+        else
+            z_bi_tile<K, Real, MPT, false>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane, tab_lds,
+                                           mi_lds, Ni_part, jump);
     }
     __syncthreads();
     flush_mi_rows<K>(mi_lds, nci, Mi + (int64_t)i0 * K);
 }
 
 // ---------------------------------------------------------------------------
-// GEN tiles: any allele count (< 64) and ploidy mix.  A genotype decodes
-// from a 64-bit descriptor; P is gathered per slot; the lane's allele counts
-// live in an LDS column owned by the lane (or in global atomics when the
-// K x alleles table does not fit).
+// GEN tiles: any allele count (< 64) and ploidy mix.  A block holds one tile
+// and up to 4 consecutive chunks of individuals, one wavefront per chunk, so
+// the wavefronts share the tile's tables: with PLDS its P values sit in LDS
+// as [k][allele][lane] values (A alleles per lane column), so a slot's K
+// weights are K conflict-free LDS reads rather than K scattered HBM/L2
+// gathers (without PLDS, P is gathered from Freq); the allele counts of
+// clusters k < K-1 are one LDS table [k][allele][lane] that the wavefronts
+// add into (or global atomics into the wavefront's chunk partial when the
+// table does not fit).  A genotype decodes from a 64-bit descriptor, loaded
+// one individual ahead.
+//
+// LDS: [PLDS] P values, then [ni_in_lds] counts, then packed Mi rows of the
+// block's individuals.  The block's counts go to the partial of its first
+// chunk; the partials of its other chunks are written as zeros.
 // ---------------------------------------------------------------------------
-template <int K, typename Real>
+template <int K, typename Real, bool PLDS>
 __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, const double* __restrict__ Freq,
                                                     const double* __restrict__ Q, uint64_t seed,
                                                     const int32_t* __restrict__ mp,
                                                     int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
-                                                    int A, int ni_in_lds, int Nc, const uint32_t* __restrict__ jump)
+                                                    int A, int ni_in_lds, int Nc, int G,
+                                                    const uint32_t* __restrict__ jump)
 {
     const int m = *mp;
     extern __shared__ int32_t lds[];
@@ -267,26 +320,37 @@ __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, con
     // live in scalar registers
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwv = blockDim.x >> 6;
-    const int c = blockIdx.y;
-    const int i0 = c * Nc;
+    const int c0 = blockIdx.y * nwv;            // the block's first chunk
+    const int c = c0 + wv;                      // this wavefront's chunk
+    const int gi0 = c0 * Nc;
+    const int gi1 = min(ds.N, (c0 + nwv) * Nc);
+    const int i0 = min(ds.N, c * Nc);
     const int i1 = min(ds.N, i0 + Nc);
-    const int nci = i1 - i0;
     constexpr int KH = (K + 1) / 2;
-    uint32_t* mi_lds = reinterpret_cast<uint32_t*>(lds);   // packed rows, as in the BI kernel
-    int32_t* ni_lds = lds + Nc * KH + wv * (K * A * kTileLoci);
-    const int ntot = Nc * KH + (ni_in_lds ? nwv * K * A * kTileLoci : 0);
-    for (int x = threadIdx.x; x < ntot; x += blockDim.x) lds[x] = 0;
-    uint32_t mval = 0;
+    constexpr int KC = K > 1 ? K - 1 : 1;       // count rows
+    const int nP = PLDS ? K * A * kTileLoci : 0;
+    const int nni = ni_in_lds ? KC * A * kTileLoci : 0;
+    Real* p_lds = reinterpret_cast<Real*>(lds);
+    int32_t* ni_lds = reinterpret_cast<int32_t*>(p_lds + nP);
+    uint32_t* mi_lds = reinterpret_cast<uint32_t*>(ni_lds + nni);
+    for (int x = threadIdx.x; x < nni + (gi1 - gi0) * KH; x += blockDim.x) ni_lds[x] = 0;   // counts, Mi rows
+
+    const TileDesc t = ds.tiles[ds.gen_list[blockIdx.x]];
+    const bool valid = lane < t.nloc;
+    const int64_t l = ds.tloc[t.lmap + lane];
+    const int64_t KT = ds.KT;
+    const int afo = ds.afo[l];
+    const int k2 = ds.k2[l];
+    const double* pl = Freq + afo;
+    if (PLDS) {
+        // wavefront w fills cluster rows w, w + nwv, ... of every lane column
+        for (int k = wv; k < K; k += nwv)
+            for (int x = 0; x < A; ++x)
+                p_lds[(k * A + x) * kTileLoci + lane] = x < k2 ? (Real)pl[(int64_t)k * KT + x] : (Real)0;
+    }
     __syncthreads();
 
-    const int64_t tix = (int64_t)blockIdx.x * nwv + wv;
-    if (tix < ds.ngen) {
-        const TileDesc t = ds.tiles[ds.gen_list[tix]];
-        const bool valid = lane < t.nloc;
-        const int64_t l = t.l0 + (valid ? lane : 0);
-        const int64_t KT = ds.KT;
-        const int afo = ds.afo[l];
-        const int k2 = ds.k2[l];
+    if (i0 < i1) {
         const uint64_t* gd = ds.gdesc + ds.gbase[l];
         Xs128 rng;
         rng.seed(seed + (uint64_t)m * (uint64_t)ds.L + (uint64_t)l, kSaltUpdateZ);
@@ -298,19 +362,29 @@ __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, con
         int w = i0 / ipw, j = i0 - w * ipw;
         const uint32_t* wp = ds.words + t.wbase + lane;
         uint32_t word = wp[(int64_t)w * kTileLoci] >> (j * bits);
-        const int MP = ds.maxploidy;
-        const double* pl = Freq + afo;
-        int32_t* nig = Ni_part + (int64_t)c * K * KT + afo;
-
-        for (int i = i0; i < i1; ++i) {
-            const uint32_t gid = word & mask;
+        // id of individual i, taken in order (the next word is read only
+        // while the chunk has individuals left)
+        auto take = [&](int i) -> uint32_t {
+            const uint32_t g = word & mask;
             word >>= bits;
             if (++j == ipw) {
                 j = 0; ++w;
                 if (i + 1 < i1) word = wp[(int64_t)w * kTileLoci];
             }
-            const uint64_t desc = valid ? gd[gid] : 0ull;
-            const int v = (int)(desc & 15u);
+            return g;
+        };
+        const int MP = ds.maxploidy;
+        int32_t* nig = Ni_part + (int64_t)c * K * KT + afo;
+        uint32_t mval = 0;
+
+        uint64_t desc = valid ? gd[take(i0)] : 0ull;
+        for (int i = i0; i < i1; ++i) {
+            const uint64_t cur = desc;
+            if (i + 1 < i1) {
+                const uint32_t g = take(i + 1);
+                desc = valid ? gd[g] : 0ull;
+            }
+            const int v = (int)(cur & 15u);
             Real q[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) q[k] = (Real)Q[(int64_t)i * K + k];
@@ -323,11 +397,17 @@ __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, con
                 const bool typed = a < v;
                 const uint64_t tmask = ballot(typed);
                 if (tmask == 0) continue;   // no lane types this slot: the draw only
-                const int al = a < VPS_MAX_PLOIDY ? (int)((desc >> (4 + 6 * a)) & 63u) : 0;
+                const int al = a < VPS_MAX_PLOIDY ? (int)((cur >> (4 + 6 * a)) & 63u) : 0;
                 const int x = typed ? al : 0;
                 Real wk[K];
+                if (PLDS) {
+                    const Real* px = p_lds + x * kTileLoci + lane;
 #pragma unroll
-                for (int k = 0; k < K; ++k) wk[k] = (Real)pl[k * KT + x] * q[k] + eps;
+                    for (int k = 0; k < K; ++k) wk[k] = px[k * A * kTileLoci] * q[k] + eps;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < K; ++k) wk[k] = (Real)pl[k * KT + x] * q[k] + eps;
+                }
                 Real s = wk[0];
 #pragma unroll
                 for (int k = 1; k < K; ++k) s = s + wk[k];
@@ -341,48 +421,56 @@ __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, con
                     nneg += neg ? 1 : 0;
                 }
                 cum[K - 1] += __builtin_popcountll(tmask);
-                if (typed) {
-                    const int ks = K - 1 - nneg;
-                    if (ni_in_lds) ni_lds[(ks * A + x) * kTileLoci + lane] += 1;
+                // the last cluster's counts follow from the typed copies
+                const int ks = K - 1 - nneg;
+                if (typed && ks < K - 1) {
+                    if (ni_in_lds) atomicAdd(ni_lds + (ks * A + x) * kTileLoci + lane, 1);
                     else atomicAdd(nig + (int64_t)ks * KT + x, 1);
                 }
             }
-            flush_mi<K>(mi_lds + (i - i0) * KH, cum, lane, mval);
-        }
-
-        if (valid && ni_in_lds) {
-            for (int k = 0; k < K - 1; ++k)
-                for (int a = 0; a < k2; ++a) nig[(int64_t)k * KT + a] = ni_lds[(k * A + a) * kTileLoci + lane];
+            flush_mi<K>(mi_lds + (i - gi0) * KH, cum, lane, mval);
         }
     }
     __syncthreads();
-    flush_mi_rows<K>(mi_lds, nci, Mi + (int64_t)i0 * K);
+    if (ni_in_lds && valid) {
+        for (int k = wv; k < K - 1; k += nwv) {
+            int32_t* dst = Ni_part + ((int64_t)c0 * K + k) * KT + afo;
+            for (int x = 0; x < k2; ++x) dst[x] = ni_lds[(k * A + x) * kTileLoci + lane];
+            for (int cc = 1; cc < nwv && c0 + cc < G; ++cc)
+                for (int x = 0; x < k2; ++x) dst[(int64_t)cc * K * KT + x] = 0;
+        }
+    }
+    flush_mi_rows<K>(mi_lds, gi1 - gi0, Mi + (int64_t)gi0 * K);
 }
 
 // ---------------------------------------------------------------------------
 // UpdateZNoAdmix (src/structureAVX.cpp:432-552): Ni[Z_i][l][x] += 1 per
 // typed copy; Mi[i][Z_i] = vt_i.  Lanes = loci, counts in lane-owned LDS
-// columns, each chunk writes its own Ni partial.
+// columns, each chunk writes its own Ni partial (in_lds = 0: the table does
+// not fit in LDS; counts go to the partials by global atomics, cleared first).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_z_noadmix(const DeviceDataset ds, const uint16_t* __restrict__ Z, int K,
-                                                 int32_t* __restrict__ Ni_part, int A, int list_kind, int Nc)
+                                                 int32_t* __restrict__ Ni_part, int A, int list_kind, int Nc,
+                                                 int in_lds)
 {
     extern __shared__ int32_t ni_lds[];   // [K][A][64]
     const int lane = threadIdx.x;
     const int c = blockIdx.y;
     const int i0 = c * Nc;
     const int i1 = min(ds.N, i0 + Nc);
-    for (int x = lane; x < K * A * kTileLoci; x += 64) ni_lds[x] = 0;
+    if (in_lds)
+        for (int x = lane; x < K * A * kTileLoci; x += 64) ni_lds[x] = 0;
     __syncthreads();
     const int64_t tix = blockIdx.x;
     const int64_t ntl = list_kind == kKindBi ? ds.nbi : ds.ngen;
     if (tix >= ntl) return;
     const TileDesc t = ds.tiles[list_kind == kKindBi ? ds.bi_list[tix] : ds.gen_list[tix]];
     const bool valid = lane < t.nloc;
-    const int64_t l = t.l0 + (valid ? lane : 0);
+    const int64_t l = ds.tloc[t.lmap + lane];
     const int afo = ds.afo[l];
     const int k2 = ds.k2[l];
     const uint64_t tab = valid ? ds.bitab[l] : 0ull;
+    const uint8_t* btab = ds.btab + t.dbase + lane;   // BI tiles above 3 bits
     const uint64_t* gd = ds.gdesc + ds.gbase[l];
     const int bits = t.bits, ipw = t.ipw;
     const uint32_t mask = (1u << bits) - 1u;
@@ -398,18 +486,28 @@ __global__ __launch_bounds__(64) void k_z_noadmix(const DeviceDataset ds, const 
         }
         const int zk = Z[i];
         int32_t* col = ni_lds + (zk * A) * kTileLoci + lane;
+        int32_t* gcol = Ni_part + ((int64_t)c * K + zk) * ds.KT + afo;
         if (t.kind == kKindBi) {
-            const uint32_t desc = (uint32_t)(tab >> (gid * 8)) & 0xFFu;
+            const uint32_t desc = bits > kBiRegBits ? (uint32_t)btab[gid * kTileLoci] : (uint32_t)(tab >> (gid * 8)) & 0xFFu;
             const int v = (int)(desc & 15u), cc = (int)(desc >> 4);
-            col[0] += v - cc;
-            col[kTileLoci] += cc;
+            if (in_lds) {
+                col[0] += v - cc;
+                col[kTileLoci] += cc;
+            } else if (v) {
+                atomicAdd(gcol, v - cc);
+                if (cc) atomicAdd(gcol + 1, cc);
+            }
         } else {
             const uint64_t desc = valid ? gd[gid] : 0ull;
             const int v = (int)(desc & 15u);
-            for (int a = 0; a < v; ++a) col[(int)((desc >> (4 + 6 * a)) & 63u) * kTileLoci] += 1;
+            for (int a = 0; a < v; ++a) {
+                const int x = (int)((desc >> (4 + 6 * a)) & 63u);
+                if (in_lds) col[x * kTileLoci] += 1;
+                else atomicAdd(gcol + x, 1);
+            }
         }
     }
-    if (valid) {
+    if (valid && in_lds) {
         int32_t* dst = Ni_part + (int64_t)c * K * ds.KT + afo;
         for (int k = 0; k < K; ++k)
             for (int a = 0; a < k2; ++a) dst[(int64_t)k * ds.KT + a] = ni_lds[(k * A + a) * kTileLoci + lane];
@@ -470,25 +568,61 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 
+// LDS available to one block (gfx950: the CU's 160 KiB)
+int max_block_lds()
+{
+    static int bytes = [] {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 64 * 1024;
+        return v;
+    }();
+    return bytes;
+}
+
+// Launch shape of the GEN kernel: one tile x `waves` chunks per block (4,
+// fewer when there are fewer chunks); P staged when P, the count table and
+// the Mi rows fit one block's LDS, else counts alone in LDS, else counts by
+// global atomics.
+struct GenPlan {
+    int waves;
+    bool plds;
+    bool ni_in_lds;
+    size_t shm;
+};
+
+template <int K, typename Real>
+GenPlan gen_plan(const DeviceDataset& ds, int Nc, int G)
+{
+    const int wv = std::max(1, std::min(kWavesPerBlock, G));
+    const size_t A = (size_t)std::max(ds.maxK, 1);
+    const size_t kc = K > 1 ? K - 1 : 1;
+    const size_t mi = (size_t)wv * Nc * ((K + 1) / 2) * sizeof(uint32_t);
+    const size_t ni = kc * A * kTileLoci * sizeof(int32_t);
+    const size_t pb = (size_t)K * A * kTileLoci * sizeof(Real);
+    const size_t cap = (size_t)max_block_lds();
+    if (pb + ni + mi <= cap) return {wv, true, true, pb + ni + mi};
+    if (ni + mi <= cap) return {wv, false, true, ni + mi};
+    return {wv, false, false, mi};
+}
+
 template <int K, typename Real>
 int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     const int nthr = 64 * kWavesPerBlock;
-    // GEN tiles keep their per-lane allele counts in LDS when the K x alleles
-    // table fits; otherwise they add into the chunk partials, which must then
-    // be cleared before any tile writes its counts.
-    const int A = ds.maxK;
+    const int A = std::max(ds.maxK, 1);
     const size_t mi_packed = (size_t)c.Nc * ((K + 1) / 2) * sizeof(uint32_t);
-    size_t ni_bytes = (size_t)kWavesPerBlock * K * A * kTileLoci * sizeof(int32_t);
-    const int in_lds = mi_packed + ni_bytes <= 64 * 1024;
-    if (ds.ngen > 0 && !in_lds) {
+    const GenPlan g = gen_plan<K, Real>(ds, c.Nc, c.G);
+    // GEN counts in global atomics add into the chunk partials, which must
+    // then be cleared before any tile (of either kind) writes its counts
+    if (ds.ngen > 0 && !g.ni_in_lds) {
         int rc = launch_fill_i32(c.Ni_part, (int64_t)c.G * K * ds.KT, 0, st);
         if (rc) return rc;
-        ni_bytes = 0;
     }
     if (ds.nbi > 0) {
         dim3 grid((unsigned)((ds.nbi + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)c.G);
-        const size_t p_bytes = (size_t)kWavesPerBlock * K * 2 * kTileLoci * sizeof(Real);
+        const size_t p_bytes = bi_p_bytes<K, Real>(kWavesPerBlock) + bi_tab_bytes(kWavesPerBlock, ds.bi_tab_rows);
         if (ds.maxploidy == 2)
             hipLaunchKernelGGL((k_z_admix_bi<K, Real, 2>), grid, dim3(nthr), p_bytes + mi_packed, st, ds, c.Freq, c.Q, c.seed, c.mdev, c.Ni_part, c.Mi, c.Nc, c.jump);
         else if (ds.maxploidy == 4)
@@ -498,21 +632,26 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
         VPS_HIP(hipGetLastError());
     }
     if (ds.ngen > 0) {
-        dim3 grid((unsigned)((ds.ngen + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)c.G);
-        hipLaunchKernelGGL((k_z_admix_gen<K, Real>), grid, dim3(nthr), mi_packed + ni_bytes, st, ds, c.Freq, c.Q, c.seed,
-                           c.mdev, c.Ni_part, c.Mi, A, in_lds, c.Nc, c.jump);
+        dim3 grid((unsigned)ds.ngen, (unsigned)((c.G + g.waves - 1) / g.waves));
+        if (g.plds)
+            hipLaunchKernelGGL((k_z_admix_gen<K, Real, true>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q, c.seed,
+                               c.mdev, c.Ni_part, c.Mi, A, 1, c.Nc, c.G, c.jump);
+        else
+            hipLaunchKernelGGL((k_z_admix_gen<K, Real, false>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q, c.seed,
+                               c.mdev, c.Ni_part, c.Mi, A, g.ni_in_lds ? 1 : 0, c.Nc, c.G, c.jump);
         VPS_HIP(hipGetLastError());
     }
     return VPS_OK;
 }
 
-// resident blocks per CU of the Z kernels for a chunk of Nc individuals
+// resident blocks per CU of the BI kernel, and resident GEN wavefronts per
+// SIMD (GEN blocks may hold fewer than 4 wavefronts), for chunks of Nc
 template <int K, typename Real>
-int z_occupancy(const DeviceDataset& ds, int Nc, int* bi_blocks, int* gen_blocks)
+int z_occupancy(const DeviceDataset& ds, int Nc, int* bi_blocks, int* gen_waves, int* gen_block_waves)
 {
     const size_t mi_packed = (size_t)Nc * ((K + 1) / 2) * sizeof(uint32_t);
-    const size_t p_bytes = (size_t)kWavesPerBlock * K * 2 * kTileLoci * sizeof(Real);
-    *bi_blocks = *gen_blocks = 1;
+    const size_t p_bytes = bi_p_bytes<K, Real>(kWavesPerBlock) + bi_tab_bytes(kWavesPerBlock, ds.bi_tab_rows);
+    *bi_blocks = 1;
     hipError_t e = hipSuccess;
     if (ds.maxploidy == 2)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 2>, 256, p_bytes + mi_packed);
@@ -521,20 +660,23 @@ int z_occupancy(const DeviceDataset& ds, int Nc, int* bi_blocks, int* gen_blocks
     else
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 0>, 256, p_bytes + mi_packed);
     if (e != hipSuccess) return fail(VPS_ERR_DEVICE, "occupancy query: %s", hipGetErrorString(e));
-    const size_t ni_bytes = (size_t)kWavesPerBlock * K * ds.maxK * kTileLoci * sizeof(int32_t);
-    const size_t gshm = mi_packed + (mi_packed + ni_bytes <= 64 * 1024 ? ni_bytes : 0);
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(gen_blocks, (const void*)k_z_admix_gen<K, Real>, 256, gshm);
+    const GenPlan g = gen_plan<K, Real>(ds, Nc, (ds.N + Nc - 1) / Nc);
+    int gb = 1;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&gb, g.plds ? (const void*)k_z_admix_gen<K, Real, true>
+                                                                 : (const void*)k_z_admix_gen<K, Real, false>,
+                                                     64 * g.waves, g.shm);
     if (e != hipSuccess) return fail(VPS_ERR_DEVICE, "occupancy query: %s", hipGetErrorString(e));
     *bi_blocks = std::max(*bi_blocks, 1);
-    *gen_blocks = std::max(*gen_blocks, 1);
+    *gen_waves = std::max(1, std::max(gb, 1) * g.waves / kWavesPerBlock);
+    *gen_block_waves = g.waves;
     return VPS_OK;
 }
 
 template <typename Real>
-int z_occupancy_k(const DeviceDataset& ds, int K, int Nc, int* b, int* g)
+int z_occupancy_k(const DeviceDataset& ds, int K, int Nc, int* b, int* g, int* gw)
 {
     switch (K) {
-#define VPS_CASE(k) case k: return z_occupancy<k, Real>(ds, Nc, b, g);
+#define VPS_CASE(k) case k: return z_occupancy<k, Real>(ds, Nc, b, g, gw);
         VPS_CASE(1) VPS_CASE(2) VPS_CASE(3) VPS_CASE(4) VPS_CASE(5) VPS_CASE(6) VPS_CASE(7) VPS_CASE(8)
         VPS_CASE(9) VPS_CASE(10) VPS_CASE(11) VPS_CASE(12) VPS_CASE(13) VPS_CASE(14) VPS_CASE(15) VPS_CASE(16)
 #undef VPS_CASE
@@ -556,7 +698,7 @@ int dispatch_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 
 }  // namespace
 
-// Chunking model.  A block (4 wavefronts, one per SIMD) costs
+// Chunking model.  A group of 4 wavefronts (one per SIMD) costs
 // (Nc * maxploidy + 8) slot-steps (jump-ahead and set-up ~ 8 slots); a CU
 // streams its blocks back to back, so the pass takes ceil(blocks / CUs)
 // block-times, stretched when fewer than 3 wavefronts per SIMD are resident
@@ -583,8 +725,10 @@ int choose_chunks(const DeviceDataset& ds, int K, int precision, int device, int
         if (Nc == lastNc) continue;
         lastNc = Nc;
         const int Gr = (N + Nc - 1) / Nc;
-        int bb = 1, gb = 1;
-        int rc = precision == VPS_PREC_MIXED ? z_occupancy_k<float>(ds, K, Nc, &bb, &gb) : z_occupancy_k<double>(ds, K, Nc, &bb, &gb);
+        // occupancy as wavefronts per SIMD (a 4-wave BI block puts one on each)
+        int bb = 1, gb = 1, gw = kWavesPerBlock;
+        int rc = precision == VPS_PREC_MIXED ? z_occupancy_k<float>(ds, K, Nc, &bb, &gb, &gw)
+                                             : z_occupancy_k<double>(ds, K, Nc, &bb, &gb, &gw);
         if (rc) return rc;
         const double work = ((double)Nc * ds.maxploidy + 8.0) * slot_us;
         double t = (double)Gr * K * ds.KT * 8.0 / hbm_bytes_per_us;
@@ -642,12 +786,17 @@ int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 int launch_z_noadmix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     const int A = ds.maxK > 2 ? ds.maxK : 2;
-    const size_t shm = (size_t)c.K * A * kTileLoci * sizeof(int32_t);
-    if (shm > 64 * 1024) return fail(VPS_ERR_UNSUPPORTED, "no-admix count table K=%d x %d alleles exceeds LDS", c.K, A);
+    size_t shm = (size_t)c.K * A * kTileLoci * sizeof(int32_t);
+    const int in_lds = shm <= 64 * 1024;
+    if (!in_lds) {
+        int rc = launch_fill_i32(c.Ni_part, (int64_t)c.G * c.K * ds.KT, 0, st);
+        if (rc) return rc;
+        shm = 0;
+    }
     if (ds.nbi > 0)
-        hipLaunchKernelGGL(k_z_noadmix, dim3((unsigned)ds.nbi, (unsigned)c.G), dim3(64), shm, st, ds, c.Z, c.K, c.Ni_part, A, kKindBi, c.Nc);
+        hipLaunchKernelGGL(k_z_noadmix, dim3((unsigned)ds.nbi, (unsigned)c.G), dim3(64), shm, st, ds, c.Z, c.K, c.Ni_part, A, kKindBi, c.Nc, in_lds);
     if (ds.ngen > 0)
-        hipLaunchKernelGGL(k_z_noadmix, dim3((unsigned)ds.ngen, (unsigned)c.G), dim3(64), shm, st, ds, c.Z, c.K, c.Ni_part, A, kKindGen, c.Nc);
+        hipLaunchKernelGGL(k_z_noadmix, dim3((unsigned)ds.ngen, (unsigned)c.G), dim3(64), shm, st, ds, c.Z, c.K, c.Ni_part, A, kKindGen, c.Nc, in_lds);
     VPS_HIP(hipGetLastError());
     return launch_reduce_ni(ds, c, st);
 }
