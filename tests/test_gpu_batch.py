@@ -36,7 +36,10 @@ def make_chains(vps, ds, specs):
     out = []
     for run_id, (model, K, extra) in enumerate(specs, start=1):
         kw = dict(MODELS[model])
-        kw.update(nburnin=6, nreps=14, nthinning=3, metrofreq=4, writelnl_val=1, g_seed=17)
+        # z_chunk=-1: the modelled chunking, the same for every copy of a
+        # chain (the Record lnL sums in chunk order; timed trials may pick
+        # different chunkings for two copies)
+        kw.update(nburnin=6, nreps=14, nthinning=3, metrofreq=4, writelnl_val=1, g_seed=17, z_chunk=-1)
         kw.update(extra)
         out.append(vps.Chain(ds, vps.params_default(**kw), K, run_id))
     return out

@@ -32,7 +32,7 @@ def job_specs():
                      alpha_q=0.5, fst=0.05, missing=0.02, ploidy=ploidy, seed=100 + j,
                      n_pop=4 if model == "locprior" else 0)
         par = dict(MODELS[model], nburnin=int(rng.integers(8, 20)), nreps=int(rng.integers(15, 40)),
-                   nthinning=3, writelnl_val=1, g_seed=3)
+                   nthinning=3, writelnl_val=1, g_seed=3, z_chunk=-1)   # the server's chunking
         out.append((synth, par, int(rng.integers(1, 7)), j + 1))
     return out
 

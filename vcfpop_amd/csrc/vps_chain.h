@@ -4,6 +4,24 @@
 
 namespace vps {
 
+// A log accumulator that keeps a product as mantissa in [0.5, 1) times 2^e
+// (frexp after each factor): the reference's exponent-stripping log
+// accumulator (src/math2.h:716-902) with the strip done every time.  Scaling
+// by powers of two is exact, so only the product order differs from the
+// reference's products, and sums agree to rounding.
+struct FrexpAcc {
+    double m;
+    int e;
+    __device__ void open() { m = 1.0; e = 0; }
+    __device__ void charge(double x)
+    {
+        const double y = m * x;
+        e += __builtin_amdgcn_frexp_exp(y);
+        m = __builtin_amdgcn_frexp_mant(y);
+    }
+    __device__ double close() const { return ((double)e + log2(m)) * kLn2Trunc; }
+};
+
 // Pointers and scalars of one chain, passed by value to every kernel.
 struct ChainDev {
     int32_t K, N, S, maxploidy;
@@ -29,6 +47,7 @@ struct ChainDev {
     double*   bufNK1;    // [N*K] Metropolis proposals / no-admix log likelihoods
     double*   bufN1;     // [N]   Metropolis log ratios
     double*   part;      // partial sums of the likelihood passes
+    double*   zlik;      // [z_lik_parts(G)] Record lnL partials of the FP64 Z pass
     int64_t*  MiSum;     // [N*K]
     double*   FreqM;     // [K*KT]
     double*   FreqA;     // [3*KT]
@@ -55,8 +74,13 @@ constexpr int kLikLociPerWave = 1024;
 
 // launchers (vps_kernels.hip / vps_updates.hip); all asynchronous on `st`
 // clear_mi = false: the caller has already zeroed Mi on an ordered stream
+// rec = true: the FP64 Z pass also sums the Record lnL (mode 0) into c.scal[1]
 int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1,
-                   bool clear_mi = true);
+                   bool clear_mi = true, bool rec = false);
+// number of Record partials the Z pass writes with G chunks
+int64_t z_lik_parts(const DeviceDataset& ds, int G);
+// out = the fixed-order sum of n partials (one block)
+int launch_reduce_lnl(const double* part, int64_t n, double* out, hipStream_t st);
 int launch_z_noadmix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);

@@ -24,25 +24,13 @@
 // Each (locus range, individual) closes one partial; record passes sum a
 // block's partials in LDS in a fixed tree, and a fixed-order pass adds the
 // block sums (or, per individual, the range partials).
+#include <type_traits>
 #include "vps_chain.h"
 
 namespace vps {
 
 constexpr int kLikThreads = 256;
 constexpr int kLikTilesPerRange = 8;
-
-struct FrexpAcc {
-    double m;
-    int e;
-    __device__ void open() { m = 1.0; e = 0; }
-    __device__ void charge(double x)
-    {
-        const double y = m * x;
-        e += __builtin_amdgcn_frexp_exp(y);
-        m = __builtin_amdgcn_frexp_mant(y);
-    }
-    __device__ double close() const { return ((double)e + log2(m)) * kLn2Trunc; }
-};
 
 // modes 0/1: sum the block's 256 partials in a fixed tree
 __device__ void block_store_sum(double v, double* dst)
@@ -106,9 +94,9 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
     FrexpAcc acc[NA];
 #pragma unroll
     for (int a = 0; a < NA; ++a) acc[a].open();
-    constexpr int kUnroll = MPT > 0 ? (MPT + 1) / 2 : 1;
     const int ncopies = MPT > 0 ? MPT : ds.maxploidy;
 
+    const int lane = threadIdx.x & 63;
     const int64_t r0 = (int64_t)blockIdx.x * tiles_per_range;
     const int64_t r1 = min(ds.nbi, r0 + tiles_per_range);
     for (int64_t r = r0; r < r1; ++r) {
@@ -119,59 +107,85 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
         const uint32_t* wp = ds.words + td.wbase + (int64_t)w * kTileLoci;
         const bool big = td.bits > kBiRegBits;
         const uint8_t* btab = ds.btab + td.dbase;
-#pragma unroll 2
-        for (int j = 0; j < td.nloc; ++j) {
-            const int64_t l = ds.tloc[td.lmap + j];
-            const uint32_t gid = (wp[j] >> sh) & mask;
-            const uint32_t desc = big ? (uint32_t)btab[gid * kTileLoci + j] : (uint32_t)(ds.bitab[l] >> (gid * 8)) & 0xFFu;
-            const int v = ok ? (int)(desc & 15u) : 0;
-            const int h = v - (int)(desc >> 4);    // copies a >= h carry allele 1
-            const double* pl = c.Freq + ds.afo[l];
-            const bool mono = ds.k2[l] < 2;        // monomorphic loci have no allele 1
-            double p0[K], p1[K];
+        // the tile's per-locus fields, lane j holding locus j (one vector
+        // load each instead of a chain of scalar loads per locus)
+        const int64_t lj = ds.tloc[td.lmap + lane];
+        const int afo_v = ds.afo[lj];
+        const int k2_v = ds.k2[lj];
+        const uint64_t tab_v = ds.bitab[lj];
+        // one copy of the loop per table kind: the register-table loop then
+        // has no vector load besides the id prefetch to wait for
+        auto tile_loop = [&](auto big_tag) {
+            constexpr bool BIG = decltype(big_tag)::value;
+            constexpr int kPairs = MPT > 0 ? (MPT + 1) / 2 : 1;   // copy pairs of the unrolled loop
+            // ids of 4 loci per 16-byte load (a tile's word rows hold 64
+            // entries, so a group never leaves its row); the next group's load
+            // is issued before this group's work
+            uint4 wn = *reinterpret_cast<const uint4*>(wp);
+            for (int j0 = 0; j0 < td.nloc; j0 += 4) {
+                const uint4 w4 = wn;
+                if (j0 + 4 < td.nloc) wn = *reinterpret_cast<const uint4*>(wp + j0 + 4);
+                const uint32_t wq[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                // both alleles in one 16-byte scalar load (P is padded past KT)
-                const double x0 = pl[k * KT], x1 = pl[k * KT + 1];
-                p0[k] = x0;
-                p1[k] = mono ? x0 : x1;
-            }
-            // Copies are charged in pairs: the product of two factors is
-            // formed first.  Every factor is a frequency or a mixture of
-            // frequencies, above 1e-154 in any chain, so a pair cannot
-            // underflow; only the product order differs from per-copy
-            // charging.
-            if (MODE == 3) {
-#pragma unroll kUnroll
-                for (int a = 0; a < ncopies; a += 2) {
-                    const bool on0 = a < v, hi0 = a >= h, on1 = a + 1 < v, hi1 = a + 1 >= h;
+                for (int u = 0; u < 4; ++u) {
+                    const int j = j0 + u;
+                    if (j >= td.nloc) break;
+                    const uint64_t tab = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(tab_v >> 32), j) << 32) |
+                                         (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)tab_v, j);
+                    const uint32_t gid = (wq[u] >> sh) & mask;
+                    const uint32_t desc = BIG ? (uint32_t)btab[gid * kTileLoci + j] : (uint32_t)(tab >> (gid * 8)) & 0xFFu;
+                    const int v = ok ? (int)(desc & 15u) : 0;
+                    const int h = v - (int)(desc >> 4);    // copies a >= h carry allele 1
+                    const double* pl = c.Freq + __builtin_amdgcn_readlane(afo_v, j);
+                    const bool mono = __builtin_amdgcn_readlane(k2_v, j) < 2;   // monomorphic loci have no allele 1
+                    double p0[K], p1[K];
 #pragma unroll
-                    for (int k = 0; k < K; ++k)
-                        acc[k].charge((on0 ? (hi0 ? p1[k] : p0[k]) : 1.0) * (on1 ? (hi1 ? p1[k] : p0[k]) : 1.0));
-                }
-            } else {
-                // sums start from the k = 0 term (0 + x == x for the
-                // non-negative terms here)
-                double f0 = p0[0] * q[0], f1 = p1[0] * q[0];
-                double g0 = MODE == 2 ? p0[0] * qp[0] : 0.0, g1 = MODE == 2 ? p1[0] * qp[0] : 0.0;
+                    for (int k = 0; k < K; ++k) {
+                        // both alleles in one 16-byte scalar load (P is padded past KT)
+                        const double x0 = pl[k * KT], x1 = pl[k * KT + 1];
+                        p0[k] = x0;
+                        p1[k] = mono ? x0 : x1;
+                    }
+                    // Copies are charged in pairs: the product of two factors is
+                    // formed first.  Every factor is a frequency or a mixture of
+                    // frequencies, above 1e-154 in any chain, so a pair cannot
+                    // underflow; only the product order differs from per-copy
+                    // charging.
+                    if (MODE == 3) {
+#pragma unroll kPairs
+                        for (int a = 0; a < ncopies; a += 2) {
+                            const bool on0 = a < v, hi0 = a >= h, on1 = a + 1 < v, hi1 = a + 1 >= h;
 #pragma unroll
-                for (int k = 1; k < K; ++k) {
-                    f0 = f0 + p0[k] * q[k];
-                    f1 = f1 + p1[k] * q[k];
-                    if (MODE == 2) {
-                        g0 = g0 + p0[k] * qp[k];
-                        g1 = g1 + p1[k] * qp[k];
+                            for (int k = 0; k < K; ++k)
+                                acc[k].charge((on0 ? (hi0 ? p1[k] : p0[k]) : 1.0) * (on1 ? (hi1 ? p1[k] : p0[k]) : 1.0));
+                        }
+                    } else {
+                        // sums start from the k = 0 term (0 + x == x for the
+                        // non-negative terms here)
+                        double f0 = p0[0] * q[0], f1 = p1[0] * q[0];
+                        double g0 = MODE == 2 ? p0[0] * qp[0] : 0.0, g1 = MODE == 2 ? p1[0] * qp[0] : 0.0;
+#pragma unroll
+                        for (int k = 1; k < K; ++k) {
+                            f0 = f0 + p0[k] * q[k];
+                            f1 = f1 + p1[k] * q[k];
+                            if (MODE == 2) {
+                                g0 = g0 + p0[k] * qp[k];
+                                g1 = g1 + p1[k] * qp[k];
+                            }
+                        }
+#pragma unroll kPairs
+                        for (int a = 0; a < ncopies; a += 2) {
+                            const bool on0 = a < v, hi0 = a >= h, on1 = a + 1 < v, hi1 = a + 1 >= h;
+                            if (MODE == 2)
+                                acc[0].charge((on0 ? (hi0 ? g1 : g0) : 1.0) * (on1 ? (hi1 ? g1 : g0) : 1.0));
+                            acc[MODE == 2 ? 1 : 0].charge((on0 ? (hi0 ? f1 : f0) : 1.0) * (on1 ? (hi1 ? f1 : f0) : 1.0));
+                        }
                     }
                 }
-#pragma unroll kUnroll
-                for (int a = 0; a < ncopies; a += 2) {
-                    const bool on0 = a < v, hi0 = a >= h, on1 = a + 1 < v, hi1 = a + 1 >= h;
-                    if (MODE == 2)
-                        acc[0].charge((on0 ? (hi0 ? g1 : g0) : 1.0) * (on1 ? (hi1 ? g1 : g0) : 1.0));
-                    acc[MODE == 2 ? 1 : 0].charge((on0 ? (hi0 ? f1 : f0) : 1.0) * (on1 ? (hi1 ? f1 : f0) : 1.0));
-                }
             }
-        }
+        };
+        if (big) tile_loop(std::true_type{});
+        else tile_loop(std::false_type{});
     }
     lik_write<K, MODE>(ds, c, acc, ok, i, (int64_t)blockIdx.x, (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
 }
@@ -192,6 +206,7 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_gen(const DeviceDataset ds,
 #pragma unroll
     for (int a = 0; a < NA; ++a) acc[a].open();
 
+    const int lane = threadIdx.x & 63;
     const int64_t r0 = (int64_t)blockIdx.x * tiles_per_range;
     const int64_t r1 = min(ds.ngen, r0 + tiles_per_range);
     for (int64_t r = r0; r < r1; ++r) {
@@ -200,30 +215,45 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_gen(const DeviceDataset ds,
         const int sh = (ii - w * td.ipw) * td.bits;
         const uint32_t mask = (1u << td.bits) - 1u;
         const uint32_t* wp = ds.words + td.wbase + (int64_t)w * kTileLoci;
-        for (int j = 0; j < td.nloc; ++j) {
-            const int64_t l = ds.tloc[td.lmap + j];
-            const uint32_t gid = (wp[j] >> sh) & mask;
-            const uint64_t d64 = ds.gdesc[ds.gbase[l] + gid];
-            const int v = ok ? (int)(d64 & 15u) : 0;
-            const double* pl = c.Freq + ds.afo[l];
-            for (int a = 0; a < v; ++a) {
-                const double* px = pl + (int)((d64 >> (4 + 6 * a)) & 63u);
-                if (MODE == 3) {
+        // the tile's per-locus fields, lane j holding locus j
+        const int64_t lj = ds.tloc[td.lmap + lane];
+        const int afo_v = ds.afo[lj];
+        const int64_t gb_v = ds.gbase[lj];
+        // ids of 4 loci per 16-byte load, the next group's issued first
+        uint4 wn = *reinterpret_cast<const uint4*>(wp);
+        for (int j0 = 0; j0 < td.nloc; j0 += 4) {
+            const uint4 w4 = wn;
+            if (j0 + 4 < td.nloc) wn = *reinterpret_cast<const uint4*>(wp + j0 + 4);
+            const uint32_t wq[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
-                    for (int k = 0; k < K; ++k) acc[k].charge(px[k * KT]);
-                } else {
-                    double f = 0, g = 0;
+            for (int u = 0; u < 4; ++u) {
+                const int j = j0 + u;
+                if (j >= td.nloc) break;
+                const int64_t gb = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(gb_v >> 32), j) << 32) |
+                                             (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)gb_v, j));
+                const uint32_t gid = (wq[u] >> sh) & mask;
+                const uint64_t d64 = ds.gdesc[gb + gid];
+                const int v = ok ? (int)(d64 & 15u) : 0;
+                const double* pl = c.Freq + __builtin_amdgcn_readlane(afo_v, j);
+                for (int a = 0; a < v; ++a) {
+                    const double* px = pl + (int)((d64 >> (4 + 6 * a)) & 63u);
+                    if (MODE == 3) {
 #pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        const double p = px[k * KT];
-                        f = f + p * q[k];
-                        if (MODE == 2) g = g + p * qp[k];
-                    }
-                    if (MODE == 2) {
-                        acc[0].charge(g);
-                        acc[1].charge(f);
+                        for (int k = 0; k < K; ++k) acc[k].charge(px[k * KT]);
                     } else {
-                        acc[0].charge(f);
+                        double f = 0, g = 0;
+#pragma unroll
+                        for (int k = 0; k < K; ++k) {
+                            const double p = px[k * KT];
+                            f = f + p * q[k];
+                            if (MODE == 2) g = g + p * qp[k];
+                        }
+                        if (MODE == 2) {
+                            acc[0].charge(g);
+                            acc[1].charge(f);
+                        } else {
+                            acc[0].charge(f);
+                        }
                     }
                 }
             }
@@ -341,6 +371,13 @@ int launch_lik_mode(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, 
 }
 
 }  // namespace
+
+int launch_reduce_lnl(const double* part, int64_t n, double* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce_lnl, dim3(1), dim3(256), 0, st, part, n, out);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
 
 int lik_part_count(const DeviceDataset& ds, int mode, int K)
 {

@@ -168,17 +168,23 @@ int tune_chunks(vps_chain* ch)
     if (rc) return rc;
     VPS_HIP(hipStreamSynchronize(ch->stream));
     if (bestG == c.G) return VPS_OK;
-    // swap the partial buffer for the chosen chunking
-    auto it = std::find(ch->allocs.begin(), ch->allocs.end(), (void*)c.Ni_part);
-    if (it != ch->allocs.end()) {
-        ch->device_bytes -= (int64_t)c.G * c.K * c.KT * sizeof(int32_t);
+    // swap the per-chunk buffers for the chosen chunking
+    auto drop = [&](void* p, int64_t bytes) -> int {
+        auto it = std::find(ch->allocs.begin(), ch->allocs.end(), p);
+        if (it == ch->allocs.end()) return VPS_OK;
+        ch->device_bytes -= bytes;
         ch->allocs.erase(it);
-        VPS_HIP(hipFreeAsync(c.Ni_part, ch->stream));
-    }
+        VPS_HIP(hipFreeAsync(p, ch->stream));
+        return VPS_OK;
+    };
+    rc = drop(c.Ni_part, (int64_t)c.G * c.K * c.KT * sizeof(int32_t));
+    if (!rc) rc = drop(c.zlik, std::max<int64_t>(z_lik_parts(d, c.G), 1) * sizeof(double));
+    if (rc) return rc;
     c.Nc = (N + bestG - 1) / bestG;
     c.G = (N + c.Nc - 1) / c.Nc;
     rc = dataset_jump(ch->ds, c.Nc, &c.jump);
     if (!rc) rc = chain_alloc(ch, &c.Ni_part, (int64_t)c.G * c.K * c.KT);
+    if (!rc) rc = chain_alloc(ch, &c.zlik, z_lik_parts(d, c.G));
     return rc;
 }
 
@@ -255,6 +261,7 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
         }
     }
     if (!rc) rc = chain_alloc(ch, &c.Ni_part, (int64_t)c.G * K * KT);
+    if (!rc) rc = chain_alloc(ch, &c.zlik, z_lik_parts(d, c.G));
     if (!rc) rc = chain_alloc(ch, &c.Mi, N * K);
     if (!rc) rc = chain_alloc(ch, &c.Alpha, K);
     if (!rc) rc = chain_alloc(ch, &c.Lambda, K);
@@ -377,7 +384,7 @@ int do_q_metro(vps_chain* ch)
     return rc;
 }
 
-int do_z_admix(vps_chain* ch, bool clear_mi = true)
+int do_z_admix(vps_chain* ch, bool clear_mi = true, bool rec = false)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ch->timing && (ch->timing_every <= 1 || ch->timed_calls++ % ch->timing_every == 0)) {
@@ -392,7 +399,7 @@ int do_z_admix(vps_chain* ch, bool clear_mi = true)
         e1 = ch->evs[ch->ev_used + 1];
         ch->ev_used += 2;
     }
-    return launch_z_admix(ch->ds->d, ch->c, ch->stream, e0, e1, clear_mi);
+    return launch_z_admix(ch->ds->d, ch->c, ch->stream, e0, e1, clear_mi, rec);
 }
 
 int do_z_noadmix(vps_chain* ch)
@@ -459,13 +466,16 @@ int launch_sweep(vps_chain* ch, const SweepPlan& p)
     if (!rc) rc = launch_update_lambda(d, c, side);
     if (!rc) rc = launch_update_a(d, c, side);
     if (!rc) rc = launch_update_f(d, c, side);
-    if (!rc) rc = p.q == kQNoAdmix ? do_z_noadmix(ch) : do_z_admix(ch, !q_beside);
+    // Record's admixture lnL comes from the FP64 Z pass itself: its weights
+    // start from the same products P[k][l][x] Q[i][k] (the same P and Q)
+    const bool zrec = p.record && p.q != kQNoAdmix && c.precision != VPS_PREC_MIXED;
+    if (!rc) rc = p.q == kQNoAdmix ? do_z_noadmix(ch) : do_z_admix(ch, !q_beside, zrec);
     if (!rc && side != ch->stream) {
         VPS_HIP(hipEventRecord(ch->ev_join, side));
         VPS_HIP(hipStreamWaitEvent(ch->stream, ch->ev_join, 0));
     }
     if (!rc && p.record) {
-        rc = launch_lik_pass(d, c, p.q == kQNoAdmix ? 1 : 0, ch->stream);
+        if (!zrec) rc = launch_lik_pass(d, c, p.q == kQNoAdmix ? 1 : 0, ch->stream);
         if (!rc) rc = launch_record_accumulate(d, c, p.post, p.trace, 1, ch->stream);
     }
     if (!rc) rc = launch_advance_m(c, ch->stream);

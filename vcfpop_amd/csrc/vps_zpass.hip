@@ -46,16 +46,43 @@ __device__ __forceinline__ void jump_apply(const uint32_t* __restrict__ tab, Xs1
     r.y = (uint64_t)a2 | ((uint64_t)a3 << 32);
 }
 
-template <typename Real> __device__ __forceinline__ Real unit_minus1(uint64_t r);
-template <> __device__ __forceinline__ double unit_minus1<double>(uint64_t r) { return unit12(r) - 1.0; }
-// FP32 weights: U keeps the 23 high bits of the 52-bit mantissa
-template <> __device__ __forceinline__ float unit_minus1<float>(uint64_t r)
+// U - 1 for U in [1, 2) built from a draw.  The exponent word is set with
+// one bitfield insert whose mask and exponent bits sit in registers (a
+// 3-operand VALU instruction takes no literal here); the registers are set
+// once per kernel, outside the loops.
+struct UnitBits {
+    uint32_t m52, one64, m23, one32;
+    __device__ __forceinline__ void init()
+    {
+        asm("s_mov_b32 %0, 0xfffff" : "=s"(m52));
+        asm("v_mov_b32 %0, 0x3ff00000" : "=v"(one64));
+        asm("s_mov_b32 %0, 0x7fffff" : "=s"(m23));
+        asm("v_mov_b32 %0, 0x3f800000" : "=v"(one32));
+    }
+};
+__device__ __forceinline__ uint32_t bfi(uint32_t mask, uint32_t a, uint32_t b)
 {
-    const uint32_t m = (uint32_t)(r >> 29) & 0x007FFFFFu;
-    return __builtin_bit_cast(float, m | 0x3F800000u) - 1.0f;
+    uint32_t d;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(d) : "s"(mask), "v"(a), "v"(b));
+    return d;
+}
+
+template <typename Real> __device__ __forceinline__ Real unit_minus1(uint64_t r, const UnitBits& ub);
+// FP64 weights: U keeps the 52 low bits of the draw
+template <> __device__ __forceinline__ double unit_minus1<double>(uint64_t r, const UnitBits& ub)
+{
+    const uint32_t hi = bfi(ub.m52, (uint32_t)(r >> 32), ub.one64);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint32_t)r) - 1.0;
+}
+// FP32 weights: U keeps the 23 high bits of the 52-bit mantissa
+template <> __device__ __forceinline__ float unit_minus1<float>(uint64_t r, const UnitBits& ub)
+{
+    return __builtin_bit_cast(float, bfi(ub.m23, (uint32_t)(r >> 29), ub.one32)) - 1.0f;
 }
 
 __device__ __forceinline__ uint64_t ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
+
+constexpr int kWavesMax = 4;   // wavefronts per block of the Z kernels
 
 // Flush the K per-individual cluster counts of one wavefront (cumulative
 // form, wave-uniform) into the block's LDS row, two counts per 32-bit word
@@ -114,8 +141,8 @@ __device__ __forceinline__ uint32_t slot_flags(uint32_t e)
 
 // one wavefront's chunk of one BI tile; BIG: ids of more than 3 bits, the
 // genotype bytes come from the lane's LDS column `tab_lds`
-template <int K, typename Real, int MPT, bool BIG>
-__device__ __forceinline__ void z_bi_tile(const DeviceDataset& ds, const TileDesc& t, const double* __restrict__ Q,
+template <int K, typename Real, int MPT, bool BIG, bool REC>
+__device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileDesc& t, const double* __restrict__ Q,
                                           uint64_t seed, int m, int64_t l, bool valid, int afo, int k2, int lane,
                                           int c, int i0, int i1, const Real* pl0, uint8_t* tab_lds,
                                           uint32_t* mi_lds, int32_t* __restrict__ Ni_part,
@@ -159,6 +186,12 @@ __device__ __forceinline__ void z_bi_tile(const DeviceDataset& ds, const TileDes
     const uint32_t* wp = ds.words + t.wbase + lane;
     uint32_t word = wp[(int64_t)w * kTileLoci] >> (j * bits);
     const int MP = MPT > 0 ? MPT : ds.maxploidy;
+    UnitBits ub;
+    ub.init();
+    // REC: the lane's Record lnL, log f over its typed copies with
+    // f = sum_k P[k][l][x] Q[i][k] (the products the weights start from)
+    FrexpAcc acc;
+    acc.open();
 
     for (int i = i0; i < i1; ++i) {
         const uint32_t gid = word & mask;
@@ -177,6 +210,7 @@ __device__ __forceinline__ void z_bi_tile(const DeviceDataset& ds, const TileDes
 #pragma unroll
         for (int k = 0; k < K; ++k) cum[k] = 0;
 
+        double pr = 1.0;   // REC, diploid: the individual's two factors, charged as one pair
         constexpr int kSlotUnroll = MPT > 0 ? MPT : 1;
 #pragma unroll kSlotUnroll
         for (int a = 0; a < MP; ++a) {
@@ -190,12 +224,22 @@ __device__ __forceinline__ void z_bi_tile(const DeviceDataset& ds, const TileDes
             const uint32_t al1b = (MPT == 2 || MPT == 4) ? ((desc >> (4 + a)) & 1u) : (a >= h ? 1u : 0u);
             const Real* pa = pl0 + al1b * kTileLoci;
             Real wk[K];
+            Real f = 0;
 #pragma unroll
-            for (int k = 0; k < K; ++k) wk[k] = pa[k * 2 * kTileLoci] * q[k] + eps;
+            for (int k = 0; k < K; ++k) {
+                const Real pq = pa[k * 2 * kTileLoci] * q[k];
+                wk[k] = pq + eps;
+                if (REC) f = k ? f + pq : pq;
+            }
+            if (REC) {
+                const double fc = typed ? (double)f : 1.0;
+                if (MPT == 2) pr = pr * fc;
+                else acc.charge(fc);
+            }
             Real s = wk[0];
 #pragma unroll
             for (int k = 1; k < K; ++k) s = s + wk[k];
-            Real tt = unit_minus1<Real>(r) * s;
+            Real tt = unit_minus1<Real>(r, ub) * s;
             const uint32_t inc = (al1b << 16) | 1u;
 #pragma unroll
             for (int k = 0; k < K - 1; ++k) {
@@ -206,6 +250,7 @@ __device__ __forceinline__ void z_bi_tile(const DeviceDataset& ds, const TileDes
             }
             cum[K - 1] += __builtin_popcountll(tmask);
         }
+        if (REC && MPT == 2) acc.charge(pr);
         flush_mi<K>(mi_lds + (i - i0) * KH, cum, lane, mval);
     }
 
@@ -221,6 +266,15 @@ __device__ __forceinline__ void z_bi_tile(const DeviceDataset& ds, const TileDes
             if (k2 > 1) dst[k * KT + 1] = one;
         }
     }
+    return REC ? acc.close() : 0.0;
+}
+
+// fixed-order sum over the wavefront (every lane ends with the sum)
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
 // LDS of the BI kernel: [wave][k][allele][lane] P values, then
@@ -230,12 +284,15 @@ template <int K, typename Real>
 __host__ __device__ constexpr size_t bi_p_bytes(int nwv) { return (size_t)nwv * K * 2 * kTileLoci * sizeof(Real); }
 __host__ __device__ inline size_t bi_tab_bytes(int nwv, int rows) { return (size_t)nwv * rows * kTileLoci; }
 
-template <int K, typename Real, int MPT>
+// REC: each block also writes the Record lnL partial of its lanes to
+// zlik[blockIdx.y * gridDim.x + blockIdx.x]
+template <int K, typename Real, int MPT, bool REC>
 __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, const double* __restrict__ Freq,
                                                    const double* __restrict__ Q, uint64_t seed,
                                                    const int32_t* __restrict__ mp,
                                                    int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
-                                                   int Nc, const uint32_t* __restrict__ jump)
+                                                   int Nc, const uint32_t* __restrict__ jump,
+                                                   double* __restrict__ zlik)
 {
     const int m = *mp;
     extern __shared__ int32_t lds_i32[];
@@ -277,15 +334,26 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
     }
     __syncthreads();
 
+    double lnl = 0.0;
     if (has_tile) {
         if (t.bits > kBiRegBits)
-            z_bi_tile<K, Real, MPT, true>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane, tab_lds,
-                                          mi_lds, Ni_part, jump);
+            lnl = z_bi_tile<K, Real, MPT, true, REC>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane,
+                                                    tab_lds, mi_lds, Ni_part, jump);
         else
-            z_bi_tile<K, Real, MPT, false>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane, tab_lds,
-                                           mi_lds, Ni_part, jump);
+            lnl = z_bi_tile<K, Real, MPT, false, REC>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane,
+                                                     tab_lds, mi_lds, Ni_part, jump);
+    }
+    __shared__ double wsum[kWavesMax];
+    if (REC) {
+        lnl = wave_sum(lnl);
+        if (lane == 0) wsum[wv] = lnl;
     }
     __syncthreads();
+    if (REC && threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < nwv; ++w) s += wsum[w];
+        zlik[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = s;
+    }
     flush_mi_rows<K>(mi_lds, nci, Mi + (int64_t)i0 * K);
 }
 
@@ -305,13 +373,17 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
 // block's individuals.  The block's counts go to the partial of its first
 // chunk; the partials of its other chunks are written as zeros.
 // ---------------------------------------------------------------------------
-template <int K, typename Real, bool PLDS>
+//
+// REC: each block also writes the Record lnL partial of its lanes to
+// zlik[zbase + blockIdx.y * gridDim.x + blockIdx.x].
+template <int K, typename Real, bool PLDS, bool REC>
 __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, const double* __restrict__ Freq,
                                                     const double* __restrict__ Q, uint64_t seed,
                                                     const int32_t* __restrict__ mp,
                                                     int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
                                                     int A, int ni_in_lds, int Nc, int G,
-                                                    const uint32_t* __restrict__ jump)
+                                                    const uint32_t* __restrict__ jump, double* __restrict__ zlik,
+                                                    int64_t zbase)
 {
     const int m = *mp;
     extern __shared__ int32_t lds[];
@@ -350,6 +422,7 @@ __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, con
     }
     __syncthreads();
 
+    double lnl = 0.0;   // REC: the lane's Record lnL over its chunk
     if (i0 < i1) {
         const uint64_t* gd = ds.gdesc + ds.gbase[l];
         Xs128 rng;
@@ -377,6 +450,10 @@ __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, con
         int32_t* nig = Ni_part + (int64_t)c * K * KT + afo;
         uint32_t mval = 0;
 
+        UnitBits ub;
+        ub.init();
+        FrexpAcc acc;
+        acc.open();
         uint64_t desc = valid ? gd[take(i0)] : 0ull;
         for (int i = i0; i < i1; ++i) {
             const uint64_t cur = desc;
@@ -400,18 +477,19 @@ __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, con
                 const int al = a < VPS_MAX_PLOIDY ? (int)((cur >> (4 + 6 * a)) & 63u) : 0;
                 const int x = typed ? al : 0;
                 Real wk[K];
-                if (PLDS) {
-                    const Real* px = p_lds + x * kTileLoci + lane;
+                Real f = 0;
+                const Real* px = p_lds + x * kTileLoci + lane;
 #pragma unroll
-                    for (int k = 0; k < K; ++k) wk[k] = px[k * A * kTileLoci] * q[k] + eps;
-                } else {
-#pragma unroll
-                    for (int k = 0; k < K; ++k) wk[k] = (Real)pl[k * KT + x] * q[k] + eps;
+                for (int k = 0; k < K; ++k) {
+                    const Real pq = (PLDS ? px[k * A * kTileLoci] : (Real)pl[k * KT + x]) * q[k];
+                    wk[k] = pq + eps;
+                    if (REC) f = k ? f + pq : pq;
                 }
+                if (REC) acc.charge(typed ? (double)f : 1.0);
                 Real s = wk[0];
 #pragma unroll
                 for (int k = 1; k < K; ++k) s = s + wk[k];
-                Real tt = unit_minus1<Real>(r) * s;
+                Real tt = unit_minus1<Real>(r, ub) * s;
                 int nneg = 0;
 #pragma unroll
                 for (int k = 0; k < K - 1; ++k) {
@@ -430,8 +508,19 @@ __global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, con
             }
             flush_mi<K>(mi_lds + (i - gi0) * KH, cum, lane, mval);
         }
+        if (REC) lnl = acc.close();
+    }
+    __shared__ double wsum[kWavesMax];
+    if (REC) {
+        lnl = wave_sum(lnl);
+        if (lane == 0) wsum[wv] = lnl;
     }
     __syncthreads();
+    if (REC && threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < nwv; ++w) s += wsum[w];
+        zlik[zbase + (int64_t)blockIdx.y * gridDim.x + blockIdx.x] = s;
+    }
     if (ni_in_lds && valid) {
         for (int k = wv; k < K - 1; k += nwv) {
             int32_t* dst = Ni_part + ((int64_t)c0 * K + k) * KT + afo;
@@ -607,8 +696,9 @@ GenPlan gen_plan(const DeviceDataset& ds, int Nc, int G)
     return {wv, false, false, mi};
 }
 
-template <int K, typename Real>
-int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
+// nparts (may be null): the number of Record partials written
+template <int K, typename Real, bool REC>
+int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, int64_t* nparts)
 {
     const int nthr = 64 * kWavesPerBlock;
     const int A = std::max(ds.maxK, 1);
@@ -620,27 +710,34 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
         int rc = launch_fill_i32(c.Ni_part, (int64_t)c.G * K * ds.KT, 0, st);
         if (rc) return rc;
     }
+    int64_t zbase = 0;   // Record partials: the BI blocks', then the GEN blocks'
     if (ds.nbi > 0) {
         dim3 grid((unsigned)((ds.nbi + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)c.G);
-        const size_t p_bytes = bi_p_bytes<K, Real>(kWavesPerBlock) + bi_tab_bytes(kWavesPerBlock, ds.bi_tab_rows);
+        zbase = (int64_t)grid.x * grid.y;
+        const size_t shm = bi_p_bytes<K, Real>(kWavesPerBlock) + bi_tab_bytes(kWavesPerBlock, ds.bi_tab_rows) + mi_packed;
         if (ds.maxploidy == 2)
-            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 2>), grid, dim3(nthr), p_bytes + mi_packed, st, ds, c.Freq, c.Q, c.seed, c.mdev, c.Ni_part, c.Mi, c.Nc, c.jump);
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 2, REC>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
+                               c.Ni_part, c.Mi, c.Nc, c.jump, c.zlik);
         else if (ds.maxploidy == 4)
-            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 4>), grid, dim3(nthr), p_bytes + mi_packed, st, ds, c.Freq, c.Q, c.seed, c.mdev, c.Ni_part, c.Mi, c.Nc, c.jump);
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 4, REC>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
+                               c.Ni_part, c.Mi, c.Nc, c.jump, c.zlik);
         else
-            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 0>), grid, dim3(nthr), p_bytes + mi_packed, st, ds, c.Freq, c.Q, c.seed, c.mdev, c.Ni_part, c.Mi, c.Nc, c.jump);
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 0, REC>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
+                               c.Ni_part, c.Mi, c.Nc, c.jump, c.zlik);
         VPS_HIP(hipGetLastError());
     }
     if (ds.ngen > 0) {
         dim3 grid((unsigned)ds.ngen, (unsigned)((c.G + g.waves - 1) / g.waves));
         if (g.plds)
-            hipLaunchKernelGGL((k_z_admix_gen<K, Real, true>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q, c.seed,
-                               c.mdev, c.Ni_part, c.Mi, A, 1, c.Nc, c.G, c.jump);
+            hipLaunchKernelGGL((k_z_admix_gen<K, Real, true, REC>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q,
+                               c.seed, c.mdev, c.Ni_part, c.Mi, A, 1, c.Nc, c.G, c.jump, c.zlik, zbase);
         else
-            hipLaunchKernelGGL((k_z_admix_gen<K, Real, false>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q, c.seed,
-                               c.mdev, c.Ni_part, c.Mi, A, g.ni_in_lds ? 1 : 0, c.Nc, c.G, c.jump);
+            hipLaunchKernelGGL((k_z_admix_gen<K, Real, false, REC>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q,
+                               c.seed, c.mdev, c.Ni_part, c.Mi, A, g.ni_in_lds ? 1 : 0, c.Nc, c.G, c.jump, c.zlik, zbase);
         VPS_HIP(hipGetLastError());
+        zbase += (int64_t)grid.x * grid.y;
     }
+    if (nparts) *nparts = zbase;
     return VPS_OK;
 }
 
@@ -654,16 +751,16 @@ int z_occupancy(const DeviceDataset& ds, int Nc, int* bi_blocks, int* gen_waves,
     *bi_blocks = 1;
     hipError_t e = hipSuccess;
     if (ds.maxploidy == 2)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 2>, 256, p_bytes + mi_packed);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 2, false>, 256, p_bytes + mi_packed);
     else if (ds.maxploidy == 4)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 4>, 256, p_bytes + mi_packed);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 4, false>, 256, p_bytes + mi_packed);
     else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 0>, 256, p_bytes + mi_packed);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 0, false>, 256, p_bytes + mi_packed);
     if (e != hipSuccess) return fail(VPS_ERR_DEVICE, "occupancy query: %s", hipGetErrorString(e));
     const GenPlan g = gen_plan<K, Real>(ds, Nc, (ds.N + Nc - 1) / Nc);
     int gb = 1;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&gb, g.plds ? (const void*)k_z_admix_gen<K, Real, true>
-                                                                 : (const void*)k_z_admix_gen<K, Real, false>,
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&gb, g.plds ? (const void*)k_z_admix_gen<K, Real, true, false>
+                                                                 : (const void*)k_z_admix_gen<K, Real, false, false>,
                                                      64 * g.waves, g.shm);
     if (e != hipSuccess) return fail(VPS_ERR_DEVICE, "occupancy query: %s", hipGetErrorString(e));
     *bi_blocks = std::max(*bi_blocks, 1);
@@ -684,11 +781,11 @@ int z_occupancy_k(const DeviceDataset& ds, int K, int Nc, int* b, int* g, int* g
     }
 }
 
-template <typename Real>
-int dispatch_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
+template <typename Real, bool REC>
+int dispatch_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, int64_t* nparts = nullptr)
 {
     switch (c.K) {
-#define VPS_CASE(k) case k: return launch_z_admix_k<k, Real>(ds, c, st);
+#define VPS_CASE(k) case k: return launch_z_admix_k<k, Real, REC>(ds, c, st, nparts);
         VPS_CASE(1) VPS_CASE(2) VPS_CASE(3) VPS_CASE(4) VPS_CASE(5) VPS_CASE(6) VPS_CASE(7) VPS_CASE(8)
         VPS_CASE(9) VPS_CASE(10) VPS_CASE(11) VPS_CASE(12) VPS_CASE(13) VPS_CASE(14) VPS_CASE(15) VPS_CASE(16)
 #undef VPS_CASE
@@ -759,20 +856,33 @@ int launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st)
 int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     int rc = launch_fill_i32(c.Mi, (int64_t)ds.N * c.K, 0, st);
-    if (!rc) rc = c.precision == VPS_PREC_MIXED ? dispatch_k<float>(ds, c, st) : dispatch_k<double>(ds, c, st);
+    if (!rc) rc = c.precision == VPS_PREC_MIXED ? dispatch_k<float, false>(ds, c, st) : dispatch_k<double, false>(ds, c, st);
     if (!rc) rc = launch_reduce_ni(ds, c, st);
     return rc;
 }
 
-int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, bool clear_mi)
+int64_t z_lik_parts(const DeviceDataset& ds, int G)
 {
+    // BI blocks: 4 tiles x 1 chunk; GEN blocks: 1 tile x 1..4 chunks
+    return (ds.nbi + kWavesPerBlock - 1) / kWavesPerBlock * G + ds.ngen * (int64_t)G;
+}
+
+int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, bool clear_mi,
+                   bool rec)
+{
+    if (rec && c.precision == VPS_PREC_MIXED) return fail(VPS_ERR_INVALID, "the Record lnL of the Z pass needs FP64 weights");
     int rc = clear_mi ? launch_fill_i32(c.Mi, (int64_t)ds.N * c.K, 0, st) : VPS_OK;
     if (rc) return rc;
     if (ev0) VPS_HIP(hipEventRecord(ev0, st));
-    rc = c.precision == VPS_PREC_MIXED ? dispatch_k<float>(ds, c, st) : dispatch_k<double>(ds, c, st);
+    int64_t nparts = 0;
+    rc = c.precision == VPS_PREC_MIXED ? dispatch_k<float, false>(ds, c, st)
+         : rec                         ? dispatch_k<double, true>(ds, c, st, &nparts)
+                                       : dispatch_k<double, false>(ds, c, st);
     if (ev1) VPS_HIP(hipEventRecord(ev1, st));
     if (rc) return rc;
-    return launch_reduce_ni(ds, c, st);
+    rc = launch_reduce_ni(ds, c, st);
+    if (!rc && rec) rc = launch_reduce_lnl(c.zlik, nparts, c.scal + 1, st);
+    return rc;
 }
 
 int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
