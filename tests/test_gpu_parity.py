@@ -268,6 +268,8 @@ SWEEP_MODELS = {
     "fmodel_parallel": dict(admix_val=1, f_val=1, rng_topology=1),
     "fmodel_single": dict(admix_val=1, f_val=1, singlef_val=1),
     "lambda": dict(admix_val=1, inferlambda_val=0, difflambda_val=1),
+    "lambda_parallel": dict(admix_val=1, inferlambda_val=0, difflambda_val=1, rng_topology=1),
+    "locprior_admix_parallel": dict(admix_val=1, locpriori_val=1, rng_topology=1),
 }
 
 FED = ["Freq", "Q", "Z", "Ni", "Mi", "Alpha", "Lambda", "loc", "Fst", "FreqA", "binaryq", "MiSum", "FreqM", "rout"]

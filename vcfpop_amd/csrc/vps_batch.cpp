@@ -164,6 +164,7 @@ extern "C" int vps_batch_step(vps_batch* b, int n_sweeps)
                 vps_chain* ch = b->chains[i];
                 VPS_HIP(hipStreamWaitEvent(ch->stream, b->fork, 0));
                 rc = launch_sweep(ch, plans[i]);
+                if (!rc) rc = settle_side(ch);
                 if (!rc) {
                     VPS_HIP(hipEventRecord(b->join[i], ch->stream));
                     VPS_HIP(hipStreamWaitEvent(b->root, b->join[i], 0));

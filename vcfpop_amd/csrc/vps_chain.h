@@ -63,6 +63,7 @@ struct ChainDev {
     Rng*      rngbuf;    // [N] streams kept between a draw kernel and its acceptance kernel
     int32_t*  mdev;      // [1] sweep index m; kernels read it, the sweep's last kernel advances it,
                          // so one captured sweep replays at any m
+    int32_t*  mside;     // [1] copy of m for the kernels beside the Z pass (written by UpdateP/Q)
     // individual chunking of the genotype passes: G chunks of Nc individuals;
     // jump[c] advances a locus stream to individual c*Nc
     int32_t   G, Nc;
@@ -75,21 +76,24 @@ constexpr int kLikLociPerWave = 1024;
 // launchers (vps_kernels.hip / vps_updates.hip); all asynchronous on `st`
 // clear_mi = false: the caller has already zeroed Mi on an ordered stream
 // rec = true: the FP64 Z pass also sums the Record lnL (mode 0) into c.scal[1]
+// advance = true: the pass's last kernel also advances the sweep index
 int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1,
-                   bool clear_mi = true, bool rec = false);
+                   bool clear_mi = true, bool rec = false, bool advance = false);
 // number of Record partials the Z pass writes with G chunks
 int64_t z_lik_parts(const DeviceDataset& ds, int G);
 // out = the fixed-order sum of n partials (one block)
 int launch_reduce_lnl(const double* part, int64_t n, double* out, hipStream_t st);
 int launch_z_noadmix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
-int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, bool advance = false);
 // likelihood passes: mode 0 = Record lnL (admix), 1 = Record lnL (no admix),
 // 2 = Metropolis log ratio per individual, 3 = no-admix per-(i,k) log likelihood
 int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c, int mode, hipStream_t st);
 int lik_part_count(const DeviceDataset& ds, int mode, int K);
 
 int launch_update_p(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+// UpdateP + Dirichlet UpdateQ + Mi clear in one launch (parallel topology)
+int launch_update_pq(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 int launch_update_q_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 int launch_q_metro_propose(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 int launch_q_metro_accept(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);

@@ -122,6 +122,20 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
             // entries, so a group never leaves its row); the next group's load
             // is issued before this group's work
             uint4 wn = *reinterpret_cast<const uint4*>(wp);
+            // a locus's 2K frequencies (both alleles in one 16-byte scalar
+            // load; P is padded past KT), loaded one locus ahead: they are
+            // first used a locus later, so the wait for them covers loads
+            // issued a whole locus of work earlier
+            double x0n[K], x1n[K];
+            auto load_p = [&](int j) {
+                const double* pl = c.Freq + __builtin_amdgcn_readlane(afo_v, j);
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    x0n[k] = pl[k * KT];
+                    x1n[k] = pl[k * KT + 1];
+                }
+            };
+            load_p(0);
             for (int j0 = 0; j0 < td.nloc; j0 += 4) {
                 const uint4 w4 = wn;
                 if (j0 + 4 < td.nloc) wn = *reinterpret_cast<const uint4*>(wp + j0 + 4);
@@ -130,22 +144,20 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
                 for (int u = 0; u < 4; ++u) {
                     const int j = j0 + u;
                     if (j >= td.nloc) break;
+                    double p0[K], p1[K];
+                    const bool mono = __builtin_amdgcn_readlane(k2_v, j) < 2;   // monomorphic loci have no allele 1
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        p0[k] = x0n[k];
+                        p1[k] = mono ? x0n[k] : x1n[k];
+                    }
+                    if (j + 1 < td.nloc) load_p(j + 1);
                     const uint64_t tab = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(tab_v >> 32), j) << 32) |
                                          (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)tab_v, j);
                     const uint32_t gid = (wq[u] >> sh) & mask;
                     const uint32_t desc = BIG ? (uint32_t)btab[gid * kTileLoci + j] : (uint32_t)(tab >> (gid * 8)) & 0xFFu;
                     const int v = ok ? (int)(desc & 15u) : 0;
                     const int h = v - (int)(desc >> 4);    // copies a >= h carry allele 1
-                    const double* pl = c.Freq + __builtin_amdgcn_readlane(afo_v, j);
-                    const bool mono = __builtin_amdgcn_readlane(k2_v, j) < 2;   // monomorphic loci have no allele 1
-                    double p0[K], p1[K];
-#pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        // both alleles in one 16-byte scalar load (P is padded past KT)
-                        const double x0 = pl[k * KT], x1 = pl[k * KT + 1];
-                        p0[k] = x0;
-                        p1[k] = mono ? x0 : x1;
-                    }
                     // Copies are charged in pairs: the product of two factors is
                     // formed first.  Every factor is a frequency or a mixture of
                     // frequencies, above 1e-154 in any chain, so a pair cannot

@@ -103,8 +103,6 @@ void chain_free(vps_chain* ch)
     if (ch->aux) (void)hipStreamDestroy(ch->aux);
     if (ch->ev_fork) (void)hipEventDestroy(ch->ev_fork);
     if (ch->ev_join) (void)hipEventDestroy(ch->ev_join);
-    if (ch->ev_fork2) (void)hipEventDestroy(ch->ev_fork2);
-    if (ch->ev_join2) (void)hipEventDestroy(ch->ev_join2);
     if (ch->stream) (void)hipStreamDestroy(ch->stream);
     delete ch;
 }
@@ -211,8 +209,6 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ch->aux, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ch->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ch->ev_join, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&ch->ev_fork2, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&ch->ev_join2, hipEventDisableTiming);
     if (e != hipSuccess) { chain_free(ch); return fail(VPS_ERR_DEVICE, "chain streams: %s", hipGetErrorString(e)); }
 
     // ReadPar, src/structure.cpp:405-464
@@ -282,6 +278,7 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     if (!rc) rc = chain_alloc(ch, &c.lnltrace, ch->lnl_cap);
     if (!rc) rc = chain_alloc(ch, &c.rngbuf, N);
     if (!rc) rc = chain_alloc(ch, &c.mdev, 1);
+    if (!rc) rc = chain_alloc(ch, &c.mside, 1);
     if (!rc && hipMemsetAsync(c.mdev, 0, sizeof(int32_t), ch->stream) != hipSuccess) rc = fail(VPS_ERR_DEVICE, "clear sweep index");
     if (rc) { chain_free(ch); return rc; }
 
@@ -384,7 +381,7 @@ int do_q_metro(vps_chain* ch)
     return rc;
 }
 
-int do_z_admix(vps_chain* ch, bool clear_mi = true, bool rec = false)
+int do_z_admix(vps_chain* ch, bool clear_mi = true, bool rec = false, bool advance = false)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ch->timing && (ch->timing_every <= 1 || ch->timed_calls++ % ch->timing_every == 0)) {
@@ -399,7 +396,7 @@ int do_z_admix(vps_chain* ch, bool clear_mi = true, bool rec = false)
         e1 = ch->evs[ch->ev_used + 1];
         ch->ev_used += 2;
     }
-    return launch_z_admix(ch->ds->d, ch->c, ch->stream, e0, e1, clear_mi, rec);
+    return launch_z_admix(ch->ds->d, ch->c, ch->stream, e0, e1, clear_mi, rec, advance);
 }
 
 int do_z_noadmix(vps_chain* ch)
@@ -433,52 +430,68 @@ SweepPlan plan_sweep(const vps_chain* ch)
 
 // Every launch of sweep m, all on the chain's stream.  Kernels read m from
 // the device, and the last launch advances it.
+int settle_side(vps_chain* ch)
+{
+    if (!ch->side_pending) return VPS_OK;
+    ch->side_pending = false;
+    VPS_HIP(hipStreamWaitEvent(ch->stream, ch->ev_join, 0));
+    return VPS_OK;
+}
+
 int launch_sweep(vps_chain* ch, const SweepPlan& p)
 {
     const DeviceDataset& d = ch->ds->d;
     const ChainDev& c = ch->c;
-    int rc = VPS_OK;
-    // A Dirichlet UpdateQ reads Mi and Alpha, not P: it runs beside UpdateP on
-    // the second stream and then clears Mi for the Z pass
-    const bool q_beside = p.q == kQAdmix && !ch->capturing;
-    if (q_beside) {
-        VPS_HIP(hipEventRecord(ch->ev_fork2, ch->stream));
-        VPS_HIP(hipStreamWaitEvent(ch->aux, ch->ev_fork2, 0));
-        rc = launch_update_q_admix(d, c, ch->aux);
-        if (!rc) rc = launch_fill_i32(c.Mi, (int64_t)d.N * c.K, 0, ch->aux);
-        if (!rc) VPS_HIP(hipEventRecord(ch->ev_join2, ch->aux));
+    // the previous sweep's alpha/lambda/A/F: read by this sweep's UpdateQ and UpdateP
+    int rc = settle_side(ch);
+    if (rc) return rc;
+    // Uncaptured sweeps of the parallel topology with a Dirichlet UpdateQ:
+    // UpdateP, UpdateQ and the Mi clear in one launch, and the parameter
+    // updates beside the Z pass joined at the start of the next sweep, or
+    // before Record (they read the sweep index from a copy, so the Z pass's
+    // last kernel can advance it)
+    const bool fast = p.q == kQAdmix && !ch->capturing && c.topology == VPS_RNG_PARALLEL;
+    if (fast) {
+        rc = launch_update_pq(d, c, ch->stream);
+    } else {
+        rc = do_update_p(ch);
+        if (!rc) rc = p.q == kQNoAdmix ? do_q_noadmix(ch) : p.q == kQMetro ? do_q_metro(ch) : do_q_admix(ch);
     }
-    if (!rc) rc = do_update_p(ch);
-    if (!rc && q_beside) VPS_HIP(hipStreamWaitEvent(ch->stream, ch->ev_join2, 0));
-    if (!rc && !q_beside) rc = p.q == kQNoAdmix ? do_q_noadmix(ch) : p.q == kQMetro ? do_q_metro(ch) : do_q_admix(ch);
     if (!rc && !c.admix && c.locpriori) rc = launch_update_di(d, c, ch->stream);
     if (!rc) rc = launch_update_locpriori(d, c, ch->stream);
     // UpdateAlpha, UpdateLambda, UpdateA and UpdateF read Q and P and write
     // only their own parameters, none of which the Z pass reads or writes:
-    // they run beside it on the chain's second stream and join before Record
-    // (captured sweeps keep one stream: graph branches cost more than they hide)
+    // they run beside it on the chain's second stream (captured sweeps keep
+    // one stream: graph branches cost more than they hide)
     hipStream_t side = ch->capturing ? ch->stream : ch->aux;
+    ChainDev cs = c;
+    if (fast) cs.mdev = c.mside;
     if (!rc && side != ch->stream) {
         VPS_HIP(hipEventRecord(ch->ev_fork, ch->stream));
         VPS_HIP(hipStreamWaitEvent(side, ch->ev_fork, 0));
     }
-    if (!rc) rc = launch_update_alpha(d, c, side);
-    if (!rc) rc = launch_update_lambda(d, c, side);
-    if (!rc) rc = launch_update_a(d, c, side);
-    if (!rc) rc = launch_update_f(d, c, side);
+    if (!rc) rc = launch_update_alpha(d, cs, side);
+    if (!rc) rc = launch_update_lambda(d, cs, side);
+    if (!rc) rc = launch_update_a(d, cs, side);
+    if (!rc) rc = launch_update_f(d, cs, side);
     // Record's admixture lnL comes from the FP64 Z pass itself: its weights
     // start from the same products P[k][l][x] Q[i][k] (the same P and Q)
     const bool zrec = p.record && p.q != kQNoAdmix && c.precision != VPS_PREC_MIXED;
-    if (!rc) rc = p.q == kQNoAdmix ? do_z_noadmix(ch) : do_z_admix(ch, !q_beside, zrec);
+    // without Record, the Z pass's last kernel advances the sweep index when
+    // nothing beside it still reads the index
+    const bool zadv = !p.record && p.q != kQNoAdmix && (fast || side == ch->stream);
+    if (!rc) rc = p.q == kQNoAdmix ? do_z_noadmix(ch) : do_z_admix(ch, !fast, zrec, zadv);
+    // Record reads alpha, lambda and F: a Record sweep joins here
     if (!rc && side != ch->stream) {
         VPS_HIP(hipEventRecord(ch->ev_join, side));
-        VPS_HIP(hipStreamWaitEvent(ch->stream, ch->ev_join, 0));
+        if (fast && !p.record) ch->side_pending = true;
+        else VPS_HIP(hipStreamWaitEvent(ch->stream, ch->ev_join, 0));
     }
     if (!rc && p.record) {
         if (!zrec) rc = launch_lik_pass(d, c, p.q == kQNoAdmix ? 1 : 0, ch->stream);
         if (!rc) rc = launch_record_accumulate(d, c, p.post, p.trace, 1, ch->stream);
     }
-    if (!rc) rc = launch_advance_m(c, ch->stream);
+    if (!rc && !zadv) rc = launch_advance_m(c, ch->stream);
     return rc;
 }
 
@@ -554,7 +567,7 @@ extern "C" int vps_chain_step(vps_chain* ch, int n)
         int rc = do_sweep(ch);
         if (rc) return rc;
     }
-    return VPS_OK;
+    return settle_side(ch);   // the chain's stream is again the point of completion
 }
 
 extern "C" int vps_chain_sync(vps_chain* ch)

@@ -32,7 +32,8 @@ struct vps_chain {
     int64_t lnl_cap = 0;
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;                  // parameter updates beside the Z pass
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool side_pending = false;                  // the second stream's updates are not joined yet
     bool capturing = false;                     // launch_sweep is being captured into a graph
     std::vector<void*> allocs;
     int64_t device_bytes = 0;
@@ -49,6 +50,7 @@ namespace vps {
 
 SweepPlan plan_sweep(const vps_chain* ch);
 int launch_sweep(vps_chain* ch, const SweepPlan& p);   // every launch of the sweep, on ch->stream
+int settle_side(vps_chain* ch);                        // ch->stream waits for the second stream's updates
 void commit_sweep(vps_chain* ch, const SweepPlan& p);  // host bookkeeping after the launch
 
 }  // namespace vps

@@ -114,6 +114,45 @@ int launch_update_q_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_
     return VPS_OK;
 }
 
+// UpdateP and the Dirichlet UpdateQ of the parallel topology in one launch
+// (the two read disjoint state): blocks [0, nbq) draw Q, one individual per
+// thread, and then clear the individual's Mi row for the Z pass; the rest
+// draw P, one (k, l) per thread.  Thread 0 also copies the sweep index for
+// the kernels that run beside the Z pass.
+__global__ void k_update_pq(const DeviceDataset ds, const ChainDev c, unsigned nbq)
+{
+    const int m = *c.mdev;
+    if (blockIdx.x == 0 && threadIdx.x == 0) c.mside[0] = m;
+    if (blockIdx.x < nbq) {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= ds.N) return;
+        int32_t* mi = c.Mi + (int64_t)i * c.K;
+        if (ds.vt[i]) {
+            Rng r;
+            r.init(c.seed + (uint64_t)m * ds.N + i, kSaltUpdateQ);
+            dirichlet_ab(r, c.Q + (int64_t)i * c.K, alpha_row(ds, c, i), mi, c.K);
+        }
+        for (int k = 0; k < c.K; ++k) mi[k] = 0;
+        return;
+    }
+    const int64_t kl = (int64_t)(blockIdx.x - nbq) * blockDim.x + threadIdx.x;
+    if (kl >= (int64_t)c.K * ds.L) return;
+    const int64_t k = kl / ds.L, l = kl % ds.L;
+    Rng r;
+    r.init(c.seed + ((uint64_t)m * c.K + k) * (uint64_t)ds.L + l, kSaltUpdateP);
+    draw_p_one(ds, c, r, k, l);
+}
+
+int launch_update_pq(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
+{
+    if (c.topology != VPS_RNG_PARALLEL) return fail(VPS_ERR_INVALID, "the joint P/Q launch draws the parallel streams");
+    const unsigned nbq = (unsigned)((ds.N + 255) / 256);
+    const unsigned nbp = (unsigned)(((int64_t)c.K * ds.L + 255) / 256);
+    hipLaunchKernelGGL(k_update_pq, dim3(nbq + nbp), dim3(256), 0, st, ds, c, nbq);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
 // ---------------------------------------------------------------------------
 // UpdateQMetro, src/structure.cpp:1090-1136: proposals q'_i ~ Dir(alpha),
 // the log-ratio pass, then acceptance with conditional uniform draws from

@@ -609,10 +609,13 @@ __global__ __launch_bounds__(64) void k_z_noadmix(const DeviceDataset ds, const 
 // (the Z passes leave the last cluster's partials unwritten).  A block covers
 // 64 alleles x 4 cluster rows; the row totals meet in LDS in a fixed order.
 constexpr int kReduceRows = 4;
+// (advance, when not null: the sweep index, advanced once by the first thread)
 __global__ __launch_bounds__(64 * kReduceRows) void k_reduce_ni(const int32_t* __restrict__ Ni_part,
                                                                const int32_t* __restrict__ ncopies,
-                                                               int32_t* __restrict__ Ni, int64_t KT, int K, int G)
+                                                               int32_t* __restrict__ Ni, int64_t KT, int K, int G,
+                                                               int32_t* __restrict__ advance)
 {
+    if (advance && blockIdx.x == 0 && threadIdx.x == 0) advance[0] += 1;
     __shared__ int32_t tot[kReduceRows][64];
     const int al = threadIdx.x & 63, kk = threadIdx.x >> 6;
     const int64_t a = (int64_t)blockIdx.x * 64 + al;
@@ -868,8 +871,9 @@ int64_t z_lik_parts(const DeviceDataset& ds, int G)
 }
 
 int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, bool clear_mi,
-                   bool rec)
+                   bool rec, bool advance)
 {
+    if (rec && advance) return fail(VPS_ERR_INVALID, "Record reads the sweep index after the Z pass");
     if (rec && c.precision == VPS_PREC_MIXED) return fail(VPS_ERR_INVALID, "the Record lnL of the Z pass needs FP64 weights");
     int rc = clear_mi ? launch_fill_i32(c.Mi, (int64_t)ds.N * c.K, 0, st) : VPS_OK;
     if (rc) return rc;
@@ -880,15 +884,15 @@ int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, h
                                        : dispatch_k<double, false>(ds, c, st);
     if (ev1) VPS_HIP(hipEventRecord(ev1, st));
     if (rc) return rc;
-    rc = launch_reduce_ni(ds, c, st);
+    rc = launch_reduce_ni(ds, c, st, advance);
     if (!rc && rec) rc = launch_reduce_lnl(c.zlik, nparts, c.scal + 1, st);
     return rc;
 }
 
-int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
+int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, bool advance)
 {
     hipLaunchKernelGGL(k_reduce_ni, dim3((unsigned)((ds.KT + 63) / 64)), dim3(64 * kReduceRows), 0, st, c.Ni_part,
-                       ds.ncopies, c.Ni, ds.KT, c.K, c.G);
+                       ds.ncopies, c.Ni, ds.KT, c.K, c.G, advance ? c.mdev : nullptr);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
