@@ -68,6 +68,7 @@ struct ChainDev {
     // jump[c] advances a locus stream to individual c*Nc
     int32_t   G, Nc;
     const uint32_t* jump;
+    int32_t   lik_tpr[2];   // tiles per locus range of the likelihood passes (BI, GEN tiles)
 };
 
 // likelihood-pass partials: lanes = individuals, one partial per locus range
@@ -89,7 +90,10 @@ int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st,
 // likelihood passes: mode 0 = Record lnL (admix), 1 = Record lnL (no admix),
 // 2 = Metropolis log ratio per individual, 3 = no-admix per-(i,k) log likelihood
 int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c, int mode, hipStream_t st);
-int lik_part_count(const DeviceDataset& ds, int mode, int K);
+int lik_part_count(const DeviceDataset& ds, int mode, int K, const int32_t* tpr);
+// tiles per locus range of the likelihood passes for this device: [0]
+// biallelic tiles, [1] multiallelic tiles
+int choose_lik_tpr(const DeviceDataset& ds, int K, int device, int32_t* tpr);
 
 int launch_update_p(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 // UpdateP + Dirichlet UpdateQ + Mi clear in one launch (parallel topology)

@@ -30,7 +30,6 @@
 namespace vps {
 
 constexpr int kLikThreads = 256;
-constexpr int kLikTilesPerRange = 8;
 
 // modes 0/1: sum the block's 256 partials in a fixed tree
 __device__ void block_store_sum(double v, double* dst)
@@ -111,7 +110,6 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
         // load each instead of a chain of scalar loads per locus)
         const int64_t lj = ds.tloc[td.lmap + lane];
         const int afo_v = ds.afo[lj];
-        const int k2_v = ds.k2[lj];
         const uint64_t tab_v = ds.bitab[lj];
         // one copy of the loop per table kind: the register-table loop then
         // has no vector load besides the id prefetch to wait for
@@ -126,16 +124,17 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
             // load; P is padded past KT), loaded one locus ahead: they are
             // first used a locus later, so the wait for them covers loads
             // issued a whole locus of work earlier
-            double x0n[K], x1n[K];
-            auto load_p = [&](int j) {
+            // (two buffers, alternating by locus parity, so no copies)
+            double xa0[K], xa1[K], xb0[K], xb1[K];
+            auto load_p = [&](int j, double* d0, double* d1) {
                 const double* pl = c.Freq + __builtin_amdgcn_readlane(afo_v, j);
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    x0n[k] = pl[k * KT];
-                    x1n[k] = pl[k * KT + 1];
+                    d0[k] = pl[k * KT];
+                    d1[k] = pl[k * KT + 1];
                 }
             };
-            load_p(0);
+            load_p(0, xa0, xa1);
             for (int j0 = 0; j0 < td.nloc; j0 += 4) {
                 const uint4 w4 = wn;
                 if (j0 + 4 < td.nloc) wn = *reinterpret_cast<const uint4*>(wp + j0 + 4);
@@ -144,14 +143,14 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
                 for (int u = 0; u < 4; ++u) {
                     const int j = j0 + u;
                     if (j >= td.nloc) break;
-                    double p0[K], p1[K];
-                    const bool mono = __builtin_amdgcn_readlane(k2_v, j) < 2;   // monomorphic loci have no allele 1
-#pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        p0[k] = x0n[k];
-                        p1[k] = mono ? x0n[k] : x1n[k];
+                    // (on a monomorphic locus no copy carries allele 1, so the
+                    // second value, the next locus's, is never charged)
+                    const double* p0 = (u & 1) ? xb0 : xa0;
+                    const double* p1 = (u & 1) ? xb1 : xa1;
+                    if (j + 1 < td.nloc) {
+                        if (u & 1) load_p(j + 1, xa0, xa1);
+                        else load_p(j + 1, xb0, xb1);
                     }
-                    if (j + 1 < td.nloc) load_p(j + 1);
                     const uint64_t tab = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(tab_v >> 32), j) << 32) |
                                          (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)tab_v, j);
                     const uint32_t gid = (wq[u] >> sh) & mask;
@@ -339,11 +338,11 @@ struct LikGrid {
     int nr_bi, nr_gen, gy;
 };
 
-LikGrid lik_grid(const DeviceDataset& ds)
+LikGrid lik_grid(const DeviceDataset& ds, const int32_t* tpr)
 {
     LikGrid g;
-    g.nr_bi = (int)((ds.nbi + kLikTilesPerRange - 1) / kLikTilesPerRange);
-    g.nr_gen = (int)((ds.ngen + kLikTilesPerRange - 1) / kLikTilesPerRange);
+    g.nr_bi = (int)((ds.nbi + tpr[0] - 1) / tpr[0]);
+    g.nr_gen = (int)((ds.ngen + tpr[1] - 1) / tpr[1]);
     g.gy = (ds.N + kLikThreads - 1) / kLikThreads;
     return g;
 }
@@ -354,16 +353,16 @@ int launch_lik_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, con
     if (g.nr_bi > 0) {
         const dim3 grid((unsigned)g.nr_bi, (unsigned)g.gy);
         if (ds.maxploidy == 2)
-            hipLaunchKernelGGL((k_lik_bi<K, MODE, 2>), grid, dim3(kLikThreads), 0, st, ds, c, kLikTilesPerRange);
+            hipLaunchKernelGGL((k_lik_bi<K, MODE, 2>), grid, dim3(kLikThreads), 0, st, ds, c, c.lik_tpr[0]);
         else if (ds.maxploidy == 4)
-            hipLaunchKernelGGL((k_lik_bi<K, MODE, 4>), grid, dim3(kLikThreads), 0, st, ds, c, kLikTilesPerRange);
+            hipLaunchKernelGGL((k_lik_bi<K, MODE, 4>), grid, dim3(kLikThreads), 0, st, ds, c, c.lik_tpr[0]);
         else
-            hipLaunchKernelGGL((k_lik_bi<K, MODE, 0>), grid, dim3(kLikThreads), 0, st, ds, c, kLikTilesPerRange);
+            hipLaunchKernelGGL((k_lik_bi<K, MODE, 0>), grid, dim3(kLikThreads), 0, st, ds, c, c.lik_tpr[0]);
         VPS_HIP(hipGetLastError());
     }
     if (g.nr_gen > 0) {
         const dim3 grid((unsigned)g.nr_gen, (unsigned)g.gy);
-        hipLaunchKernelGGL((k_lik_gen<K, MODE>), grid, dim3(kLikThreads), 0, st, ds, c, kLikTilesPerRange, g.nr_bi,
+        hipLaunchKernelGGL((k_lik_gen<K, MODE>), grid, dim3(kLikThreads), 0, st, ds, c, c.lik_tpr[1], g.nr_bi,
                            (int64_t)g.nr_bi * g.gy);
         VPS_HIP(hipGetLastError());
     }
@@ -391,9 +390,62 @@ int launch_reduce_lnl(const double* part, int64_t n, double* out, hipStream_t st
     return VPS_OK;
 }
 
-int lik_part_count(const DeviceDataset& ds, int mode, int K)
+namespace {
+
+template <int K>
+int lik_blocks_per_cu(const DeviceDataset& ds, int* bi, int* gen)
 {
-    const LikGrid g = lik_grid(ds);
+    const void* f = ds.maxploidy == 2 ? (const void*)k_lik_bi<K, 2, 2>
+                    : ds.maxploidy == 4 ? (const void*)k_lik_bi<K, 2, 4> : (const void*)k_lik_bi<K, 2, 0>;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi, f, kLikThreads, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(gen, (const void*)k_lik_gen<K, 2>, kLikThreads, 0);
+    if (e != hipSuccess) return fail(VPS_ERR_DEVICE, "occupancy query: %s", hipGetErrorString(e));
+    *bi = std::max(*bi, 1);
+    *gen = std::max(*gen, 1);
+    return VPS_OK;
+}
+
+// tiles per range for one launch of `ntiles` equal tiles: the fewest, from 4
+// to 16, whose blocks all fit in one round of resident blocks (a block that
+// has to wait for a second round then runs alone, latency-bound); when no
+// such count exists, 8 (measured: many rounds of short blocks balance better
+// than one round of long ones)
+int tiles_per_range(int64_t ntiles, int64_t gy, int64_t cap)
+{
+    if (ntiles <= 0) return 1;
+    for (int t = 4; t <= 16; ++t)
+        if ((ntiles + t - 1) / t * gy <= cap) return (int)std::min<int64_t>(t, ntiles);
+    return (int)std::min<int64_t>(8, ntiles);
+}
+
+}  // namespace
+
+// Tiles per locus range of the likelihood passes, for the biallelic and the
+// multiallelic launch (blocks = one range x 256 individuals).  Resident
+// block counts are those of the Metropolis pass, the heaviest.
+int choose_lik_tpr(const DeviceDataset& ds, int K, int device, int32_t* tpr)
+{
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
+    int bpc = 1, gpc = 1, rc = VPS_OK;
+    switch (K) {
+#define VPS_CASE(k) case k: rc = lik_blocks_per_cu<k>(ds, &bpc, &gpc); break;
+        VPS_CASE(1) VPS_CASE(2) VPS_CASE(3) VPS_CASE(4) VPS_CASE(5) VPS_CASE(6) VPS_CASE(7) VPS_CASE(8)
+        VPS_CASE(9) VPS_CASE(10) VPS_CASE(11) VPS_CASE(12) VPS_CASE(13) VPS_CASE(14) VPS_CASE(15) VPS_CASE(16)
+#undef VPS_CASE
+    default: return fail(VPS_ERR_UNSUPPORTED, "K=%d outside 1..%d", K, VPS_MAX_K);
+    }
+    if (rc) return rc;
+    const int64_t gy = (ds.N + kLikThreads - 1) / kLikThreads;
+    tpr[0] = tiles_per_range(ds.nbi, gy, (int64_t)cus * bpc);
+    tpr[1] = tiles_per_range(ds.ngen, gy, (int64_t)cus * gpc);
+    return VPS_OK;
+}
+
+int lik_part_count(const DeviceDataset& ds, int mode, int K, const int32_t* tpr)
+{
+    const LikGrid g = lik_grid(ds, tpr);
     const int64_t nr = (int64_t)g.nr_bi + g.nr_gen;
     if (mode <= 1) return (int)(nr * g.gy);
     return (int)(nr * ds.N * (mode == 3 ? K : 1));
@@ -401,7 +453,7 @@ int lik_part_count(const DeviceDataset& ds, int mode, int K)
 
 int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c, int mode, hipStream_t st)
 {
-    const LikGrid g = lik_grid(ds);
+    const LikGrid g = lik_grid(ds, c.lik_tpr);
     int rc;
     switch (mode) {
     case 0: rc = launch_lik_mode<0>(ds, c, st, g); break;

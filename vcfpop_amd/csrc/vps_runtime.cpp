@@ -237,7 +237,8 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
 
     const int64_t N = d.N, KT = d.KT, S = d.S;
     int rc = 0;
-    const int nparts = std::max({lik_part_count(d, 0, K), lik_part_count(d, 3, K)});
+    rc = choose_lik_tpr(d, K, ds->device, c.lik_tpr);
+    const int nparts = rc ? 0 : std::max({lik_part_count(d, 0, K, c.lik_tpr), lik_part_count(d, 3, K, c.lik_tpr)});
     if (!rc) rc = chain_alloc(ch, &c.Freq, K * KT + 2);   // +2: paired allele loads past the last locus
     if (!rc) rc = chain_alloc(ch, &c.Q, N * K);
     if (!rc) rc = chain_alloc(ch, &c.Z, N + 32);
@@ -280,6 +281,9 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     if (!rc) rc = chain_alloc(ch, &c.mdev, 1);
     if (!rc) rc = chain_alloc(ch, &c.mside, 1);
     if (!rc && hipMemsetAsync(c.mdev, 0, sizeof(int32_t), ch->stream) != hipSuccess) rc = fail(VPS_ERR_DEVICE, "clear sweep index");
+    // the padding past P: read by paired allele loads, never charged
+    if (!rc && hipMemsetAsync(c.Freq + (int64_t)K * KT, 0, 2 * sizeof(double), ch->stream) != hipSuccess)
+        rc = fail(VPS_ERR_DEVICE, "clear P padding");
     if (rc) { chain_free(ch); return rc; }
 
     // InitFix (src/structure.cpp:468-553): P = 0.5 (1/k2 off biallelic), Q = 1/K,

@@ -112,7 +112,7 @@ int64_t estimate_bytes(const HostImage& h, int K)
                             ntiles * (56 + 64 * 4) + btab + ((N + 15) / 16) * 8192 + (1 << 20);
     const int64_t parts = std::min<int64_t>(kEstChunkParts, (N + 15) / 16);
     const int64_t chain = K * KT * (8 + 4 + 8) + parts * K * KT * 4 + 3 * KT * 8 + N * K * (8 + 4 + 8 + 8 + 8) +
-                          N * 64 + (ntiles / 8 + 2) * N * K * 8 + (1 << 20);
+                          N * 64 + (ntiles / 4 + 2) * N * K * 8 + (1 << 20);
     return dataset + chain;
 }
 
