@@ -77,7 +77,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (chains shard across GPUs "
-              "with no communication, see DESIGN.md)", file=sys.stderr)
+              "with no communication, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import vcfpop_amd as V
@@ -139,7 +139,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
-            "kernel": "z_admix", "kernel_avg_ms": round(z_avg_s * 1e3, 4), "launches_timed": z_n,
+            "kernel": "z_admix (sweeps without Record)", "kernel_avg_ms": round(z_avg_s * 1e3, 4), "launches_timed": z_n,
             "alg_bytes_per_launch": int(b_alg), "alg_fp64_ops_per_launch": int(f_alg),
             "valu": {"bound": "fp64-valu (binding)", "achieved": round(tops, 3), "peak": FP64_VALU_PEAK_TOPS,
                      "unit": "TFLOP/s", "frac": round(tops / FP64_VALU_PEAK_TOPS, 4)},

@@ -361,7 +361,8 @@ int  vps_server_destroy(vps_server* s);
 /* Timing hook for measurement: average device time of the last launches of  */
 /* a named kernel family, measured with HIP events on the chain's stream.     */
 /* ------------------------------------------------------------------------- */
-/* every = n: time every n-th Z pass (1 = all, 0 = off) */
+/* every = n: time every n-th Z pass (1 = all, 0 = off); the passes of Record
+   sweeps, which also sum the lnL, are neither timed nor counted */
 int vps_chain_enable_timing(vps_chain* ch, int every);
 /* kernel: 0 = Z admix pass; returns total ms and number of timed launches */
 int vps_chain_kernel_time(vps_chain* ch, int kernel, double* total_ms, int64_t* launches);

@@ -388,7 +388,9 @@ int do_q_metro(vps_chain* ch)
 int do_z_admix(vps_chain* ch, bool clear_mi = true, bool rec = false, bool advance = false)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (ch->timing && (ch->timing_every <= 1 || ch->timed_calls++ % ch->timing_every == 0)) {
+    // timing samples the plain pass: a Record sweep's pass (which also sums
+    // lnL) neither is timed nor counts
+    if (ch->timing && !rec && (ch->timing_every <= 1 || ch->timed_calls++ % ch->timing_every == 0)) {
         if (ch->ev_used + 2 > ch->evs.size()) {
             for (int x = 0; x < 64; ++x) {
                 hipEvent_t e;

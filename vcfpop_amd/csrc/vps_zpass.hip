@@ -200,7 +200,10 @@ __device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileD
             j = 0; ++w;
             if (i + 1 < i1) word = wp[(int64_t)w * kTileLoci];
         }
-        const uint32_t desc = BIG ? (uint32_t)tab_lds[gid * kTileLoci + lane] : (uint32_t)(tab >> (gid * 8));
+        // register table: a byte permute picks byte gid of the 8-byte table
+        // (only the low byte of desc is read)
+        const uint32_t desc = BIG ? (uint32_t)tab_lds[gid * kTileLoci + lane]
+                                  : __builtin_amdgcn_perm((uint32_t)(tab >> 32), (uint32_t)tab, gid);
         const int v = (int)(desc & 15u);
         const int h = v - (int)((desc >> 4) & 15u);   // the generic path's dosage form
         Real q[K];
