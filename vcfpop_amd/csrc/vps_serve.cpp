@@ -89,30 +89,36 @@ int copy_image(const vps_dataset_desc* in, HostImage& h)
 
 // Device bytes of a job, from its sizes: the dataset (transposed ids padded
 // to 64-locus tiles, per-locus tables, genotype descriptors, one chunk
-// jump-table set) and the chain (P, Ni and per-chunk partials for
-// kEstChunkParts chunks, FreqM, FreqA, N x K state, likelihood partials).
+// jump-table set) and the chain (P, Ni, FreqM, FreqA, N x K state,
+// likelihood partials, Ni and Record partials for kEstChunkParts chunks).
 // The reservation is corrected to the measured bytes once the job is loaded.
 constexpr int64_t kEstChunkParts = 16;
 
 int64_t estimate_bytes(const HostImage& h, int K)
 {
     const int64_t N = h.N, L = h.L;
-    int64_t KT = 0, words = 0;
+    int64_t KT = 0;
     for (int64_t l = 0; l < L; ++l) KT += h.n_allele[l];
-    for (int64_t l = 0; l < L; l += 64) {
-        const int bits = std::max<int>(1, h.bitsize[l]);
-        const int64_t ipw = std::max(1, 32 / std::min(bits, 32));
-        words += ((N + ipw - 1) / ipw) * 64;
+    // tiles: per id width, its loci split by kind, so at most one partial
+    // tile per width and kind
+    int64_t per_bits[33] = {0};
+    for (int64_t l = 0; l < L; ++l) per_bits[std::min<int>(std::max<int>(1, h.bitsize[l]), 32)]++;
+    int64_t ntiles = 0, words = 0;
+    for (int b = 1; b <= 32; ++b) {
+        if (!per_bits[b]) continue;
+        const int64_t t = (per_bits[b] + 63) / 64 + 1;
+        const int64_t ipw = std::max(1, 32 / b);
+        ntiles += t;
+        words += t * ((N + ipw - 1) / ipw) * 64;
     }
-    // tiles split by bitsize and kind: at most two partial tiles per bitsize
-    const int64_t ntiles = (L + 63) / 64 + 2 * 24;
     int64_t btab = 0;   // biallelic genotype bytes, 2^bits per locus
     for (int64_t l = 0; l < L; ++l) btab += (int64_t)1 << std::min<int>(std::max<int>(1, h.bitsize[l]), 6);
     const int64_t dataset = words * 4 + L * (4 + 4 + 8 + 8 + 4) + (int64_t)h.gt_ploidy.size() * 8 + N * 8 +
-                            ntiles * (56 + 64 * 4) + btab + ((N + 15) / 16) * 8192 + (1 << 20);
+                            ntiles * (56 + 64 * 4) + btab + ((N + 15) / 16) * 8192 + (64 << 10);
     const int64_t parts = std::min<int64_t>(kEstChunkParts, (N + 15) / 16);
-    const int64_t chain = K * KT * (8 + 4 + 8) + parts * K * KT * 4 + 3 * KT * 8 + N * K * (8 + 4 + 8 + 8 + 8) +
-                          N * 64 + (ntiles / 4 + 2) * N * K * 8 + (1 << 20);
+    const int64_t chain = K * KT * (8 + 4 + 8) + parts * (K * KT * 4 + (ntiles / 4 + ntiles) * 8) + 3 * KT * 8 +
+                          N * K * (8 + 4 + 8 + 8 + 8) +
+                          N * 64 + (ntiles / 4 + 2) * N * K * 8 + (64 << 10);
     return dataset + chain;
 }
 

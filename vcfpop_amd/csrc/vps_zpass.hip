@@ -83,6 +83,7 @@ template <> __device__ __forceinline__ float unit_minus1<float>(uint64_t r, cons
 __device__ __forceinline__ uint64_t ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
 
 constexpr int kWavesMax = 4;   // wavefronts per block of the Z kernels
+constexpr int kXcds = 8;       // XCDs of the device; blocks are dealt to them round-robin
 
 // Flush the K per-individual cluster counts of one wavefront (cumulative
 // form, wave-uniform) into the block's LDS row, two counts per 32-bit word
@@ -287,16 +288,24 @@ template <int K, typename Real>
 __host__ __device__ constexpr size_t bi_p_bytes(int nwv) { return (size_t)nwv * K * 2 * kTileLoci * sizeof(Real); }
 __host__ __device__ inline size_t bi_tab_bytes(int nwv, int rows) { return (size_t)nwv * rows * kTileLoci; }
 
-// REC: each block also writes the Record lnL partial of its lanes to
-// zlik[blockIdx.y * gridDim.x + blockIdx.x]
+// Work items are (tile group of nwv tiles, chunk) pairs, w = group * G +
+// chunk.  Blocks are dealt to the XCDs round-robin by block index, so block
+// b runs item (b % kXcds) * per + b / kXcds (gridDim.x = kXcds * per): each
+// XCD takes a contiguous run of items, and the chunks of one tile group run
+// back to back on it and read the group's P from its L2 rather than HBM.
+// REC: each block also writes the Record lnL partial of its lanes to zlik[w].
 template <int K, typename Real, int MPT, bool REC>
 __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, const double* __restrict__ Freq,
                                                    const double* __restrict__ Q, uint64_t seed,
                                                    const int32_t* __restrict__ mp,
                                                    int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
-                                                   int Nc, const uint32_t* __restrict__ jump,
+                                                   int Nc, int G, const uint32_t* __restrict__ jump,
                                                    double* __restrict__ zlik)
 {
+    const int64_t per = gridDim.x / kXcds;
+    const int64_t w = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    const int64_t ngroups = ((int64_t)ds.nbi + (blockDim.x >> 6) - 1) / (blockDim.x >> 6);
+    if (w >= ngroups * G) return;   // the whole block: grid padding
     const int m = *mp;
     extern __shared__ int32_t lds_i32[];
     const int lane = threadIdx.x & 63;
@@ -304,7 +313,7 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
     // live in scalar registers
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwv = blockDim.x >> 6;
-    const int c = blockIdx.y;
+    const int c = (int)(w % G);
     const int i0 = c * Nc;
     const int i1 = min(ds.N, i0 + Nc);
     const int nci = i1 - i0;
@@ -315,7 +324,7 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
     uint32_t* mi_lds = reinterpret_cast<uint32_t*>(lds_u8 + bi_p_bytes<K, Real>(nwv) + bi_tab_bytes(nwv, ds.bi_tab_rows));
     for (int x = threadIdx.x; x < nci * KH; x += blockDim.x) mi_lds[x] = 0;
 
-    const int64_t tix = (int64_t)blockIdx.x * nwv + wv;
+    const int64_t tix = (w / G) * nwv + wv;
     const bool has_tile = tix < ds.nbi;
     TileDesc t{};
     int afo = 0, k2 = 1;
@@ -354,8 +363,8 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
     __syncthreads();
     if (REC && threadIdx.x == 0) {
         double s = 0.0;
-        for (int w = 0; w < nwv; ++w) s += wsum[w];
-        zlik[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = s;
+        for (int x = 0; x < nwv; ++x) s += wsum[x];
+        zlik[w] = s;
     }
     flush_mi_rows<K>(mi_lds, nci, Mi + (int64_t)i0 * K);
 }
@@ -716,20 +725,25 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st,
         int rc = launch_fill_i32(c.Ni_part, (int64_t)c.G * K * ds.KT, 0, st);
         if (rc) return rc;
     }
-    int64_t zbase = 0;   // Record partials: the BI blocks', then the GEN blocks'
+    int64_t zbase = 0;   // Record partials: the BI work items', then the GEN blocks'
     if (ds.nbi > 0) {
-        dim3 grid((unsigned)((ds.nbi + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)c.G);
-        zbase = (int64_t)grid.x * grid.y;
+        // one block per (group of 4 tiles, chunk) item, dealt round-robin to
+        // the XCDs; padded to a multiple of kXcds (padding blocks exit)
+        const int64_t items = (ds.nbi + kWavesPerBlock - 1) / kWavesPerBlock * (int64_t)c.G;
+        const int64_t per = (items + kXcds - 1) / kXcds;
+        if (per * kXcds > 0x7FFFFFFF) return fail(VPS_ERR_UNSUPPORTED, "Z pass grid of %lld blocks", (long long)(per * kXcds));
+        const dim3 grid((unsigned)(per * kXcds));
+        zbase = items;
         const size_t shm = bi_p_bytes<K, Real>(kWavesPerBlock) + bi_tab_bytes(kWavesPerBlock, ds.bi_tab_rows) + mi_packed;
         if (ds.maxploidy == 2)
             hipLaunchKernelGGL((k_z_admix_bi<K, Real, 2, REC>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
-                               c.Ni_part, c.Mi, c.Nc, c.jump, c.zlik);
+                               c.Ni_part, c.Mi, c.Nc, c.G, c.jump, c.zlik);
         else if (ds.maxploidy == 4)
             hipLaunchKernelGGL((k_z_admix_bi<K, Real, 4, REC>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
-                               c.Ni_part, c.Mi, c.Nc, c.jump, c.zlik);
+                               c.Ni_part, c.Mi, c.Nc, c.G, c.jump, c.zlik);
         else
             hipLaunchKernelGGL((k_z_admix_bi<K, Real, 0, REC>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
-                               c.Ni_part, c.Mi, c.Nc, c.jump, c.zlik);
+                               c.Ni_part, c.Mi, c.Nc, c.G, c.jump, c.zlik);
         VPS_HIP(hipGetLastError());
     }
     if (ds.ngen > 0) {
@@ -869,7 +883,7 @@ int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_
 
 int64_t z_lik_parts(const DeviceDataset& ds, int G)
 {
-    // BI blocks: 4 tiles x 1 chunk; GEN blocks: 1 tile x 1..4 chunks
+    // BI items: 4 tiles x 1 chunk; GEN blocks: 1 tile x 1..4 chunks
     return (ds.nbi + kWavesPerBlock - 1) / kWavesPerBlock * G + ds.ngen * (int64_t)G;
 }
 
