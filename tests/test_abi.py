@@ -15,7 +15,7 @@ def declared(header):
 
 def test_every_declared_function_is_exported(vps):
     lib = ctypes.CDLL(vps.LIB_PATH)
-    names = declared("vps.h") + declared("vps_synth.h")
+    names = declared("vps.h") + declared("vps_synth.h") + declared("vps_ingest.h")
     assert len(names) > 30
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
@@ -23,7 +23,7 @@ def test_every_declared_function_is_exported(vps):
 
 def test_python_mirror_binds_every_declared_function(vps):
     from vcfpop_amd import _lib
-    names = declared("vps.h") + declared("vps_synth.h")
+    names = declared("vps.h") + declared("vps_synth.h") + declared("vps_ingest.h")
     unbound = [n for n in names if n not in _lib._SIGS]
     assert not unbound, unbound
 

@@ -19,6 +19,7 @@ from ._lib import (  # noqa: F401
     VpsDatasetDesc,
     VpsSynthConfig,
     Synth,
+    Image,
     Dataset,
     Chain,
     Batch,

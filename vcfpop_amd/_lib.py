@@ -185,6 +185,10 @@ _SIGS = {
     "vps_synth_truth": ([_P, _P], C.c_int),
     "vps_synth_sizes": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "vps_synth_destroy": ([_P], C.c_int),
+    "vps_image_read_polygene": ([C.c_char_p, C.POINTER(C.c_void_p)], C.c_int),
+    "vps_image_desc": ([_P, C.POINTER(VpsDatasetDesc)], C.c_int),
+    "vps_image_sizes": ([_P] + [C.POINTER(C.c_int64)] * 5, C.c_int),
+    "vps_image_destroy": ([_P], C.c_int),
 }
 
 
@@ -300,6 +304,70 @@ class Synth:
             pass
 
 
+class _PackedArrays:
+    """numpy copies of a packed description (for tests)."""
+
+    def _arrays(self, pb, ng, na):
+        d, L, N = self.desc, self.desc.n_loc, self.desc.n_ind
+        return dict(
+            N=N, L=L, S=d.n_pop,
+            ind_pop=np.ctypeslib.as_array(d.ind_pop, (N,)).copy(),
+            geno_bits=np.ctypeslib.as_array(d.geno_bits, (max(pb, 1),)).copy(),
+            geno_byte_offset=np.ctypeslib.as_array(d.geno_byte_offset, (L,)).copy(),
+            bitsize=np.ctypeslib.as_array(d.bitsize, (L,)).copy(),
+            n_allele=np.ctypeslib.as_array(d.n_allele, (L,)).copy(),
+            gtab_offset=np.ctypeslib.as_array(d.gtab_offset, (L + 1,)).copy(),
+            gt_ploidy=np.ctypeslib.as_array(d.gt_ploidy, (ng,)).copy(),
+            gt_allele_offset=np.ctypeslib.as_array(d.gt_allele_offset, (ng,)).copy(),
+            allele_pool=np.ctypeslib.as_array(d.allele_pool, (max(na, 1),)).copy(),
+        )
+
+
+class Image(_PackedArrays):
+    """A genotype file read into vcfpop's packed layout (include/vps_ingest.h)."""
+
+    def __init__(self, path, fmt="polygene"):
+        if fmt != "polygene":
+            raise ValueError(f"unsupported genotype format {fmt!r}")
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.vps_image_read_polygene(os.fsencode(str(path)), C.byref(h)))
+        self.h = h
+        self.desc = VpsDatasetDesc()
+        _check(lib.vps_image_desc(h, C.byref(self.desc)))
+        v = [C.c_int64() for _ in range(5)]
+        _check(lib.vps_image_sizes(h, *[C.byref(x) for x in v]))
+        self.loci_read, self.loci_kept, self._pb, self._ng, self._na = (x.value for x in v)
+
+    @property
+    def N(self):
+        return self.desc.n_ind
+
+    @property
+    def L(self):
+        return self.desc.n_loc
+
+    def names(self, kind):
+        """'ind', 'pop' or 'loc' names"""
+        d = self.desc
+        arr, n = {"ind": (d.ind_name, d.n_ind), "pop": (d.pop_name, d.n_pop), "loc": (d.loc_name, d.n_loc)}[kind]
+        return [arr[i].decode() for i in range(n)]
+
+    def arrays(self):
+        return self._arrays(self._pb, self._ng, self._na)
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().vps_image_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def desc_from_arrays(a):
     """Build a VpsDatasetDesc over numpy arrays (the arrays must stay alive)."""
     d = VpsDatasetDesc()
@@ -321,7 +389,7 @@ class Dataset:
 
     def __init__(self, desc, device=0):
         self._src = desc
-        if isinstance(desc, Synth):
+        if isinstance(desc, (Synth, Image)):
             d = desc.desc
         elif isinstance(desc, dict):
             d = desc_from_arrays(desc)

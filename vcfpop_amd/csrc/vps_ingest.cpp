@@ -1,0 +1,304 @@
+// vps_ingest.cpp — genotype text files read into the packed genotype image
+// (vps_dataset_desc): the polygene format.
+//
+// The reference's reader, restated (src/load.cpp):
+//   IND::polygene (:3047-3120): a line per individual; name, population and
+//     ploidy, then per locus the allele identifiers separated by commas (an
+//     empty field is missing).  Alleles are sorted: genotypes are unphased.
+//   Genotype ids per locus in order of first appearance over individuals;
+//     an individual registers the missing genotype of its ploidy before its
+//     own genotype (:3093-3110), so every ploidy present has a missing entry.
+//   IndexAlleleLength (:1535-1590): allele indices in order of first
+//     appearance over the genotype table (copies in stored order, missing
+//     entries skipped); each genotype's copies are then re-sorted by index.
+//     A locus with fewer than two alleles fails the locus filter (:1575) and
+//     is removed before any analysis (RemoveLocus, src/filter.cpp:137-160).
+//   Ids are packed at CeilLog2(#genotypes) bits, LSB-first from a byte-aligned
+//     start (src/math2.h:503-512, src/misc.cpp:1624-1636).
+// One deviation: a single allele in a field of ploidy > 1 is an error here;
+// the reference pads such a field with missing copies.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+#include "../../include/vps_ingest.h"
+#include "vps_internal.h"
+
+struct vps_image {
+    int32_t N = 0;
+    int64_t L = 0, L_read = 0;
+    int32_t S = 0;
+    std::vector<uint8_t> bits;
+    std::vector<uint64_t> byteoff;
+    std::vector<uint8_t> bitsize;
+    std::vector<int32_t> nallele;
+    std::vector<int64_t> gtoff;
+    std::vector<uint8_t> gtploidy;
+    std::vector<int64_t> gtaoff;
+    std::vector<uint16_t> pool;
+    std::vector<int32_t> pop;
+    std::vector<std::string> ind_names, pop_names, loc_names;
+    std::vector<const char*> ind_ptr, pop_ptr, loc_ptr;
+};
+
+using namespace vps;
+
+namespace {
+
+int ceil_log2(int64_t v)
+{
+    if (v <= 1) return 1;
+    int b = 0;
+    uint64_t x = (uint64_t)(v - 1);
+    while (x) { ++b; x >>= 1; }
+    return b;
+}
+
+std::vector<std::string> split(const std::string& s, char sep)
+{
+    std::vector<std::string> out;
+    size_t a = 0;
+    for (;;) {
+        const size_t b = s.find(sep, a);
+        out.emplace_back(s.substr(a, b == std::string::npos ? std::string::npos : b - a));
+        if (b == std::string::npos) break;
+        a = b + 1;
+    }
+    return out;
+}
+
+std::string trim(const std::string& s)
+{
+    size_t a = 0, b = s.size();
+    while (a < b && (s[a] == ' ' || s[a] == '\r')) ++a;
+    while (b > a && (s[b - 1] == ' ' || s[b - 1] == '\r')) --b;
+    return s.substr(a, b - a);
+}
+
+bool parse_int(const std::string& s, int64_t lo, int64_t hi, int64_t* v)
+{
+    if (s.empty() || s.size() > 18) return false;
+    int64_t x = 0;
+    for (char ch : s) {
+        if (ch < '0' || ch > '9') return false;
+        x = x * 10 + (ch - '0');
+    }
+    if (x < lo || x > hi) return false;
+    *v = x;
+    return true;
+}
+
+// a genotype of one locus: ploidy and sorted allele identifiers (none if missing)
+struct Geno {
+    int v = 0;
+    bool missing = false;
+    std::vector<uint16_t> al;
+};
+
+std::string geno_key(const Geno& g)
+{
+    std::string k(1, (char)g.v);
+    k.push_back(g.missing ? 'm' : 't');
+    for (uint16_t a : g.al) { k.push_back((char)(a & 0xFF)); k.push_back((char)(a >> 8)); }
+    return k;
+}
+
+}  // namespace
+
+extern "C" int vps_image_read_polygene(const char* path, vps_image** out)
+{
+    if (!path || !out) return fail(VPS_ERR_INVALID, "null argument");
+    *out = nullptr;
+    FILE* f = fopen(path, "rb");
+    if (!f) return fail(VPS_ERR_INVALID, "cannot open %s", path);
+    std::string text;
+    char buf[1 << 16];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, n);
+    fclose(f);
+
+    std::vector<std::string> lines = split(text, '\n');
+    size_t li = 0;
+    while (li < lines.size() && trim(lines[li]).empty()) ++li;
+    if (li == lines.size()) return fail(VPS_ERR_INVALID, "%s: empty file", path);
+    const std::vector<std::string> head = split(trim(lines[li]), '\t');
+    if (head.size() < 4) return fail(VPS_ERR_INVALID, "%s: the header needs ID, Pop, Ploidy and at least one locus", path);
+    const int64_t L = (int64_t)head.size() - 3;
+    ++li;
+
+    vps_image* im = new vps_image();
+    im->L_read = L;
+    // per locus: genotype table (first appearance) and each individual's id
+    std::vector<std::unordered_map<std::string, uint32_t>> gid(L);
+    std::vector<std::vector<Geno>> gtab(L);
+    std::vector<std::vector<uint32_t>> ids(L);
+    std::map<std::string, int32_t> pop_index;
+    for (; li < lines.size(); ++li) {
+        const std::string line = trim(lines[li]);
+        if (line.empty()) continue;
+        const std::vector<std::string> fld = split(line, '\t');
+        const int lineno = (int)li + 1;
+        if ((int64_t)fld.size() < 3 + L) {
+            delete im;
+            return fail(VPS_ERR_INVALID, "%s line %d: %zu fields, expected %lld", path, lineno, fld.size(), (long long)(3 + L));
+        }
+        for (size_t x = 3 + L; x < fld.size(); ++x)
+            if (!trim(fld[x]).empty()) {
+                delete im;
+                return fail(VPS_ERR_INVALID, "%s line %d: more fields than loci", path, lineno);
+            }
+        int64_t v = 0;
+        if (!parse_int(trim(fld[2]), 1, VPS_MAX_PLOIDY, &v)) {
+            delete im;
+            return fail(VPS_ERR_INVALID, "%s line %d: ploidy '%s' outside 1..%d", path, lineno, fld[2].c_str(), VPS_MAX_PLOIDY);
+        }
+        im->ind_names.push_back(trim(fld[0]));
+        const std::string pname = trim(fld[1]);
+        auto pit = pop_index.find(pname);
+        if (pit == pop_index.end()) {
+            pit = pop_index.emplace(pname, (int32_t)im->pop_names.size()).first;
+            im->pop_names.push_back(pname);
+        }
+        im->pop.push_back(pit->second);
+        for (int64_t l = 0; l < L; ++l) {
+            const std::string gs = trim(fld[3 + l]);
+            Geno g;
+            g.v = (int)v;
+            if (gs.empty()) {
+                g.missing = true;
+            } else {
+                const std::vector<std::string> al = split(gs, ',');
+                if ((int64_t)al.size() != v) {
+                    delete im;
+                    return fail(VPS_ERR_INVALID, "%s line %d locus %lld: %zu alleles for ploidy %lld (allelic phenotypes are not supported)",
+                                path, lineno, (long long)(l + 1), al.size(), (long long)v);
+                }
+                for (const std::string& a : al) {
+                    int64_t x = 0;
+                    if (!parse_int(trim(a), 0, 0xFFFE, &x)) {
+                        delete im;
+                        return fail(VPS_ERR_INVALID, "%s line %d locus %lld: allele '%s' is not an integer in 0..65534",
+                                    path, lineno, (long long)(l + 1), a.c_str());
+                    }
+                    g.al.push_back((uint16_t)x);
+                }
+                std::sort(g.al.begin(), g.al.end());   // unphased
+            }
+            // the missing genotype of this ploidy first, then the genotype
+            Geno miss;
+            miss.v = (int)v;
+            miss.missing = true;
+            uint32_t id = 0;
+            for (const Geno* e : {&miss, &g}) {
+                const std::string k = geno_key(*e);
+                auto it = gid[l].find(k);
+                if (it == gid[l].end()) {
+                    it = gid[l].emplace(k, (uint32_t)gtab[l].size()).first;
+                    gtab[l].push_back(*e);
+                }
+                id = it->second;
+                if (g.missing) break;
+            }
+            ids[l].push_back(id);
+        }
+    }
+    const int64_t N = (int64_t)im->ind_names.size();
+    if (N == 0) { delete im; return fail(VPS_ERR_INVALID, "%s: no individuals", path); }
+    if (N > 0x7FFFFFFF) { delete im; return fail(VPS_ERR_UNSUPPORTED, "%s: too many individuals", path); }
+    im->N = (int32_t)N;
+    im->S = (int32_t)im->pop_names.size();
+
+    // allele indices by first appearance over the genotype table, copies
+    // re-sorted by index; loci with fewer than two alleles are dropped
+    im->gtoff.push_back(0);
+    uint64_t bytes = 0;
+    for (int64_t l = 0; l < L; ++l) {
+        std::unordered_map<uint16_t, uint16_t> aix;
+        for (const Geno& g : gtab[l])
+            for (uint16_t a : g.al)
+                if (!aix.count(a)) aix.emplace(a, (uint16_t)aix.size());
+        const int k = (int)aix.size();
+        if (k < 2) continue;
+        if (k > VPS_MAX_ALLELES) {
+            delete im;
+            return fail(VPS_ERR_UNSUPPORTED, "%s locus %s: %d alleles, the device path takes at most %d", path,
+                        head[3 + l].c_str(), k, VPS_MAX_ALLELES);
+        }
+        const int64_t ng = (int64_t)gtab[l].size();
+        const int b = ceil_log2(ng);
+        if (b > 24) { delete im; return fail(VPS_ERR_UNSUPPORTED, "%s locus %s: %lld genotypes", path, head[3 + l].c_str(), (long long)ng); }
+        im->loc_names.push_back(head[3 + l]);
+        im->nallele.push_back(k);
+        im->bitsize.push_back((uint8_t)b);
+        im->byteoff.push_back(bytes);
+        for (const Geno& g : gtab[l]) {
+            im->gtploidy.push_back(g.missing ? 0 : (uint8_t)g.v);
+            im->gtaoff.push_back((int64_t)im->pool.size());
+            std::vector<uint16_t> m;
+            for (uint16_t a : g.al) m.push_back(aix[a]);
+            std::sort(m.begin(), m.end());
+            im->pool.insert(im->pool.end(), m.begin(), m.end());
+        }
+        im->gtoff.push_back((int64_t)im->gtploidy.size());
+        const uint64_t nb = ((uint64_t)b * N + 7) / 8;
+        im->bits.resize(bytes + nb, 0);
+        for (int64_t i = 0; i < N; ++i) {
+            const uint64_t v = ids[l][i];
+            const uint64_t bit = (uint64_t)i * b;
+            for (int x = 0; x < b; ++x)
+                if (v >> x & 1) im->bits[bytes + ((bit + x) >> 3)] |= (uint8_t)(1u << ((bit + x) & 7));
+        }
+        bytes += nb;
+    }
+    im->L = (int64_t)im->nallele.size();
+    if (im->L == 0) { delete im; return fail(VPS_ERR_INVALID, "%s: every locus has fewer than two alleles", path); }
+    if (im->pool.empty()) im->pool.push_back(0);
+    for (auto& s : im->ind_names) im->ind_ptr.push_back(s.c_str());
+    for (auto& s : im->pop_names) im->pop_ptr.push_back(s.c_str());
+    for (auto& s : im->loc_names) im->loc_ptr.push_back(s.c_str());
+    *out = im;
+    return VPS_OK;
+}
+
+extern "C" int vps_image_desc(const vps_image* im, vps_dataset_desc* d)
+{
+    if (!im || !d) return fail(VPS_ERR_INVALID, "null argument");
+    memset(d, 0, sizeof *d);
+    d->n_ind = im->N;
+    d->n_loc = im->L;
+    d->n_pop = im->S;
+    d->ind_pop = im->pop.data();
+    d->geno_bits = im->bits.data();
+    d->geno_byte_offset = im->byteoff.data();
+    d->bitsize = im->bitsize.data();
+    d->n_allele = im->nallele.data();
+    d->gtab_offset = im->gtoff.data();
+    d->gt_ploidy = im->gtploidy.data();
+    d->gt_allele_offset = im->gtaoff.data();
+    d->allele_pool = im->pool.data();
+    d->ind_name = im->ind_ptr.data();
+    d->pop_name = im->pop_ptr.data();
+    d->loc_name = im->loc_ptr.data();
+    return VPS_OK;
+}
+
+extern "C" int vps_image_sizes(const vps_image* im, int64_t* loci_read, int64_t* loci_kept, int64_t* packed_bytes,
+                               int64_t* n_genotypes, int64_t* n_alleles)
+{
+    if (!im) return fail(VPS_ERR_INVALID, "null image");
+    if (loci_read) *loci_read = im->L_read;
+    if (loci_kept) *loci_kept = im->L;
+    if (packed_bytes) *packed_bytes = (int64_t)im->bits.size();
+    if (n_genotypes) *n_genotypes = (int64_t)im->gtploidy.size();
+    if (n_alleles) *n_alleles = (int64_t)im->pool.size();
+    return VPS_OK;
+}
+
+extern "C" int vps_image_destroy(vps_image* im)
+{
+    delete im;
+    return VPS_OK;
+}

@@ -1,0 +1,128 @@
+"""Run the Bayesian clustering of a vcfpop parameter file on the device.
+
+    python -m vcfpop_amd.par par.polygene.txt [--device 0] [--sweeps N] [--topology reference|parallel]
+
+Reads the options the clustering uses and ignores the rest (other analyses
+are not part of this engine):
+  -g_input, -g_format (polygene), -g_output, -g_seed, -g_decimal
+  -structure_* (src/parameters.cpp:1230-1315): models, K range, runs,
+  burn-in / sampling / thinning lengths, lambda/alpha/r/F settings.
+The runs are the reference's task list (src/structure.cpp:2402-2413): for
+K = krange_min..krange_max and rep = 1..nruns, run id = 1, 2, ... in that
+order and run seed = HashULong(g_seed + id).  All runs are one batch on
+one device; each finished run writes its WriteStructure table and the
+runs' WriteStructureSummary table goes to <g_output>.structure.txt.
+Populations come from the file's population column.
+"""
+import argparse
+import os
+import re
+import sys
+import time
+
+from ._lib import RNG_PARALLEL, RNG_REFERENCE, Batch, Chain, Dataset, Image, params_default, write_summary
+
+_YESNO = {"yes": 1, "no": 2}
+_BOOL_OPTS = {"admix", "locpriori", "f", "inferlambda", "difflambda", "diversity", "inferalpha", "diffalpha",
+              "uniformalpha", "singlef", "writelnl"}
+_INT_OPTS = {"nburnin", "nreps", "nthinning", "nruns", "nadmburnin", "metrofreq"}
+_DBL_OPTS = {"lambda", "stdlambda", "maxlambda", "alpha", "stdalpha", "maxalpha", "alphapriora", "alphapriorb",
+             "r", "maxr", "stdr", "epseta", "epsgamma", "pmeanf", "pstdf", "stdf"}
+_IGNORED = {"plot", "nstream", "eval"}
+
+
+def read_par(path):
+    """(global options, structure options) of a parameter file"""
+    g, st = {}, {}
+    for raw in open(path, encoding="utf-8", errors="replace"):
+        line = raw.strip()
+        m = re.match(r"^-(g|structure)_([A-Za-z0-9]+)=(.*)$", line)
+        if not m:
+            continue
+        (g if m.group(1) == "g" else st)[m.group(2).lower()] = m.group(3).strip().strip('"')
+    return g, st
+
+
+def structure_params(st, g_seed):
+    """vps_params from -structure_* options (the reference's defaults otherwise)"""
+    kw = {}
+    krange = (1, 5)
+    for k, v in st.items():
+        if k in _BOOL_OPTS:
+            if v.lower() not in _YESNO:
+                raise ValueError(f"-structure_{k}={v}: expected yes or no")
+            kw[k + "_val"] = _YESNO[v.lower()]
+        elif k in _INT_OPTS:
+            kw[k] = int(v)
+        elif k in _DBL_OPTS:
+            kw[k] = float(v)
+        elif k == "krange":
+            m = re.match(r"^\[?\s*(\d+)\s*,\s*(\d+)\s*\]?$", v)
+            if not m:
+                raise ValueError(f"-structure_krange={v}: expected [min,max]")
+            krange = (int(m.group(1)), int(m.group(2)))
+        elif k not in _IGNORED:
+            raise ValueError(f"unrecognized parameter -structure_{k}")
+    kw["g_seed"] = g_seed
+    kw["krange_min"], kw["krange_max"] = krange
+    return kw, krange
+
+
+def run(par_path, device=0, sweeps=None, topology="reference", log=print):
+    g, st = read_par(par_path)
+    base = os.path.dirname(os.path.abspath(par_path))
+    if g.get("format", "polygene").lower() != "polygene":
+        raise ValueError(f"-g_format={g['format']}: this engine reads the polygene format")
+    inp = g.get("input")
+    if not inp:
+        raise ValueError("-g_input is missing")
+    inp = os.path.join(base, inp.replace("\\", os.sep))
+    out = os.path.join(base, g.get("output", "vcfpop").replace("\\", os.sep))
+    os.makedirs(os.path.dirname(out) or ".", exist_ok=True)
+    decimal = int(g.get("decimal", 5))
+    kw, (kmin, kmax) = structure_params(st, int(g.get("seed", 0)))
+    if sweeps is not None:   # a shorter run: burn-in a fifth, sampling the rest
+        kw["nburnin"], kw["nreps"] = sweeps // 5, sweeps - sweeps // 5
+    kw["rng_topology"] = RNG_PARALLEL if topology == "parallel" else RNG_REFERENCE
+    p = params_default(**kw)
+    nruns = p.nruns
+    nadm = p.nadmburnin if (p.admix_val == 2 and p.locpriori_val == 2) else 0
+    total = nadm + p.nburnin + p.nreps
+
+    im = Image(inp)
+    log(f"{inp}: {im.N} individuals, {im.L} of {im.loci_read} loci (fewer than 2 alleles dropped)")
+    ds = Dataset(im, device)
+    runs = [(k, rep) for k in range(kmin, kmax + 1) for rep in range(1, nruns + 1)]
+    chains = [Chain(ds, p, k, i + 1) for i, (k, rep) in enumerate(runs)]
+    t0 = time.perf_counter()
+    b = Batch(chains)
+    b.step(total)
+    b.sync()
+    dt = time.perf_counter() - t0
+    log(f"{len(runs)} runs x {total} sweeps in {dt:.2f} s")
+    summaries = []
+    for c, (k, rep) in zip(chains, runs):
+        c.finish()
+        summaries.append(c.write_tables(out, rep=rep, decimal=decimal, diversity=p.diversity_val))
+    write_summary(out + ".structure.txt", summaries, decimal=decimal)
+    b.close()
+    for c in chains:
+        c.close()
+    ds.close()
+    im.close()
+    return summaries
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="Bayesian clustering of a vcfpop parameter file on the device")
+    ap.add_argument("par")
+    ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--sweeps", type=int, default=None, help="override burn-in + sampling length (a quick run)")
+    ap.add_argument("--topology", choices=["reference", "parallel"], default="reference")
+    args = ap.parse_args(argv)
+    for s in run(args.par, args.device, args.sweeps, args.topology):
+        print(f"id {s['id']} K={s['k']} rep {s['rep']}: mean lnL {s['mean_lnl']:.3f}, lnP(D) {s['lnpd']:.3f}")
+
+
+if __name__ == "__main__":
+    sys.exit(main())
