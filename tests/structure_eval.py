@@ -60,11 +60,18 @@ def run_eval(synth, K=5, seed=1, models=("admix", "noadmix"), backends=("oracle"
     rows = []
     for model in models:
         kw, passes = MODELS[model]
-        chains = {}
-        for b in backends:
-            p = V.params_default(nburnin=0, nreps=1, nthinning=1, nadmburnin=0, g_seed=seed, z_chunk=-1,
-                                 precision=V.PREC_MIXED if b == "mixed" else V.PREC_FP64, **kw)
-            chains[b] = O.Chain(od, p, K, 1) if b == "oracle" else V.Chain(ds, p, K, 1)
+        params = {b: V.params_default(nburnin=0, nreps=1, nthinning=1, nadmburnin=0, g_seed=seed, z_chunk=-1,
+                                      precision=V.PREC_MIXED if b == "mixed" else V.PREC_FP64, **kw)
+                  for b in backends}
+        for b in backends:      # a throwaway sweep, so that module loading is not timed
+            if b != "oracle":
+                w = V.Chain(ds, params[b], K, 2)
+                for name in passes:
+                    getattr(w, name)()
+                w.sync()
+                w.close()
+        chains = {b: O.Chain(od, params[b], K, 1) if b == "oracle" else V.Chain(ds, params[b], K, 1)
+                  for b in backends}
         for b, c in chains.items():
             if b != "oracle":
                 c.sync()
