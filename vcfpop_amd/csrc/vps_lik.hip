@@ -172,16 +172,23 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
                         }
                     } else {
                         // sums start from the k = 0 term (0 + x == x for the
-                        // non-negative terms here)
+                        // non-negative terms here).  The Metropolis ratio's
+                        // mixtures are fused multiply-adds (one rounding per
+                        // term instead of two; the ratio is held to 1e-13
+                        // per term, far above either rounding); Record keeps
+                        // the product-then-sum rounding.
                         double f0 = p0[0] * q[0], f1 = p1[0] * q[0];
                         double g0 = MODE == 2 ? p0[0] * qp[0] : 0.0, g1 = MODE == 2 ? p1[0] * qp[0] : 0.0;
 #pragma unroll
                         for (int k = 1; k < K; ++k) {
-                            f0 = f0 + p0[k] * q[k];
-                            f1 = f1 + p1[k] * q[k];
                             if (MODE == 2) {
-                                g0 = g0 + p0[k] * qp[k];
-                                g1 = g1 + p1[k] * qp[k];
+                                f0 = __builtin_fma(p0[k], q[k], f0);
+                                f1 = __builtin_fma(p1[k], q[k], f1);
+                                g0 = __builtin_fma(p0[k], qp[k], g0);
+                                g1 = __builtin_fma(p1[k], qp[k], g1);
+                            } else {
+                                f0 = f0 + p0[k] * q[k];
+                                f1 = f1 + p1[k] * q[k];
                             }
                         }
 #pragma unroll kPairs

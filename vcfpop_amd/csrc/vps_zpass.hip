@@ -88,6 +88,11 @@ template <> __device__ __forceinline__ float unit_minus1<float>(uint64_t r, cons
 
 __device__ __forceinline__ uint64_t ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
 
+// v_writelane with a lane index in a register: the compiler's intrinsic
+// (it routes the index through M0), which this HIP version declares no
+// builtin for
+__device__ int writelane_i32(int val, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+
 constexpr int kWavesMax = 4;   // wavefronts per block of the Z kernels
 constexpr int kXcds = 8;       // XCDs of the device; blocks are dealt to them round-robin
 
@@ -134,16 +139,42 @@ __device__ __forceinline__ void flush_mi_rows(const uint32_t* mi_lds, int nci, i
 // MPT > 0 fixes maxploidy at compile time (unrolled slot loop).
 // ---------------------------------------------------------------------------
 
-// a genotype byte recoded as ploidy | (allele-1 flag of slot a) << (4 + a),
+// a genotype byte recoded as (allele-1 flag of slot a) << a | ploidy << 4,
 // a < 4, so a slot's allele is one bit extract instead of a compare and
-// select (tables of maximum ploidy 2 and 4)
+// select, and slot a is typed iff the byte is >= (a + 1) * 16 (tables of
+// maximum ploidy 2 and 4)
 __device__ __forceinline__ uint32_t slot_flags(uint32_t e)
 {
     const int v = (int)(e & 15u), h = v - (int)((e >> 4) & 15u);
     uint32_t f = 0;
 #pragma unroll
     for (int a = 0; a < 4; ++a) f |= (a >= h ? 1u : 0u) << a;
-    return (uint32_t)v | (f << 4);
+    return f | ((uint32_t)v << 4);
+}
+
+// Per-individual Mi rows of the BI kernel: the K cluster counts of one
+// individual summed over the block's wavefronts, FB-bit fields, FPW per
+// 32-bit word.  A block adds at most 4 waves x 64 loci x MPT slots per
+// count: 512 < 2^10 for diploid tables, < 2^16 otherwise (<= 10 slots).
+template <int K, int MPT>
+struct MiPack {
+    static constexpr int FB = MPT == 2 ? 10 : 16;
+    static constexpr int FPW = 32 / FB;
+    static constexpr int MW = (K + FPW - 1) / FPW;   // words per row
+};
+template <int K, int MPT>
+__host__ __device__ constexpr int mi_row_words() { return MiPack<K, MPT>::MW; }
+
+// add a BI block's packed per-individual rows to Mi
+template <int K, int MPT>
+__device__ __forceinline__ void flush_bi_rows(const uint32_t* mi_lds, int nci, int32_t* __restrict__ Mi_blk)
+{
+    using MP = MiPack<K, MPT>;
+    for (int x = threadIdx.x; x < nci * K; x += blockDim.x) {
+        const int i = x / K, k = x - i * K;
+        const int v = (int)((mi_lds[i * MP::MW + k / MP::FPW] >> ((k % MP::FPW) * MP::FB)) & ((1u << MP::FB) - 1u));
+        if (v) atomicAdd(Mi_blk + x, v);
+    }
 }
 
 // one wavefront's chunk of one BI tile; BIG: ids of more than 3 bits, the
@@ -155,7 +186,6 @@ __device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileD
                                           uint32_t* mi_lds, int32_t* __restrict__ Ni_part,
                                           const uint32_t* __restrict__ jump)
 {
-    constexpr int KH = (K + 1) / 2;
     const int64_t KT = ds.KT;
     uint64_t tab = 0;
     if (BIG) {
@@ -185,13 +215,25 @@ __device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileD
     uint32_t nc[K > 1 ? K - 1 : 1];
 #pragma unroll
     for (int k = 0; k < K - 1; ++k) nc[k] = 0;
-    uint32_t mval = 0;
+    using MPK = MiPack<K, MPT>;
+    // lane x of mrow[] holds the packed row of individual i0 + 64 g + x of
+    // the current group g of 64 individuals; a group's rows go to LDS at once
+    uint32_t mrow[MPK::MW];
+#pragma unroll
+    for (int x = 0; x < MPK::MW; ++x) mrow[x] = 0;
     const Real eps = (Real)kMinFreq;
     const int bits = t.bits, ipw = t.ipw;
-    const uint32_t mask = (1u << bits) - 1u;
-    int w = i0 / ipw, j = i0 - w * ipw;
+    // ids: word by word, each id a bitfield extract at a uniform offset; the
+    // next word's load is issued before the current word's ids (clamped to
+    // the chunk's last word, so every load is in bounds)
+    const int w_first = i0 / ipw, w_last = (i1 - 1) / ipw;
     const uint32_t* wp = ds.words + t.wbase + lane;
-    uint32_t word = wp[(int64_t)w * kTileLoci] >> (j * bits);
+    uint32_t nxt = wp[(int64_t)w_first * kTileLoci];
+    // the extract's width in a vector register, set once (the extract
+    // takes its offset from a scalar register, and one scalar operand is
+    // the limit)
+    uint32_t vbits;
+    asm("v_mov_b32 %0, %1" : "=v"(vbits) : "s"(bits));
     const int MP = MPT > 0 ? MPT : ds.maxploidy;
     UnitBits ub;
     ub.init();
@@ -200,68 +242,92 @@ __device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileD
     FrexpAcc acc;
     acc.open();
 
-    for (int i = i0; i < i1; ++i) {
-        const uint32_t gid = word & mask;
-        word >>= bits;
-        if (++j == ipw) {
-            j = 0; ++w;
-            if (i + 1 < i1) word = wp[(int64_t)w * kTileLoci];
-        }
-        // register table: a byte permute picks byte gid of the 8-byte table
-        // (only the low byte of desc is read)
-        const uint32_t desc = BIG ? (uint32_t)tab_lds[gid * kTileLoci + lane]
-                                  : __builtin_amdgcn_perm((uint32_t)(tab >> 32), (uint32_t)tab, gid);
-        const int v = (int)(desc & 15u);
-        const int h = v - (int)((desc >> 4) & 15u);   // the generic path's dosage form
-        Real q[K];
+    for (int w = w_first; w <= w_last; ++w) {
+        const uint32_t word = nxt;
+        nxt = wp[(int64_t)min(w + 1, w_last) * kTileLoci];
+        const int ia = max(i0, w * ipw), ib = min(i1, (w + 1) * ipw);
+        for (int i = ia, sh = (ia - w * ipw) * bits; i < ib; ++i, sh += bits) {
+            const uint32_t gid = __builtin_amdgcn_ubfe(word, (uint32_t)sh, vbits);
+            // register table: a byte permute picks byte gid of the 8-byte table
+            // (only the low byte of desc is read)
+            const uint32_t desc = BIG ? (uint32_t)tab_lds[gid * kTileLoci + lane]
+                                      : __builtin_amdgcn_perm((uint32_t)(tab >> 32), (uint32_t)tab, gid);
+            const uint32_t db = desc & 0xFFu;
+            const int v = (int)(desc & 15u);
+            const int h = v - (int)((desc >> 4) & 15u);   // the generic path's dosage form
+            Real q[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) q[k] = (Real)Q[(int64_t)i * K + k];
-        int cum[K];
+            for (int k = 0; k < K; ++k) q[k] = (Real)Q[(int64_t)i * K + k];
+            int cum[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) cum[k] = 0;
+            for (int k = 0; k < K; ++k) cum[k] = 0;
 
-        double pr = 1.0;   // REC, diploid: the individual's two factors, charged as one pair
-        constexpr int kSlotUnroll = MPT > 0 ? MPT : 1;
+            double pr = 1.0;   // REC, diploid: the individual's two factors, charged as one pair
+            constexpr int kSlotUnroll = MPT > 0 ? MPT : 1;
 #pragma unroll kSlotUnroll
-        for (int a = 0; a < MP; ++a) {
-            const uint64_t r = rng.next();
-            const bool typed = a < v;
-            const uint64_t tmask = ballot(typed);
-            // a slot no lane types (an individual below maxploidy) only
-            // consumes its draw; diploid-only tiles keep straight-line code
-            if (MPT != 2 && tmask == 0) continue;
-            // allele of slot a: a flag bit on the recoded tables, a >= h otherwise
-            const uint32_t al1b = (MPT == 2 || MPT == 4) ? ((desc >> (4 + a)) & 1u) : (a >= h ? 1u : 0u);
-            const Real* pa = pl0 + al1b * kTileLoci;
-            Real wk[K];
-            Real f = 0;
+            for (int a = 0; a < MP; ++a) {
+                const uint64_t r = rng.next();
+                const bool typed = (MPT == 2 || MPT == 4) ? db >= (uint32_t)(a + 1) * 16u : a < v;
+                const uint64_t tmask = ballot(typed);
+                // a slot no lane types (an individual below maxploidy) only
+                // consumes its draw; diploid-only tiles keep straight-line code
+                if (MPT != 2 && tmask == 0) continue;
+                // allele of slot a: a flag bit on the recoded tables, a >= h otherwise
+                const uint32_t al1b = (MPT == 2 || MPT == 4) ? ((desc >> a) & 1u) : (a >= h ? 1u : 0u);
+                const Real* pa = pl0 + al1b * kTileLoci;
+                Real wk[K];
+                Real f = 0;
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const Real pq = pa[k * 2 * kTileLoci] * q[k];
-                wk[k] = pq + eps;
-                if (REC) f = k ? f + pq : pq;
-            }
-            if (REC) {
-                const double fc = typed ? (double)f : 1.0;
-                if (MPT == 2) pr = pr * fc;
-                else acc.charge(fc);
-            }
-            Real s = wk[0];
+                for (int k = 0; k < K; ++k) {
+                    const Real pq = pa[k * 2 * kTileLoci] * q[k];
+                    wk[k] = pq + eps;
+                    if (REC) f = k ? f + pq : pq;
+                }
+                if (REC) {
+                    const double fc = typed ? (double)f : 1.0;
+                    if (MPT == 2) pr = pr * fc;
+                    else acc.charge(fc);
+                }
+                Real s = wk[0];
 #pragma unroll
-            for (int k = 1; k < K; ++k) s = s + wk[k];
-            Real tt = unit_minus1<Real>(r, ub) * s;
-            const uint32_t inc = (al1b << 16) | 1u;
+                for (int k = 1; k < K; ++k) s = s + wk[k];
+                Real tt = unit_minus1<Real>(r, ub) * s;
+                const uint32_t inc = (al1b << 16) | 1u;
 #pragma unroll
-            for (int k = 0; k < K - 1; ++k) {
-                tt = tt - wk[k];
-                const bool neg = tt < (Real)0;
-                cum[k] += __builtin_popcountll(ballot(neg) & tmask);
-                nc[k] += (neg && typed) ? inc : 0u;
+                for (int k = 0; k < K - 1; ++k) {
+                    tt = tt - wk[k];
+                    const bool neg = tt < (Real)0;
+                    cum[k] += __builtin_popcountll(ballot(neg) & tmask);
+                    nc[k] += (neg && typed) ? inc : 0u;
+                }
+                cum[K - 1] += __builtin_popcountll(tmask);
             }
-            cum[K - 1] += __builtin_popcountll(tmask);
+            if (REC && MPT == 2) acc.charge(pr);
+            // the individual's K counts, packed, into lane (i - i0) % 64 of mrow
+            const int rr = i - i0;
+#pragma unroll
+            for (int x = 0; x < MPK::MW; ++x) {
+                uint32_t pk = 0;
+#pragma unroll
+                for (int f = 0; f < MPK::FPW; ++f) {
+                    const int k = x * MPK::FPW + f;
+                    if (k < K) pk |= (uint32_t)(cum[k] - (k ? cum[k - 1] : 0)) << (f * MPK::FB);
+                }
+                mrow[x] = (uint32_t)writelane_i32((int)pk, rr & 63, (int)mrow[x]);
+            }
+            if ((rr & 63) == 63) {
+#pragma unroll
+                for (int x = 0; x < MPK::MW; ++x) atomicAdd(mi_lds + ((rr & ~63) + lane) * MPK::MW + x, mrow[x]);
+            }
         }
-        if (REC && MPT == 2) acc.charge(pr);
-        flush_mi<K>(mi_lds + (i - i0) * KH, cum, lane, mval);
+    }
+    // the last group, when partial: its rows are lanes 0 .. (i1 - i0 - 1) % 64
+    {
+        const int rr = i1 - 1 - i0;
+        if ((rr & 63) != 63 && lane <= (rr & 63)) {
+#pragma unroll
+            for (int x = 0; x < MPK::MW; ++x) atomicAdd(mi_lds + ((rr & ~63) + lane) * MPK::MW + x, mrow[x]);
+        }
     }
 
     if (valid) {
@@ -293,6 +359,9 @@ __device__ __forceinline__ double wave_sum(double v)
 template <int K, typename Real>
 __host__ __device__ constexpr size_t bi_p_bytes(int nwv) { return (size_t)nwv * K * 2 * kTileLoci * sizeof(Real); }
 __host__ __device__ inline size_t bi_tab_bytes(int nwv, int rows) { return (size_t)nwv * rows * kTileLoci; }
+// words per Mi row of the BI kernel launched for this dataset (its MPT)
+template <int K>
+inline int bi_row_words(const DeviceDataset& ds) { return ds.maxploidy == 2 ? mi_row_words<K, 2>() : mi_row_words<K, 0>(); }
 
 // Work items are (tile group of nwv tiles, chunk) pairs, w = group * G +
 // chunk.  Blocks are dealt to the XCDs round-robin by block index, so block
@@ -323,12 +392,12 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
     const int i0 = c * Nc;
     const int i1 = min(ds.N, i0 + Nc);
     const int nci = i1 - i0;
-    constexpr int KH = (K + 1) / 2;
+    constexpr int MW = mi_row_words<K, MPT>();
     uint8_t* lds_u8 = reinterpret_cast<uint8_t*>(lds_i32);
     Real* p_lds = reinterpret_cast<Real*>(lds_u8) + (size_t)wv * (K * 2 * kTileLoci);
     uint8_t* tab_lds = lds_u8 + bi_p_bytes<K, Real>(nwv) + (size_t)wv * ds.bi_tab_rows * kTileLoci;
     uint32_t* mi_lds = reinterpret_cast<uint32_t*>(lds_u8 + bi_p_bytes<K, Real>(nwv) + bi_tab_bytes(nwv, ds.bi_tab_rows));
-    for (int x = threadIdx.x; x < nci * KH; x += blockDim.x) mi_lds[x] = 0;
+    for (int x = threadIdx.x; x < nci * MW; x += blockDim.x) mi_lds[x] = 0;
 
     const int64_t tix = (w / G) * nwv + wv;
     const bool has_tile = tix < ds.nbi;
@@ -372,7 +441,7 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
         for (int x = 0; x < nwv; ++x) s += wsum[x];
         zlik[w] = s;
     }
-    flush_mi_rows<K>(mi_lds, nci, Mi + (int64_t)i0 * K);
+    flush_bi_rows<K, MPT>(mi_lds, nci, Mi + (int64_t)i0 * K);
 }
 
 // ---------------------------------------------------------------------------
@@ -723,7 +792,7 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st,
 {
     const int nthr = 64 * kWavesPerBlock;
     const int A = std::max(ds.maxK, 1);
-    const size_t mi_packed = (size_t)c.Nc * ((K + 1) / 2) * sizeof(uint32_t);
+    const size_t mi_packed = (size_t)c.Nc * bi_row_words<K>(ds) * sizeof(uint32_t);
     const GenPlan g = gen_plan<K, Real>(ds, c.Nc, c.G);
     // GEN counts in global atomics add into the chunk partials, which must
     // then be cleared before any tile (of either kind) writes its counts
@@ -772,7 +841,7 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st,
 template <int K, typename Real>
 int z_occupancy(const DeviceDataset& ds, int Nc, int* bi_blocks, int* gen_waves, int* gen_block_waves)
 {
-    const size_t mi_packed = (size_t)Nc * ((K + 1) / 2) * sizeof(uint32_t);
+    const size_t mi_packed = (size_t)Nc * bi_row_words<K>(ds) * sizeof(uint32_t);
     const size_t p_bytes = bi_p_bytes<K, Real>(kWavesPerBlock) + bi_tab_bytes(kWavesPerBlock, ds.bi_tab_rows);
     *bi_blocks = 1;
     hipError_t e = hipSuccess;
