@@ -120,7 +120,8 @@ def main():
         cpu = cpu_baseline(s, p, K, args.cpu_seconds)
 
     out = {
-        "metric": "MCMC sweeps/s at N=2000,L=100k,K=5",
+        "metric": ("MCMC sweeps/s at N=2000,L=100k,K=5" if args.workload == "admix_n2000_l100k_k5"
+                   else f"MCMC sweeps/s ({args.workload})"),
         "value": round(sweeps_s, 3),
         "unit": "sweeps/s",
         "n_gpus": 1,
@@ -132,7 +133,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if prec == V.PREC_FP64 else "f32-weights/f64",
         "data": "synthetic (seeded generator: Fst 0.05 drift, 2% missing)",
-        "config": {"workload": args.workload, "N": ds.N, "L": ds.L, "K": K, "model": "admixture+alpha Metropolis",
+        "config": {"workload": args.workload, "N": ds.N, "L": ds.L, "K": K, "model": "admixture+alpha Metropolis" + ("+locprior" if spec.get("locpriori") else ""),
                    "rng_topology": args.topology, "precision": args.precision,
                    "record_every": 10, "metro_every": 10,
                    "z_chunks": {"G": G, "individuals_per_chunk": Nc, "chosen_by": "timed candidates"}},
