@@ -26,3 +26,22 @@ def test_example_par_runs(vps, tmp_path):
         assert "Model=ADM,LOC,F" in text and f"Number of cluster={s['k']}" in text
     rows = (tmp_path / "res" / "out.structure.txt").read_text().splitlines()
     assert sum(1 for r in rows if r.split("\t")[0].isdigit()) == 20
+
+
+def test_example_par_eval_mode(vps, tmp_path):
+    """-structure_eval=yes: K = 5, one sweep, both models, every pass on the
+    FP64 and mixed device paths (vcfpop's evaluation mode)"""
+    from vcfpop_amd import par
+    shutil.copy(os.path.join(HERE, "golden", "data.polygene.txt"), tmp_path / "data.polygene.txt")
+    text = open(os.path.join(HERE, "golden", "par.polygene.txt")).read().replace("polygene\\out", "res\\out")
+    (tmp_path / "par.txt").write_text(text + "\n-structure_eval=yes\n")
+    print("example par, eval mode", flush=True)
+    lines = []
+    rows = par.run(str(tmp_path / "par.txt"), device=0, log=lines.append)
+    assert {r["backend"] for r in rows} == {"fp64", "mixed"}
+    assert {(r["model"], r["pass"]) for r in rows} >= {("admix", "update_z_admix"), ("noadmix", "update_z_noadmix"),
+                                                      ("admix", "record"), ("noadmix", "update_q_noadmix")}
+    by = {(r["model"], r["pass"], r["backend"]): r for r in rows}
+    for model in ("admix", "noadmix"):    # the same seed: initialisation agrees on both paths
+        assert by[(model, "init", "fp64")]["Mi"] == by[(model, "init", "mixed")]["Mi"]
+    assert any("update_z_admix" in ln for ln in lines)

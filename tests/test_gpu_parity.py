@@ -20,6 +20,8 @@ RTOL = 1e-12
 DATASETS = {
     # biallelic diploid (BI tiles only)
     "bi_diploid": dict(n_ind=96, n_loc=700, k_true=3, alpha_q=0.5, fst=0.05, missing=0.02, seed=21),
+    # N a multiple of 64: chunks that end exactly on a 64-individual row group
+    "bi_diploid_128": dict(n_ind=128, n_loc=300, k_true=3, alpha_q=0.5, fst=0.05, missing=0.02, seed=25),
     # mixed ploidy 2/4/6 with multiallelic loci and locations (BI and GEN tiles)
     "mixed": dict(n_ind=75, n_loc=500, k_true=4, alpha_q=0.5, fst=0.05, missing=0.03, frac_multi=0.25,
                   min_alleles=3, max_alleles=8, multi_dir=0.3, ploidy=[(2, 0.6), (4, 0.3), (6, 0.1)], n_pop=4,
@@ -115,8 +117,8 @@ def test_count_tables_beyond_lds(vps, orc, pairs, K):
     assert (g.get("Mi") == o.get("Mi")).all()
 
 
-@pytest.mark.parametrize("name", ["bi_diploid", "mixed"])
-@pytest.mark.parametrize("z_chunk", [1, 7, 32, 512])
+@pytest.mark.parametrize("name", ["bi_diploid", "bi_diploid_128", "mixed"])
+@pytest.mark.parametrize("z_chunk", [1, 7, 32, 64, 512])
 def test_z_chunking_invariance(vps, orc, pairs, name, z_chunk):
     """every individual chunking of the genotype passes gives the oracle's
     Z pass (admixture and no-admixture) bit for bit"""

@@ -29,3 +29,11 @@ def test_unknown_structure_option_is_refused(vps):
         par.structure_params({"bogus": "1"}, 0)
     with pytest.raises(ValueError, match="yes or no"):
         par.structure_params({"admix": "maybe"}, 0)
+
+
+def test_eval_option(vps):
+    from vcfpop_amd import par
+    par.structure_params({"eval": "yes"}, 0)
+    par.structure_params({"eval": "no"}, 0)
+    with pytest.raises(ValueError, match="yes or no"):
+        par.structure_params({"eval": "1"}, 0)

@@ -13,6 +13,11 @@ order and run seed = HashULong(g_seed + id).  All runs are one batch on
 one device; each finished run writes its WriteStructure table and the
 runs' WriteStructureSummary table goes to <g_output>.structure.txt.
 Populations come from the file's population column.
+
+-structure_eval=yes switches to vcfpop's evaluation mode instead
+(src/function.cpp:4743-4766): K = 5, one run, one sweep, both models, each
+pass timed on the FP64 and the mixed-precision device paths with digests of
+its integer state (vcfpop_amd.evalmode).
 """
 import argparse
 import os
@@ -28,7 +33,7 @@ _BOOL_OPTS = {"admix", "locpriori", "f", "inferlambda", "difflambda", "diversity
 _INT_OPTS = {"nburnin", "nreps", "nthinning", "nruns", "nadmburnin", "metrofreq"}
 _DBL_OPTS = {"lambda", "stdlambda", "maxlambda", "alpha", "stdalpha", "maxalpha", "alphapriora", "alphapriorb",
              "r", "maxr", "stdr", "epseta", "epsgamma", "pmeanf", "pstdf", "stdf"}
-_IGNORED = {"plot", "nstream", "eval"}
+_IGNORED = {"plot", "nstream"}
 
 
 def read_par(path):
@@ -56,6 +61,9 @@ def structure_params(st, g_seed):
             kw[k] = int(v)
         elif k in _DBL_OPTS:
             kw[k] = float(v)
+        elif k == "eval":
+            if v.lower() not in _YESNO:
+                raise ValueError(f"-structure_{k}={v}: expected yes or no")
         elif k == "krange":
             m = re.match(r"^\[?\s*(\d+)\s*,\s*(\d+)\s*\]?$", v)
             if not m:
@@ -68,7 +76,25 @@ def structure_params(st, g_seed):
     return kw, krange
 
 
+class _LogFile:
+    """a file-like adapter that sends whole lines to a log function"""
+
+    def __init__(self, log):
+        self.log, self.buf = log, ""
+
+    def write(self, text):
+        self.buf += text
+        while "\n" in self.buf:
+            line, self.buf = self.buf.split("\n", 1)
+            self.log(line)
+
+    def flush(self):
+        pass
+
+
 def run(par_path, device=0, sweeps=None, topology="reference", log=print):
+    """Run a parameter file's clustering; returns the runs' summaries (or,
+    with -structure_eval=yes, the evaluation rows)"""
     g, st = read_par(par_path)
     base = os.path.dirname(os.path.abspath(par_path))
     if g.get("format", "polygene").lower() != "polygene":
@@ -92,6 +118,14 @@ def run(par_path, device=0, sweeps=None, topology="reference", log=print):
     im = Image(inp)
     log(f"{inp}: {im.N} individuals, {im.L} of {im.loci_read} loci (fewer than 2 alleles dropped)")
     ds = Dataset(im, device)
+    if st.get("eval", "no").lower() == "yes":
+        from . import evalmode
+        base = {k: v for k, v in kw.items() if k not in ("krange_min", "krange_max", "nruns", "rng_topology")}
+        rows = evalmode.run_eval(ds, K=5, seed=int(g.get("seed", 0)), base=base)
+        evalmode.print_table(rows, file=_LogFile(log))
+        ds.close()
+        im.close()
+        return rows
     runs = [(k, rep) for k in range(kmin, kmax + 1) for rep in range(1, nruns + 1)]
     chains = [Chain(ds, p, k, i + 1) for i, (k, rep) in enumerate(runs)]
     t0 = time.perf_counter()
@@ -121,7 +155,8 @@ def main(argv=None):
     ap.add_argument("--topology", choices=["reference", "parallel"], default="reference")
     args = ap.parse_args(argv)
     for s in run(args.par, args.device, args.sweeps, args.topology):
-        print(f"id {s['id']} K={s['k']} rep {s['rep']}: mean lnL {s['mean_lnl']:.3f}, lnP(D) {s['lnpd']:.3f}")
+        if "id" in s:
+            print(f"id {s['id']} K={s['k']} rep {s['rep']}: mean lnL {s['mean_lnl']:.3f}, lnP(D) {s['lnpd']:.3f}")
 
 
 if __name__ == "__main__":
