@@ -46,7 +46,10 @@ struct ChainDev {
     double*   Lambda;    // [K]
     double*   bufNK1;    // [N*K] Metropolis proposals / no-admix log likelihoods
     double*   bufN1;     // [N]   Metropolis log ratios
+    double*   bufN2;     // [N]   Metropolis pass: each individual's log likelihood under its current Q,
+                         //       then (after acceptance) under the Q it keeps
     double*   part;      // partial sums of the likelihood passes
+    double*   part_cur;  // Metropolis pass only: the current-Q partials (set by the launch)
     double*   zlik;      // [z_lik_parts(G)] Record lnL partials of the FP64 Z pass
     int64_t*  MiSum;     // [N*K]
     double*   FreqM;     // [K*KT]

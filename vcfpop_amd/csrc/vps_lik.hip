@@ -63,7 +63,11 @@ __device__ void lik_write(const DeviceDataset& ds, const ChainDev& c, FrexpAcc* 
     if (MODE <= 1) {
         block_store_sum(ok ? acc[0].close() : 0.0, c.part + block_slot);
     } else if (MODE == 2) {
-        if (ok) c.part[range * ds.N + i] = acc[0].close() - acc[1].close();
+        if (ok) {
+            const double cur = acc[1].close();
+            c.part[range * ds.N + i] = acc[0].close() - cur;
+            c.part_cur[range * ds.N + i] = cur;
+        }
     } else {
         if (ok) {
             double* out = c.part + range * ds.N * K + (int64_t)i * K;
@@ -302,30 +306,41 @@ __global__ __launch_bounds__(256) void k_reduce_lnl(const double* __restrict__ p
 // 4 threads each; thread s sums ranges s, s+4, ... and the four partials are
 // added in a fixed order.
 constexpr int kRowSplit = 4;
+// rows_sum: one split's share of the range partials of entry x, in a fixed order
+__device__ __forceinline__ double rows_sum(const double* __restrict__ p, int64_t n, int nranges, int split)
+{
+    int r = split;
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (; r + 3 * kRowSplit < nranges; r += 4 * kRowSplit) {
+        a0 += p[(int64_t)r * n];
+        a1 += p[(int64_t)(r + kRowSplit) * n];
+        a2 += p[(int64_t)(r + 2 * kRowSplit) * n];
+        a3 += p[(int64_t)(r + 3 * kRowSplit) * n];
+    }
+    for (; r < nranges; r += kRowSplit) a0 += p[(int64_t)r * n];
+    return (a0 + a1) + (a2 + a3);
+}
+
+// cur/dst_cur (the Metropolis pass): the current-Q partials, summed the same way
 __global__ __launch_bounds__(256) void k_reduce_rows(const DeviceDataset ds, const ChainDev c, int nranges, int NA,
-                                                     int add_prior, double* __restrict__ dst)
+                                                     int add_prior, double* __restrict__ dst,
+                                                     const double* __restrict__ cur, double* __restrict__ dst_cur)
 {
     __shared__ double sh[kRowSplit][64];
+    __shared__ double sc[kRowSplit][64];
     const int lane = threadIdx.x & 63, split = threadIdx.x >> 6;
     const int64_t x = (int64_t)blockIdx.x * 64 + lane;
     const int64_t n = (int64_t)ds.N * NA;
-    double s = 0;
+    double s = 0, sk = 0;
     if (x < n) {
-        const double* p = c.part + x;
-        int r = split;
-        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-        for (; r + 3 * kRowSplit < nranges; r += 4 * kRowSplit) {
-            a0 += p[(int64_t)r * n];
-            a1 += p[(int64_t)(r + kRowSplit) * n];
-            a2 += p[(int64_t)(r + 2 * kRowSplit) * n];
-            a3 += p[(int64_t)(r + 3 * kRowSplit) * n];
-        }
-        for (; r < nranges; r += kRowSplit) a0 += p[(int64_t)r * n];
-        s = (a0 + a1) + (a2 + a3);
+        s = rows_sum(c.part + x, n, nranges, split);
+        if (cur) sk = rows_sum(cur + x, n, nranges, split);
     }
     sh[split][lane] = s;
+    sc[split][lane] = sk;
     __syncthreads();
     if (split != 0 || x >= n) return;
+    if (cur) dst_cur[x] = (sc[0][lane] + sc[1][lane]) + (sc[2][lane] + sc[3][lane]);
     double t = 0;
     if (add_prior) {
         const int i = (int)(x / NA), k = (int)(x % NA);
@@ -455,12 +470,16 @@ int lik_part_count(const DeviceDataset& ds, int mode, int K, const int32_t* tpr)
     const LikGrid g = lik_grid(ds, tpr);
     const int64_t nr = (int64_t)g.nr_bi + g.nr_gen;
     if (mode <= 1) return (int)(nr * g.gy);
-    return (int)(nr * ds.N * (mode == 3 ? K : 1));
+    return (int)(nr * ds.N * (mode == 3 ? K : 2));   // mode 2: ratios, then current-Q partials
 }
 
-int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c, int mode, hipStream_t st)
+int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c0, int mode, hipStream_t st)
 {
-    const LikGrid g = lik_grid(ds, c.lik_tpr);
+    const LikGrid g = lik_grid(ds, c0.lik_tpr);
+    const int nranges = g.nr_bi + g.nr_gen;
+    // the Metropolis pass also keeps its current-Q partials, after the ratios
+    ChainDev c = c0;
+    c.part_cur = c0.part + (int64_t)nranges * ds.N;
     int rc;
     switch (mode) {
     case 0: rc = launch_lik_mode<0>(ds, c, st, g); break;
@@ -470,7 +489,6 @@ int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c, int mode, hipStr
     default: return fail(VPS_ERR_INVALID, "likelihood mode %d", mode);
     }
     if (rc) return rc;
-    const int nranges = g.nr_bi + g.nr_gen;
     if (mode <= 1) {
         hipLaunchKernelGGL(k_reduce_lnl, dim3(1), dim3(256), 0, st, c.part, (int64_t)nranges * g.gy, c.scal + 1);
     } else {
@@ -478,7 +496,8 @@ int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c, int mode, hipStr
         const int64_t n = (int64_t)ds.N * NA;
         double* dst = mode == 2 ? c.bufN1 : c.bufNK1;
         hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)((n + 63) / 64)), dim3(64 * kRowSplit), 0, st, ds, c, nranges, NA,
-                           (mode == 3 && c.locpriori) ? 1 : 0, dst);
+                           (mode == 3 && c.locpriori) ? 1 : 0, dst, mode == 2 ? (const double*)c.part_cur : nullptr,
+                           mode == 2 ? c.bufN2 : nullptr);
     }
     VPS_HIP(hipGetLastError());
     return VPS_OK;

@@ -238,7 +238,8 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     const int64_t N = d.N, KT = d.KT, S = d.S;
     int rc = 0;
     rc = choose_lik_tpr(d, K, ds->device, c.lik_tpr);
-    const int nparts = rc ? 0 : std::max({lik_part_count(d, 0, K, c.lik_tpr), lik_part_count(d, 3, K, c.lik_tpr)});
+    const int nparts = rc ? 0 : std::max({lik_part_count(d, 0, K, c.lik_tpr), lik_part_count(d, 2, K, c.lik_tpr),
+                                          lik_part_count(d, 3, K, c.lik_tpr)});
     if (!rc) rc = chain_alloc(ch, &c.Freq, K * KT + 2);   // +2: paired allele loads past the last locus
     if (!rc) rc = chain_alloc(ch, &c.Q, N * K);
     if (!rc) rc = chain_alloc(ch, &c.Z, N + 32);
@@ -264,6 +265,7 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     if (!rc) rc = chain_alloc(ch, &c.Lambda, K);
     if (!rc) rc = chain_alloc(ch, &c.bufNK1, N * K);
     if (!rc) rc = chain_alloc(ch, &c.bufN1, N);
+    if (!rc) rc = chain_alloc(ch, &c.bufN2, N);
     if (!rc) rc = chain_alloc(ch, &c.part, nparts);
     if (!rc) rc = chain_alloc(ch, &c.MiSum, N * K);
     if (!rc) rc = chain_alloc(ch, &c.FreqM, K * KT);
@@ -306,6 +308,7 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     if (!rc) rc = fill_f64(ch, c.rout, c.rlen, 0.0);
     if (!rc) rc = fill_f64(ch, c.bufNK1, N * K, 0.0);
     if (!rc) rc = fill_f64(ch, c.bufN1, N, 0.0);
+    if (!rc) rc = fill_f64(ch, c.bufN2, N, 0.0);
     if (!rc) rc = fill_f64(ch, c.scal, 8, 0.0);
     if (!rc) rc = fill_f64(ch, c.scal, 1, 1.0);   // r = 1 (InitFix)
     if (!rc) rc = fill_f64(ch, c.FreqA, 3 * KT, 0.0);
@@ -480,9 +483,12 @@ int launch_sweep(vps_chain* ch, const SweepPlan& p)
     if (!rc) rc = launch_update_lambda(d, cs, side);
     if (!rc) rc = launch_update_a(d, cs, side);
     if (!rc) rc = launch_update_f(d, cs, side);
-    // Record's admixture lnL comes from the FP64 Z pass itself: its weights
-    // start from the same products P[k][l][x] Q[i][k] (the same P and Q)
-    const bool zrec = p.record && p.q != kQNoAdmix && c.precision != VPS_PREC_MIXED;
+    // Record's admixture lnL: after a Metropolis UpdateQ, the sum of the
+    // per-individual log likelihoods that pass left under the kept Q (the
+    // same P and Q Record reads); otherwise from the FP64 Z pass itself, whose
+    // weights start from the same products P[k][l][x] Q[i][k]
+    const bool mrec = p.record && p.q == kQMetro;
+    const bool zrec = p.record && !mrec && p.q != kQNoAdmix && c.precision != VPS_PREC_MIXED;
     // without Record, the Z pass's last kernel advances the sweep index when
     // nothing beside it still reads the index
     const bool zadv = !p.record && p.q != kQNoAdmix && (fast || side == ch->stream);
@@ -494,7 +500,8 @@ int launch_sweep(vps_chain* ch, const SweepPlan& p)
         else VPS_HIP(hipStreamWaitEvent(ch->stream, ch->ev_join, 0));
     }
     if (!rc && p.record) {
-        if (!zrec) rc = launch_lik_pass(d, c, p.q == kQNoAdmix ? 1 : 0, ch->stream);
+        if (mrec) rc = launch_reduce_lnl(c.bufN2, d.N, c.scal + 1, ch->stream);
+        else if (!zrec) rc = launch_lik_pass(d, c, p.q == kQNoAdmix ? 1 : 0, ch->stream);
         if (!rc) rc = launch_record_accumulate(d, c, p.post, p.trace, 1, ch->stream);
     }
     if (!rc && !zadv) rc = launch_advance_m(c, ch->stream);

@@ -118,7 +118,7 @@ int64_t estimate_bytes(const HostImage& h, int K)
     const int64_t parts = std::min<int64_t>(kEstChunkParts, (N + 15) / 16);
     const int64_t chain = K * KT * (8 + 4 + 8) + parts * (K * KT * 4 + (ntiles / 4 + ntiles) * 8) + 3 * KT * 8 +
                           N * K * (8 + 4 + 8 + 8 + 8) +
-                          N * 64 + (ntiles / 4 + 2) * N * K * 8 + (64 << 10);
+                          N * 64 + (ntiles / 4 + 2) * N * std::max<int64_t>(K, 2) * 8 + (64 << 10);
     return dataset + chain;
 }
 

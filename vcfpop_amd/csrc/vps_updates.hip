@@ -178,6 +178,11 @@ __global__ void k_q_metro_propose(const DeviceDataset ds, const ChainDev c)
     c.rngbuf[i] = r;
 }
 
+// Acceptance (src/structure.cpp:1039): a proposal is kept when its log
+// ratio is >= -1e-10 or, only then drawn, a uniform falls below exp(ratio).
+// bufN2 arrives as each individual's log likelihood under its current Q and
+// leaves as the one under the Q it keeps (current + ratio when accepted):
+// a Record in the same sweep sums it instead of another genotype pass.
 __global__ void k_q_metro_accept(const DeviceDataset ds, const ChainDev c)
 {
     const int K = c.K;
@@ -186,16 +191,20 @@ __global__ void k_q_metro_accept(const DeviceDataset ds, const ChainDev c)
         Rng r = c.rngbuf[0];
         for (int i = 0; i < ds.N; ++i) {
             if (!ds.vt[i]) continue;
-            if (c.bufN1[i] >= kNZero || r.uniform() < exp(c.bufN1[i]))
+            if (c.bufN1[i] >= kNZero || r.uniform() < exp(c.bufN1[i])) {
                 for (int k = 0; k < K; ++k) c.Q[(int64_t)i * K + k] = c.bufNK1[(int64_t)i * K + k];
+                c.bufN2[i] += c.bufN1[i];
+            }
         }
         return;
     }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ds.N || !ds.vt[i]) return;
     Rng r = c.rngbuf[i];
-    if (c.bufN1[i] >= kNZero || r.uniform() < exp(c.bufN1[i]))
+    if (c.bufN1[i] >= kNZero || r.uniform() < exp(c.bufN1[i])) {
         for (int k = 0; k < K; ++k) c.Q[(int64_t)i * K + k] = c.bufNK1[(int64_t)i * K + k];
+        c.bufN2[i] += c.bufN1[i];   // the log likelihood under the Q it keeps
+    }
 }
 
 int launch_q_metro_propose(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
