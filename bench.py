@@ -111,7 +111,8 @@ def main():
     traffic = None
     tp = os.path.join(ROOT, "profiles", "zpass_admix_n2000_l100k_k5_traffic.json")
     if os.path.exists(tp) and args.workload == "admix_n2000_l100k_k5":
-        traffic = json.load(open(tp)).get("bytes_per_launch")
+        tj = json.load(open(tp))   # rocprof FETCH_SIZE/WRITE_SIZE per launch, by chunk count
+        traffic = tj.get("bytes_per_launch_by_G", {}).get(str(G), tj.get("bytes_per_launch") if G == tj.get("chunks_G") else None)
     ch.close()
 
     cpu = None
