@@ -244,8 +244,9 @@ def make_jobs(V, n, rate, scale, seed=7):
         kw = (dict(admix_val=1) if model == "admix" else
               dict(admix_val=2, nadmburnin=sweeps // 10) if model == "noadmix" else dict(admix_val=1, f_val=1))
         burn = sweeps // 4
+        # z_chunk=-1: the chunk model, as the server uses it, for the plain runs too (no timing trials)
         p = V.params_default(nburnin=burn, nreps=sweeps - burn - kw.get("nadmburnin", 0),
-                             rng_topology=V.RNG_PARALLEL, g_seed=j, **kw)
+                             rng_topology=V.RNG_PARALLEL, g_seed=j, z_chunk=-1, **kw)
         jobs.append(dict(arrival=t, synth=s, params=p, K=K, N=N, L=L, sweeps=sweeps, model=model))
     return jobs
 
