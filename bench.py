@@ -191,10 +191,24 @@ def bench_batched(V, S):
     s = V.make_synth("batch_n1000_l50k_k2to9x8")
     ds = V.Dataset(s, 0)
 
-    def chains():
+    def chains(precision=V.PREC_FP64):
         return [V.Chain(ds, V.params_default(admix_val=1, nburnin=0, nreps=5000, g_seed=1,
-                                             rng_topology=V.RNG_PARALLEL), K, r)
+                                             rng_topology=V.RNG_PARALLEL, precision=precision), K, r)
                 for K in range(2, 10) for r in range(1, 9)]
+
+    def run_batch(cs):
+        b = V.Batch(cs)
+        b.step(2)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(S)
+        b.sync()
+        t = (time.perf_counter() - t0) / S
+        st = b.stats()
+        b.close()
+        for c in cs:
+            c.close()
+        return t, st
 
     cs = chains()
     for c in cs:
@@ -207,24 +221,18 @@ def bench_batched(V, S):
     plain = (time.perf_counter() - t0) / S
     for c in cs:
         c.close()
-    cs = chains()
-    b = V.Batch(cs)
-    b.step(2)
-    b.sync()
-    t0 = time.perf_counter()
-    b.step(S)
-    b.sync()
-    batched = (time.perf_counter() - t0) / S
-    st = b.stats()
-    b.close()
-    for c in cs:
-        c.close()
+    batched, st = run_batch(chains())
+    # the same batch with FP32 weights (FP64 lnL, draws and Metropolis steps)
+    mixed, _ = run_batch(chains(V.PREC_MIXED))
     ds.close()
     per_hr = lambda t: 64 * 3600.0 / (t * 5000)
     return {"workload": "batch_n1000_l50k_k2to9x8", "chains": 64, "sweeps_per_chain": 5000,
             "measured_sweeps": S, "plain_chains_per_hr": round(per_hr(plain), 1),
             "batched_chains_per_hr": round(per_hr(batched), 1), "speedup": round(plain / batched, 3),
-            "ms_per_sweep_of_64": {"plain": round(plain * 1e3, 3), "batched": round(batched * 1e3, 3)},
+            "batched_mixed_chains_per_hr": round(per_hr(mixed), 1),
+            "speedup_mixed_vs_plain_fp64": round(plain / mixed, 3),
+            "ms_per_sweep_of_64": {"plain": round(plain * 1e3, 3), "batched": round(batched * 1e3, 3),
+                                   "batched_mixed": round(mixed * 1e3, 3)},
             "graphs": st}
 
 
@@ -281,7 +289,7 @@ def bench_served(V, n, rate, scale):
         end = max(end, j["arrival"]) + d_
         lat_plain.append(end - j["arrival"])
 
-    def serve(at_arrival):
+    def serve(at_arrival, mixed=False):
         srv = V.Server(hbm_budget_bytes=128 << 30, max_active=32, slice_sweeps=16, loaders=2)
         ids = []
         t0 = time.perf_counter()
@@ -289,7 +297,11 @@ def bench_served(V, n, rate, scale):
             now = time.perf_counter() - t0
             if at_arrival and now < j["arrival"]:
                 time.sleep(j["arrival"] - now)
-            ids.append(srv.submit(j["synth"], j["params"], j["K"], 1))
+            p = j["params"]
+            if mixed:
+                p = V.params_default(**{f: getattr(p, f) for f, _ in p._fields_})
+                p.precision = V.PREC_MIXED
+            ids.append(srv.submit(j["synth"], p, j["K"], 1))
         for i in ids:
             srv.wait(i, 900.0)
         span = time.perf_counter() - t0
@@ -300,6 +312,7 @@ def bench_served(V, n, rate, scale):
 
     open_span, lat, st = serve(True)
     sat_span, _, st_sat = serve(False)
+    mix_span, _, _ = serve(False, mixed=True)
     pct = lambda a, q: round(float(np.percentile(a, q)), 3)
     return {"jobs": n, "arrival_rate_per_s": rate,
             "sweeps_per_job": f"U[2000,10000]/{scale}", "model_mix": "admix/no-admix/F",
@@ -310,6 +323,9 @@ def bench_served(V, n, rate, scale):
                           "served_chains_per_hr": round(n * 3600.0 / sat_span, 1),
                           "plain_chains_per_hr": round(n * 3600.0 / plain_sat, 1),
                           "speedup": round(plain_sat / sat_span, 3),
+                          "served_mixed_makespan_s": round(mix_span, 3),
+                          "served_mixed_chains_per_hr": round(n * 3600.0 / mix_span, 1),
+                          "speedup_mixed_vs_plain_fp64": round(plain_sat / mix_span, 3),
                           "peak_reserved_GB": round(st_sat["peak_reserved_bytes"] / 1e9, 3)},
             "policy": "FIFO admission under a 128 GB budget, at most 32 jobs on the device; "
                       "latency counts queueing; the stream is drained"}
