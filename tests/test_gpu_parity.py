@@ -28,6 +28,9 @@ DATASETS = {
                   seed=22),
     # ragged: N not a multiple of anything, a single short tile
     "ragged": dict(n_ind=13, n_loc=37, k_true=2, alpha_q=1.0, missing=0.2, ploidy=[(1, 0.3), (3, 0.7)], seed=23),
+    # the maximum ploidy (8 and 10 copies; BI tiles on the runtime slot loop, GEN descriptors of 10 alleles)
+    "high_ploidy": dict(n_ind=40, n_loc=200, k_true=3, alpha_q=0.5, missing=0.05, frac_multi=0.4, min_alleles=3,
+                        max_alleles=12, multi_dir=0.5, ploidy=[(8, 0.5), (10, 0.5)], n_pop=2, seed=26),
     # many alleles: K x alleles tables beyond one block's LDS at large K
     "many_alleles": dict(n_ind=40, n_loc=160, k_true=3, alpha_q=0.5, missing=0.05, frac_multi=0.6, min_alleles=12,
                          max_alleles=40, multi_dir=0.5, ploidy=[(2, 0.5), (4, 0.5)], n_pop=2, seed=24),
