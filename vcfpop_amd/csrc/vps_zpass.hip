@@ -93,7 +93,7 @@ __device__ __forceinline__ uint64_t ballot(bool b) { return __builtin_amdgcn_bal
 // builtin for
 __device__ int writelane_i32(int val, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
 
-constexpr int kWavesMax = 4;   // wavefronts per block of the Z kernels
+constexpr int kWavesMax = 8;   // wavefronts per block of the Z kernels (BI: 4; GEN: up to 8)
 constexpr int kXcds = 8;       // XCDs of the device; blocks are dealt to them round-robin
 
 // Flush the K per-individual cluster counts of one wavefront (cumulative
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
 
 // ---------------------------------------------------------------------------
 // GEN tiles: any allele count (< 64) and ploidy mix.  A block holds one tile
-// and up to 4 consecutive chunks of individuals, one wavefront per chunk, so
+// and up to 8 consecutive chunks of individuals, one wavefront per chunk, so
 // the wavefronts share the tile's tables: with PLDS its P values sit in LDS
 // as [k][allele][lane] values (A alleles per lane column), so a slot's K
 // weights are K conflict-free LDS reads rather than K scattered HBM/L2
@@ -464,7 +464,7 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
 // REC: each block also writes the Record lnL partial of its lanes to
 // zlik[zbase + blockIdx.y * gridDim.x + blockIdx.x].
 template <int K, typename Real, bool PLDS, bool REC>
-__global__ __launch_bounds__(256) void k_z_admix_gen(const DeviceDataset ds, const double* __restrict__ Freq,
+__global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceDataset ds, const double* __restrict__ Freq,
                                                     const double* __restrict__ Q, uint64_t seed,
                                                     const int32_t* __restrict__ mp,
                                                     int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
@@ -760,10 +760,12 @@ int max_block_lds()
     return bytes;
 }
 
-// Launch shape of the GEN kernel: one tile x `waves` chunks per block (4,
-// fewer when there are fewer chunks); P staged when P, the count table and
-// the Mi rows fit one block's LDS, else counts alone in LDS, else counts by
-// global atomics.
+// Launch shape of the GEN kernel: one tile x `waves` chunks per block; P
+// staged when P, the count table and the Mi rows fit one block's LDS, else
+// counts alone in LDS, else counts by global atomics.  The tile's P and
+// count tables are shared by the block's waves, so when LDS limits the
+// resident blocks, 8 chunks per block keep more waves resident than 4 (the
+// shape with more resident waves is taken; fewer when there are fewer chunks).
 struct GenPlan {
     int waves;
     bool plds;
@@ -772,9 +774,8 @@ struct GenPlan {
 };
 
 template <int K, typename Real>
-GenPlan gen_plan(const DeviceDataset& ds, int Nc, int G)
+GenPlan gen_plan_waves(const DeviceDataset& ds, int Nc, int wv)
 {
-    const int wv = std::max(1, std::min(kWavesPerBlock, G));
     const size_t A = (size_t)std::max(ds.maxK, 1);
     const size_t kc = K > 1 ? K - 1 : 1;
     const size_t mi = (size_t)wv * Nc * ((K + 1) / 2) * sizeof(uint32_t);
@@ -784,6 +785,22 @@ GenPlan gen_plan(const DeviceDataset& ds, int Nc, int G)
     if (pb + ni + mi <= cap) return {wv, true, true, pb + ni + mi};
     if (ni + mi <= cap) return {wv, false, true, ni + mi};
     return {wv, false, false, mi};
+}
+
+template <int K, typename Real>
+GenPlan gen_plan(const DeviceDataset& ds, int Nc, int G)
+{
+    const GenPlan g4 = gen_plan_waves<K, Real>(ds, Nc, std::max(1, std::min(kWavesPerBlock, G)));
+    if (G < kWavesMax) return g4;
+    const GenPlan g8 = gen_plan_waves<K, Real>(ds, Nc, kWavesMax);
+    if (g8.plds != g4.plds || g8.ni_in_lds != g4.ni_in_lds) return g4;
+    // resident waves per CU as LDS allows (160 KiB per CU), capped by 32 waves
+    const size_t cu_lds = 160 * 1024;
+    auto resident = [&](const GenPlan& g) {
+        const size_t blocks = g.shm ? cu_lds / g.shm : 64;
+        return std::min<size_t>(blocks * g.waves, 32);
+    };
+    return resident(g8) > resident(g4) ? g8 : g4;
 }
 
 // nparts (may be null): the number of Record partials written
