@@ -241,8 +241,15 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_gen(const DeviceDataset ds,
         const int64_t lj = ds.tloc[td.lmap + lane];
         const int afo_v = ds.afo[lj];
         const int64_t gb_v = ds.gbase[lj];
-        // ids of 4 loci per 16-byte load, the next group's issued first
+        // ids of 4 loci per 16-byte load, the next group's issued first;
+        // each locus's genotype descriptor is loaded one locus ahead
+        auto desc_of = [&](int j, uint32_t word) -> uint64_t {
+            const int64_t gb = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(gb_v >> 32), j) << 32) |
+                                         (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)gb_v, j));
+            return ds.gdesc[gb + ((word >> sh) & mask)];
+        };
         uint4 wn = *reinterpret_cast<const uint4*>(wp);
+        uint64_t dn = desc_of(0, wn.x);
         for (int j0 = 0; j0 < td.nloc; j0 += 4) {
             const uint4 w4 = wn;
             if (j0 + 4 < td.nloc) wn = *reinterpret_cast<const uint4*>(wp + j0 + 4);
@@ -251,10 +258,8 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_gen(const DeviceDataset ds,
             for (int u = 0; u < 4; ++u) {
                 const int j = j0 + u;
                 if (j >= td.nloc) break;
-                const int64_t gb = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(gb_v >> 32), j) << 32) |
-                                             (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)gb_v, j));
-                const uint32_t gid = (wq[u] >> sh) & mask;
-                const uint64_t d64 = ds.gdesc[gb + gid];
+                const uint64_t d64 = dn;
+                if (j + 1 < td.nloc) dn = desc_of(j + 1, u < 3 ? wq[u + 1] : wn.x);
                 const int v = ok ? (int)(d64 & 15u) : 0;
                 const double* pl = c.Freq + __builtin_amdgcn_readlane(afo_v, j);
                 for (int a = 0; a < v; ++a) {
@@ -263,12 +268,17 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_gen(const DeviceDataset ds,
 #pragma unroll
                         for (int k = 0; k < K; ++k) acc[k].charge(px[k * KT]);
                     } else {
+                        // (the Metropolis ratio's mixtures fuse their multiply-adds, as on BI tiles)
                         double f = 0, g = 0;
 #pragma unroll
                         for (int k = 0; k < K; ++k) {
                             const double p = px[k * KT];
-                            f = f + p * q[k];
-                            if (MODE == 2) g = g + p * qp[k];
+                            if (MODE == 2) {
+                                f = __builtin_fma(p, q[k], f);
+                                g = __builtin_fma(p, qp[k], g);
+                            } else {
+                                f = f + p * q[k];
+                            }
                         }
                         if (MODE == 2) {
                             acc[0].charge(g);
