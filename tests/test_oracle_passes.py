@@ -159,3 +159,61 @@ def test_full_sweeps_run_all_models(vps, orc):
         assert np.allclose(Q[vt > 0].sum(1), 1.0, atol=1e-9), kw
         ch.finish()
         assert np.isfinite(ch.get("rout")).all(), kw
+
+
+def test_z_noadmix_counts_closed_form(vps, orc):
+    """UpdateZNoAdmix (src/structure.cpp:1259-1301): Mi[i][Z_i] = typed
+    copies of i, Ni[Z_i][l][x] counts every typed copy x, from a dense decode"""
+    s, a, d, order, ch, geno, k2 = _setup(vps, orc)
+    K = 3
+    rng = np.random.default_rng(4)
+    Z = rng.integers(0, K, a["N"]).astype(np.uint16)
+    ch.set("Z", Z)
+    ch.call("update_z_noadmix")
+    afo, vt = order[1], order[2]
+    Ni = np.zeros((K, d.KT), dtype=np.int64)
+    Mi = np.zeros((a["N"], K), dtype=np.int64)
+    for l, row in enumerate(geno):
+        for i, als in enumerate(row):
+            for x in als:
+                Ni[Z[i], afo[l] + x] += 1
+    Mi[np.arange(a["N"]), Z] = vt
+    assert (ch.get("Ni") == Ni).all()
+    assert (ch.get("Mi") == Mi).all()
+
+
+# UPDATEQ stream salt (src/global.h)
+_SALT_UPDATE_Q = 0xE7853EB09C92935A
+
+
+@pytest.mark.parametrize("topology", [0, 1])
+def test_q_noadmix_draw_restated(vps, orc, topology):
+    """UpdateQNoAdmix's draw (src/structureAVX.cpp:279-287): per typed
+    individual, in order, PolyLog over its K log likelihoods (pinned above)
+    picks Z_i and Q_i becomes that one-hot row; the reference topology draws
+    from one UPDATEQ stream, the parallel one from one stream per individual"""
+    s, a, d, order, ch, geno, k2 = _setup(vps, orc)
+    K, N = 3, a["N"]
+    p = vps.params_default(admix_val=2, g_seed=5, rng_topology=topology, nadmburnin=0)
+    ch = orc.Chain(d, p, K, 1)
+    seed = ch.seed()
+    rng = np.random.default_rng(6)
+    vt = order[2]
+    for t in range(5):
+        m = 300 + t
+        Freq, Q = random_state(rng, N, K, d.KT, order[1], k2)
+        ch.set("Freq", Freq)
+        ch.set("m", [m])
+        ll = ch.noadmix_loglik()
+        ch.call("update_q_noadmix")
+        Zg, Qg = ch.get("Z"), ch.get("Q")
+        g = orc.Rng(seed + m, _SALT_UPDATE_Q) if topology == 0 else None
+        for i in range(N):
+            if not vt[i]:
+                assert (Qg[i] == 0).all()
+                continue
+            if topology == 1:
+                g = orc.Rng(seed + m * N + i, _SALT_UPDATE_Q)
+            k = g.polylog(ll[i])
+            assert Zg[i] == k, (t, i)
+            assert Qg[i, k] == 1 and Qg[i].sum() == 1
