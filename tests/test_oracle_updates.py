@@ -514,3 +514,67 @@ def test_update_f_restated(vps, orc, fsame):
         assert np.allclose(got, np.concatenate([F2, f2]), rtol=1e-13, atol=0), (t, got, F2, f2)
         moved += int(np.any(got[:K] != F))
     assert moved > 5, moved
+
+
+# UPDATELambda stream salt (src/global.h)
+_SALT_UPDATE_LAMBDA = 0x67099E2FE2510092
+
+
+@pytest.mark.parametrize("diff, fmodel", [(False, False), (True, False), (False, True)])
+def test_update_lambda_restated(vps, orc, diff, fmodel):
+    """UpdateLambda (src/structure.cpp:1598-1635; never reached in the
+    reference, whose option parsing cannot make inferlambda true, but kept
+    reachable here with inferlambda_val = 0) against a restatement with
+    scipy's log-gamma: the Dirichlet normalisers weighted by the number of
+    loci with i alleles, and the log product of the cluster (or, in the F
+    model, ancestral) frequencies"""
+    s, d = _data(vps, orc, frac_multi=0.4, min_alleles=3, max_alleles=5)
+    K = 3
+    p = vps.params_default(admix_val=1, f_val=1 if fmodel else 2, inferlambda_val=0,
+                           difflambda_val=1 if diff else 2, stdlambda=0.2, g_seed=15)
+    ch = orc.Chain(d, p, K, 1)
+    seed, KT = ch.seed(), d.KT
+    orig, afo, _ = d.order()
+    k2 = np.asarray(s.arrays()["n_allele"])[orig]
+    kdis = np.bincount(k2)
+    rng = np.random.default_rng(9)
+    lg = special.gammaln
+    moved = 0
+    for t in range(60):
+        m = 400 + t
+        P, A, F = _fmodel_state(rng, K, KT, k2, afo)
+        lam = rng.uniform(0.3, 2.0, K) if diff else np.full(K, rng.uniform(0.3, 2.0))
+        ch.set("Freq", P)
+        ch.set("FreqA", A)
+        ch.set("Lambda", lam)
+        ch.set("m", [m])
+        ch.call("update_lambda")
+        got = ch.get("Lambda")
+        g = orc.Rng(seed + m, _SALT_UPDATE_LAMBDA)
+        e = lam.copy()
+
+        def dl_of(lo, lm, np_, logprod):
+            gd = lg(lo) - lg(lm)
+            s_ = sum(kdis[i] * np_ * (lg(i * lm) - lg(i * lo) + i * gd) for i in range(2, len(kdis)))
+            return s_ + (lm - lo) * logprod
+
+        if diff:
+            for k in range(K):
+                lo = e[k]
+                lm = g.normal(lo, p.stdlambda)
+                if lm <= 0 or lm >= p.maxlambda:
+                    continue
+                dl = dl_of(lo, lm, 1, np.log(P[k]).sum())
+                if dl >= -1e-10 or g.uniform() < np.exp(dl):
+                    e[k] = lm
+        else:
+            lo = e[0]
+            lm = g.normal(lo, p.stdlambda)
+            if 0 < lm < p.maxlambda:
+                lp = np.log(A[0]).sum() if fmodel else np.log(P).sum()
+                dl = dl_of(lo, lm, 1 if fmodel else K, lp)
+                if dl >= -1e-10 or g.uniform() < np.exp(dl):
+                    e[:] = lm
+        assert np.allclose(got, e, rtol=1e-13, atol=0), (t, got, e)
+        moved += int(np.any(got != lam))
+    assert moved > 5, moved
