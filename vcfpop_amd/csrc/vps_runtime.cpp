@@ -108,7 +108,8 @@ void chain_free(vps_chain* ch)
 }
 
 // Chunk-size autotuner: time the whole Z pass (count kernels and chunk
-// reduction) for the model's chunking and a ladder of chunk sizes on the
+// reduction; the fastest of 3 passes after a warm-up) for the model's
+// chunking and a ladder of chunk sizes on the
 // freshly initialised chain, and keep the fastest.  Chunking changes no draw
 // and no count, so the choice only moves time.  The trial passes write a
 // scratch Mi (the Metropolis buffer, unused until the first sweep), scratch
@@ -150,12 +151,18 @@ int tune_chunks(vps_chain* ch)
         t.Ni = tmp + (size_t)Gmax * c.K * c.KT;
         t.Mi = reinterpret_cast<int32_t*>(c.bufNK1);   // N*K doubles >= N*K ints
         rc = launch_z_admix_counts(d, t, ch->stream);  // warm-up
-        hipError_t e = hipEventRecord(e0, ch->stream);
-        for (int r = 0; r < 2 && !rc && e == hipSuccess; ++r) rc = launch_z_admix_counts(d, t, ch->stream);
-        if (!rc && e == hipSuccess) e = hipEventRecord(e1, ch->stream);
-        if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+        // the fastest of 3 timed passes: robust to a pass slowed by other work
         float ms = 0;
-        if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        hipError_t e = hipSuccess;
+        for (int r = 0; r < 3 && !rc && e == hipSuccess; ++r) {
+            e = hipEventRecord(e0, ch->stream);
+            if (e == hipSuccess) rc = launch_z_admix_counts(d, t, ch->stream);
+            if (!rc && e == hipSuccess) e = hipEventRecord(e1, ch->stream);
+            if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+            float one = 0;
+            if (!rc && e == hipSuccess) e = hipEventElapsedTime(&one, e0, e1);
+            if (!rc && e == hipSuccess && (r == 0 || one < ms)) ms = one;
+        }
         if (!rc && e != hipSuccess) rc = fail(VPS_ERR_DEVICE, "tuner timing: %s", hipGetErrorString(e));
         if (!rc && (x == 0 || ms < best_ms * 0.99f)) { best_ms = ms; bestG = t.G; }
     }
