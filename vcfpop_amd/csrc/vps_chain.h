@@ -64,6 +64,10 @@ struct ChainDev {
     double*   rout;      // [rlen]
     double*   lnltrace;  // [nlnl capacity]
     Rng*      rngbuf;    // [N] streams kept between a draw kernel and its acceptance kernel
+    Rng*      frng;      // [1] the UpdateF stream between the kernels of one UpdateF
+    double*   fctl;      // [4] UpdateF step: Fo, Fm, out of range, done
+    double*   fpart;     // [256] UpdateF block partial sums
+    uint32_t* fcnt;      // [1] UpdateF blocks finished (the last one decides; reset by it)
     int32_t*  mdev;      // [1] sweep index m; kernels read it, the sweep's last kernel advances it,
                          // so one captured sweep replays at any m
     int32_t*  mside;     // [1] copy of m for the kernels beside the Z pass (written by UpdateP/Q)

@@ -278,6 +278,15 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     if (!rc) rc = chain_alloc(ch, &c.FreqM, K * KT);
     if (!rc) rc = chain_alloc(ch, &c.FreqA, 3 * KT);
     if (!rc) rc = chain_alloc(ch, &c.Fst, 2 * K);
+    if (!rc) rc = chain_alloc(ch, &c.frng, 1);
+    if (!rc) rc = chain_alloc(ch, &c.fctl, 4);
+    if (!rc) rc = chain_alloc(ch, &c.fpart, 256);
+    if (!rc) rc = chain_alloc(ch, &c.fcnt, 1);
+    if (!rc) {
+        const uint32_t zero = 0;
+        VPS_HIP(hipMemcpyAsync(c.fcnt, &zero, sizeof(zero), hipMemcpyHostToDevice, ch->stream));
+        VPS_HIP(hipStreamSynchronize(ch->stream));
+    }
     if (!rc) rc = chain_alloc(ch, &c.Eta, K);
     if (!rc) rc = chain_alloc(ch, &c.Gamma, S * K);
     if (!rc) rc = chain_alloc(ch, &c.Di, S * K);

@@ -633,66 +633,101 @@ int launch_update_a(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
     return VPS_OK;
 }
 
-__global__ __launch_bounds__(kScalarThreads) void k_update_f(const DeviceDataset ds, const ChainDev c)
+// UpdateF runs as K steps of a grid-wide sum.  The per-cluster log ratio
+// sums a term over every allele of the cluster (all clusters with a single
+// F), lgamma twice per term: spread over many blocks, with the proposal and
+// the decision on one thread.  k_update_f_start draws the first proposal;
+// step k sums its terms per block, and the last block to finish (a counter
+// in device memory) adds the block sums in block order, decides, and draws
+// the next proposal from the same stream.  The stream's draws (a normal per
+// cluster, a uniform only when the log ratio is below -1e-10) are the
+// reference's, in its order.
+constexpr int kFBlocks = 256;
+
+__device__ void f_propose(const ChainDev& c, Rng& r, int k)
 {
-    const int m = *c.mdev;
+    const double Fo = c.Fst[k];
+    const double Fm = r.normal(Fo, c.stdf);
+    c.fctl[0] = Fo;
+    c.fctl[1] = Fm;
+    c.fctl[2] = (Fm < 0 || Fm > 1) ? 1.0 : 0.0;
+}
+
+__global__ void k_update_f_start(const ChainDev c)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    Rng r;
+    r.init(c.seed + (uint64_t)*c.mdev, kSaltUpdateF);
+    f_propose(c, r, 0);
+    c.fctl[3] = 0.0;
+    c.frng[0] = r;
+}
+
+__global__ __launch_bounds__(kScalarThreads) void k_update_f_step(const DeviceDataset ds, const ChainDev c, int k)
+{
     __shared__ double sh[kScalarThreads];
-    __shared__ double bc[4];
+    __shared__ uint32_t ticket;
     const int K = c.K;
     const int64_t KT = ds.KT;
-    Rng r;
-    if (threadIdx.x == 0) r.init(c.seed + (uint64_t)m, kSaltUpdateF);
-    for (int k = 0; k < K; ++k) {
-        if (threadIdx.x == 0) {
-            const double Fo = c.Fst[k];
-            const double Fm = r.normal(Fo, c.stdf);
-            bc[0] = Fo; bc[1] = Fm;
-            bc[2] = (Fm < 0 || Fm > 1) ? 1.0 : 0.0;
+    if (c.fctl[3] != 0.0) return;   // done: a single F decided, or all clusters
+    const bool skip = c.fctl[2] != 0.0;
+    const double Fo = c.fctl[0], Fm = c.fctl[1];
+    const double fo = c.Fst[K + k], fm = (1 - Fm) / Fm;
+    double s = 0;
+    if (!skip) {
+        // terms: cluster k (clusters k..K-1 with a single F) x every allele
+        const int64_t n = (c.fsame ? (int64_t)(K - k) : 1) * KT;
+        for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+            const int64_t k2 = k + x / KT, a = x % KT;
+            const double pa2 = c.FreqA[a];
+            s += pa2 * (fm - fo) * my_log(c.Freq[k2 * KT + a]) + lgamma(fo * pa2) - lgamma(fm * pa2);
         }
-        __syncthreads();
-        if (bc[2] != 0.0) {
-            if (c.fsame) break;
-            __syncthreads();
-            continue;
-        }
-        const double Fo = bc[0], Fm = bc[1];
-        const double fo = c.Fst[K + k], fm = (1 - Fm) / Fm;
-        double part = 0;
-        for (int k2 = k; k2 < K; ++k2) {
-            const double* p = c.Freq + k2 * KT;
-            double s = 0;
-            for (int64_t a = threadIdx.x; a < KT; a += blockDim.x) {
-                const double pa2 = c.FreqA[a];
-                s += pa2 * (fm - fo) * my_log(p[a]) + lgamma(fo * pa2) - lgamma(fm * pa2);
-            }
-            part += s;
-            if (!c.fsame) break;
-        }
-        sh[threadIdx.x] = part;
-        __syncthreads();
-        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            double dlnL = c.pmeanf * (Fo - Fm) / (c.pstdf * c.pstdf) +
-                          (c.pmeanf * c.pmeanf / (c.pstdf * c.pstdf) - 1) * my_log(Fm / Fo) +
-                          (double)((int64_t)(c.fsame ? K : 1) * ds.L) * (lgamma(fm) - lgamma(fo));
-            dlnL += sh[0];
-            if (dlnL >= kNZero || r.uniform() < exp(dlnL)) {
-                if (c.fsame) for (int j = 0; j < K; ++j) { c.Fst[j] = Fm; c.Fst[K + j] = fm; }
-                else { c.Fst[k] = Fm; c.Fst[K + k] = fm; }
-            }
-        }
-        __syncthreads();
-        if (c.fsame) break;
     }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        c.fpart[blockIdx.x] = sh[0];
+        __threadfence();
+        ticket = atomicAdd(c.fcnt, 1u);
+    }
+    __syncthreads();
+    if (ticket != gridDim.x - 1 || threadIdx.x != 0) return;
+    // the last block: every partial is visible
+    __threadfence();
+    *c.fcnt = 0;
+    Rng r = c.frng[0];
+    if (!skip) {
+        double total = 0;
+        for (unsigned b = 0; b < gridDim.x; ++b) total += c.fpart[b];
+        double dlnL = c.pmeanf * (Fo - Fm) / (c.pstdf * c.pstdf) +
+                      (c.pmeanf * c.pmeanf / (c.pstdf * c.pstdf) - 1) * my_log(Fm / Fo) +
+                      (double)((int64_t)(c.fsame ? K : 1) * ds.L) * (lgamma(fm) - lgamma(fo));
+        dlnL += total;
+        if (dlnL >= kNZero || r.uniform() < exp(dlnL)) {
+            if (c.fsame) for (int j = 0; j < K; ++j) { c.Fst[j] = Fm; c.Fst[K + j] = fm; }
+            else { c.Fst[k] = Fm; c.Fst[K + k] = fm; }
+        }
+    }
+    if (c.fsame || k + 1 >= K) c.fctl[3] = 1.0;
+    else f_propose(c, r, k + 1);
+    c.frng[0] = r;
 }
 
 int launch_update_f(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     if (!c.fmodel) return VPS_OK;
-    hipLaunchKernelGGL(k_update_f, dim3(1), dim3(kScalarThreads), 0, st, ds, c);
+    hipLaunchKernelGGL(k_update_f_start, dim3(1), dim3(64), 0, st, c);
+    // a single F decides at k = 0, over every cluster's terms
+    const int steps = c.fsame ? 1 : c.K;
+    for (int k = 0; k < steps; ++k) {
+        const int64_t n = (c.fsame ? (int64_t)c.K : 1) * ds.KT;
+        const unsigned nb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(kFBlocks, (n + 4 * kScalarThreads - 1) / (4 * kScalarThreads)));
+        hipLaunchKernelGGL(k_update_f_step, dim3(nb), dim3(kScalarThreads), 0, st, ds, c, k);
+    }
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
