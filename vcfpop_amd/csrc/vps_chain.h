@@ -68,6 +68,8 @@ struct ChainDev {
     double*   fctl;      // [4] UpdateF step: Fo, Fm, out of range, done
     double*   fpart;     // [256] UpdateF block partial sums
     uint32_t* fcnt;      // [1] UpdateF blocks finished (the last one decides; reset by it)
+    double*   lpart;     // [2 * max(kLpCap, S*K)] UpdateAlpha/UpdateLambda LogProd partials
+    uint32_t* lcnt;      // [1] their blocks finished (reset by the last)
     int32_t*  mdev;      // [1] sweep index m; kernels read it, the sweep's last kernel advances it,
                          // so one captured sweep replays at any m
     int32_t*  mside;     // [1] copy of m for the kernels beside the Z pass (written by UpdateP/Q)
@@ -77,6 +79,10 @@ struct ChainDev {
     const uint32_t* jump;
     int32_t   lik_tpr[2];   // tiles per locus range of the likelihood passes (BI, GEN tiles)
 };
+
+// partials of the grid-wide LogProds of UpdateAlpha and UpdateLambda (more
+// when S*K columns exceed it: one block per column then)
+constexpr int kLpCap = 1024;
 
 // likelihood-pass partials: lanes = individuals, one partial per locus range
 constexpr int kLikLociPerWave = 1024;

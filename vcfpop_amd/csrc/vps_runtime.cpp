@@ -282,9 +282,12 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     if (!rc) rc = chain_alloc(ch, &c.fctl, 4);
     if (!rc) rc = chain_alloc(ch, &c.fpart, 256);
     if (!rc) rc = chain_alloc(ch, &c.fcnt, 1);
+    if (!rc) rc = chain_alloc(ch, &c.lpart, 2 * std::max<int64_t>(kLpCap, (int64_t)S * K));
+    if (!rc) rc = chain_alloc(ch, &c.lcnt, 1);
     if (!rc) {
         const uint32_t zero = 0;
         VPS_HIP(hipMemcpyAsync(c.fcnt, &zero, sizeof(zero), hipMemcpyHostToDevice, ch->stream));
+        VPS_HIP(hipMemcpyAsync(c.lcnt, &zero, sizeof(zero), hipMemcpyHostToDevice, ch->stream));
         VPS_HIP(hipStreamSynchronize(ch->stream));
     }
     if (!rc) rc = chain_alloc(ch, &c.Eta, K);

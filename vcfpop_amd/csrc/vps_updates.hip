@@ -285,15 +285,37 @@ int launch_update_di(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
     return VPS_OK;
 }
 
+constexpr int kScalarThreads = 256;
+
 // ---------------------------------------------------------------------------
-// block-cooperative LogProd (any order; agrees with the sequential one to
-// rounding).  Every thread of the block gets the result.
+// Grid-wide LogProd for UpdateAlpha and UpdateLambda.  Their log ratios need
+// LogProd over columns of Q or P (up to N*K or K*KT values), and the
+// Metropolis chain itself is a few scalars: the grid is ncol x B blocks,
+// block (col, b) reduces slice b of column col to an exponent and a mantissa,
+// and the last block to finish (a counter the last block resets) combines
+// each column's B partials in block order, then runs the chain on thread 0.
 // ---------------------------------------------------------------------------
-__device__ double block_logprod(const double* a, int64_t n, int64_t sep, int64_t* sh_s, double* sh_p)
+struct LpCol {
+    const double* a;
+    int64_t n, sep;   // values a[x * sep], x < n
+};
+
+// one block's share of column `col`, written as (exponent bits, mantissa)
+__device__ void lp_block(const LpCol col, int b, int B, int64_t* sh_s, double* sh_p, double* out)
 {
+    const int64_t x1 = col.n * (b + 1) / B, st = blockDim.x;
     LogAcc acc;
     acc.open();
-    for (int64_t x = threadIdx.x; x < n; x += blockDim.x) acc.charge(a[x * sep]);
+    int64_t x = col.n * b / B + threadIdx.x;
+    for (; x + 3 * st < x1; x += 4 * st) {   // four loads in flight per thread
+        const double v0 = col.a[x * col.sep], v1 = col.a[(x + st) * col.sep];
+        const double v2 = col.a[(x + 2 * st) * col.sep], v3 = col.a[(x + 3 * st) * col.sep];
+        acc.charge(v0);
+        acc.charge(v1);
+        acc.charge(v2);
+        acc.charge(v3);
+    }
+    for (; x < x1; x += st) acc.charge(col.a[x * col.sep]);
     LogAcc::strip(acc.slog, acc.prod);
     sh_s[threadIdx.x] = acc.slog;
     sh_p[threadIdx.x] = acc.prod;
@@ -308,47 +330,88 @@ __device__ double block_logprod(const double* a, int64_t n, int64_t sep, int64_t
         }
         __syncthreads();
     }
-    const double r = ((double)sh_s[0] + log2(sh_p[0])) * kLn2Trunc;
-    __syncthreads();
-    return r;
-}
-
-__device__ double block_sum(const double* a, int64_t n, int64_t sep, double* sh)
-{
-    double s = 0;
-    for (int64_t x = threadIdx.x; x < n; x += blockDim.x) s += a[x * sep];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = as_f64((uint64_t)sh_s[0]);
+        out[1] = sh_p[0];
     }
-    const double r = sh[0];
-    __syncthreads();
-    return r;
 }
 
-constexpr int kScalarThreads = 256;
+// true in every thread of the last block of the grid to arrive
+__device__ bool lp_arrive(uint32_t* cnt, uint32_t nblocks)
+{
+    __shared__ uint32_t ticket;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        ticket = atomicAdd(cnt, 1u);
+    }
+    __syncthreads();
+    if (ticket != nblocks - 1) return false;
+    __threadfence();
+    if (threadIdx.x == 0) *cnt = 0;
+    return true;
+}
+
+// the last block: column col's B partials combined in block order; the
+// result replaces the mantissa of the column's first partial
+__device__ void lp_combine_all(double* part, int ncol, int B)
+{
+    for (int col = threadIdx.x; col < ncol; col += blockDim.x) {
+        double* q = part + (int64_t)2 * col * B;
+        int64_t s = 0;
+        double p = 1;
+        for (int b = 0; b < B; ++b) {
+            s += (int64_t)as_u64(q[2 * b]);
+            p *= q[2 * b + 1];
+            LogAcc::strip(s, p);
+        }
+        q[1] = ((double)s + log2(p)) * kLn2Trunc;
+    }
+    __syncthreads();
+}
+
+constexpr int kLpThreads = 256;
+constexpr int kLpItems = 8;       // values per thread a column slice aims at
+
+// blocks per column: about kLpItems values per thread, within the partials' capacity
+int lp_blocks(int ncol, int64_t n)
+{
+    const int64_t want = (n + (int64_t)kLpThreads * kLpItems - 1) / ((int64_t)kLpThreads * kLpItems);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(want, std::max(1, kLpCap / ncol)));
+}
 
 // ---------------------------------------------------------------------------
-// UpdateAlpha, src/structure.cpp:1501-1594 (one block; the likelihood sums
-// are block reductions, the Metropolis chain runs on thread 0)
+// UpdateAlpha, src/structure.cpp:1501-1594
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kScalarThreads) void k_update_alpha(const DeviceDataset ds, const ChainDev c)
+enum { kAlphaOne = 0, kAlphaDiff = 1, kAlphaLoc = 2 };
+
+__device__ LpCol alpha_col(const DeviceDataset& ds, const ChainDev& c, int mode, int col)
+{
+    const int K = c.K;
+    if (mode == kAlphaLoc) {
+        const int s = col / K, k = col % K;
+        return {c.Q + (int64_t)ds.pop_ind0[s] * K + k, ds.pop_nind[s], K};
+    }
+    if (mode == kAlphaDiff) return {c.Q + col, ds.N, K};
+    return {c.Q, (int64_t)ds.N * K, 1};
+}
+
+__global__ __launch_bounds__(kLpThreads) void k_update_alpha(const DeviceDataset ds, const ChainDev c, int mode, int B)
 {
     const int m = *c.mdev;
-    __shared__ int64_t sh_s[kScalarThreads];
-    __shared__ double sh_p[kScalarThreads];
-    __shared__ double lp[VPS_MAX_K * 64 + 1];
+    __shared__ int64_t sh_s[kLpThreads];
+    __shared__ double sh_p[kLpThreads];
     const int K = c.K, N = ds.N, S = c.S;
-    if (c.locpriori && c.admix) {
-        // log prod of Q column k over the individuals of each location
-        for (int s = 0; s < S && s < 64; ++s)
-            for (int k = 0; k < K; ++k)
-                lp[s * K + k] = block_logprod(c.Q + (int64_t)ds.pop_ind0[s] * K + k, ds.pop_nind[s], K, sh_s, sh_p);
-        if (threadIdx.x != 0) return;
-        Rng r;
-        r.init(c.seed + (uint64_t)m, kSaltUpdateAlpha);
+    if (mode != kAlphaLoc && !c.admix && m >= c.nadmburnin) return;
+    const int col = blockIdx.x / B, ncol = gridDim.x / B;
+    lp_block(alpha_col(ds, c, mode, col), blockIdx.x % B, B, sh_s, sh_p, c.lpart + 2 * (int64_t)blockIdx.x);
+    if (!lp_arrive(c.lcnt, gridDim.x)) return;
+    lp_combine_all(c.lpart, ncol, B);
+    if (threadIdx.x != 0) return;
+    const double* lp = c.lpart + 1;   // LogProd of column j at lp[2 * j * B]
+    const int64_t ls = 2 * (int64_t)B;
+    Rng r;
+    r.init(c.seed + (uint64_t)m, kSaltUpdateAlpha);
+    if (mode == kAlphaLoc) {
         const double rr = c.scal[0];
         for (int k = 0; k < K; ++k) {
             const double am = r.normal(c.Alpha[k], c.stdalpha);
@@ -372,8 +435,7 @@ __global__ __launch_bounds__(kScalarThreads) void k_update_alpha(const DeviceDat
                     const double d = am - ao;
                     double dlnL = (rr * c.Alpha[k] - 1) * my_log(am / ao) - d * rr +
                                   ds.pop_nind[s] * (lgamma(c.SumAlpha[s] + d) - lgamma(am) - lgamma(c.SumAlpha[s]) + lgamma(ao));
-                    const double lq = s < 64 ? lp[s * K + k] : logprod_sep(c.Q + (int64_t)ds.pop_ind0[s] * K + k, ds.pop_nind[s], K);
-                    dlnL += d * lq;
+                    dlnL += d * lp[(s * K + k) * ls];
                     if (dlnL >= kNZero || r.uniform() < exp(dlnL)) {
                         c.AlphaLocal[s * K + k] = am;
                         c.SumAlpha[s] += d;
@@ -383,13 +445,7 @@ __global__ __launch_bounds__(kScalarThreads) void k_update_alpha(const DeviceDat
         }
         return;
     }
-    if (!c.inferalpha) return;
-    if (!c.admix && m >= c.nadmburnin) return;
-    if (c.diffalpha) {
-        for (int k = 0; k < K; ++k) lp[k] = block_logprod(c.Q + k, N, K, sh_s, sh_p);
-        if (threadIdx.x != 0) return;
-        Rng r;
-        r.init(c.seed + (uint64_t)m, kSaltUpdateAlpha);
+    if (mode == kAlphaDiff) {
         double sumalpha = 0;
         for (int k = 0; k < K; ++k) sumalpha += c.Alpha[k];
         for (int k = 0; k < K; ++k) {
@@ -398,51 +454,66 @@ __global__ __launch_bounds__(kScalarThreads) void k_update_alpha(const DeviceDat
             const double ao = c.Alpha[k];
             const double d = am - ao;
             double dlnL = c.uniformalpha ? 0 : ((c.alphapriora - 1) * my_log(am / ao) + (ao - am) / c.alphapriorb);
-            dlnL += (am - ao) * lp[k] + (lgamma(sumalpha + d) - lgamma(sumalpha) - lgamma(am) + lgamma(ao)) * N;
+            dlnL += (am - ao) * lp[k * ls] + (lgamma(sumalpha + d) - lgamma(sumalpha) - lgamma(am) + lgamma(ao)) * N;
             if (dlnL >= kNZero || r.uniform() < exp(dlnL)) c.Alpha[k] = am;
         }
         return;
     }
-    const double lq = block_logprod(c.Q, (int64_t)N * K, 1, sh_s, sh_p);
-    if (threadIdx.x != 0) return;
-    Rng r;
-    r.init(c.seed + (uint64_t)m, kSaltUpdateAlpha);
     const double am = r.normal(c.Alpha[0], c.stdalpha);
     if (am <= 0 || (am >= c.maxalpha && c.uniformalpha)) return;
     const double ao = c.Alpha[0];
     double dlnL = c.uniformalpha ? 0 : ((c.alphapriora - 1) * my_log(am / ao) + (ao - am) / c.alphapriorb);
-    dlnL += (am - ao) * lq + (lgamma(K * am) - lgamma(K * ao) - K * lgamma(am) + K * lgamma(ao)) * N;
+    dlnL += (am - ao) * lp[0] + (lgamma(K * am) - lgamma(K * ao) - K * lgamma(am) + K * lgamma(ao)) * N;
     if (dlnL >= kNZero || r.uniform() < exp(dlnL))
         for (int k = 0; k < K; ++k) c.Alpha[k] = am;
 }
 
 int launch_update_alpha(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
-    if (c.locpriori && c.admix && c.S > 64) {
-        // per-location sums beyond 64 locations run on thread 0 (correct, slower)
+    int mode, ncol;
+    int64_t n;
+    if (c.locpriori && c.admix) {
+        mode = kAlphaLoc;
+        ncol = c.S * c.K;
+        n = (ds.N + c.S - 1) / c.S;
+    } else if (!c.inferalpha) {
+        return VPS_OK;
+    } else if (c.diffalpha) {
+        mode = kAlphaDiff;
+        ncol = c.K;
+        n = ds.N;
+    } else {
+        mode = kAlphaOne;
+        ncol = 1;
+        n = (int64_t)ds.N * c.K;
     }
-    hipLaunchKernelGGL(k_update_alpha, dim3(1), dim3(kScalarThreads), 0, st, ds, c);
+    const int B = lp_blocks(ncol, n);
+    hipLaunchKernelGGL(k_update_alpha, dim3((unsigned)(ncol * B)), dim3(kLpThreads), 0, st, ds, c, mode, B);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
 
 // ---------------------------------------------------------------------------
-// UpdateLambda, src/structure.cpp:1598-1635 (runs only when inferlambda)
+// UpdateLambda, src/structure.cpp:1598-1635 (runs only when inferlambda):
+// LogProd of each cluster's frequencies (difflambda), else of all of them
+// (of the ancestral frequencies under the F model)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kScalarThreads) void k_update_lambda(const DeviceDataset ds, const ChainDev c)
+__global__ __launch_bounds__(kLpThreads) void k_update_lambda(const DeviceDataset ds, const ChainDev c, int B)
 {
     const int m = *c.mdev;
-    __shared__ int64_t sh_s[kScalarThreads];
-    __shared__ double sh_p[kScalarThreads];
-    __shared__ double lp[VPS_MAX_K + 1];
+    __shared__ int64_t sh_s[kLpThreads];
+    __shared__ double sh_p[kLpThreads];
     const int K = c.K;
     const int64_t KT = ds.KT;
-    if (c.difflambda) {
-        for (int k = 0; k < K; ++k) lp[k] = block_logprod(c.Freq + k * KT, KT, 1, sh_s, sh_p);
-    } else {
-        lp[0] = c.fmodel ? block_logprod(c.FreqA, KT, 1, sh_s, sh_p) : block_logprod(c.Freq, K * KT, 1, sh_s, sh_p);
-    }
+    const int col = blockIdx.x / B, ncol = gridDim.x / B;
+    const LpCol lc = c.difflambda ? LpCol{c.Freq + col * KT, KT, 1}
+                                  : c.fmodel ? LpCol{c.FreqA, KT, 1} : LpCol{c.Freq, K * KT, 1};
+    lp_block(lc, blockIdx.x % B, B, sh_s, sh_p, c.lpart + 2 * (int64_t)blockIdx.x);
+    if (!lp_arrive(c.lcnt, gridDim.x)) return;
+    lp_combine_all(c.lpart, ncol, B);
     if (threadIdx.x != 0) return;
+    const double* lp = c.lpart + 1;
+    const int64_t ls = 2 * (int64_t)B;
     Rng r;
     r.init(c.seed + (uint64_t)m, kSaltUpdateLambda);
     if (c.difflambda) {
@@ -451,7 +522,7 @@ __global__ __launch_bounds__(kScalarThreads) void k_update_lambda(const DeviceDa
             if (lm <= 0 || lm >= c.maxlambda) continue;
             double dlnL = 0, gd = lgamma(lo) - lgamma(lm);
             for (int i = 2; i <= ds.maxK; ++i) dlnL += ds.kdis[i] * (lgamma(i * lm) - lgamma(i * lo) + i * gd);
-            dlnL += (lm - lo) * lp[k];
+            dlnL += (lm - lo) * lp[k * ls];
             if (dlnL >= kNZero || r.uniform() < exp(dlnL)) c.Lambda[k] = lm;
         }
     } else {
@@ -469,7 +540,10 @@ __global__ __launch_bounds__(kScalarThreads) void k_update_lambda(const DeviceDa
 int launch_update_lambda(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     if (!c.inferlambda) return VPS_OK;
-    hipLaunchKernelGGL(k_update_lambda, dim3(1), dim3(kScalarThreads), 0, st, ds, c);
+    const int ncol = c.difflambda ? c.K : 1;
+    const int64_t n = c.difflambda || c.fmodel ? ds.KT : c.K * ds.KT;
+    const int B = lp_blocks(ncol, n);
+    hipLaunchKernelGGL(k_update_lambda, dim3((unsigned)(ncol * B)), dim3(kLpThreads), 0, st, ds, c, B);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
