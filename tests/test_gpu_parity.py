@@ -27,6 +27,8 @@ DATASETS = {
                   min_alleles=3, max_alleles=8, multi_dir=0.3, ploidy=[(2, 0.6), (4, 0.3), (6, 0.1)], n_pop=4,
                   seed=22),
     # ragged: N not a multiple of anything, a single short tile
+    # enough individuals that UpdateAlpha's LogProd columns span several blocks
+    "many_ind": dict(n_ind=5000, n_loc=24, n_pop=2, k_true=3, alpha_q=0.5, fst=0.05, missing=0.02, seed=27),
     "ragged": dict(n_ind=13, n_loc=37, k_true=2, alpha_q=1.0, missing=0.2, ploidy=[(1, 0.3), (3, 0.7)], seed=23),
     # the maximum ploidy (8 and 10 copies; BI tiles on the runtime slot loop, GEN descriptors of 10 alleles)
     "high_ploidy": dict(n_ind=40, n_loc=200, k_true=3, alpha_q=0.5, missing=0.05, frac_multi=0.4, min_alleles=3,
@@ -247,19 +249,24 @@ def test_dirichlet_updates(vps, orc, pairs, name, topology):
     assert close(g.get("Q"), o.get("Q"))
 
 
-@pytest.mark.parametrize("kw", [dict(), dict(diffalpha_val=1), dict(uniformalpha_val=2)])
-def test_update_alpha(vps, orc, pairs, kw):
-    say("alpha", kw)
-    ds, od, g, o = make(vps, orc, pairs, "bi_diploid", 3, admix_val=1, g_seed=11, **kw)
+@pytest.mark.parametrize("name", ["bi_diploid", "many_ind"])
+@pytest.mark.parametrize("kw", [dict(), dict(diffalpha_val=1), dict(uniformalpha_val=2), dict(locpriori_val=1)])
+def test_update_alpha(vps, orc, pairs, name, kw):
+    """global, per-cluster and per-location alpha; on many_ind every LogProd
+    column spans several blocks (2,048 values a block)"""
+    say("alpha", name, kw)
+    ds, od, g, o = make(vps, orc, pairs, name, 3, admix_val=1, g_seed=11, **kw)
     Q = np.random.default_rng(3).dirichlet(np.ones(3), size=ds.N)
     for m in range(5):
         for c in (g, o):
             c.set("Q", Q)
             c.set("m", [m])
-        feed(g, o, ["Alpha"])
+        feed(g, o, ["Alpha", "loc"] if kw.get("locpriori_val") else ["Alpha"])
         g.update_alpha()
         o.call("update_alpha")
         assert close(g.get("Alpha"), o.get("Alpha"))
+        if kw.get("locpriori_val"):
+            assert close(g.get("loc"), o.get("loc"))
 
 
 SWEEP_MODELS = {
@@ -273,6 +280,7 @@ SWEEP_MODELS = {
     "fmodel_parallel": dict(admix_val=1, f_val=1, rng_topology=1),
     "fmodel_single": dict(admix_val=1, f_val=1, singlef_val=1),
     "lambda": dict(admix_val=1, inferlambda_val=0, difflambda_val=1),
+    "lambda_single": dict(admix_val=1, inferlambda_val=0, difflambda_val=2),
     "lambda_parallel": dict(admix_val=1, inferlambda_val=0, difflambda_val=1, rng_topology=1),
     "locprior_admix_parallel": dict(admix_val=1, locpriori_val=1, rng_topology=1),
 }
