@@ -361,7 +361,51 @@ def bench_precision(V, S=20):
     res["rel_dlnL"] = float(abs(a["lnL"][0] - b["lnL"][0]) / abs(a["lnL"][0]))
     res["frac_Mi_cells_differing"] = float((a["Mi"] != b["Mi"]).mean())
     ds.close()
+    res["posterior"] = posterior_precision(V)
     return res
+
+
+def posterior_ancestry(V, ds, K, seed, prec):
+    """posterior-mean ancestry (normalised MiSum) and mean lnL of one full chain
+    of the plain admixture workload (1,000 burn-in + 2,000 sampling sweeps)"""
+    p = V.params_default(admix_val=1, nburnin=1000, nreps=2000, g_seed=seed, rng_topology=V.RNG_PARALLEL,
+                         precision=prec)
+    c = V.Chain(ds, p, K, 1)
+    c.step(3000)
+    c.finish()
+    ms = c.get("MiSum")
+    out = ms / ms.sum(axis=1, keepdims=True), float(c.get("rout")[0])
+    c.close()
+    return out
+
+
+def posterior_precision(V, seeds=(1, 2, 3, 4, 5, 6, 7, 8, 9, 10)):
+    """FP32 weights against FP64 over full chains of the plain admixture
+    workload: posterior-mean ancestry and mean lnL, beside the seed-to-seed
+    spread of the FP64 chains (the Monte Carlo error of the same estimate)"""
+    import itertools
+
+    import numpy as np
+    s = V.make_synth("admix_n200_l2k_k3")
+    ds = V.Dataset(s, 0)
+    d_mix, d_seed, l_mix, l_seed = [], [], [], []
+    for seed in seeds:
+        q64, l64 = posterior_ancestry(V, ds, 3, seed, V.PREC_FP64)
+        q32, l32 = posterior_ancestry(V, ds, 3, seed, V.PREC_MIXED)
+        q64b, l64b = posterior_ancestry(V, ds, 3, seed + 1000, V.PREC_FP64)
+        d_mix.append(np.abs(q32 - q64))
+        # the other seed's chain may settle on other cluster labels
+        perm = min(itertools.permutations(range(3)), key=lambda pm: np.abs(q64b[:, list(pm)] - q64).mean())
+        d_seed.append(np.abs(q64b[:, list(perm)] - q64))
+        l_mix.append(abs(l32 - l64) / abs(l64))
+        l_seed.append(abs(l64b - l64) / abs(l64))
+    ds.close()
+    d_mix, d_seed = np.stack(d_mix), np.stack(d_seed)
+    return {"workload": "admix_n200_l2k_k3", "chains": len(seeds), "sweeps": 3000,
+            "mean_abs_dQ_mixed_vs_fp64": float(d_mix.mean()), "max_abs_dQ_mixed_vs_fp64": float(d_mix.max()),
+            "mean_abs_dQ_fp64_other_seed": float(d_seed.mean()), "max_abs_dQ_fp64_other_seed": float(d_seed.max()),
+            "max_rel_dmeanlnL_mixed_vs_fp64": float(max(l_mix)),
+            "max_rel_dmeanlnL_fp64_other_seed": float(max(l_seed))}
 
 
 if __name__ == "__main__":

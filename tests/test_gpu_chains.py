@@ -78,3 +78,46 @@ def test_tables_match_oracle_run(vps, orc, config0, tmp_path):
                 assert abs(float(x.split("=")[1]) - float(y.split("=")[1])) <= max(scale, 1e-5), (x, y)
             else:
                 assert x == y, (suffix, x, y)
+
+
+def _ancestry(vps, ds, seed, prec):
+    p = vps.params_default(admix_val=1, nburnin=1000, nreps=2000, g_seed=seed, rng_topology=1, precision=prec)
+    c = vps.Chain(ds, p, 3, 1)
+    c.step(3000)
+    c.finish()
+    ms = c.get("MiSum")
+    out = ms / ms.sum(axis=1, keepdims=True), float(c.get("rout")[0])
+    c.close()
+    return out
+
+
+def _aligned(q, ref):
+    """q with its cluster columns permuted to match ref best (chains of
+    different seeds may settle on different cluster labels)"""
+    import itertools
+    perms = list(itertools.permutations(range(q.shape[1])))
+    best = min(perms, key=lambda pm: np.abs(q[:, list(pm)] - ref).mean())
+    return q[:, list(best)]
+
+
+def test_mixed_posterior_within_monte_carlo_error(vps, config0):
+    """FP32 weights: the chain leaves the FP64 trajectory at its first
+    differing draw, so its posterior-mean ancestry is another Monte Carlo
+    estimate of the same quantity.  Its distance from the FP64 estimate of the
+    same seed stays within the spread between FP64 estimates of two seeds,
+    and mean lnL agrees to that spread as well."""
+    s, ds, od = config0
+    d_mix, d_seed, l_mix, l_seed = [], [], [], []
+    for seed in (1, 2, 3, 4):
+        say("posterior ancestry, seed", seed)
+        q64, l64 = _ancestry(vps, ds, seed, vps.PREC_FP64)
+        q32, l32 = _ancestry(vps, ds, seed, vps.PREC_MIXED)
+        q64b, l64b = _ancestry(vps, ds, seed + 1000, vps.PREC_FP64)
+        d_mix.append(np.abs(q32 - q64).mean())
+        d_seed.append(np.abs(_aligned(q64b, q64) - q64).mean())
+        l_mix.append(abs(l32 - l64))
+        l_seed.append(abs(l64b - l64))
+    say("mean |dQ| mixed", d_mix, "other seed", d_seed, "|d mean lnL| mixed", l_mix, "other seed", l_seed)
+    assert np.mean(d_mix) <= np.mean(d_seed)
+    assert max(d_mix) < 0.01
+    assert np.mean(l_mix) <= 3 * np.mean(l_seed) + 1.0
