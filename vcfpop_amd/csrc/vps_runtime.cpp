@@ -473,13 +473,14 @@ int launch_sweep(vps_chain* ch, const SweepPlan& p)
     // the previous sweep's alpha/lambda/A/F: read by this sweep's UpdateQ and UpdateP
     int rc = settle_side(ch);
     if (rc) return rc;
-    // Uncaptured sweeps of the parallel topology with a Dirichlet UpdateQ:
-    // UpdateP, UpdateQ and the Mi clear in one launch, and the parameter
-    // updates beside the Z pass joined at the start of the next sweep, or
-    // before Record (they read the sweep index from a copy, so the Z pass's
-    // last kernel can advance it)
-    const bool fast = p.q == kQAdmix && !ch->capturing && c.topology == VPS_RNG_PARALLEL;
-    if (fast) {
+    // Sweeps of the parallel topology with a Dirichlet UpdateQ: UpdateP,
+    // UpdateQ and the Mi clear in one launch.  Uncaptured, the parameter
+    // updates also run beside the Z pass, joined at the start of the next
+    // sweep or before Record (they read the sweep index from a copy, so the
+    // Z pass's last kernel can advance it); captured, they run in series.
+    const bool joint = p.q == kQAdmix && c.topology == VPS_RNG_PARALLEL;
+    const bool fast = joint && !ch->capturing;
+    if (joint) {
         rc = launch_update_pq(d, c, ch->stream);
     } else {
         rc = do_update_p(ch);
@@ -511,7 +512,7 @@ int launch_sweep(vps_chain* ch, const SweepPlan& p)
     // without Record, the Z pass's last kernel advances the sweep index when
     // nothing beside it still reads the index
     const bool zadv = !p.record && p.q != kQNoAdmix && (fast || side == ch->stream);
-    if (!rc) rc = p.q == kQNoAdmix ? do_z_noadmix(ch) : do_z_admix(ch, !fast, zrec, zadv);
+    if (!rc) rc = p.q == kQNoAdmix ? do_z_noadmix(ch) : do_z_admix(ch, !joint, zrec, zadv);
     // Record reads alpha, lambda and F: a Record sweep joins here
     if (!rc && side != ch->stream) {
         VPS_HIP(hipEventRecord(ch->ev_join, side));
