@@ -269,6 +269,30 @@ def test_update_alpha(vps, orc, pairs, name, kw):
             assert close(g.get("loc"), o.get("loc"))
 
 
+def test_update_alpha_many_locations(vps, orc):
+    """locprior alpha with more location x cluster columns (80 x 16) than the
+    LogProd partial buffer's default capacity (1,024): one block per column"""
+    say("alpha, 80 locations, K = 16")
+    s = vps.Synth(n_ind=400, n_loc=24, n_pop=80, k_true=3, alpha_q=0.5, fst=0.05, missing=0.02, seed=29)
+    ds, od = vps.Dataset(s, 0), orc.Data(s.desc)
+    p = vps.params_default(admix_val=1, locpriori_val=1, g_seed=13)
+    g, o = vps.Chain(ds, p, 16, 1), orc.Chain(od, p, 16, 1)
+    Q = np.random.default_rng(5).dirichlet(np.ones(16), size=ds.N)
+    for m in range(3):
+        for c in (g, o):
+            c.set("Q", Q)
+            c.set("m", [m])
+        feed(g, o, ["Alpha", "loc"])
+        g.update_alpha()
+        o.call("update_alpha")
+        assert close(g.get("Alpha"), o.get("Alpha"))
+        assert close(g.get("loc"), o.get("loc"))
+    g.close()
+    o.close()
+    ds.close()
+    od.close()
+
+
 SWEEP_MODELS = {
     "admix": dict(admix_val=1),
     "admix_parallel": dict(admix_val=1, rng_topology=1),
