@@ -191,7 +191,8 @@ int vps_chain_destroy(vps_chain* ch);
    UpdateP, UpdateQ, UpdateLocPriori, UpdateZ, UpdateAlpha, UpdateLambda,
    UpdateA, UpdateF, Record.  Asynchronous on the chain's stream. */
 int vps_chain_step(vps_chain* ch, int n_sweeps);
-/* Arrange (src/structure.cpp:1862-1869) after the last sweep */
+/* Arrange (src/structure.cpp:1862-1869) after the last sweep;
+   VPS_ERR_STATE while no sampling sweep has been recorded (nr = 0) */
 int vps_chain_finish(vps_chain* ch);
 /* WaitCUDA (src/structureCUDA.h:8) */
 int vps_chain_sync(vps_chain* ch);
@@ -342,7 +343,10 @@ typedef struct vps_server_stats_t {
 
 void vps_server_config_default(vps_server_config* c);
 int  vps_server_create(const vps_server_config* cfg, vps_server** out);
-/* copies the genotype image; VPS_ERR_BUDGET if the job can never fit */
+/* copies the genotype image; VPS_ERR_BUDGET if the job can never fit.
+   n_sweeps is the run length nburnin + nreps (+ nadmburnin for the
+   no-admixture model without a location prior); 0 takes that length, and
+   any other value is VPS_ERR_INVALID */
 int  vps_server_submit(vps_server* s, const vps_dataset_desc* desc, const vps_params* par, int K, int run_id,
                        int64_t n_sweeps, int64_t* job_id);
 /* blocks until the job is done (timeout_s < 0: no limit); VPS_ERR_STATE on timeout */

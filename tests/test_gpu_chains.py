@@ -113,7 +113,8 @@ def test_mixed_posterior_within_monte_carlo_error(vps, config0):
         q64, l64 = _ancestry(vps, ds, seed, vps.PREC_FP64)
         q32, l32 = _ancestry(vps, ds, seed, vps.PREC_MIXED)
         q64b, l64b = _ancestry(vps, ds, seed + 1000, vps.PREC_FP64)
-        d_mix.append(np.abs(q32 - q64).mean())
+        # once off the FP64 trajectory the FP32 chain may settle on other labels too
+        d_mix.append(np.abs(_aligned(q32, q64) - q64).mean())
         d_seed.append(np.abs(_aligned(q64b, q64) - q64).mean())
         l_mix.append(abs(l32 - l64))
         l_seed.append(abs(l64b - l64))
@@ -121,3 +122,20 @@ def test_mixed_posterior_within_monte_carlo_error(vps, config0):
     assert np.mean(d_mix) <= np.mean(d_seed)
     assert max(d_mix) < 0.01
     assert np.mean(l_mix) <= 3 * np.mean(l_seed) + 1.0
+
+
+def test_finish_before_sampling_is_refused(vps):
+    """Arrange divides by the recorded sampling sweeps: finishing a chain
+    still in burn-in is a state error, and the chain can go on and finish"""
+    s = vps.Synth(n_ind=40, n_loc=200, k_true=2, seed=3)
+    ds = vps.Dataset(s, 0)
+    c = vps.Chain(ds, vps.params_default(admix_val=1, nburnin=5, nreps=4, g_seed=1), 2, 1)
+    c.step(5)
+    with pytest.raises(vps.VpsError):
+        c.finish()
+    c.step(4)
+    c.finish()
+    r = c.get("rout")
+    assert np.isfinite(r).all()
+    c.close()
+    ds.close()

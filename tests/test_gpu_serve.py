@@ -104,3 +104,24 @@ def test_destroy_with_running_jobs(vps):
     for r in range(1, 5):
         srv.submit(s, p, 3, r)
     srv.close()   # abandons the running and queued jobs without hanging
+
+
+def test_submit_checks_run_length(vps):
+    """a job's sweep count is its run length: nburnin + nreps (+ nadmburnin
+    for no-admixture without a location prior); another count is refused"""
+    s = vps.Synth(n_ind=50, n_loc=300, k_true=2, seed=11)
+    p = vps.params_default(admix_val=1, nburnin=4, nreps=6)
+    srv = vps.Server(hbm_budget_bytes=2 << 30, max_active=1)
+    for bad in (4, 9, 11):
+        with pytest.raises(vps.VpsError):
+            srv.submit(s, p, 2, 1, sweeps=bad)
+    j = srv.submit(s, p, 2, 1, sweeps=0)        # 0 takes the run length
+    srv.wait(j, 60.0)
+    assert srv.info(j)["sweeps_total"] == 10
+    q = vps.params_default(admix_val=2, nburnin=3, nreps=5, nadmburnin=2)
+    with pytest.raises(vps.VpsError):
+        srv.submit(s, q, 2, 2, sweeps=8)
+    j = srv.submit(s, q, 2, 2, sweeps=10)
+    srv.wait(j, 60.0)
+    assert srv.result(j, "rout").size > 0
+    srv.close()

@@ -3,7 +3,13 @@
 //   mode 1  Record, binary Q          lnL = sum log P[Z_i][l][x]
 //           (RecordAVX, src/structureAVX.cpp:986-1187)
 //   mode 2  Metropolis log ratio     r_i = sum log(sum_k q'_ik p / sum_k q_ik p)
-//           (UpdateQMetroAVX, src/structureAVX.cpp:557-783)
+//           (UpdateQMetroAVX, src/structureAVX.cpp:557-783): each copy's
+//           ratio is formed as in SumProdDiv (src/math2.cpp:1557-1580),
+//           products and sums rounded separately, then divided, and the
+//           ratios are charged into one accumulator (FP64 mode)
+//   mode 4  the same log ratio as the difference of two accumulated
+//           mixtures formed with fused multiply-adds (mixed-precision mode:
+//           half the FP64 instructions, no division)
 //   mode 3  no-admix log likelihood  b_ik = sum log P[k][l][x]
 //           (UpdateQNoAdmixAVX, src/structureAVX.cpp:254-428)
 // sums run over typed allele copies x of individual i at locus l.
@@ -52,7 +58,7 @@ __device__ void lik_load_q(const ChainDev& c, int i, double* q, double* qp)
         if (MODE == 1) q[k] = c.Z[i] == k ? 1.0 : 0.0;   // one-hot: sum_k q_k p_k == p_Z exactly
         else if (MODE != 3) q[k] = c.Q[(int64_t)i * K + k];
         else q[k] = 0.0;
-        qp[k] = MODE == 2 ? c.bufNK1[(int64_t)i * K + k] : 0.0;
+        qp[k] = (MODE == 2 || MODE == 4) ? c.bufNK1[(int64_t)i * K + k] : 0.0;
     }
 }
 
@@ -62,10 +68,10 @@ __device__ void lik_write(const DeviceDataset& ds, const ChainDev& c, FrexpAcc* 
 {
     if (MODE <= 1) {
         block_store_sum(ok ? acc[0].close() : 0.0, c.part + block_slot);
-    } else if (MODE == 2) {
+    } else if (MODE == 2 || MODE == 4) {
         if (ok) {
             const double cur = acc[1].close();
-            c.part[range * ds.N + i] = acc[0].close() - cur;
+            c.part[range * ds.N + i] = MODE == 2 ? acc[0].close() : acc[0].close() - cur;
             c.part_cur[range * ds.N + i] = cur;
         }
     } else {
@@ -79,7 +85,7 @@ __device__ void lik_write(const DeviceDataset& ds, const ChainDev& c, FrexpAcc* 
 
 // number of accumulators per lane
 template <int K, int MODE>
-struct LikNA { static constexpr int value = MODE == 3 ? K : (MODE == 2 ? 2 : 1); };
+struct LikNA { static constexpr int value = MODE == 3 ? K : ((MODE == 2 || MODE == 4) ? 2 : 1); };
 
 // biallelic tiles (at most two alleles; genotype bytes in ds.bitab, or in
 // the tile's ds.btab columns above 3 bits per id).  MPT is
@@ -176,16 +182,16 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
                         }
                     } else {
                         // sums start from the k = 0 term (0 + x == x for the
-                        // non-negative terms here).  The Metropolis ratio's
-                        // mixtures are fused multiply-adds (one rounding per
-                        // term instead of two; the ratio is held to 1e-13
-                        // per term, far above either rounding); Record keeps
-                        // the product-then-sum rounding.
+                        // non-negative terms here).  Record and the FP64
+                        // Metropolis ratio keep the product-then-sum rounding
+                        // of SumProdDiv; mode 4 fuses the multiply-adds (one
+                        // rounding per term instead of two).
+                        constexpr bool MET = MODE == 2 || MODE == 4;
                         double f0 = p0[0] * q[0], f1 = p1[0] * q[0];
-                        double g0 = MODE == 2 ? p0[0] * qp[0] : 0.0, g1 = MODE == 2 ? p1[0] * qp[0] : 0.0;
+                        double g0 = MET ? p0[0] * qp[0] : 0.0, g1 = MET ? p1[0] * qp[0] : 0.0;
 #pragma unroll
                         for (int k = 1; k < K; ++k) {
-                            if (MODE == 2) {
+                            if (MODE == 4) {
                                 f0 = __builtin_fma(p0[k], q[k], f0);
                                 f1 = __builtin_fma(p1[k], q[k], f1);
                                 g0 = __builtin_fma(p0[k], qp[k], g0);
@@ -193,14 +199,23 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
                             } else {
                                 f0 = f0 + p0[k] * q[k];
                                 f1 = f1 + p1[k] * q[k];
+                                if (MODE == 2) {
+                                    g0 = g0 + p0[k] * qp[k];
+                                    g1 = g1 + p1[k] * qp[k];
+                                }
                             }
                         }
+                        // the per-copy ratios (correctly rounded divisions);
+                        // a ratio of an allele no copy carries is never charged
+                        const double r0 = MODE == 2 ? g0 / f0 : 0.0, r1 = MODE == 2 ? g1 / f1 : 0.0;
 #pragma unroll kPairs
                         for (int a = 0; a < ncopies; a += 2) {
                             const bool on0 = a < v, hi0 = a >= h, on1 = a + 1 < v, hi1 = a + 1 >= h;
                             if (MODE == 2)
+                                acc[0].charge((on0 ? (hi0 ? r1 : r0) : 1.0) * (on1 ? (hi1 ? r1 : r0) : 1.0));
+                            else if (MODE == 4)
                                 acc[0].charge((on0 ? (hi0 ? g1 : g0) : 1.0) * (on1 ? (hi1 ? g1 : g0) : 1.0));
-                            acc[MODE == 2 ? 1 : 0].charge((on0 ? (hi0 ? f1 : f0) : 1.0) * (on1 ? (hi1 ? f1 : f0) : 1.0));
+                            acc[MET ? 1 : 0].charge((on0 ? (hi0 ? f1 : f0) : 1.0) * (on1 ? (hi1 ? f1 : f0) : 1.0));
                         }
                     }
                 }
@@ -268,19 +283,24 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_gen(const DeviceDataset ds,
 #pragma unroll
                         for (int k = 0; k < K; ++k) acc[k].charge(px[k * KT]);
                     } else {
-                        // (the Metropolis ratio's mixtures fuse their multiply-adds, as on BI tiles)
+                        // (rounding as on BI tiles: separate products and sums
+                        // and a per-copy ratio in mode 2, fused in mode 4)
                         double f = 0, g = 0;
 #pragma unroll
                         for (int k = 0; k < K; ++k) {
                             const double p = px[k * KT];
-                            if (MODE == 2) {
+                            if (MODE == 4) {
                                 f = __builtin_fma(p, q[k], f);
                                 g = __builtin_fma(p, qp[k], g);
                             } else {
                                 f = f + p * q[k];
+                                if (MODE == 2) g = g + p * qp[k];
                             }
                         }
                         if (MODE == 2) {
+                            acc[0].charge(g / f);
+                            acc[1].charge(f);
+                        } else if (MODE == 4) {
                             acc[0].charge(g);
                             acc[1].charge(f);
                         } else {
@@ -494,7 +514,7 @@ int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c0, int mode, hipSt
     switch (mode) {
     case 0: rc = launch_lik_mode<0>(ds, c, st, g); break;
     case 1: rc = launch_lik_mode<1>(ds, c, st, g); break;
-    case 2: rc = launch_lik_mode<2>(ds, c, st, g); break;
+    case 2: rc = c.precision == VPS_PREC_MIXED ? launch_lik_mode<4>(ds, c, st, g) : launch_lik_mode<2>(ds, c, st, g); break;
     case 3: rc = launch_lik_mode<3>(ds, c, st, g); break;
     default: return fail(VPS_ERR_INVALID, "likelihood mode %d", mode);
     }

@@ -614,6 +614,10 @@ extern "C" int vps_chain_finish(vps_chain* ch)
 {
     if (!ch) return fail(VPS_ERR_INVALID, "null chain");
     if (ch->finished) return VPS_OK;
+    // Arrange divides the accumulators by the number of recorded sampling
+    // sweeps; a run stopped before its first one has no result
+    if (ch->nr == 0)
+        return fail(VPS_ERR_STATE, "no sampling sweep recorded yet (%d sweeps run, burn-in %d)", ch->m, ch->nburnin);
     int rc = launch_arrange(ch->ds->d, ch->c, ch->nr, ch->stream);
     if (rc) return rc;
     VPS_HIP(hipStreamSynchronize(ch->stream));

@@ -237,24 +237,30 @@ void loader_main(vps_server* s)
     }
 }
 
-int collect_results(Job& j)
+// A finished job's results, copied to host memory by the scheduler thread
+// without the server lock and moved into the job under it.
+struct JobResults {
+    std::vector<double> Q, ancestry, rout, lnltrace;
+    double lnl_last = 0;
+};
+
+int collect_results(vps_chain* ch, JobResults& r)
 {
-    vps_chain* ch = j.ch;
     const int64_t N = ch->c.N, K = ch->K;
     int rc = vps_chain_finish(ch);
-    j.Q.resize(N * K);
-    j.ancestry.resize(N * K);
+    r.Q.resize(N * K);
+    r.ancestry.resize(N * K);
     int64_t rlen = 0, nl = 0;
     if (!rc) rc = vps_chain_get_sizes(ch, &rlen, &nl);
-    j.rout.resize(std::max<int64_t>(rlen, 1));
-    j.lnltrace.resize(std::max<int64_t>(nl, 1));
-    if (!rc) rc = vps_chain_get_state(ch, VPS_STATE_Q, j.Q.data(), j.Q.size() * 8);
-    if (!rc) rc = vps_chain_get_state(ch, VPS_STATE_MISUM, j.ancestry.data(), j.ancestry.size() * 8);
-    if (!rc && rlen) rc = vps_chain_get_state(ch, VPS_STATE_ROUT, j.rout.data(), rlen * 8);
-    if (!rc && nl) rc = vps_chain_get_state(ch, VPS_STATE_LNLTRACE, j.lnltrace.data(), nl * 8);
-    if (!rc) rc = vps_chain_get_state(ch, VPS_STATE_LNL, &j.lnl_last, 8);
-    j.rout.resize(rlen);
-    j.lnltrace.resize(nl);
+    r.rout.resize(std::max<int64_t>(rlen, 1));
+    r.lnltrace.resize(std::max<int64_t>(nl, 1));
+    if (!rc) rc = vps_chain_get_state(ch, VPS_STATE_Q, r.Q.data(), r.Q.size() * 8);
+    if (!rc) rc = vps_chain_get_state(ch, VPS_STATE_MISUM, r.ancestry.data(), r.ancestry.size() * 8);
+    if (!rc && rlen) rc = vps_chain_get_state(ch, VPS_STATE_ROUT, r.rout.data(), rlen * 8);
+    if (!rc && nl) rc = vps_chain_get_state(ch, VPS_STATE_LNLTRACE, r.lnltrace.data(), nl * 8);
+    if (!rc) rc = vps_chain_get_state(ch, VPS_STATE_LNL, &r.lnl_last, 8);
+    r.rout.resize(rlen);
+    r.lnltrace.resize(nl);
     return rc;
 }
 
@@ -292,10 +298,18 @@ void scheduler_main(vps_server* s)
             }
             // finished (or failed): results to the host, device memory back
             std::string msg;
-            if (!rc) rc = collect_results(j);
+            JobResults res;
+            if (!rc) rc = collect_results(j.ch, res);
             if (rc) msg = vps_last_error();
             release_device(j);
             std::lock_guard<std::mutex> lk(s->mu);
+            if (!rc) {
+                j.Q = std::move(res.Q);
+                j.ancestry = std::move(res.ancestry);
+                j.rout = std::move(res.rout);
+                j.lnltrace = std::move(res.lnltrace);
+                j.lnl_last = res.lnl_last;
+            }
             j.state = rc ? VPS_JOB_FAILED : VPS_JOB_DONE;
             j.status = rc;
             j.err = msg;
@@ -349,7 +363,16 @@ extern "C" int vps_server_submit(vps_server* s, const vps_dataset_desc* desc, co
 {
     if (!s || !desc || !par || !job_id) return fail(VPS_ERR_INVALID, "null argument");
     if (K < 1 || K > VPS_MAX_K) return fail(VPS_ERR_UNSUPPORTED, "K=%d outside 1..%d", K, VPS_MAX_K);
-    if (n_sweeps < 0) return fail(VPS_ERR_INVALID, "n_sweeps");
+    // a run is nburnin + nreps sweeps, plus the admixture burn-in of the
+    // no-admixture model without a location prior (ReadPar, src/structure.cpp:460-462)
+    const int64_t run_len = (int64_t)par->nburnin + par->nreps +
+                            ((par->admix_val != 1 && par->locpriori_val != 1) ? par->nadmburnin : 0);
+    if (par->nburnin < 0 || par->nreps < 1 || par->nadmburnin < 0)
+        return fail(VPS_ERR_INVALID, "nburnin %d, nreps %d, nadmburnin %d", par->nburnin, par->nreps, par->nadmburnin);
+    if (n_sweeps == 0) n_sweeps = run_len;
+    if (n_sweeps != run_len)
+        return fail(VPS_ERR_INVALID, "n_sweeps %lld differs from the run length %lld the parameters give",
+                    (long long)n_sweeps, (long long)run_len);
     auto j = std::make_shared<Job>();
     int rc = copy_image(desc, j->img);
     if (rc) return rc;
