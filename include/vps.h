@@ -235,6 +235,23 @@ int vps_pass_update_p(vps_chain* ch);          /* UpdatePCUDA (dead in the refer
 int vps_pass_update_q_admix(vps_chain* ch);    /* UpdateQAdmix,       src/structure.cpp:986-1004 */
 int vps_pass_update_alpha(vps_chain* ch);      /* UpdateAlpha,        src/structure.cpp:1501-1594 */
 
+/* Record's likelihood pass with no device allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops).  It runs the genotype pass of
+   vps_pass_record on caller-owned device buffers and writes the lnL to the
+   device double *lnl:
+     freq  [K*KT + 2] doubles, P in sorted locus order (the 2 trailing values
+           are read by paired loads and never charged);
+     q     [N*K] doubles (admixture form), or
+     z     [N] uint16 cluster of each individual (binary-Q form; q unused).
+   The pass's block partials live in one static device array per GPU (4 MiB,
+   held until the process ends): a dataset needing more partials than it holds
+   (about N = 100,000 with L = 1,000,000) is refused with VPS_ERR_UNSUPPORTED.
+   Calls that overlap on one GPU race on that array.  Asynchronous on `stream`
+   (a hipStream_t; NULL = the null stream). */
+int vps_record_lnl_noalloc(const vps_dataset* ds, int K, const double* freq, const double* q,
+                           const uint16_t* z, double* lnl, void* stream);
+
 /* ------------------------------------------------------------------------- */
 /* Result tables (WriteStructure / WriteStructureSummary,                     */
 /* src/structure.cpp:88-401): the run file "<prefix>.structure.k=K_rep=R_id=I */

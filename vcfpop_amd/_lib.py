@@ -145,6 +145,7 @@ _SIGS = {
     "vps_chain_get_state": ([_P, C.c_int, _P, C.c_size_t], C.c_int),
     "vps_chain_set_state": ([_P, C.c_int, _P, C.c_size_t], C.c_int),
     "vps_chain_get_sizes": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+    "vps_record_lnl_noalloc": ([_P, C.c_int, _P, _P, _P, _P, _P], C.c_int),
     "vps_pass_update_z_admix": ([_P], C.c_int),
     "vps_pass_update_z_noadmix": ([_P], C.c_int),
     "vps_pass_update_q_noadmix": ([_P], C.c_int),

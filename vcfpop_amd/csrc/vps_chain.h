@@ -101,8 +101,9 @@ int launch_z_noadmix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, bool advance = false);
 // likelihood passes: mode 0 = Record lnL (admix), 1 = Record lnL (no admix),
-// 2 = Metropolis log ratio per individual, 3 = no-admix per-(i,k) log likelihood
-int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c, int mode, hipStream_t st);
+// 2 = Metropolis log ratio per individual, 3 = no-admix per-(i,k) log likelihood.
+// Modes 0/1 write the lnL to lnl_out, or to c.scal[1] when it is null.
+int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c, int mode, hipStream_t st, double* lnl_out = nullptr);
 int lik_part_count(const DeviceDataset& ds, int mode, int K, const int32_t* tpr);
 // tiles per locus range of the likelihood passes for this device: [0]
 // biallelic tiles, [1] multiallelic tiles

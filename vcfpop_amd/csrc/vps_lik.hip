@@ -503,7 +503,7 @@ int lik_part_count(const DeviceDataset& ds, int mode, int K, const int32_t* tpr)
     return (int)(nr * ds.N * (mode == 3 ? K : 2));   // mode 2: ratios, then current-Q partials
 }
 
-int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c0, int mode, hipStream_t st)
+int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c0, int mode, hipStream_t st, double* lnl_out)
 {
     const LikGrid g = lik_grid(ds, c0.lik_tpr);
     const int nranges = g.nr_bi + g.nr_gen;
@@ -520,7 +520,8 @@ int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c0, int mode, hipSt
     }
     if (rc) return rc;
     if (mode <= 1) {
-        hipLaunchKernelGGL(k_reduce_lnl, dim3(1), dim3(256), 0, st, c.part, (int64_t)nranges * g.gy, c.scal + 1);
+        hipLaunchKernelGGL(k_reduce_lnl, dim3(1), dim3(256), 0, st, c.part, (int64_t)nranges * g.gy,
+                           lnl_out ? lnl_out : c.scal + 1);
     } else {
         const int NA = mode == 3 ? c.K : 1;
         const int64_t n = (int64_t)ds.N * NA;
