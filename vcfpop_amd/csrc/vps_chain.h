@@ -22,6 +22,28 @@ struct FrexpAcc {
     __device__ double close() const { return ((double)e + log2(m)) * kLn2Trunc; }
 };
 
+// apply a jump table (32 nibbles x 16 entries x 128 bits) to a stream.
+// One state word (8 nibbles) per outer step: at most 8 table loads are in
+// flight, so the prologue needs no more registers than the sampling loop
+// (the loop, not the jump, then sets the kernel's occupancy).
+__device__ __forceinline__ void jump_apply(const uint32_t* __restrict__ tab, Xs128& r)
+{
+    uint32_t w0 = (uint32_t)r.x, w1 = (uint32_t)(r.x >> 32), w2 = (uint32_t)r.y, w3 = (uint32_t)(r.y >> 32);
+    uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    const uint4* t4 = reinterpret_cast<const uint4*>(tab);
+#pragma unroll 1
+    for (int q = 0; q < 4; ++q, t4 += 8 * 16) {
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+            const uint4 e = t4[p * 16 + ((w0 >> (p * 4)) & 15u)];
+            a0 ^= e.x; a1 ^= e.y; a2 ^= e.z; a3 ^= e.w;
+        }
+        w0 = w1; w1 = w2; w2 = w3;
+    }
+    r.x = (uint64_t)a0 | ((uint64_t)a1 << 32);
+    r.y = (uint64_t)a2 | ((uint64_t)a3 << 32);
+}
+
 // Pointers and scalars of one chain, passed by value to every kernel.
 struct ChainDev {
     int32_t K, N, S, maxploidy;
@@ -108,6 +130,15 @@ int lik_part_count(const DeviceDataset& ds, int mode, int K, const int32_t* tpr)
 // tiles per locus range of the likelihood passes for this device: [0]
 // biallelic tiles, [1] multiallelic tiles
 int choose_lik_tpr(const DeviceDataset& ds, int K, int device, int32_t* tpr);
+
+// The reference topology's sequential Gamma streams, parsed in parallel
+// (vps_spp.hip): UpdateP's 128 streams when do_p, and UpdateQ's stream with
+// draws of kind qkind (kSppQ*); then the rows are normalised, Mi is cleared
+// when clear_mi, and the sweep index is copied for the side-stream kernels.
+constexpr int kSppQNone = -1, kSppQAdmix = 1, kSppQProposals = 2;
+int launch_spp(const DeviceDataset& ds, const ChainDev& c, bool do_p, int qkind, bool clear_mi, hipStream_t st);
+int launch_spp_metro_accept(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+int launch_spp_polylog(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 
 int launch_update_p(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 // UpdateP + Dirichlet UpdateQ + Mi clear in one launch (parallel topology)

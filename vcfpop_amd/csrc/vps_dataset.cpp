@@ -8,6 +8,7 @@
 #include <numeric>
 #include <string.h>
 #include <thread>
+#include <mutex>
 #include "vps_internal.h"
 
 using namespace vps;
@@ -79,6 +80,24 @@ void parallel_for(int64_t n, F&& f)
     for (unsigned t = 0; t < nt; ++t)
         th.emplace_back([&, t]() { for (int64_t i = t; i < n; i += nt) f(i); });
     for (auto& x : th) x.join();
+}
+
+}  // namespace
+
+namespace {
+
+// GF(2) jump tables of the parallel stream parse: table r advances an
+// xorshift128+ state by 8 * 2^r draws (8, 16, ..., 4096).  The same for every
+// dataset; built once.
+const std::vector<uint32_t>& ring_jump_tables()
+{
+    static std::once_flag once;
+    static std::vector<uint32_t> tabs;
+    std::call_once(once, [] {
+        tabs.resize((size_t)kRingJumps * kJumpWords);
+        for (int r = 0; r < kRingJumps; ++r) jump_table((uint64_t)8 << r, tabs.data() + (size_t)r * kJumpWords);
+    });
+    return tabs;
 }
 
 }  // namespace
@@ -380,6 +399,14 @@ extern "C" int vps_dataset_create(const vps_dataset_desc* in, int device, vps_da
     if (!rc) rc = upload(&d.nhaplo, nhaplo, ds);
     if (!rc) rc = upload(&d.ncopies, ncopies, ds);
     if (!rc) rc = upload(&d.kdis, ds->kdis, ds);
+    {
+        std::vector<int32_t> typed;
+        for (int i = 0; i < N; ++i)
+            if (vt[i]) typed.push_back(i);
+        d.ntyped = (int32_t)typed.size();
+        if (!rc) rc = upload(&d.typed, typed, ds);
+    }
+    if (!rc) rc = upload(&d.ring_jump, ring_jump_tables(), ds);
     if (rc) { vps_dataset_destroy(ds); return rc; }
     *out = ds;
     return VPS_OK;

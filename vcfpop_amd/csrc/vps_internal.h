@@ -60,6 +60,7 @@ struct TileDesc {
 // nibbles of the 128-bit state; entry [p][v] is the 128-bit image of nibble
 // p holding v (4 dwords: x_lo, x_hi, y_lo, y_hi).
 constexpr int kJumpWords = 32 * 16 * 4;
+constexpr int kRingJumps = 10;   // 8, 16, ..., 4096 draws (the parallel stream parse)
 
 struct DeviceDataset {
     // sizes
@@ -93,6 +94,9 @@ struct DeviceDataset {
     int32_t*  nhaplo;          // [L]
     int32_t*  ncopies;         // [KT] typed copies of each allele (sum over clusters of Ni)
     int64_t*  kdis;            // [maxK+1] loci per allele count
+    int32_t*  typed;           // [ntyped] individuals with vt > 0, ascending
+    int32_t   ntyped;
+    uint32_t* ring_jump;       // [kRingJumps * kJumpWords]: table r advances a stream 8 * 2^r draws
 };
 
 }  // namespace vps

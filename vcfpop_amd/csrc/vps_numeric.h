@@ -261,8 +261,9 @@ VPS_HD void dirichlet_a(Rng& r, double* res, const double* a, int n)
     for (int i = 0; i < n; ++i) res[i] *= inv;
 }
 
-// PolyLog(a, n), src/statistics.cpp:116-155 (overwrites a)
-VPS_HD int polylog(Rng& r, double* a, int n)
+// PolyLog(a, n), src/statistics.cpp:116-155 (overwrites a), with its one
+// draw given: `word` is the stream's next xorshift output
+VPS_HD int polylog_word(uint64_t word, double* a, int n)
 {
     double mx = a[0];
     for (int i = 1; i < n; ++i) mx = a[i] > mx ? a[i] : mx;
@@ -272,13 +273,14 @@ VPS_HD int polylog(Rng& r, double* a, int n)
         a[i] = diff < -23 ? kMinFreq : exp(diff);
         s += a[i];
     }
-    double t = r.uniform(0, s);
+    double t = (unit12(word) - 1.0) * (s - 0) + 0;   // Uniform(0, s)
     for (int i = 0; i < n; ++i) {
         if (t < a[i]) return i;
         t -= a[i];
     }
     return n - 1;
 }
+VPS_HD int polylog(Rng& r, double* a, int n) { return polylog_word(r.xorshift(), a, n); }
 
 // Exponent-tracking product accumulator (OpenLog/ChargeLog/CloseLog,
 // src/math2.h:716-902): the binary exponent of a factor or of the running

@@ -473,15 +473,23 @@ int launch_sweep(vps_chain* ch, const SweepPlan& p)
     // the previous sweep's alpha/lambda/A/F: read by this sweep's UpdateQ and UpdateP
     int rc = settle_side(ch);
     if (rc) return rc;
-    // Sweeps of the parallel topology with a Dirichlet UpdateQ: UpdateP,
-    // UpdateQ and the Mi clear in one launch.  Uncaptured, the parameter
-    // updates also run beside the Z pass, joined at the start of the next
-    // sweep or before Record (they read the sweep index from a copy, so the
-    // Z pass's last kernel can advance it); captured, they run in series.
-    const bool joint = p.q == kQAdmix && c.topology == VPS_RNG_PARALLEL;
+    // Sweeps with a Dirichlet UpdateQ: UpdateP, UpdateQ and the Mi clear in
+    // one step (one launch in the parallel topology; the parallel parse of the
+    // reference's streams and its normalisation in the reference topology).
+    // Uncaptured, the parameter updates also run beside the Z pass, joined at
+    // the start of the next sweep or before Record (they read the sweep index
+    // from a copy, so the Z pass's last kernel can advance it); captured, they
+    // run in series.
+    const bool joint = p.q == kQAdmix;
     const bool fast = joint && !ch->capturing;
     if (joint) {
-        rc = launch_update_pq(d, c, ch->stream);
+        rc = c.topology == VPS_RNG_PARALLEL ? launch_update_pq(d, c, ch->stream)
+                                            : launch_spp(d, c, true, kSppQAdmix, true, ch->stream);
+    } else if (p.q == kQMetro && c.topology == VPS_RNG_REFERENCE) {
+        // UpdateP's streams and the proposals' stream in one parse
+        rc = launch_spp(d, c, true, kSppQProposals, false, ch->stream);
+        if (!rc) rc = launch_lik_pass(d, c, 2, ch->stream);
+        if (!rc) rc = launch_q_metro_accept(d, c, ch->stream);
     } else {
         rc = do_update_p(ch);
         if (!rc) rc = p.q == kQNoAdmix ? do_q_noadmix(ch) : p.q == kQMetro ? do_q_metro(ch) : do_q_admix(ch);

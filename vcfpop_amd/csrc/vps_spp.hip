@@ -1,0 +1,445 @@
+// vps_spp.hip — the reference's sequential Gamma streams drawn in parallel.
+//
+// In the reference's draw topology UpdateP runs 128 streams
+// RNG(seed + m*128 + b, UPDATEP), each drawing the Dirichlet rows of its
+// slice kl in [b*KL/128, (b+1)*KL/128) one after another
+// (src/structure.cpp:843-861), and UpdateQ runs one stream
+// RNG(seed + m, UPDATEQ) over all typed individuals (src/structure.cpp:
+// 990-1003; the Metropolis proposals, :1012-1020).  Each Gamma draw
+// (Marsaglia–Tsang, src/statistics.cpp:212-242) consumes a data-dependent
+// number of words: a Box–Muller pair when no second normal is cached, one
+// uniform for the acceptance test, one more for a shape below 1, and more for
+// every rejection.  A stream therefore has to be parsed in order.
+//
+// Parsing in parallel.  One workgroup of 1024 lanes owns a stream; lane j
+// takes draw front + j.  Almost every draw takes its first candidate, so the
+// lane assumes every earlier draw of the window did: draws then alternate
+// between "pair + uniform" (3 words, leaves a normal cached) and "cached
+// normal + uniform" (1 word), plus one word per shape below 1 (a prefix
+// count).  Each lane evaluates its draw exactly, from that speculated start,
+// and checks whether its own consumption was the typical one.  The first lane
+// that deviates (a rejection, v <= 0, or a Box–Muller value whose low word
+// is the cache's empty marker) still started in the right place, so its draw
+// is correct; the window commits lanes 0..f and the next window starts from
+// lane f's true end state.  Rejections are rare (about 0.3% of draws at shape
+// 10, under 0.03% at shape 1000), so a window commits ~1000 draws.
+//
+// Words come from an LDS ring of 8,192 consecutive stream outputs: thread t
+// holds the generator state 8t draws past the ring's start and writes 8
+// words; the ring moves on in steps of 4,096 with one GF(2) jump per thread.
+// Every lane reads words at known positions, so no lane steps a generator
+// serially.
+//
+// The values equal the serial parse exactly: each committed draw is the
+// reference's arithmetic (vps_numeric.h) on the words the serial parse would
+// give it.  A draw that would need more than the ring holds (impossible for a
+// finite shape; a NaN shape spins) ends the parse's wait: it is stored as NaN
+// and its stream moves to the end of the ring.
+#include <climits>
+#include "vps_chain.h"
+
+namespace vps {
+
+constexpr int kSppThreads = 1024;
+constexpr int kRingWords = 8 * kSppThreads;
+constexpr int64_t kRingStep = kRingWords / 2;
+constexpr int kPBlocks = 128;       // the reference's UpdateP streams (src/structure.cpp:843)
+
+// A stream's parse state between two draws: words consumed, and the
+// Box–Muller cache (the pair drawn at `bm`, values U1 and U2).
+struct SppFront {
+    int64_t d;        // next draw
+    int64_t pos;      // words consumed
+    int64_t bm;       // position of the cached pair
+    double U1, U2;
+    int32_t cached;
+    int32_t pad;
+};
+
+// this thread's generator: the state `8 * threadIdx.x` draws into the ring
+__device__ __forceinline__ void ring_seed(Xs128& st, const uint32_t* __restrict__ J)
+{
+    const int t = threadIdx.x;
+#pragma unroll 1
+    for (int r = 0; r < kRingJumps; ++r)
+        if ((t >> r) & 1) jump_apply(J + (size_t)r * kJumpWords, st);
+}
+
+__device__ __forceinline__ void ring_fill(const Xs128& st, uint64_t* ring)
+{
+    Xs128 s = st;
+    uint64_t* o = ring + 8 * threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) o[u] = s.next();
+}
+
+// moves the ring's window forward by whole steps until `pos` lies in its
+// first half (a block-uniform condition; every thread calls it)
+__device__ __forceinline__ void ring_follow(int64_t pos, int64_t& ws, Xs128& st, uint64_t* ring,
+                                            const uint32_t* __restrict__ J)
+{
+    if (pos - ws < kRingStep) return;
+    const int64_t n = (pos - ws) / kRingStep;
+#pragma unroll 1
+    for (int64_t x = 0; x < n; ++x) jump_apply(J + (size_t)(kRingJumps - 1) * kJumpWords, st);
+    ws += n * kRingStep;
+    __syncthreads();   // (the ring's last readers are done)
+    ring_fill(st, ring);
+    __syncthreads();
+}
+
+// exclusive prefix count of `flag` over the block's lanes; *total = all
+__device__ __forceinline__ int block_prefix(bool flag, int* s_cnt, int* total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t bal = __ballot(flag);
+    const int in = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) s_cnt[w] = __popcll(bal);
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int x = 0; x < kSppThreads / 64; ++x) {
+        if (x < w) before += s_cnt[x];
+        all += s_cnt[x];
+    }
+    *total = all;
+    return before + in;
+}
+
+// the draw kinds of one parse
+constexpr int kDrawP = 0;       // Freq[g] <- Gamma(prior + Ni[g]), g = k*KT + allele
+constexpr int kDrawQ = 1;       // Q[i][k] <- Gamma(alpha_row(i)[k] + Mi[i][k]), typed i
+constexpr int kDrawProp = 2;    // bufNK1[i][k] <- Gamma(alpha_row(i)[k]), typed i
+
+__device__ __forceinline__ double* draw_target(const DeviceDataset& ds, const ChainDev& c, int kind, int64_t g,
+                                               double* alpha)
+{
+    if (kind == kDrawP) {
+        const int64_t k = g / ds.KT, x = g - k * ds.KT;
+        const double prior = c.fmodel ? c.FreqA[x] * c.Fst[c.K + (c.fsame ? 0 : k)] : c.Lambda[k];
+        *alpha = prior + (double)c.Ni[g];
+        return c.Freq + g;
+    }
+    const int64_t t = g / c.K, k = g - t * c.K;
+    const int i = ds.typed[t];
+    const double* ar = c.locpriori ? c.AlphaLocal + (int64_t)ds.popid[i] * c.K : c.Alpha;
+    if (kind == kDrawQ) {
+        *alpha = ar[k] + (double)c.Mi[(int64_t)i * c.K + k];
+        return c.Q + (int64_t)i * c.K + k;
+    }
+    *alpha = ar[k] * 1.0;
+    return c.bufNK1 + (int64_t)i * c.K + k;
+}
+
+// The parse of one stream per block.  Blocks [0, np) are UpdateP's 128
+// streams (np = 0 or 128); block np, when qkind >= 0, is UpdateQ's stream with
+// draws of kind qkind.  Values are unnormalised Gamma draws; k_spp_normalize
+// turns each row into its Dirichlet draw.  The Metropolis proposals' stream
+// state after the parse goes to rngbuf[0] for the acceptance kernel.
+__global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset ds, const ChainDev c, int np, int qkind)
+{
+    __shared__ uint64_t ring[kRingWords];
+    __shared__ int s_cnt[kSppThreads / 64];
+    __shared__ uint8_t s_lt1[kSppThreads];
+    __shared__ int s_first;
+    __shared__ SppFront s_front;
+
+    const int m = *c.mdev;
+    const int j = threadIdx.x;
+    int kind;
+    uint64_t sseed, salt;
+    int64_t d1;
+    if ((int)blockIdx.x < np) {
+        kind = kDrawP;
+        const int b = blockIdx.x;
+        const int64_t KL = (int64_t)c.K * ds.L;
+        const int64_t kl0 = (int64_t)b * KL / kPBlocks, kl1 = (int64_t)(b + 1) * KL / kPBlocks;
+        // a slice of kl is a contiguous range of Freq: g = k*KT + afo[l] + allele
+        auto first_g = [&](int64_t kl) -> int64_t {
+            if (kl >= KL) return (int64_t)c.K * ds.KT;
+            const int64_t k = kl / ds.L;
+            return k * ds.KT + ds.afo[kl - k * ds.L];
+        };
+        if (j == 0) s_front.d = first_g(kl0);
+        d1 = first_g(kl1);
+        sseed = c.seed + (uint64_t)m * kPBlocks + b;
+        salt = kSaltUpdateP;
+    } else {
+        kind = qkind;
+        if (j == 0) s_front.d = 0;
+        d1 = (int64_t)ds.ntyped * c.K;
+        sseed = c.seed + (uint64_t)m;
+        salt = kSaltUpdateQ;
+    }
+    const uint32_t* J = ds.ring_jump;
+    Xs128 st;
+    st.seed(sseed, salt);
+    ring_seed(st, J);
+    int64_t ws = 0;
+    ring_fill(st, ring);
+    if (j == 0) {
+        s_front.pos = 0;
+        s_front.bm = 0;
+        s_front.U1 = as_f64(0xFFFFFFFFull);
+        s_front.U2 = 0;
+        s_front.cached = 0;
+    }
+    __syncthreads();
+
+    for (;;) {
+        if (s_front.d >= d1) break;
+        ring_follow(s_front.pos, ws, st, ring, J);
+        const SppFront f = s_front;
+        const int64_t nact = min((int64_t)kSppThreads, d1 - f.d);
+        const bool act = j < nact;
+        double alpha = 2.0;
+        double* out = act ? draw_target(ds, c, kind, f.d + j, &alpha) : nullptr;
+        const bool lt1 = act && alpha < 1;
+        s_lt1[j] = lt1 ? 1 : 0;
+        if (j == 0) s_first = INT_MAX;
+        int nlt1;
+        const int A = block_prefix(lt1, s_cnt, &nlt1);   // (its barrier also publishes s_lt1, s_first)
+
+        // the speculated start: every earlier draw of the window typical
+        const bool cj = (f.cached != 0) ^ (j & 1);
+        const int64_t pj = f.pos + (f.cached ? 2 * j - (j & 1) : 2 * j + (j & 1)) + A;
+        const int64_t bmj = cj ? (j == 0 ? f.bm : pj - 3 - s_lt1[j - 1]) : pj;
+
+        bool ovf = false;
+        auto u01 = [&](int64_t p) -> double {
+            const int64_t o = p - ws;
+            if (o < 0 || o >= kRingWords) { ovf = true; return 0.5; }
+            return unit12(ring[o]) - 1.0;
+        };
+        // the draw's first normal: the cached pair's sine, or a new pair's
+        // cosine (one form for both, so the wave stays convergent).  Lane 0
+        // takes a cached pair from the front (its words may have left the
+        // ring) and reads in-ring words only to keep the form.
+        const int64_t bsrc = (j == 0 && cj) ? f.pos : bmj;
+        double U1 = my_sqrt(-2.0 * log(u01(bsrc)));
+        double U2 = 2.0 * kPi * u01(bsrc + 1);
+        if (j == 0 && cj) { U1 = f.U1; U2 = f.U2; }
+        double t = U1 * my_sin(cj ? U2 : U2 + (kPi / 2));
+        int64_t pos = pj, bm = f.bm;
+        bool cache;
+        if (cj) {
+            cache = false;
+        } else {
+            pos += 2;
+            bm = pj;
+            cache = (uint32_t)as_u64(U1) != 0xFFFFFFFFu;
+        }
+        auto normal = [&]() -> double {
+            if (cache) {
+                cache = false;
+                return U1 * my_sin(U2);
+            }
+            const double v1 = u01(pos), v2 = u01(pos + 1);
+            bm = pos;
+            pos += 2;
+            U1 = my_sqrt(-2.0 * log(v1));
+            U2 = 2.0 * kPi * v2;
+            cache = (uint32_t)as_u64(U1) != 0xFFFFFFFFu;
+            return U1 * my_cos(U2);
+        };
+        // Gamma(alpha) of RNG<double>, src/statistics.cpp:212-242: shapes
+        // below 1 draw Gamma(1 + alpha) and a boost
+        const double ag = alpha < 1 ? 1.0 + alpha : alpha;
+        const double dd = ag - 0.333333333333333;
+        const double cc = 1.0 / (sqrt(dd) * 3.0);
+        double v = 1.0, u;
+        bool first = true;
+        for (int guard = 0; guard < 4096 && !ovf; ++guard) {
+            do {
+                if (!first) t = normal();
+                first = false;
+                const double v1 = cc * t;
+                v = 1.0 + v1;
+            } while (v <= 0 && !ovf);
+            v = v * v * v;
+            u = u01(pos++);
+            if (u < 1.0 - 0.0331 * t * t * t * t) break;
+            if (log(u) < 0.5 * t * t + dd * (1.0 - v + log(v))) break;
+        }
+        double val = dd * v;
+        if (alpha < 1) {
+            const double boost = pow(u01(pos++), 1.0 / alpha);
+            val = val * boost;
+        }
+        // typical end: the next lane's speculated start
+        const int64_t tpos = pj + (cj ? 1 : 3) + (lt1 ? 1 : 0);
+        const bool dev = act && (ovf || pos != tpos || cache == cj || (cache && bm != pj));
+        if (dev) atomicMin(&s_first, 2 * j + (ovf ? 1 : 0));
+        __syncthreads();
+        const int code = s_first;
+        int64_t ncommit;
+        bool forced = false;
+        if (code == INT_MAX || (code >> 1) >= nact) ncommit = nact;
+        else if (!(code & 1)) ncommit = (code >> 1) + 1;
+        else if ((code >> 1) > 0) ncommit = code >> 1;
+        else { ncommit = 1; forced = true; }
+        if (act && j < ncommit) *out = forced ? __builtin_nan("") : val;
+        if (j == ncommit - 1) {
+            SppFront nf;
+            nf.d = f.d + ncommit;
+            nf.pos = forced ? ws + kRingWords : pos;
+            nf.cached = forced ? 0 : (cache ? 1 : 0);
+            nf.bm = bm;
+            nf.U1 = cache ? U1 : as_f64(0xFFFFFFFFull);
+            nf.U2 = U2;
+            nf.pad = 0;
+            s_front = nf;
+        }
+        __syncthreads();
+    }
+
+    // the proposals' stream continues in the acceptance kernel: its state
+    // after the last draw (generator at `pos`, and the cache)
+    if (kind == kDrawProp) {
+        ring_follow(s_front.pos, ws, st, ring, J);
+        const int64_t off = s_front.pos - ws;
+        if (j == (int)(off >> 3)) {
+            Xs128 s = st;
+            for (int x = 0; x < (int)(off & 7); ++x) s.next();
+            Rng r;
+            r.s = s;
+            r.U1 = s_front.cached ? s_front.U1 : as_f64(0xFFFFFFFFull);
+            r.U2 = s_front.U2;
+            c.rngbuf[0] = r;
+        }
+    }
+}
+
+// Dirichlet rows from the Gamma draws: each P row (k, l) and each typed
+// individual's Q (or proposal) row is scaled by 1/sum (the reference's
+// Dirichlet, src/statistics.cpp:269-296).  clear_mi zeroes every Mi row for
+// the Z pass; thread 0 copies the sweep index for the kernels that run
+// beside the Z pass.
+__global__ void k_spp_normalize(const DeviceDataset ds, const ChainDev c, int do_p, int qkind, int clear_mi)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x == 0) c.mside[0] = *c.mdev;
+    const int64_t np = do_p ? (int64_t)c.K * ds.L : 0;
+    if (x < np) {
+        const int64_t k = x / ds.L, l = x - k * ds.L;
+        double* p = c.Freq + k * ds.KT + ds.afo[l];
+        const int k2 = ds.k2[l];
+        double s = 0;
+        for (int a = 0; a < k2; ++a) s += p[a];
+        const double inv = 1.0 / s;
+        for (int a = 0; a < k2; ++a) p[a] *= inv;
+        return;
+    }
+    const int64_t i = x - np;
+    if (i >= ds.N) return;
+    const int K = c.K;
+    if (qkind >= 0 && ds.vt[i]) {
+        double* r = (qkind == kDrawQ ? c.Q : c.bufNK1) + i * K;
+        double s = 0;
+        for (int k = 0; k < K; ++k) s += r[k];
+        const double inv = 1.0 / s;
+        for (int k = 0; k < K; ++k) r[k] *= inv;
+    }
+    if (clear_mi)
+        for (int k = 0; k < K; ++k) c.Mi[i * K + k] = 0;
+}
+
+// Metropolis acceptance on the proposals' stream (src/structure.cpp:1034-
+// 1042): individuals in order, a uniform drawn only for a log ratio below
+// NZERO.  The position of each such uniform is a prefix count, so one block
+// decides 1024 individuals at a time.  bufN2 arrives as each individual's log
+// likelihood under its current Q and leaves as the one under the Q it keeps.
+__global__ __launch_bounds__(kSppThreads) void k_spp_metro_accept(const DeviceDataset ds, const ChainDev c)
+{
+    __shared__ uint64_t ring[kRingWords];
+    __shared__ int s_cnt[kSppThreads / 64];
+    const uint32_t* J = ds.ring_jump;
+    Xs128 st = c.rngbuf[0].s;
+    ring_seed(st, J);
+    int64_t ws = 0, used = 0;
+    ring_fill(st, ring);
+    __syncthreads();
+    const int K = c.K;
+    for (int base = 0; base < ds.N; base += kSppThreads) {
+        ring_follow(used, ws, st, ring, J);
+        const int i = base + threadIdx.x;
+        const bool typed = i < ds.N && ds.vt[i];
+        const double r = typed ? c.bufN1[i] : 0.0;
+        const bool need = typed && !(r >= kNZero);
+        int total;
+        const int off = block_prefix(need, s_cnt, &total);
+        bool acc = typed && r >= kNZero;
+        if (need) acc = unit12(ring[used + off - ws]) - 1.0 < exp(r);
+        if (acc) {
+            for (int k = 0; k < K; ++k) c.Q[(int64_t)i * K + k] = c.bufNK1[(int64_t)i * K + k];
+            c.bufN2[i] += r;
+        }
+        used += total;
+        __syncthreads();   // (s_cnt and the ring are reused)
+    }
+}
+
+// UpdateQNoAdmix's PolyLog draws (src/structureAVX.cpp:279-287): typed
+// individual number t takes the stream's t-th word; Q rows are cleared and
+// Q[i][Z_i] = 1.
+__global__ __launch_bounds__(kSppThreads) void k_spp_polylog(const DeviceDataset ds, const ChainDev c)
+{
+    __shared__ uint64_t ring[kRingWords];
+    __shared__ int s_cnt[kSppThreads / 64];
+    const int m = *c.mdev;
+    const uint32_t* J = ds.ring_jump;
+    Xs128 st;
+    st.seed(c.seed + (uint64_t)m, kSaltUpdateQ);
+    ring_seed(st, J);
+    int64_t ws = 0, used = 0;
+    ring_fill(st, ring);
+    __syncthreads();
+    const int K = c.K;
+    for (int base = 0; base < ds.N; base += kSppThreads) {
+        ring_follow(used, ws, st, ring, J);
+        const int i = base + threadIdx.x;
+        const bool typed = i < ds.N && ds.vt[i];
+        int total;
+        const int off = block_prefix(typed, s_cnt, &total);
+        if (i < ds.N) {
+            for (int k = 0; k < K; ++k) c.Q[(int64_t)i * K + k] = 0;
+            if (typed) {
+                const int k2 = polylog_word(ring[used + off - ws], c.bufNK1 + (int64_t)i * K, K);
+                c.Q[(int64_t)i * K + k2] = 1;
+                c.Z[i] = (uint16_t)k2;
+            }
+        }
+        used += total;
+        __syncthreads();
+    }
+}
+
+int launch_spp(const DeviceDataset& ds, const ChainDev& c, bool do_p, int qkind, bool clear_mi, hipStream_t st)
+{
+    const int np = do_p ? kPBlocks : 0;
+    const int nb = np + (qkind >= 0 ? 1 : 0);
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_spp_gamma, dim3(nb), dim3(kSppThreads), 0, st, ds, c, np, qkind);
+        VPS_HIP(hipGetLastError());
+    }
+    const int64_t n = (do_p ? (int64_t)c.K * ds.L : 0) + ds.N;
+    hipLaunchKernelGGL(k_spp_normalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ds, c, do_p ? 1 : 0,
+                       qkind, clear_mi ? 1 : 0);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+int launch_spp_metro_accept(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_spp_metro_accept, dim3(1), dim3(kSppThreads), 0, st, ds, c);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+int launch_spp_polylog(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_spp_polylog, dim3(1), dim3(kSppThreads), 0, st, ds, c);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
+}
+
+}  // namespace vps

@@ -10,7 +10,8 @@
 //                      (src/structure.cpp:843-861); Q in one stream
 //                      RNG(seed + m, UPDATEQ) over individuals
 //                      (src/structure.cpp:990-1003); A in 32 blocks.
-//                      Parity mode: 128 (resp. 1, 32) device lanes.
+//                      The P and Q streams are parsed in parallel
+//                      (vps_spp.hip); A runs 32 device lanes.
 //   VPS_RNG_PARALLEL   one stream per (k, l): RNG(seed + (m*K + k)*L + l,
 //                      UPDATEP); per individual RNG(seed + m*N + i, UPDATEQ);
 //                      per locus RNG(seed + m*L + l, UPDATEA).  Same
@@ -56,22 +57,10 @@ __global__ void k_update_p_par(const DeviceDataset ds, const ChainDev c)
     draw_p_one(ds, c, r, k, l);
 }
 
-__global__ void k_update_p_ref(const DeviceDataset ds, const ChainDev c)
-{
-    const int m = *c.mdev;
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= 128) return;
-    const int64_t KL = (int64_t)c.K * ds.L;
-    const int64_t kl0 = (int64_t)b * KL / 128, kl1 = (int64_t)(b + 1) * KL / 128;
-    Rng r;
-    r.init(c.seed + (uint64_t)m * 128 + b, kSaltUpdateP);
-    for (int64_t kl = kl0; kl < kl1; ++kl) draw_p_one(ds, c, r, kl / ds.L, kl % ds.L);
-}
-
 int launch_update_p(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     if (c.topology == VPS_RNG_REFERENCE) {
-        hipLaunchKernelGGL(k_update_p_ref, dim3(2), dim3(64), 0, st, ds, c);
+        return launch_spp(ds, c, true, kSppQNone, false, st);
     } else {
         const int64_t n = (int64_t)c.K * ds.L;
         hipLaunchKernelGGL(k_update_p_par, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ds, c);
@@ -91,14 +80,6 @@ __device__ __forceinline__ const double* alpha_row(const DeviceDataset& ds, cons
 __global__ void k_q_admix(const DeviceDataset ds, const ChainDev c)
 {
     const int m = *c.mdev;
-    if (c.topology == VPS_RNG_REFERENCE) {
-        if (blockIdx.x != 0 || threadIdx.x != 0) return;
-        Rng r;
-        r.init(c.seed + (uint64_t)m, kSaltUpdateQ);
-        for (int i = 0; i < ds.N; ++i)
-            if (ds.vt[i]) dirichlet_ab(r, c.Q + (int64_t)i * c.K, alpha_row(ds, c, i), c.Mi + (int64_t)i * c.K, c.K);
-        return;
-    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ds.N || !ds.vt[i]) return;
     Rng r;
@@ -108,8 +89,9 @@ __global__ void k_q_admix(const DeviceDataset ds, const ChainDev c)
 
 int launch_update_q_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
-    const unsigned nb = c.topology == VPS_RNG_REFERENCE ? 1 : (unsigned)((ds.N + 255) / 256);
-    hipLaunchKernelGGL(k_q_admix, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c);
+    if (c.topology == VPS_RNG_REFERENCE) return launch_spp(ds, c, false, kSppQAdmix, false, st);
+    const unsigned nb = (unsigned)((ds.N + 255) / 256);
+    hipLaunchKernelGGL(k_q_admix, dim3(nb), dim3(256), 0, st, ds, c);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
@@ -161,15 +143,6 @@ int launch_update_pq(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 __global__ void k_q_metro_propose(const DeviceDataset ds, const ChainDev c)
 {
     const int m = *c.mdev;
-    if (c.topology == VPS_RNG_REFERENCE) {
-        if (blockIdx.x != 0 || threadIdx.x != 0) return;
-        Rng r;
-        r.init(c.seed + (uint64_t)m, kSaltUpdateQ);
-        for (int i = 0; i < ds.N; ++i)
-            if (ds.vt[i]) dirichlet_a(r, c.bufNK1 + (int64_t)i * c.K, alpha_row(ds, c, i), c.K);
-        c.rngbuf[0] = r;
-        return;
-    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ds.N) return;
     Rng r;
@@ -186,18 +159,6 @@ __global__ void k_q_metro_propose(const DeviceDataset ds, const ChainDev c)
 __global__ void k_q_metro_accept(const DeviceDataset ds, const ChainDev c)
 {
     const int K = c.K;
-    if (c.topology == VPS_RNG_REFERENCE) {
-        if (blockIdx.x != 0 || threadIdx.x != 0) return;
-        Rng r = c.rngbuf[0];
-        for (int i = 0; i < ds.N; ++i) {
-            if (!ds.vt[i]) continue;
-            if (c.bufN1[i] >= kNZero || r.uniform() < exp(c.bufN1[i])) {
-                for (int k = 0; k < K; ++k) c.Q[(int64_t)i * K + k] = c.bufNK1[(int64_t)i * K + k];
-                c.bufN2[i] += c.bufN1[i];
-            }
-        }
-        return;
-    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ds.N || !ds.vt[i]) return;
     Rng r = c.rngbuf[i];
@@ -209,16 +170,18 @@ __global__ void k_q_metro_accept(const DeviceDataset ds, const ChainDev c)
 
 int launch_q_metro_propose(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
-    const unsigned nb = c.topology == VPS_RNG_REFERENCE ? 1 : (unsigned)((ds.N + 255) / 256);
-    hipLaunchKernelGGL(k_q_metro_propose, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c);
+    if (c.topology == VPS_RNG_REFERENCE) return launch_spp(ds, c, false, kSppQProposals, false, st);
+    const unsigned nb = (unsigned)((ds.N + 255) / 256);
+    hipLaunchKernelGGL(k_q_metro_propose, dim3(nb), dim3(256), 0, st, ds, c);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
 
 int launch_q_metro_accept(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
-    const unsigned nb = c.topology == VPS_RNG_REFERENCE ? 1 : (unsigned)((ds.N + 255) / 256);
-    hipLaunchKernelGGL(k_q_metro_accept, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c);
+    if (c.topology == VPS_RNG_REFERENCE) return launch_spp_metro_accept(ds, c, st);
+    const unsigned nb = (unsigned)((ds.N + 255) / 256);
+    hipLaunchKernelGGL(k_q_metro_accept, dim3(nb), dim3(256), 0, st, ds, c);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
@@ -232,19 +195,6 @@ __global__ void k_q_noadmix_draw(const DeviceDataset ds, const ChainDev c)
 {
     const int m = *c.mdev;
     const int K = c.K;
-    if (c.topology == VPS_RNG_REFERENCE) {
-        if (blockIdx.x != 0 || threadIdx.x != 0) return;
-        Rng r;
-        r.init(c.seed + (uint64_t)m, kSaltUpdateQ);
-        for (int i = 0; i < ds.N; ++i) {
-            for (int k = 0; k < K; ++k) c.Q[(int64_t)i * K + k] = 0;
-            if (!ds.vt[i]) continue;
-            const int k2 = polylog(r, c.bufNK1 + (int64_t)i * K, K);
-            c.Q[(int64_t)i * K + k2] = 1;
-            c.Z[i] = (uint16_t)k2;
-        }
-        return;
-    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ds.N) return;
     for (int k = 0; k < K; ++k) c.Q[(int64_t)i * K + k] = 0;
@@ -258,8 +208,9 @@ __global__ void k_q_noadmix_draw(const DeviceDataset ds, const ChainDev c)
 
 int launch_q_noadmix_draw(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
-    const unsigned nb = c.topology == VPS_RNG_REFERENCE ? 1 : (unsigned)((ds.N + 255) / 256);
-    hipLaunchKernelGGL(k_q_noadmix_draw, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c);
+    if (c.topology == VPS_RNG_REFERENCE) return launch_spp_polylog(ds, c, st);
+    const unsigned nb = (unsigned)((ds.N + 255) / 256);
+    hipLaunchKernelGGL(k_q_noadmix_draw, dim3(nb), dim3(256), 0, st, ds, c);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }

@@ -30,28 +30,6 @@
 
 namespace vps {
 
-// apply a jump table (32 nibbles x 16 entries x 128 bits) to a stream.
-// One state word (8 nibbles) per outer step: at most 8 table loads are in
-// flight, so the prologue needs no more registers than the sampling loop
-// (the loop, not the jump, then sets the kernel's occupancy).
-__device__ __forceinline__ void jump_apply(const uint32_t* __restrict__ tab, Xs128& r)
-{
-    uint32_t w0 = (uint32_t)r.x, w1 = (uint32_t)(r.x >> 32), w2 = (uint32_t)r.y, w3 = (uint32_t)(r.y >> 32);
-    uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-    const uint4* t4 = reinterpret_cast<const uint4*>(tab);
-#pragma unroll 1
-    for (int q = 0; q < 4; ++q, t4 += 8 * 16) {
-#pragma unroll
-        for (int p = 0; p < 8; ++p) {
-            const uint4 e = t4[p * 16 + ((w0 >> (p * 4)) & 15u)];
-            a0 ^= e.x; a1 ^= e.y; a2 ^= e.z; a3 ^= e.w;
-        }
-        w0 = w1; w1 = w2; w2 = w3;
-    }
-    r.x = (uint64_t)a0 | ((uint64_t)a1 << 32);
-    r.y = (uint64_t)a2 | ((uint64_t)a3 << 32);
-}
-
 // U - 1 for U in [1, 2) built from a draw.  The exponent word is set with
 // one bitfield insert whose mask and exponent bits sit in registers (a
 // 3-operand VALU instruction takes no literal here); the registers are set
