@@ -86,16 +86,19 @@ void parallel_for(int64_t n, F&& f)
 
 namespace {
 
-// GF(2) jump tables of the parallel stream parse: table r advances an
-// xorshift128+ state by 8 * 2^r draws (8, 16, ..., 4096).  The same for every
-// dataset; built once.
+// GF(2) jump tables of the parallel stream parse: tables l < 64 advance an
+// xorshift128+ state by 8*l draws, tables 64 + w by 512*w draws, so thread
+// 64w + l of a 1024-thread block reaches its ring offset 8*(64w + l) in two
+// jumps.  The same for every dataset; built once.
 const std::vector<uint32_t>& ring_jump_tables()
 {
     static std::once_flag once;
     static std::vector<uint32_t> tabs;
     std::call_once(once, [] {
         tabs.resize((size_t)kRingJumps * kJumpWords);
-        for (int r = 0; r < kRingJumps; ++r) jump_table((uint64_t)8 << r, tabs.data() + (size_t)r * kJumpWords);
+        for (int l = 0; l < kRingLaneTabs; ++l) jump_table((uint64_t)8 * l, tabs.data() + (size_t)l * kJumpWords);
+        for (int w = 0; w < kRingWaveTabs; ++w)
+            jump_table((uint64_t)512 * w, tabs.data() + (size_t)(kRingLaneTabs + w) * kJumpWords);
     });
     return tabs;
 }

@@ -60,7 +60,11 @@ struct TileDesc {
 // nibbles of the 128-bit state; entry [p][v] is the 128-bit image of nibble
 // p holding v (4 dwords: x_lo, x_hi, y_lo, y_hi).
 constexpr int kJumpWords = 32 * 16 * 4;
-constexpr int kRingJumps = 10;   // 8, 16, ..., 4096 draws (the parallel stream parse)
+// jump tables of the parallel stream parse: [0, 64) advance 8*l draws (l =
+// lane), [64, 80) advance 512*w draws (w = wave); table 72 is 4,096 draws
+constexpr int kRingLaneTabs = 64, kRingWaveTabs = 16;
+constexpr int kRingJumps = kRingLaneTabs + kRingWaveTabs;
+constexpr int kRingAdvanceTab = kRingLaneTabs + 8;
 
 struct DeviceDataset {
     // sizes
@@ -96,7 +100,7 @@ struct DeviceDataset {
     int64_t*  kdis;            // [maxK+1] loci per allele count
     int32_t*  typed;           // [ntyped] individuals with vt > 0, ascending
     int32_t   ntyped;
-    uint32_t* ring_jump;       // [kRingJumps * kJumpWords]: table r advances a stream 8 * 2^r draws
+    uint32_t* ring_jump;       // [kRingJumps * kJumpWords] (see kRingLaneTabs)
 };
 
 }  // namespace vps

@@ -57,12 +57,12 @@ struct SppFront {
 };
 
 // this thread's generator: the state `8 * threadIdx.x` draws into the ring
+// (two jumps: 512 per wave, then 8 per lane)
 __device__ __forceinline__ void ring_seed(Xs128& st, const uint32_t* __restrict__ J)
 {
-    const int t = threadIdx.x;
-#pragma unroll 1
-    for (int r = 0; r < kRingJumps; ++r)
-        if ((t >> r) & 1) jump_apply(J + (size_t)r * kJumpWords, st);
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (w) jump_apply(J + (size_t)(kRingLaneTabs + w) * kJumpWords, st);
+    if (l) jump_apply(J + (size_t)l * kJumpWords, st);
 }
 
 __device__ __forceinline__ void ring_fill(const Xs128& st, uint64_t* ring)
@@ -73,15 +73,21 @@ __device__ __forceinline__ void ring_fill(const Xs128& st, uint64_t* ring)
     for (int u = 0; u < 8; ++u) o[u] = s.next();
 }
 
+// the 4,096-draw jump table, copied to LDS once per block
+__device__ __forceinline__ void ring_load_advance(const uint32_t* __restrict__ J, uint32_t* tab)
+{
+    const uint32_t* src = J + (size_t)kRingAdvanceTab * kJumpWords;
+    for (int x = threadIdx.x; x < kJumpWords; x += kSppThreads) tab[x] = src[x];
+}
+
 // moves the ring's window forward by whole steps until `pos` lies in its
 // first half (a block-uniform condition; every thread calls it)
-__device__ __forceinline__ void ring_follow(int64_t pos, int64_t& ws, Xs128& st, uint64_t* ring,
-                                            const uint32_t* __restrict__ J)
+__device__ __forceinline__ void ring_follow(int64_t pos, int64_t& ws, Xs128& st, uint64_t* ring, const uint32_t* tab)
 {
     if (pos - ws < kRingStep) return;
     const int64_t n = (pos - ws) / kRingStep;
 #pragma unroll 1
-    for (int64_t x = 0; x < n; ++x) jump_apply(J + (size_t)(kRingJumps - 1) * kJumpWords, st);
+    for (int64_t x = 0; x < n; ++x) jump_apply(tab, st);
     ws += n * kRingStep;
     __syncthreads();   // (the ring's last readers are done)
     ring_fill(st, ring);
@@ -130,21 +136,52 @@ __device__ __forceinline__ double* draw_target(const DeviceDataset& ds, const Ch
     return c.bufNK1 + (int64_t)i * c.K + k;
 }
 
+// Slots.  Seen from the front of a window, the stream is a sequence of
+// slots, each one normal (a fresh Box-Muller pair, 2 words, when none is
+// cached; else the cached one, 0 words) and one acceptance uniform; the
+// pairs alternate with the cached normals.  A draw that takes its first
+// candidate fills one slot, and each rejection fills one more, so a draw
+// after r rejections in the window starts r slots later than it would
+// otherwise.  Shapes below 1 add one uniform after their draw.  Slot s of a
+// draw with A such uniforms before it starts at word
+//     front + s + 2 * (pairs among slots [0, s)) + A.
+// A window evaluates each of its draws under H hypotheses h = 0..H-1 ("h
+// rejections before me in this window"); the true hypothesis of draw j+1 is
+// h_j + r_j, so a scan over the draws that deviate picks each draw's true
+// evaluation.  A v <= 0 candidate uses a normal without a uniform and leaves
+// the slot pattern: such a draw ends the window (its own value is right).
+__device__ __forceinline__ int64_t slot_words(int64_t s, bool front_cached)
+{
+    return s + 2 * ((s + (front_cached ? 0 : 1)) >> 1);
+}
+
 // The parse of one stream per block.  Blocks [0, np) are UpdateP's 128
 // streams (np = 0 or 128); block np, when qkind >= 0, is UpdateQ's stream with
 // draws of kind qkind.  Values are unnormalised Gamma draws; k_spp_normalize
 // turns each row into its Dirichlet draw.  The Metropolis proposals' stream
 // state after the parse goes to rngbuf[0] for the acceptance kernel.
+//
+// Thread layout of a window with H hypotheses (H in 1, 2, 4, 8, 16): wave w
+// takes hypothesis w % H for draws 64 * (w / H) + lane, so a window holds
+// 1024 / H draws.  H adapts to the deviation rate seen so far: a window
+// commits about min(1024 / H, H / rate) draws.
 __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset ds, const ChainDev c, int np, int qkind)
 {
     __shared__ uint64_t ring[kRingWords];
+    __shared__ uint32_t s_adv[kJumpWords];
     __shared__ int s_cnt[kSppThreads / 64];
-    __shared__ uint8_t s_lt1[kSppThreads];
-    __shared__ int s_first;
+    __shared__ uint8_t s_lt1[kSppThreads];       // per draw of the window
+    __shared__ uint8_t s_next[kSppThreads];      // per thread: next hypothesis, or kNextStop / kNextOvf
+    __shared__ uint64_t s_dev[kSppThreads / 64]; // per wave: draws not typical under its hypothesis
+    __shared__ int s_seg_lane[17];
+    __shared__ uint8_t s_seg_h[17];
+    __shared__ int s_nseg, s_commit, s_stop_ovf, s_H;
+    __shared__ float s_rate;
     __shared__ SppFront s_front;
+    constexpr uint8_t kNextStop = 0xFF, kNextOvf = 0xFE;
 
     const int m = *c.mdev;
-    const int j = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int kind;
     uint64_t sseed, salt;
     int64_t d1;
@@ -159,50 +196,66 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
             const int64_t k = kl / ds.L;
             return k * ds.KT + ds.afo[kl - k * ds.L];
         };
-        if (j == 0) s_front.d = first_g(kl0);
+        if (tid == 0) s_front.d = first_g(kl0);
         d1 = first_g(kl1);
         sseed = c.seed + (uint64_t)m * kPBlocks + b;
         salt = kSaltUpdateP;
     } else {
         kind = qkind;
-        if (j == 0) s_front.d = 0;
+        if (tid == 0) s_front.d = 0;
         d1 = (int64_t)ds.ntyped * c.K;
         sseed = c.seed + (uint64_t)m;
         salt = kSaltUpdateQ;
     }
     const uint32_t* J = ds.ring_jump;
+    ring_load_advance(J, s_adv);
     Xs128 st;
     st.seed(sseed, salt);
     ring_seed(st, J);
     int64_t ws = 0;
     ring_fill(st, ring);
-    if (j == 0) {
+    if (tid == 0) {
         s_front.pos = 0;
         s_front.bm = 0;
         s_front.U1 = as_f64(0xFFFFFFFFull);
         s_front.U2 = 0;
         s_front.cached = 0;
+        s_H = 1;
+        s_rate = 0.f;
     }
     __syncthreads();
 
     for (;;) {
         if (s_front.d >= d1) break;
-        ring_follow(s_front.pos, ws, st, ring, J);
+        ring_follow(s_front.pos, ws, st, ring, s_adv);
         const SppFront f = s_front;
-        const int64_t nact = min((int64_t)kSppThreads, d1 - f.d);
-        const bool act = j < nact;
+        const int H = s_H;
+        const int G = kSppThreads / 64 / H;            // draw groups of 64
+        const int h = wave % H, grp = wave / H;
+        const int jd = grp * 64 + lane;                // this thread's draw in the window
+        const int64_t nact = min((int64_t)(64 * G), d1 - f.d);
+        const bool act = jd < nact;
         double alpha = 2.0;
-        double* out = act ? draw_target(ds, c, kind, f.d + j, &alpha) : nullptr;
+        double* out = act ? draw_target(ds, c, kind, f.d + jd, &alpha) : nullptr;
         const bool lt1 = act && alpha < 1;
-        s_lt1[j] = lt1 ? 1 : 0;
-        if (j == 0) s_first = INT_MAX;
-        int nlt1;
-        const int A = block_prefix(lt1, s_cnt, &nlt1);   // (its barrier also publishes s_lt1, s_first)
+        // prefix count of shapes below 1 over the window's draws (their
+        // ballot is the same in every hypothesis's wave of a group)
+        const uint64_t bal = __ballot(lt1);
+        if (h == 0) {
+            s_lt1[jd] = lt1 ? 1 : 0;
+            if (lane == 0) s_cnt[grp] = __popcll(bal);
+        }
+        __syncthreads();
+        int A = __popcll(bal & ((1ull << lane) - 1ull));
+        for (int x = 0; x < grp; ++x) A += s_cnt[x];
 
-        // the speculated start: every earlier draw of the window typical
-        const bool cj = (f.cached != 0) ^ (j & 1);
-        const int64_t pj = f.pos + (f.cached ? 2 * j - (j & 1) : 2 * j + (j & 1)) + A;
-        const int64_t bmj = cj ? (j == 0 ? f.bm : pj - 3 - s_lt1[j - 1]) : pj;
+        // start of this draw under hypothesis h
+        const int64_t sl = jd + h;
+        const bool fc = f.cached != 0;
+        const bool cj = fc ^ (sl & 1);
+        const int64_t pj = f.pos + slot_words(sl, fc) + A;
+        const bool live = act && (jd > 0 || h == 0);   // draw 0 starts at the front
+        const int64_t bmj = cj ? (sl == 0 ? f.bm : pj - 3 - (jd > 0 ? s_lt1[jd - 1] : 0)) : pj;
 
         bool ovf = false;
         auto u01 = [&](int64_t p) -> double {
@@ -211,13 +264,13 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
             return unit12(ring[o]) - 1.0;
         };
         // the draw's first normal: the cached pair's sine, or a new pair's
-        // cosine (one form for both, so the wave stays convergent).  Lane 0
-        // takes a cached pair from the front (its words may have left the
-        // ring) and reads in-ring words only to keep the form.
-        const int64_t bsrc = (j == 0 && cj) ? f.pos : bmj;
+        // cosine (one form for both, so the wave stays convergent).  At the
+        // front a cached pair comes from the front's values (its words may
+        // have left the ring); the ring is read in range only to keep the form.
+        const int64_t bsrc = (sl == 0 && cj) || !live ? f.pos : bmj;
         double U1 = my_sqrt(-2.0 * log(u01(bsrc)));
         double U2 = 2.0 * kPi * u01(bsrc + 1);
-        if (j == 0 && cj) { U1 = f.U1; U2 = f.U2; }
+        if (sl == 0 && cj) { U1 = f.U1; U2 = f.U2; }
         double t = U1 * my_sin(cj ? U2 : U2 + (kPi / 2));
         int64_t pos = pj, bm = f.bm;
         bool cache;
@@ -228,6 +281,7 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
             bm = pj;
             cache = (uint32_t)as_u64(U1) != 0xFFFFFFFFu;
         }
+        bool off_slot = !cj && !cache;   // a pair whose U1 reads as the empty marker
         auto normal = [&]() -> double {
             if (cache) {
                 cache = false;
@@ -239,6 +293,7 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
             U1 = my_sqrt(-2.0 * log(v1));
             U2 = 2.0 * kPi * v2;
             cache = (uint32_t)as_u64(U1) != 0xFFFFFFFFu;
+            if (!cache) off_slot = true;
             return U1 * my_cos(U2);
         };
         // Gamma(alpha) of RNG<double>, src/statistics.cpp:212-242: shapes
@@ -248,15 +303,20 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
         const double cc = 1.0 / (sqrt(dd) * 3.0);
         double v = 1.0, u;
         bool first = true;
+        int slots = 0;
         for (int guard = 0; guard < 4096 && !ovf; ++guard) {
+            int normals = 0;
             do {
                 if (!first) t = normal();
                 first = false;
+                ++normals;
                 const double v1 = cc * t;
                 v = 1.0 + v1;
             } while (v <= 0 && !ovf);
+            if (normals > 1) off_slot = true;
             v = v * v * v;
             u = u01(pos++);
+            ++slots;
             if (u < 1.0 - 0.0331 * t * t * t * t) break;
             if (log(u) < 0.5 * t * t + dd * (1.0 - v + log(v))) break;
         }
@@ -265,29 +325,102 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
             const double boost = pow(u01(pos++), 1.0 / alpha);
             val = val * boost;
         }
-        // typical end: the next lane's speculated start
-        const int64_t tpos = pj + (cj ? 1 : 3) + (lt1 ? 1 : 0);
-        const bool dev = act && (ovf || pos != tpos || cache == cj || (cache && bm != pj));
-        if (dev) atomicMin(&s_first, 2 * j + (ovf ? 1 : 0));
+        // where the next draw starts if this one filled `slots` slots: slot
+        // sl + slots, behind this draw's own uniform for a shape below 1
+        const int64_t nsl = sl + slots;
+        const bool ncached = fc ^ (nsl & 1);
+        const int64_t npos = f.pos + slot_words(nsl, fc) + A + (lt1 ? 1 : 0);
+        const bool aligned = !ovf && !off_slot && pos == npos && cache == ncached && (!cache || bm == npos - 3 - (lt1 ? 1 : 0));
+        const int nh = h + slots - 1;
+        uint8_t nx;
+        if (!live) nx = kNextStop;
+        else if (ovf) nx = kNextOvf;
+        else if (!aligned || nh >= H) nx = kNextStop;
+        else nx = (uint8_t)nh;
+        s_next[tid] = nx;
+        const uint64_t devb = __ballot(live && nx != (uint8_t)h);
+        if (lane == 0) s_dev[wave] = devb;
         __syncthreads();
-        const int code = s_first;
-        int64_t ncommit;
-        bool forced = false;
-        if (code == INT_MAX || (code >> 1) >= nact) ncommit = nact;
-        else if (!(code & 1)) ncommit = (code >> 1) + 1;
-        else if ((code >> 1) > 0) ncommit = code >> 1;
-        else { ncommit = 1; forced = true; }
-        if (act && j < ncommit) *out = forced ? __builtin_nan("") : val;
-        if (j == ncommit - 1) {
-            SppFront nf;
-            nf.d = f.d + ncommit;
-            nf.pos = forced ? ws + kRingWords : pos;
-            nf.cached = forced ? 0 : (cache ? 1 : 0);
-            nf.bm = bm;
-            nf.U1 = cache ? U1 : as_f64(0xFFFFFFFFull);
-            nf.U2 = U2;
-            nf.pad = 0;
-            s_front = nf;
+
+        // the path through the hypotheses: one thread follows it over the
+        // deviating draws only
+        if (tid == 0) {
+            int hh = 0, nseg = 1, commit = (int)nact, stop_ovf = 0, events = 0;
+            s_seg_lane[0] = 0;
+            s_seg_h[0] = 0;
+            for (int g = 0; g < G && commit == (int)nact; ++g) {
+                int j = 0;
+                while (j < 64) {
+                    const uint64_t msk = s_dev[g * H + hh] & (~0ull << j);
+                    if (!msk) break;
+                    const int jj = __ffsll((unsigned long long)msk) - 1;
+                    const int draw = g * 64 + jj;
+                    if (draw >= (int)nact) break;
+                    const uint8_t e = s_next[(g * H + hh) * 64 + jj];
+                    if (e == kNextStop || e == kNextOvf) {
+                        commit = e == kNextOvf ? draw : draw + 1;
+                        stop_ovf = e == kNextOvf;
+                        ++events;
+                        break;
+                    }
+                    events += e - hh;
+                    hh = e;
+                    if (nseg < 17) {
+                        s_seg_lane[nseg] = draw + 1;
+                        s_seg_h[nseg] = (uint8_t)hh;
+                        ++nseg;
+                    }
+                    j = jj + 1;
+                }
+            }
+            s_nseg = nseg;
+            s_commit = commit;
+            s_stop_ovf = stop_ovf;
+            // the next window's hypothesis count, from the deviation rate
+            const int done = max(commit, 1);
+            const float rate = 0.5f * s_rate + 0.5f * (float)events / (float)done;
+            s_rate = rate;
+            int bestH = 1;
+            float best = 0.f;
+            for (int HH = 1; HH <= 16; HH *= 2) {
+                const float expect = fminf((float)(kSppThreads / HH), (float)HH / fmaxf(rate, 1e-5f));
+                if (expect > best * 1.05f) { best = expect; bestH = HH; }
+            }
+            s_H = bestH;
+        }
+        __syncthreads();
+        const int commit = s_commit;
+        int hj = 0;
+        for (int x = 0; x < s_nseg; ++x)
+            if (s_seg_lane[x] <= jd) hj = s_seg_h[x];
+        const bool mine = live && h == hj;
+        if (commit == 0) {
+            // the front draw would need more words than the ring holds
+            if (jd == 0 && h == 0) {
+                *out = __builtin_nan("");
+                SppFront nf;
+                nf.d = f.d + 1;
+                nf.pos = ws + kRingWords;
+                nf.cached = 0;
+                nf.bm = 0;
+                nf.U1 = as_f64(0xFFFFFFFFull);
+                nf.U2 = 0;
+                nf.pad = 0;
+                s_front = nf;
+            }
+        } else if (mine && jd < commit) {
+            *out = val;
+            if (jd == commit - 1) {
+                SppFront nf;
+                nf.d = f.d + commit;
+                nf.pos = pos;
+                nf.cached = cache ? 1 : 0;
+                nf.bm = bm;
+                nf.U1 = cache ? U1 : as_f64(0xFFFFFFFFull);
+                nf.U2 = U2;
+                nf.pad = 0;
+                s_front = nf;
+            }
         }
         __syncthreads();
     }
@@ -295,9 +428,9 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
     // the proposals' stream continues in the acceptance kernel: its state
     // after the last draw (generator at `pos`, and the cache)
     if (kind == kDrawProp) {
-        ring_follow(s_front.pos, ws, st, ring, J);
+        ring_follow(s_front.pos, ws, st, ring, s_adv);
         const int64_t off = s_front.pos - ws;
-        if (j == (int)(off >> 3)) {
+        if (tid == (int)(off >> 3)) {
             Xs128 s = st;
             for (int x = 0; x < (int)(off & 7); ++x) s.next();
             Rng r;
@@ -351,8 +484,10 @@ __global__ void k_spp_normalize(const DeviceDataset ds, const ChainDev c, int do
 __global__ __launch_bounds__(kSppThreads) void k_spp_metro_accept(const DeviceDataset ds, const ChainDev c)
 {
     __shared__ uint64_t ring[kRingWords];
+    __shared__ uint32_t s_adv[kJumpWords];
     __shared__ int s_cnt[kSppThreads / 64];
     const uint32_t* J = ds.ring_jump;
+    ring_load_advance(J, s_adv);
     Xs128 st = c.rngbuf[0].s;
     ring_seed(st, J);
     int64_t ws = 0, used = 0;
@@ -360,7 +495,7 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_metro_accept(const DeviceDa
     __syncthreads();
     const int K = c.K;
     for (int base = 0; base < ds.N; base += kSppThreads) {
-        ring_follow(used, ws, st, ring, J);
+        ring_follow(used, ws, st, ring, s_adv);
         const int i = base + threadIdx.x;
         const bool typed = i < ds.N && ds.vt[i];
         const double r = typed ? c.bufN1[i] : 0.0;
@@ -384,9 +519,11 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_metro_accept(const DeviceDa
 __global__ __launch_bounds__(kSppThreads) void k_spp_polylog(const DeviceDataset ds, const ChainDev c)
 {
     __shared__ uint64_t ring[kRingWords];
+    __shared__ uint32_t s_adv[kJumpWords];
     __shared__ int s_cnt[kSppThreads / 64];
     const int m = *c.mdev;
     const uint32_t* J = ds.ring_jump;
+    ring_load_advance(J, s_adv);
     Xs128 st;
     st.seed(c.seed + (uint64_t)m, kSaltUpdateQ);
     ring_seed(st, J);
@@ -395,7 +532,7 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_polylog(const DeviceDataset
     __syncthreads();
     const int K = c.K;
     for (int base = 0; base < ds.N; base += kSppThreads) {
-        ring_follow(used, ws, st, ring, J);
+        ring_follow(used, ws, st, ring, s_adv);
         const int i = base + threadIdx.x;
         const bool typed = i < ds.N && ds.vt[i];
         int total;
