@@ -399,6 +399,9 @@ int vps_debug_decode(const vps_dataset* ds, int64_t l, int32_t i, int32_t* ploid
 int vps_debug_jump(uint64_t D, uint64_t* x, uint64_t* y);
 /* a chain's individual chunking of the genotype passes: G chunks of Nc individuals */
 int vps_debug_chunking(const vps_chain* ch, int32_t* G, int32_t* Nc);
+/* wavefronts per block of the multiallelic (GEN) Z-pass kernel for the chain's
+   chunking: 4, or 8 when LDS limits the resident blocks and G >= 8 */
+int vps_debug_gen_waves(const vps_chain* ch, int32_t* waves);
 /* one warm-up plus `iters` launches of a plain device copy of `bytes` bytes
    (scales hardware traffic counters); average ms per copy */
 int vps_debug_copy(int device, size_t bytes, int iters, double* ms_per_copy);

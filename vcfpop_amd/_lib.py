@@ -179,6 +179,7 @@ _SIGS = {
     "vps_debug_decode": ([_P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), _P], C.c_int),
     "vps_debug_jump": ([C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
     "vps_debug_chunking": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+    "vps_debug_gen_waves": ([_P, C.POINTER(C.c_int32)], C.c_int),
     "vps_debug_copy": ([C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
     "vps_synth_default": ([C.POINTER(VpsSynthConfig)], None),
     "vps_synth_create": ([C.POINTER(VpsSynthConfig), C.POINTER(_P)], C.c_int),
@@ -543,6 +544,12 @@ class Chain:
         g, nc = C.c_int32(), C.c_int32()
         _check(load().vps_debug_chunking(self.h, C.byref(g), C.byref(nc)))
         return g.value, nc.value
+
+    def gen_waves(self):
+        """wavefronts per block of the multiallelic Z-pass kernel (4 or 8)"""
+        w = C.c_int32()
+        _check(load().vps_debug_gen_waves(self.h, C.byref(w)))
+        return w.value
 
     def close(self):
         if getattr(self, "h", None):

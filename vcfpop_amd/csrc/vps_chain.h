@@ -159,6 +159,8 @@ int launch_record_accumulate(const DeviceDataset& ds, const ChainDev& c, int pos
 // m += 1 on the device (the sweep's last launch)
 int launch_advance_m(const ChainDev& c, hipStream_t st);
 int launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
+// wavefronts per block of the GEN Z kernel (4 or 8) for chunks of Nc
+int z_gen_block_waves(const DeviceDataset& ds, int K, int precision, int Nc, int* waves);
 // chunk count minimising (grid rounds x per-block work) at the kernel's occupancy
 int choose_chunks(const DeviceDataset& ds, int K, int precision, int device, int* G);
 int launch_init_admix(const DeviceDataset& ds, const ChainDev& c, int singlez, hipStream_t st);

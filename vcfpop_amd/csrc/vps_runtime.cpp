@@ -870,3 +870,12 @@ extern "C" int vps_debug_chunking(const vps_chain* ch, int32_t* G, int32_t* Nc)
     *Nc = ch->c.Nc;
     return VPS_OK;
 }
+
+extern "C" int vps_debug_gen_waves(const vps_chain* ch, int32_t* waves)
+{
+    if (!ch || !waves) return fail(VPS_ERR_INVALID, "null argument");
+    int w = 0;
+    const int rc = z_gen_block_waves(ch->ds->d, ch->K, ch->c.precision, ch->c.Nc, &w);
+    *waves = w;
+    return rc;
+}

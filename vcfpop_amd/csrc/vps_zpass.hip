@@ -951,6 +951,14 @@ int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_
     return rc;
 }
 
+// wavefronts per block of the GEN Z kernel for a chain's chunking
+int z_gen_block_waves(const DeviceDataset& ds, int K, int precision, int Nc, int* waves)
+{
+    int bb = 1, gb = 1;
+    return precision == VPS_PREC_MIXED ? z_occupancy_k<float>(ds, K, Nc, &bb, &gb, waves)
+                                       : z_occupancy_k<double>(ds, K, Nc, &bb, &gb, waves);
+}
+
 int64_t z_lik_parts(const DeviceDataset& ds, int G)
 {
     // BI items: 4 tiles x 1 chunk; GEN blocks: 1 tile x 1..4 chunks
