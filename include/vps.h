@@ -148,6 +148,19 @@ const char* vps_last_error(void);
 /* GetDeviceCountCUDA, src/structureCUDA.h:30 */
 int         vps_device_count(int* n);
 
+/* Plain device memory and device control, for callers that keep device
+   buffers of their own: MallocDeviceCUDA, FreeCUDA, MemsetCUDA, MemcpyCUDA
+   (src/structureCUDA.h:16-22), WaitCUDA (:8), ResetDeviceCUDA (:42),
+   ShowDevicesCUDA (:40).  Status codes instead of exit(). */
+int vps_device_alloc(int device, size_t bytes, void** out);
+int vps_device_free(void* p);
+int vps_device_memset(void* p, int value, size_t bytes);
+/* to_device = 1: host -> device, 0: device -> host (MemcpyCUDA's todevice) */
+int vps_device_memcpy(void* dst, const void* src, size_t bytes, int to_device);
+int vps_device_synchronize(int device);
+int vps_device_reset(int device);
+int vps_device_name(int device, char* buf, size_t n);
+
 /* ------------------------------------------------------------------------- */
 /* Dataset: device-resident genotype image shared by every chain on a device. */
 /* Replaces AllocMemoryCUDA + CopyStructureMemory (src/structureCUDA.h:24-32, */

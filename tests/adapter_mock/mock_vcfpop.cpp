@@ -1,0 +1,246 @@
+// mock_vcfpop.cpp (test fixture) — definitions of the vcfpop globals that
+// vcfpop.h (this folder) declares, and extern "C" hooks through which the
+// tests fill them from a genotype image, build a BAYESIAN<double> with host
+// buffers, and call the adapter's members.  Nothing here is part of the
+// engine.
+#include <algorithm>
+#include <stdarg.h>
+#include <stdexcept>
+#include <string.h>
+#include <string>
+#include <vector>
+#include "vcfpop.h"
+#include "../../include/vps.h"
+#include "../../vcfpop_amd/adapter/structureVPS.h"
+
+// the seam functions the adapter defines (declared by src/structureCUDA.h in vcfpop)
+extern "C" void CopyStructureMemory(int devID);
+extern "C" void FreeStructureMemory(int devID);
+extern "C" void CreateStreamCUDA(int devID);
+extern "C" void WaitCUDA();
+
+int g_seed_val;
+int structure_admix_val, structure_locpriori_val, structure_f_val;
+int structure_krange_min, structure_krange_max;
+int structure_nburnin_val, structure_nreps_val, structure_nthinning_val, structure_nruns_val, structure_nadmburnin_val;
+double structure_lambda_val, structure_stdlambda_val, structure_maxlambda_val;
+int structure_inferlambda_val, structure_difflambda_val, structure_diversity_val;
+double structure_alpha_val, structure_stdalpha_val, structure_maxalpha_val, structure_alphapriora_val,
+    structure_alphapriorb_val;
+int structure_inferalpha_val, structure_diffalpha_val, structure_uniformalpha_val, structure_metrofreq_val;
+double structure_r_val, structure_maxr_val, structure_stdr_val, structure_epseta_val, structure_epsgamma_val;
+double structure_pmeanf_val, structure_pstdf_val, structure_stdf_val;
+int structure_singlef_val, structure_writelnl_val;
+
+template <>
+IND<double>** ainds<double> = nullptr;
+template <>
+IND<float>** ainds<float> = nullptr;
+SLOCUS* slocus = nullptr;
+BUCKET geno_bucket = {nullptr, nullptr};
+int nind = 0, npop = 0;
+LOCN nloc = 0;
+int64 KT = 0;
+
+namespace {
+
+std::string g_exit;
+
+// the image the globals point into
+struct Image {
+    std::vector<uint8_t> bits;
+    std::vector<OFFSET> offsets;
+    std::vector<SLOCUS> loci;
+    std::vector<GENOTYPE> gts;
+    std::vector<ushort> pool;
+    std::vector<IND<double>> inds;
+    std::vector<IND<double>*> ind_ptrs;
+} g_img;
+
+}  // namespace
+
+void Exit(const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    throw std::runtime_error(buf);
+}
+
+extern "C" {
+
+const char* mock_last_exit() { return g_exit.c_str(); }
+
+// the vcfpop globals from a genotype image (vps_dataset_desc layout)
+int mock_set_image(const vps_dataset_desc* d)
+{
+    Image& im = g_img;
+    im = Image();
+    const int64 L = d->n_loc;
+    int64 bytes = 0;
+    for (int64 l = 0; l < L; ++l)
+        bytes = std::max<int64>(bytes, (int64)d->geno_byte_offset[l] + ((int64)d->bitsize[l] * d->n_ind + 7) / 8);
+    im.bits.assign(d->geno_bits, d->geno_bits + bytes);
+    im.bits.resize(bytes + 8, 0);
+    im.offsets.resize(L);
+    im.loci.resize(L);
+    const int64 G = d->gtab_offset[L];
+    im.gts.resize(G);
+    int64 npool = 0;
+    for (int64 g = 0; g < G; ++g) npool = std::max<int64>(npool, d->gt_allele_offset[g] + d->gt_ploidy[g]);
+    im.pool.assign(d->allele_pool, d->allele_pool + npool);
+    im.pool.push_back(0);
+    for (int64 g = 0; g < G; ++g) {
+        im.gts[g].ploidy = d->gt_ploidy[g];
+        im.gts[g].alleles = im.pool.data() + d->gt_allele_offset[g];
+    }
+    KT = 0;
+    for (int64 l = 0; l < L; ++l) {
+        im.offsets[l].offset = d->geno_byte_offset[l];
+        im.offsets[l].size = d->bitsize[l];
+        im.loci[l].k = (uint64)d->n_allele[l];
+        im.loci[l].ngeno = (uint)(d->gtab_offset[l + 1] - d->gtab_offset[l]);
+        im.loci[l].gtab = im.gts.data() + d->gtab_offset[l];
+        KT += d->n_allele[l];
+    }
+    im.inds.resize(d->n_ind);
+    im.ind_ptrs.resize(d->n_ind);
+    for (int i = 0; i < d->n_ind; ++i) {
+        im.inds[i].popid = d->ind_pop ? d->ind_pop[i] : 0;
+        im.inds[i].vt = 0;
+        im.ind_ptrs[i] = &im.inds[i];
+    }
+    geno_bucket.base_addr = im.bits.data();
+    geno_bucket.offset = im.offsets.data();
+    slocus = im.loci.data();
+    ainds<double> = im.ind_ptrs.data();
+    nind = d->n_ind;
+    nloc = L;
+    npop = d->n_pop > 0 ? d->n_pop : 1;
+    return 0;
+}
+
+// the option globals from the engine's parameter block
+void mock_set_options(const vps_params* p)
+{
+    structure_admix_val = p->admix_val; structure_locpriori_val = p->locpriori_val; structure_f_val = p->f_val;
+    structure_nburnin_val = p->nburnin; structure_nreps_val = p->nreps; structure_nthinning_val = p->nthinning;
+    structure_nruns_val = p->nruns; structure_nadmburnin_val = p->nadmburnin;
+    structure_lambda_val = p->lambda; structure_stdlambda_val = p->stdlambda; structure_maxlambda_val = p->maxlambda;
+    structure_inferlambda_val = p->inferlambda_val; structure_difflambda_val = p->difflambda_val;
+    structure_diversity_val = p->diversity_val;
+    structure_alpha_val = p->alpha; structure_stdalpha_val = p->stdalpha; structure_maxalpha_val = p->maxalpha;
+    structure_alphapriora_val = p->alphapriora; structure_alphapriorb_val = p->alphapriorb;
+    structure_inferalpha_val = p->inferalpha_val; structure_diffalpha_val = p->diffalpha_val;
+    structure_uniformalpha_val = p->uniformalpha_val; structure_metrofreq_val = p->metrofreq;
+    structure_r_val = p->r; structure_maxr_val = p->maxr; structure_stdr_val = p->stdr;
+    structure_epseta_val = p->epseta; structure_epsgamma_val = p->epsgamma;
+    structure_pmeanf_val = p->pmeanf; structure_pstdf_val = p->pstdf; structure_stdf_val = p->stdf;
+    structure_singlef_val = p->singlef_val; structure_writelnl_val = p->writelnl_val;
+    structure_krange_min = p->krange_min; structure_krange_max = p->krange_max;
+    g_seed_val = (int)p->g_seed;
+}
+
+// a run as ReadPar leaves it (src/structure.cpp:405-464), with zeroed
+// host buffers the tests fill
+struct MockRun {
+    BAYESIAN<double> b;
+    std::vector<double> Freq, FreqM, FreqA, Q, Alpha, Lambda, f, F, Eta, Gamma, Di, AlphaLocal, SumAlpha, bufNK1,
+        bufN1, lnL, rout;
+    std::vector<int64> MiSum, Mi;
+    std::vector<int> Ni;
+    std::vector<ushort> Z;
+};
+
+void* mock_run_new(int K, int id, int rlen, int nlnl)
+{
+    MockRun* r = new MockRun();
+    BAYESIAN<double>& b = r->b;
+    memset(&b, 0, sizeof(b));
+    const int N = nind, S = npop;
+    b.id = id; b.S = S; b.K = K; b.N = N; b.L = nloc;
+    b.admix = structure_admix_val == 1;
+    b.locpriori = structure_locpriori_val == 1;
+    b.fmodel = structure_f_val == 1;
+    b.nreps = structure_nreps_val; b.nthinning = structure_nthinning_val; b.nruns = structure_nruns_val;
+    b.nadmburnin = (!b.locpriori && !b.admix) ? structure_nadmburnin_val : 0;
+    b.nburnin = structure_nburnin_val + b.nadmburnin;
+    b.r = structure_r_val;
+    b.rlen = rlen;
+    r->Freq.assign((size_t)K * KT, 0); r->FreqM.assign((size_t)K * KT, 0); r->FreqA.assign(3 * (size_t)KT, 0);
+    r->Q.assign((size_t)N * K, 0); r->Alpha.assign(K, 0); r->Lambda.assign(K, 0); r->f.assign(K, 0); r->F.assign(K, 0);
+    r->Eta.assign(K, 0); r->Gamma.assign((size_t)S * K, 0); r->Di.assign((size_t)S * K, 0);
+    r->AlphaLocal.assign((size_t)S * K, 0); r->SumAlpha.assign(S, 0);
+    r->bufNK1.assign((size_t)N * K, 0); r->bufN1.assign(N, 0); r->lnL.assign(std::max(nlnl, 1), 0);
+    r->rout.assign(std::max(rlen, 1), 0);
+    r->MiSum.assign((size_t)N * K, 0); r->Mi.assign((size_t)N * K, 0); r->Ni.assign((size_t)K * KT, 0);
+    r->Z.assign(N, 0);
+    b.Freq = r->Freq.data(); b.FreqM = r->FreqM.data(); b.FreqA = r->FreqA.data(); b.Q = r->Q.data();
+    b.Alpha = r->Alpha.data(); b.Lambda = r->Lambda.data(); b.f = r->f.data(); b.F = r->F.data();
+    b.Eta = r->Eta.data(); b.Gamma = r->Gamma.data(); b.Di = r->Di.data(); b.AlphaLocal = r->AlphaLocal.data();
+    b.SumAlpha = r->SumAlpha.data(); b.bufNK1 = r->bufNK1.data(); b.bufN1 = r->bufN1.data(); b.lnL = r->lnL.data();
+    b.rout = r->rout.data(); b.MiSum = r->MiSum.data(); b.Mi = r->Mi.data(); b.Ni = r->Ni.data(); b.Z = r->Z.data();
+    return r;
+}
+
+void mock_run_free(void* p) { delete static_cast<MockRun*>(p); }
+
+// host buffer of a field, for the tests to read and write
+void* mock_run_field(void* p, const char* name)
+{
+    MockRun* r = static_cast<MockRun*>(p);
+    struct { const char* n; void* v; } tab[] = {
+        {"Freq", r->Freq.data()}, {"FreqM", r->FreqM.data()}, {"FreqA", r->FreqA.data()}, {"Q", r->Q.data()},
+        {"Alpha", r->Alpha.data()}, {"Lambda", r->Lambda.data()}, {"f", r->f.data()}, {"F", r->F.data()},
+        {"Eta", r->Eta.data()}, {"Gamma", r->Gamma.data()}, {"Di", r->Di.data()}, {"AlphaLocal", r->AlphaLocal.data()},
+        {"SumAlpha", r->SumAlpha.data()}, {"bufNK1", r->bufNK1.data()}, {"bufN1", r->bufN1.data()},
+        {"lnL", r->lnL.data()}, {"rout", r->rout.data()}, {"MiSum", r->MiSum.data()}, {"Mi", r->Mi.data()},
+        {"Ni", r->Ni.data()}, {"Z", r->Z.data()}, {"r", &r->b.r}, {"m", &r->b.m}, {"nr", &r->b.nr},
+        {"nlnL", &r->b.nlnL}, {"binaryq", &r->b.binaryq}, {"seed", &r->b.seed}};
+    for (auto& e : tab)
+        if (!strcmp(e.n, name)) return e.v;
+    return nullptr;
+}
+
+// one member of the run, or the whole-run hand-off; 0, or 1 with the Exit
+// message in mock_last_exit()
+int mock_run_call(void* p, const char* member)
+{
+    BAYESIAN<double>& b = static_cast<MockRun*>(p)->b;
+    try {
+        if (!strcmp(member, "InitCUDA")) b.InitCUDA();
+        else if (!strcmp(member, "UninitCUDA")) b.UninitCUDA();
+        else if (!strcmp(member, "UpdatePCUDA")) b.UpdatePCUDA();
+        else if (!strcmp(member, "UpdateQNoAdmixCUDA")) b.UpdateQNoAdmixCUDA();
+        else if (!strcmp(member, "UpdateQMetroCUDA")) b.UpdateQMetroCUDA();
+        else if (!strcmp(member, "UpdateZAdmixCUDA")) b.UpdateZAdmixCUDA();
+        else if (!strcmp(member, "UpdateZNoAdmixCUDA")) b.UpdateZNoAdmixCUDA();
+        else if (!strcmp(member, "RecordCUDA")) b.RecordCUDA();
+        else if (!strcmp(member, "MCMC")) VpsMCMC(b);
+        else throw std::runtime_error(std::string("unknown member ") + member);
+    } catch (const std::exception& e) {
+        g_exit = e.what();
+        return 1;
+    }
+    return 0;
+}
+
+// an extern "C" seam function through the fixture, catching Exit
+int mock_seam(const char* fn, int dev)
+{
+    try {
+        if (!strcmp(fn, "CopyStructureMemory")) CopyStructureMemory(dev);
+        else if (!strcmp(fn, "FreeStructureMemory")) FreeStructureMemory(dev);
+        else if (!strcmp(fn, "CreateStreamCUDA")) CreateStreamCUDA(dev);
+        else if (!strcmp(fn, "WaitCUDA")) WaitCUDA();
+        else throw std::runtime_error(std::string("unknown seam function ") + fn);
+    } catch (const std::exception& e) {
+        g_exit = e.what();
+        return 1;
+    }
+    return 0;
+}
+
+}  // extern "C"

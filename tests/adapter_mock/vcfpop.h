@@ -1,0 +1,136 @@
+/*
+ * vcfpop.h (test fixture) — the declarations of vcfpop that the adapter
+ * vcfpop_amd/adapter/structureVPS.cpp uses, and nothing else, so that the
+ * adapter compiles and runs here.  In vcfpop the adapter includes the real
+ * src/vcfpop.h; these declarations carry the same names and types:
+ *   - STRUCTURE_RUNINFO and the BAYESIAN<REAL> fields the five *CUDA members,
+ *     InitCUDA and UninitCUDA read or write (src/structure.h:24-306);
+ *   - the -structure option globals and g_seed_val (src/parameters.h:20,
+ *     209-257), the data-model globals nind, nloc, npop, KT, ainds, slocus and
+ *     geno_bucket (src/global.h:170-189, src/function.h:621-629), and the
+ *     accessors the adapter calls on a locus and a genotype
+ *     (src/function.h:53-209);
+ *   - Exit (vcfpop's fatal-error exit).
+ * The field layout of these structs is not vcfpop's: the adapter only names
+ * fields, it never depends on their offsets.
+ */
+#pragma once
+#include <atomic>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+typedef uint64_t uint64;
+typedef int64_t int64;
+typedef uint32_t uint;
+typedef unsigned short ushort;
+typedef unsigned char byte;
+typedef int64_t LOCN;
+
+/* fatal error: vcfpop prints and exits; the fixture throws to the test */
+[[noreturn]] void Exit(const char* fmt, ...);
+
+/* options (src/parameters.h) */
+extern int g_seed_val;
+extern int structure_admix_val, structure_locpriori_val, structure_f_val;
+extern int structure_krange_min, structure_krange_max;
+extern int structure_nburnin_val, structure_nreps_val, structure_nthinning_val, structure_nruns_val,
+    structure_nadmburnin_val;
+extern double structure_lambda_val, structure_stdlambda_val, structure_maxlambda_val;
+extern int structure_inferlambda_val, structure_difflambda_val, structure_diversity_val;
+extern double structure_alpha_val, structure_stdalpha_val, structure_maxalpha_val, structure_alphapriora_val,
+    structure_alphapriorb_val;
+extern int structure_inferalpha_val, structure_diffalpha_val, structure_uniformalpha_val, structure_metrofreq_val;
+extern double structure_r_val, structure_maxr_val, structure_stdr_val, structure_epseta_val, structure_epsgamma_val;
+extern double structure_pmeanf_val, structure_pstdf_val, structure_stdf_val;
+extern int structure_singlef_val, structure_writelnl_val;
+
+/* data model (src/function.h, src/global.h) */
+struct GENOTYPE {
+    ushort* alleles;    /* ploidy copies in stored order */
+    int ploidy;         /* 0 for the missing genotype */
+    int Nalleles() { return ploidy ? 1 : 0; }
+    int Ploidy() { return ploidy; }
+    ushort* GetAlleleArray() { return alleles; }
+};
+
+struct SLOCUS {
+    uint64 k;           /* number of alleles */
+    uint ngeno;         /* number of genotypes */
+    GENOTYPE* gtab;
+    GENOTYPE* GetGtab() { return gtab; }
+};
+
+struct OFFSET {
+    uint64 offset : 48; /* first byte of a locus's genotype ids */
+    uint64 size : 16;   /* bits per id */
+};
+
+struct BUCKET {
+    byte* base_addr;
+    OFFSET* offset;
+};
+
+template <typename REAL>
+struct IND {
+    int popid;
+    int64 vt;
+};
+
+template <typename REAL>
+extern IND<REAL>** ainds;
+extern SLOCUS* slocus;
+extern BUCKET geno_bucket;
+extern int nind, npop;
+extern LOCN nloc;
+extern int64 KT;
+#define GetLoc(x) (slocus[(x)])
+
+/* src/structure.h */
+struct STRUCTURE_RUNINFO {
+    double MeanlnL, VarlnL, lnPD;
+    int k, id, rep;
+};
+
+template <typename REAL>
+struct BAYESIAN {
+    int id, S, K, N;
+    int64 L;
+    bool admix, locpriori, fmodel;
+    int nburnin, nreps, nthinning, nruns, nadmburnin;
+    double r;
+    double* bufNK1;
+    double* bufN1;
+    REAL* Freq;
+    REAL* FreqM;
+    REAL* FreqA;
+    double* Lambda;
+    int64* MiSum;
+    ushort* Z;
+    int64* Mi;
+    REAL* Q;
+    int* Ni;
+    double* Alpha;
+    double* f;
+    double* F;
+    double* Eta;
+    double* Gamma;
+    REAL* Di;
+    double* AlphaLocal;
+    double* SumAlpha;
+    uint64 seed;
+    int m, nr, nlnL;
+    double* lnL;
+    double* rout;
+    int rlen;
+    bool usecuda, binaryq;
+
+    void InitCUDA();
+    void UninitCUDA();
+    void UpdatePCUDA();
+    void UpdateQNoAdmixCUDA();
+    void UpdateQMetroCUDA();
+    void UpdateZAdmixCUDA();
+    void UpdateZNoAdmixCUDA();
+    void RecordCUDA();
+};

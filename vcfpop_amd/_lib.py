@@ -146,6 +146,13 @@ _SIGS = {
     "vps_chain_set_state": ([_P, C.c_int, _P, C.c_size_t], C.c_int),
     "vps_chain_get_sizes": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "vps_record_lnl_noalloc": ([_P, C.c_int, _P, _P, _P, _P, _P], C.c_int),
+    "vps_device_alloc": ([C.c_int, C.c_size_t, C.POINTER(_P)], C.c_int),
+    "vps_device_free": ([_P], C.c_int),
+    "vps_device_memset": ([_P, C.c_int, C.c_size_t], C.c_int),
+    "vps_device_memcpy": ([_P, _P, C.c_size_t, C.c_int], C.c_int),
+    "vps_device_synchronize": ([C.c_int], C.c_int),
+    "vps_device_reset": ([C.c_int], C.c_int),
+    "vps_device_name": ([C.c_int, C.c_char_p, C.c_size_t], C.c_int),
     "vps_pass_update_z_admix": ([_P], C.c_int),
     "vps_pass_update_z_noadmix": ([_P], C.c_int),
     "vps_pass_update_q_noadmix": ([_P], C.c_int),
