@@ -440,6 +440,10 @@ def bench_precision(V, S=20):
         c.record()
         states[name] = {n: c.get(n) for n in ("Q", "Freq", "lnL", "Mi")}
         c.close()
+    # the per-sweep FP32 error: one sweep from identical FP64 state
+    from vcfpop_amd.precision import one_sweep_error
+    p1 = V.params_default(admix_val=1, locpriori_val=1, nburnin=0, nreps=10000, g_seed=1, rng_topology=V.RNG_REFERENCE)
+    res["one_sweep"] = one_sweep_error(ds, p1, spec["K"])
     a, b = states["fp64"], states["mixed"]
     res["after_sweeps"] = S + 2
     res["max_abs_dQ"] = float(np.abs(a["Q"] - b["Q"]).max())
