@@ -75,6 +75,9 @@ typedef struct vps_dataset_desc {
     const char* const* ind_name;       /* [N] optional (NULL): names for output tables     */
     const char* const* pop_name;       /* [S] optional (NULL)                               */
     const char* const* loc_name;       /* [L] optional (NULL)                               */
+    const char* const* pop_region;     /* [S] optional (NULL): the level-1 region of each
+                                          population (the ancestry table's RegL1 column,
+                                          src/structure.cpp:283); NULL = "Total"           */
 } vps_dataset_desc;
 
 /* ------------------------------------------------------------------------- */
@@ -269,8 +272,8 @@ int vps_record_lnl_noalloc(const vps_dataset* ds, int K, const double* freq, con
 /* Result tables (WriteStructure / WriteStructureSummary,                     */
 /* src/structure.cpp:88-401): the run file "<prefix>.structure.k=K_rep=R_id=I */
 /* .txt" (and ".lnl.txt" with writelnl) with the reference's sections and     */
-/* number format.  Regions are not part of the genotype image: the RegL1      */
-/* column reads "Total" (the reference's value for a population without one). */
+/* number format.  The RegL1 column reads the dataset's pop_region names, or  */
+/* "Total" (the reference's value for a population without a region).         */
 /* ------------------------------------------------------------------------- */
 typedef struct vps_output_options {
     int32_t     decimal;      /* digits after the point (-decimal, default 5)          */
@@ -305,8 +308,35 @@ int  vps_chain_write_tables(vps_chain* ch, const char* prefix, const vps_output_
 /* the same tables from host arrays (any dataset, including device = -1) */
 int  vps_write_tables_host(const vps_dataset* ds, const vps_params* par, int K, int run_id, const vps_run_state* st,
                            const char* prefix, const vps_output_options* opt, vps_run_summary* summary);
-/* WriteStructureSummary: one row per run (id, k, rep, mean lnL, var lnL, lnP(D)) */
-int  vps_write_summary(const char* path, const vps_run_summary* runs, int n, const vps_output_options* opt);
+/* The head of a result file (OpenResFile, src/file.cpp:269-287, and
+   WriteParameters, src/parameters.cpp:1408-1418):
+     ##Bayesian clustering calculated by vcfpop v1.08b
+     #  Input: <input>
+     #  Output: <output>
+     #  Time: YYYY-MM-DD HH:MM:SS        (local time)
+     #  Parameter: <argv[0]>
+     #    <argv[i]>                      for each later argument that starts
+                                         (case-insensitively) with -g_ (but not
+                                         -g_indtext or -g_indtab), with
+                                         -structure, or with -f_ when filters on
+   argv is vcfpop's argument list: the command line, or for a parameter file
+   (-p=) the file's blank-separated tokens with their quotes trimmed
+   (SetParameters, src/parameters.cpp:1345-1362). */
+#define VPS_VCFPOP_VERSION "1.08b"     /* VERSION, src/global.h:10 */
+typedef struct vps_result_header {
+    const char*        input;      /* -g_input value                          */
+    const char*        output;     /* -g_output value                         */
+    const char* const* argv;       /* [argc] vcfpop's arguments               */
+    int32_t            argc;
+    int32_t            filters;    /* 1 = -f given: -f_* arguments are listed */
+    int64_t            time;       /* seconds since the epoch; 0 = now        */
+} vps_result_header;
+
+/* WriteStructureSummary (src/structure.cpp:380-401): the header (hdr NULL:
+   the title line only), then one row per run (ID, k, rep, mean lnL, var lnL,
+   lnP(D)); like the reference, no line break after the last row. */
+int  vps_write_summary(const char* path, const vps_run_summary* runs, int n, const vps_output_options* opt,
+                       const vps_result_header* hdr);
 
 /* ------------------------------------------------------------------------- */
 /* Batched chains.  The reference runs K values x replicates as a task pool of */

@@ -27,6 +27,27 @@ typedef struct vps_image vps_image;
    allele identifiers outside 0..65534) return VPS_ERR_INVALID with a
    message naming the line. */
 int vps_image_read_polygene(const char* path, vps_image** out);
+/* Populations and regions from vcfpop's -g_indtext option, replacing those
+   of the file's population column (which vcfpop does not read).  NULL or an
+   empty text puts every individual in one population, "Total", as vcfpop
+   does without -g_indtext.  The text is blank-separated tokens: before the
+   first "#REG", each "name:m1,m2,..." is a population of individuals named,
+   given as "#i" (1-based; a name is tried first) or as a range "#i-#j";
+   after each "#REG", "name:m1,..." is a region of the level below (its
+   populations, or regions), named or given as "#i" / "#i-#j" (1-based).
+   Individuals not listed form "DefPop"; populations not in a region form
+   "DefRegL1".  As vcfpop does after loading (AssignInds, src/load.cpp:88-160;
+   AssignVInds and SetVReg, src/filter.cpp:166-398): empty populations are
+   dropped, populations are ordered by a walk of the region tree, and the
+   individuals move to population order (their order within a population
+   kept; the genotype tables are unchanged, the packed ids move with them).
+   With a single population left it is named "Total".  Each population's
+   level-1 region becomes its pop_region ("Total" when the populations fall
+   in fewer than two level-1 regions).  A duplicate name, an individual or
+   population listed twice, an unknown name or an index out of range is
+   VPS_ERR_INVALID with a message naming it.  The image's descriptor must be
+   fetched again afterwards. */
+int vps_image_assign_indtext(vps_image* im, const char* indtext);
 /* a view of the image's arrays and names, valid while the image lives */
 int vps_image_desc(const vps_image* im, vps_dataset_desc* desc);
 /* loci in the file, loci kept (the others have fewer than two alleles),

@@ -118,3 +118,89 @@ def test_reader_errors(vps, tmp_path, body, msg):
     p.write_text("ID\tPop\tPloidy\tA\n" + body)
     with pytest.raises(vps.VpsError, match=msg):
         vps.Image(p)
+
+
+# --- populations and regions from -g_indtext (vcfpop_amd/csrc/vps_ingest.cpp,
+# vcfpop's Initialize src/function.cpp:4352-4500, AssignInds
+# src/load.cpp:88-160, AssignVInds/SetVReg src/filter.cpp:166-398)
+
+SIX = ("ID\tPop\tPloidy\tA\tB\n"
+       "a1\tx\t2\t1,2\t1,1\n"
+       "a2\tx\t2\t2,2\t1,3\n"
+       "a3\tx\t2\t1,1\t3,3\n"
+       "a4\ty\t2\t3,3\t1,3\n"
+       "a5\ty\t2\t1,3\t\n"
+       "a6\ty\t2\t2,3\t3,3\n")
+
+
+def six(vps, tmp_path):
+    p = tmp_path / "six.txt"
+    p.write_text(SIX)
+    return vps.Image(str(p))
+
+
+def test_indtext_default_is_one_population(vps, tmp_path):
+    im = six(vps, tmp_path)
+    assert im.names("pop") == ["x", "y"]      # the population column, until -g_indtext is applied
+    im.assign_indtext(None)
+    assert im.desc.n_pop == 1 and im.names("pop") == ["Total"] and im.names("reg") == ["Total"]
+    assert list(im.arrays()["ind_pop"]) == [0] * 6
+    im.assign_indtext("")
+    assert im.names("pop") == ["Total"]
+
+
+def test_indtext_orders_individuals_by_region_walk(vps, tmp_path):
+    im = six(vps, tmp_path)
+    before = im.arrays()
+    ids0 = [unpack_ids(before, l, 6) for l in range(2)]
+    # pB holds a2 (by name) and a5..a6 (a range), pA holds #1 and #4;
+    # a3 is in no population (DefPop).  Regions in parse order: r2 = {pB}
+    # then r1 = {pA}; DefPop falls in DefRegL1, the first region walked.
+    im.assign_indtext("pA:#1,#4\npB:a2,#5-#6 #REG r2:#2 r1:pA")
+    assert im.names("pop") == ["DefPop", "pB", "pA"]
+    assert im.names("reg") == ["DefRegL1", "r2", "r1"]
+    assert im.names("ind") == ["a3", "a2", "a5", "a6", "a1", "a4"]
+    a = im.arrays()
+    assert list(a["ind_pop"]) == [0, 1, 1, 1, 2, 2]
+    order = [2, 1, 4, 5, 0, 3]
+    for l in range(2):    # genotype tables unchanged, ids moved with the individuals
+        assert unpack_ids(a, l, 6) == [ids0[l][i] for i in order]
+        assert table(a, l) == table(before, l)
+
+
+def test_indtext_single_region_reads_total(vps, tmp_path):
+    im = six(vps, tmp_path)
+    im.assign_indtext("p1:#1-#3 p2:#4-#6 #REG only:#1-#2")
+    assert im.names("pop") == ["p1", "p2"] and im.names("reg") == ["Total", "Total"]
+    im.assign_indtext("p1:#1-#3 p2:#4-#6")
+    assert im.names("reg") == ["Total", "Total"]
+    assert im.names("ind") == ["a1", "a2", "a3", "a4", "a5", "a6"]
+
+
+def test_indtext_two_region_levels(vps, tmp_path):
+    im = six(vps, tmp_path)
+    # level 1: ra = {p3}, rb = {p1, p2}; level 2: big = {rb, ra} in that
+    # order by index range, so the walk visits ra (index 1) before rb
+    im.assign_indtext("p1:#1-#2 p2:#3-#4 p3:#5-#6 #REG ra:p3 rb:#1-#2 #REG big:#1-#2 other:")
+    assert im.names("pop") == ["p3", "p1", "p2"]
+    assert im.names("reg") == ["ra", "rb", "rb"]
+    assert im.names("ind") == ["a5", "a6", "a1", "a2", "a3", "a4"]
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("p:#1-#3 p:#4", "same name"),
+    ("p:#1-#3 q:#3-#4", "same individual"),
+    ("p:#1-#7", "exceeds range"),
+    ("p:zz", "not in the genotype data"),
+    ("p:#9", "cannot find individual"),
+    ("p:#1 q:#2 #REG r:#1-#3", "exceeds range"),
+    ("p:#1 q:#2 #REG r:#1 s:#1", "the same member"),
+    ("p:#1 q:#2 #REG r:#1,p", "twice"),
+    ("p:#1 q:#2 #REG p:#2", "repeat population/region name"),
+    ("p:#1 q:#2 #REG r:nope", "cannot find nope"),
+    ("p:#1 q", "has no ':'"),
+])
+def test_indtext_errors(vps, tmp_path, text, msg):
+    im = six(vps, tmp_path)
+    with pytest.raises(vps.VpsError, match=msg):
+        im.assign_indtext(text)

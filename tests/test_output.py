@@ -99,5 +99,57 @@ def test_options_and_summary(vps, orc, tmp_path):
     assert f"Mean value of ln likelihood={'%0.3f' % st['rout'][0]}" in t2
     vps.write_summary(tmp_path / "sum.txt", [summ, out])
     rows = open(tmp_path / "sum.txt", newline="").read().split("\r\n")
-    assert rows[2] == "ID\tk\trep\tMean lnL\tVar lnL\tlnP(D)"
-    assert rows[3].split("\t")[:3] == ["7", "3", "2"]
+    assert rows[:3] == ["##Bayesian clustering calculated by vcfpop v1.08b", "", ""]
+    assert rows[3] == "ID\tk\trep\tMean lnL\tVar lnL\tlnP(D)"
+    assert rows[4].split("\t")[:3] == ["7", "3", "2"]
+    assert len(rows) == 6 and rows[5].split("\t")[0] == "7"    # no line break after the last row
+
+
+def test_summary_head(vps, tmp_path):
+    """OpenResFile (src/file.cpp:269-287) and WriteParameters
+    (src/parameters.cpp:1408-1418): title, Input, Output, local Time,
+    Parameter = argv[0], then the later -g_* (not -g_indtext / -g_indtab),
+    -structure* and, with -f, -f_* arguments"""
+    import time as _time
+    t = 1767225600 + 3 * 3600 + 25 * 60 + 7
+    argv = ["-p=run.txt", "-g_input=a.txt", "-G_seed=3", "-g_indtext=p:#1-#2", "-g_indtab=x", "-structure",
+            "-STRUCTURE_k=2", "-structures", "-f_qual=[1,2]", "-fst", "-gx", "-g_"]
+    runs = [dict(id=1, k=2, rep=1, mean_lnl=-1.5, var_lnl=2.0, lnpd=-3.25)]
+    lt = _time.localtime(t)
+    stamp = "%04d-%02d-%02d %02d:%02d:%02d" % lt[:6]
+    common = ["##Bayesian clustering calculated by vcfpop v1.08b", "#  Input: a.txt", "#  Output: out/x",
+              "#  Time: " + stamp, "#  Parameter: -p=run.txt", "#    -g_input=a.txt", "#    -G_seed=3",
+              "#    -structure", "#    -STRUCTURE_k=2", "#    -structures"]
+    tail = ["#    -g_", "", "", "ID\tk\trep\tMean lnL\tVar lnL\tlnP(D)", "1\t2\t1\t-1.500\t2.000\t-3.250"]
+    for filters, extra in ((False, []), (True, ["#    -f_qual=[1,2]"])):
+        hdr = dict(input="a.txt", output="out/x", argv=argv, filters=filters, time=t)
+        vps.write_summary(tmp_path / "s.txt", runs, header=hdr, decimal=3, linebreak="\n")
+        rows = open(tmp_path / "s.txt", newline="").read().split("\n")
+        assert rows == common + extra + tail
+
+
+def test_ancestry_regions_from_indtext(vps, orc, tmp_path):
+    """the RegL1 column (src/structure.cpp:283) carries each population's
+    level-1 region from -g_indtext"""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "data.polygene.txt")
+    im = vps.Image(path)
+    im.assign_indtext("w:#1-#40 x:#41-#60 y:#61-#100 z:#101-#120 #REG north:x,z south:w,y")
+    assert im.names("pop") == ["x", "z", "w", "y"] and im.names("reg") == ["north", "north", "south", "south"]
+    ds = vps.Dataset(im, -1)
+    p = vps.params_default(admix_val=1, nburnin=2, nreps=4, nthinning=2, g_seed=4)
+    o = orc.Chain(orc.Data(im.desc), p, 2, 1)
+    o.sweep(6)
+    o.finish()
+    st = dict(Q=o.get("Q"), ancestry=o.get("MiSum"), FreqM=o.get("FreqM"), rout=o.get("rout"), seed=o.seed())
+    vps.write_tables_host(ds, p, 2, 1, st, str(tmp_path / "r"), rep=1)
+    lines = open(str(tmp_path / "r") + ".structure.k=2_rep=1_id=1.txt", newline="").read().split("\r\n")
+    anc = section(lines, "Inferred ancestry of individuals")
+    assert anc[1] == "Ind\tPop\tRegL1\t1\t2"
+    want = ([(f"Ind{i}", "x", "north") for i in range(41, 61)] + [(f"Ind{i}", "z", "north") for i in range(101, 121)]
+            + [(f"Ind{i}", "w", "south") for i in range(1, 41)] + [(f"Ind{i}", "y", "south") for i in range(61, 101)])
+    assert [tuple(r.split("\t")[:3]) for r in anc[2:]] == want
+    mem = section(lines, "Proportion of membership of each pre-defined population in each of the 2 clusters")
+    assert [r.split("\t")[0] for r in mem[2:]] == ["x", "z", "w", "y"]
+    ds.close()
+    im.close()

@@ -37,6 +37,7 @@ class VpsDatasetDesc(C.Structure):
         ("ind_name", C.POINTER(C.c_char_p)),
         ("pop_name", C.POINTER(C.c_char_p)),
         ("loc_name", C.POINTER(C.c_char_p)),
+        ("pop_region", C.POINTER(C.c_char_p)),
     ]
 
 
@@ -63,6 +64,11 @@ class VpsParams(C.Structure):
         ("z_chunk", C.c_int32),
         ("sweep_graphs", C.c_int32),
     ]
+
+
+class VpsResultHeader(C.Structure):
+    _fields_ = [("input", C.c_char_p), ("output", C.c_char_p), ("argv", C.POINTER(C.c_char_p)), ("argc", C.c_int32),
+                ("filters", C.c_int32), ("time", C.c_int64)]
 
 
 class VpsOutputOptions(C.Structure):
@@ -177,7 +183,8 @@ _SIGS = {
     "vps_chain_write_tables": ([_P, C.c_char_p, C.POINTER(VpsOutputOptions), C.POINTER(VpsRunSummary)], C.c_int),
     "vps_write_tables_host": ([_P, C.POINTER(VpsParams), C.c_int, C.c_int, C.POINTER(VpsRunState), C.c_char_p,
                                C.POINTER(VpsOutputOptions), C.POINTER(VpsRunSummary)], C.c_int),
-    "vps_write_summary": ([C.c_char_p, C.POINTER(VpsRunSummary), C.c_int, C.POINTER(VpsOutputOptions)], C.c_int),
+    "vps_write_summary": ([C.c_char_p, C.POINTER(VpsRunSummary), C.c_int, C.POINTER(VpsOutputOptions),
+                           C.POINTER(VpsResultHeader)], C.c_int),
     "vps_batch_create": ([C.POINTER(_P), C.c_int, C.POINTER(_P)], C.c_int),
     "vps_batch_step": ([_P, C.c_int], C.c_int),
     "vps_batch_sync": ([_P], C.c_int),
@@ -196,6 +203,7 @@ _SIGS = {
     "vps_synth_destroy": ([_P], C.c_int),
     "vps_image_read_polygene": ([C.c_char_p, C.POINTER(C.c_void_p)], C.c_int),
     "vps_image_desc": ([_P, C.POINTER(VpsDatasetDesc)], C.c_int),
+    "vps_image_assign_indtext": ([_P, C.c_char_p], C.c_int),
     "vps_image_sizes": ([_P] + [C.POINTER(C.c_int64)] * 5, C.c_int),
     "vps_image_destroy": ([_P], C.c_int),
 }
@@ -356,10 +364,20 @@ class Image(_PackedArrays):
     def L(self):
         return self.desc.n_loc
 
+    def assign_indtext(self, indtext):
+        """populations and regions from vcfpop's -g_indtext (None: one
+        population, "Total"); individuals move to population order"""
+        lib = load()
+        _check(lib.vps_image_assign_indtext(self.h, indtext.encode() if indtext else None))
+        _check(lib.vps_image_desc(self.h, C.byref(self.desc)))
+
     def names(self, kind):
-        """'ind', 'pop' or 'loc' names"""
+        """'ind', 'pop', 'loc' or 'reg' (each population's level-1 region) names"""
         d = self.desc
-        arr, n = {"ind": (d.ind_name, d.n_ind), "pop": (d.pop_name, d.n_pop), "loc": (d.loc_name, d.n_loc)}[kind]
+        if kind == "reg" and not d.pop_region:
+            return ["Total"] * d.n_pop
+        arr, n = {"ind": (d.ind_name, d.n_ind), "pop": (d.pop_name, d.n_pop), "loc": (d.loc_name, d.n_loc),
+                  "reg": (d.pop_region, d.n_pop)}[kind]
         return [arr[i].decode() for i in range(n)]
 
     def arrays(self):
@@ -715,10 +733,19 @@ def write_tables_host(ds, params, K, run_id, state, prefix, **opts):
     return _summary_dict(sm)
 
 
-def write_summary(path, summaries, **opts):
-    """WriteStructureSummary over the runs' summary dicts"""
+def write_summary(path, summaries, header=None, **opts):
+    """WriteStructureSummary over the runs' summary dicts; header = dict(input,
+    output, argv (list of str), filters (bool), time (epoch seconds, 0 = now))
+    for the result-file head, or None for the title line only"""
     arr = (VpsRunSummary * max(1, len(summaries)))()
     for i, d in enumerate(summaries):
         arr[i] = VpsRunSummary(d["id"], d["k"], d["rep"], 0, d["mean_lnl"], d["var_lnl"], d["lnpd"])
     o = output_options(**opts)
-    _check(load().vps_write_summary(str(path).encode(), arr, len(summaries), C.byref(o)))
+    h = None
+    if header is not None:
+        argv = [a.encode() for a in header.get("argv", [])]
+        av = (C.c_char_p * max(1, len(argv)))(*argv)
+        h = VpsResultHeader(header.get("input", "").encode(), header.get("output", "").encode(), av, len(argv),
+                            1 if header.get("filters") else 0, int(header.get("time", 0)))
+    _check(load().vps_write_summary(str(path).encode(), arr, len(summaries), C.byref(o),
+                                    C.byref(h) if h is not None else None))

@@ -358,6 +358,7 @@ extern "C" int vps_dataset_create(const vps_dataset_desc* in, int device, vps_da
 
     if (in->ind_name) for (int i = 0; i < N; ++i) ds->ind_names.emplace_back(in->ind_name[i] ? in->ind_name[i] : "");
     if (in->pop_name) for (int s = 0; s < S; ++s) ds->pop_names.emplace_back(in->pop_name[s] ? in->pop_name[s] : "");
+    if (in->pop_region) for (int s = 0; s < S; ++s) ds->pop_regions.emplace_back(in->pop_region[s] ? in->pop_region[s] : "");
     if (in->loc_name) for (int64_t l = 0; l < L; ++l) ds->loc_names.emplace_back(in->loc_name[ds->orig[l]] ? in->loc_name[ds->orig[l]] : "");
 
     if (host_only) {

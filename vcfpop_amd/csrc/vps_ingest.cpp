@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <unordered_map>
@@ -40,8 +41,8 @@ struct vps_image {
     std::vector<int64_t> gtaoff;
     std::vector<uint16_t> pool;
     std::vector<int32_t> pop;
-    std::vector<std::string> ind_names, pop_names, loc_names;
-    std::vector<const char*> ind_ptr, pop_ptr, loc_ptr;
+    std::vector<std::string> ind_names, pop_names, loc_names, reg_names;
+    std::vector<const char*> ind_ptr, pop_ptr, loc_ptr, reg_ptr;
 };
 
 using namespace vps;
@@ -263,6 +264,260 @@ extern "C" int vps_image_read_polygene(const char* path, vps_image** out)
     return VPS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Populations and regions from -g_indtext (vcfpop's Initialize,
+// src/function.cpp:4352-4500; AssignInds, src/load.cpp:88-160; AssignVInds
+// and SetVReg, src/filter.cpp:166-398).  Each level's entry 0 is its default
+// (DefPop, DefRegL1, DefRegL2, ...).  The region walk keeps each unit's
+// parent as parsed and numbers the kept units separately; vcfpop renumbers
+// the parent field in place, which gives the same order wherever no renumbered
+// value meets a later unit's parsed index (always with one region level).
+
+namespace {
+
+struct Unit {
+    std::string name;
+    std::vector<std::string> members;
+    int parent = 0;      // index of the enclosing region one level up
+    int64_t nind = 0;
+    int npop = 0;
+};
+
+// ParseTwoNumber (src/string2.cpp:189-194)
+void two_numbers(const std::string& s, int& a, int& b)
+{
+    a = b = -1;
+    if (!s.empty() && s[0] == '#') sscanf(s.c_str(), "#%d-#%d", &a, &b);
+}
+
+bool starts_lower(const std::string& s, const char* prefix)
+{
+    size_t i = 0;
+    for (; prefix[i]; ++i) {
+        if (i >= s.size()) return false;
+        char c = s[i];
+        if (c >= 'A' && c <= 'Z') c = (char)(c + 32);
+        if (c != prefix[i]) return false;
+    }
+    return true;
+}
+
+struct Assignment {
+    std::vector<int32_t> order;            // new position -> old individual
+    std::vector<int32_t> pop;              // new position -> population
+    std::vector<std::string> pop_names, reg_names;
+};
+
+int assign(const std::vector<std::string>& ind_names, const char* indtext, Assignment* out)
+{
+    const int N = (int)ind_names.size();
+    std::vector<Unit> pops(1);
+    pops[0].name = "DefPop";
+    std::vector<std::vector<Unit>> regs(1, std::vector<Unit>(1));
+    regs[0][0].name = "DefRegL1";
+    int rl = -1;
+    std::vector<std::string> toks;
+    if (indtext) {
+        std::string t;
+        for (const char* c = indtext;; ++c) {
+            if (!*c || *c == ' ' || *c == '\t' || *c == '\r' || *c == '\n') {
+                if (!t.empty()) toks.push_back(t);
+                t.clear();
+                if (!*c) break;
+            } else {
+                t.push_back(*c);
+            }
+        }
+    }
+    for (const std::string& tok : toks) {
+        if (starts_lower(tok, "#reg")) {
+            ++rl;
+            regs.emplace_back(1);
+            regs.back()[0].name = "DefRegL" + std::to_string(rl + 2);
+            continue;
+        }
+        const size_t colon = tok.find(':');
+        if (colon == std::string::npos) return fail(VPS_ERR_INVALID, "-g_indtext: '%s' has no ':'", tok.c_str());
+        Unit u;
+        u.name = tok.substr(0, colon);
+        for (const std::string& m : split(tok.substr(colon + 1), ','))
+            if (!m.empty()) u.members.push_back(m);
+        if (rl == -1) {
+            for (const Unit& p : pops)
+                if (p.name == u.name) return fail(VPS_ERR_INVALID, "-g_indtext: two populations have the same name: %s", u.name.c_str());
+            pops.push_back(u);
+            continue;
+        }
+        for (int r = 0; r <= rl; ++r)
+            for (const Unit& x : regs[r])
+                if (x.name == u.name) return fail(VPS_ERR_INVALID, "-g_indtext: repeat population/region name: %s", u.name.c_str());
+        for (const Unit& p : pops)
+            if (p.name == u.name) return fail(VPS_ERR_INVALID, "-g_indtext: repeat population/region name: %s", u.name.c_str());
+        std::vector<Unit>& below = rl == 0 ? pops : regs[rl - 1];
+        const int id = (int)regs[rl].size();
+        auto take = [&](int i) {
+            if (below[i].parent == id)
+                return fail(VPS_ERR_INVALID, "-g_indtext: %s appears twice in region %s", below[i].name.c_str(), u.name.c_str());
+            if (below[i].parent != 0)
+                return fail(VPS_ERR_INVALID, "-g_indtext: regions %s and %s have the same member %s",
+                            regs[rl][below[i].parent].name.c_str(), u.name.c_str(), below[i].name.c_str());
+            below[i].parent = id;
+            return (int)VPS_OK;
+        };
+        for (const std::string& m : u.members) {
+            int v1, v2;
+            two_numbers(m, v1, v2);
+            if (v1 != -1 && v2 != -1) {
+                if (v1 <= 0 || v1 >= (int)below.size() || v2 <= 0 || v2 >= (int)below.size())
+                    return fail(VPS_ERR_INVALID, "-g_indtext: %s of region %s exceeds range", m.c_str(), u.name.c_str());
+                for (int i = std::min(v1, v2); i <= std::max(v1, v2); ++i)
+                    if (int e = take(i)) return e;
+            } else {
+                int found = -1;
+                for (int i = 0; i < (int)below.size() && found < 0; ++i)
+                    if (v1 == i || below[i].name == m) found = i;
+                if (found < 0) return fail(VPS_ERR_INVALID, "-g_indtext: cannot find %s of region %s", m.c_str(), u.name.c_str());
+                if (int e = take(found)) return e;
+            }
+        }
+        regs[rl].push_back(u);
+    }
+
+    // individuals (AssignInds)
+    std::unordered_map<std::string, int> name2id;
+    for (int i = 0; i < N; ++i)
+        if (!name2id.emplace(ind_names[i], i).second)
+            return fail(VPS_ERR_INVALID, "-g_indtext: individuals #%d and #%d have the same name %s", name2id[ind_names[i]] + 1,
+                        i + 1, ind_names[i].c_str());
+    std::vector<int> popid(N, 0);
+    for (int pp = (int)pops.size() - 1; pp >= 0; --pp) {
+        auto put = [&](int i) {
+            if (popid[i] != 0)
+                return fail(VPS_ERR_INVALID, "-g_indtext: populations %s and %s have the same individual %s",
+                            pops[popid[i]].name.c_str(), pops[pp].name.c_str(), ind_names[i].c_str());
+            popid[i] = pp;
+            return (int)VPS_OK;
+        };
+        for (const std::string& m : pops[pp].members) {
+            int v1, v2;
+            two_numbers(m, v1, v2);
+            if (v1 == -1 && v2 == -1) {
+                auto it = name2id.find(m);
+                if (it == name2id.end())
+                    return fail(VPS_ERR_INVALID, "-g_indtext: individual %s of population %s is not in the genotype data",
+                                m.c_str(), pops[pp].name.c_str());
+                if (int e = put(it->second)) return e;
+            } else if (v2 == -1) {
+                auto it = name2id.find(m);
+                const int i = it != name2id.end() ? it->second : v1 - 1;
+                if (i < 0 || i >= N)
+                    return fail(VPS_ERR_INVALID, "-g_indtext: cannot find individual %s of population %s", m.c_str(),
+                                pops[pp].name.c_str());
+                if (int e = put(i)) return e;
+            } else {
+                if (v1 <= 0 || v1 > N || v2 <= 0 || v2 > N)
+                    return fail(VPS_ERR_INVALID, "-g_indtext: %s of population %s exceeds range", m.c_str(), pops[pp].name.c_str());
+                for (int i = std::min(v1, v2) - 1; i <= std::max(v1, v2) - 1; ++i)
+                    if (int e = put(i)) return e;
+            }
+        }
+    }
+
+    // counts up the region tree; top levels holding a single region dropped
+    for (int i = 0; i < N; ++i) pops[popid[i]].nind++;
+    const int levels = (int)regs.size();
+    std::vector<int> nreg(levels, 0);
+    for (const Unit& p : pops)
+        if (p.nind > 0) { regs[0][p.parent].npop++; regs[0][p.parent].nind += p.nind; }
+    for (int r = 0; r < levels; ++r)
+        for (const Unit& x : regs[r])
+            if (x.npop > 0) {
+                nreg[r]++;
+                if (r + 1 < levels) { regs[r + 1][x.parent].npop++; regs[r + 1][x.parent].nind += x.nind; }
+            }
+    int lreg = levels;
+    while (lreg && nreg[lreg - 1] == 1) {
+        --lreg;
+        if (lreg == 0) break;
+    }
+    int npop = 0;
+    for (const Unit& p : pops) npop += p.nind > 0;
+
+    // population order: the walk of SetVReg from the single top region
+    std::vector<int> apops;
+    if (lreg == 0 && npop == 1) {
+        lreg = -1;
+        for (int j = 0; j < (int)pops.size(); ++j)
+            if (pops[j].nind > 0) apops.push_back(j);
+    } else {
+        int top = -1;
+        for (int i = 0; i < (int)regs[lreg].size(); ++i)
+            if (regs[lreg][i].nind > 0) top = i;
+        std::vector<int> kept_l0;     // level-0 region of each population in apops
+        std::function<void(int, int)> walk = [&](int r, int i) {
+            const Unit& x = regs[r][i];
+            if (x.npop == 0 || x.nind == 0) return;
+            if (r > 0) {
+                for (int j = 0; j < (int)regs[r - 1].size(); ++j)
+                    if (regs[r - 1][j].parent == i) walk(r - 1, j);
+            } else {
+                for (int j = 0; j < (int)pops.size(); ++j)
+                    if (pops[j].parent == i && pops[j].nind > 0) { apops.push_back(j); kept_l0.push_back(i); }
+            }
+        };
+        walk(lreg, top);
+        for (size_t a = 0; a < apops.size(); ++a)
+            out->reg_names.push_back(lreg >= 1 ? regs[0][kept_l0[a]].name : std::string("Total"));
+    }
+    if ((int)apops.size() != npop) return fail(VPS_ERR_INVALID, "-g_indtext: a population outside the region tree");
+    std::vector<int> newpop(pops.size(), -1);
+    for (int a = 0; a < npop; ++a) {
+        newpop[apops[a]] = a;
+        out->pop_names.push_back(lreg == -1 ? std::string("Total") : pops[apops[a]].name);
+    }
+    if (lreg == -1) out->reg_names.assign(1, "Total");
+    for (int a = 0; a < npop; ++a)
+        for (int i = 0; i < N; ++i)
+            if (popid[i] == apops[a]) { out->order.push_back(i); out->pop.push_back(a); }
+    return VPS_OK;
+}
+
+}  // namespace
+
+extern "C" int vps_image_assign_indtext(vps_image* im, const char* indtext)
+{
+    if (!im) return fail(VPS_ERR_INVALID, "null image");
+    Assignment as;
+    if (int e = assign(im->ind_names, indtext, &as)) return e;
+    const int64_t N = im->N;
+    // the packed ids of each locus in the new individual order
+    std::vector<uint8_t> bits(im->bits.size(), 0);
+    for (int64_t l = 0; l < im->L; ++l) {
+        const uint64_t base = im->byteoff[l] * 8;
+        const int b = im->bitsize[l];
+        for (int64_t i = 0; i < N; ++i) {
+            const uint64_t src = base + (uint64_t)as.order[i] * b, dst = base + (uint64_t)i * b;
+            for (int x = 0; x < b; ++x)
+                if (im->bits[(src + x) >> 3] >> ((src + x) & 7) & 1) bits[(dst + x) >> 3] |= (uint8_t)(1u << ((dst + x) & 7));
+        }
+    }
+    std::vector<std::string> names(N);
+    for (int64_t i = 0; i < N; ++i) names[i] = im->ind_names[as.order[i]];
+    im->bits.swap(bits);
+    im->ind_names.swap(names);
+    im->pop = as.pop;
+    im->pop_names = as.pop_names;
+    im->reg_names = as.reg_names;
+    im->S = (int32_t)im->pop_names.size();
+    im->ind_ptr.clear();
+    im->pop_ptr.clear();
+    im->reg_ptr.clear();
+    for (auto& x : im->ind_names) im->ind_ptr.push_back(x.c_str());
+    for (auto& x : im->pop_names) im->pop_ptr.push_back(x.c_str());
+    for (auto& x : im->reg_names) im->reg_ptr.push_back(x.c_str());
+    return VPS_OK;
+}
+
 extern "C" int vps_image_desc(const vps_image* im, vps_dataset_desc* d)
 {
     if (!im || !d) return fail(VPS_ERR_INVALID, "null argument");
@@ -282,6 +537,7 @@ extern "C" int vps_image_desc(const vps_image* im, vps_dataset_desc* d)
     d->ind_name = im->ind_ptr.data();
     d->pop_name = im->pop_ptr.data();
     d->loc_name = im->loc_ptr.data();
+    d->pop_region = im->reg_ptr.empty() ? nullptr : im->reg_ptr.data();
     return VPS_OK;
 }
 

@@ -120,6 +120,7 @@ struct vps_dataset {
     std::vector<int64_t> lpos;         // sorted locus -> its entry in tloc
     std::vector<uint8_t> bitsize;
     std::vector<std::string> ind_names, pop_names, loc_names;
+    std::vector<std::string> pop_regions;   // RegL1 name of each population (empty = "Total")
     std::vector<int64_t> kdis;
     int64_t device_bytes = 0;
     std::vector<void*> allocations;

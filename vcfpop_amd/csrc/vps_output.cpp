@@ -11,6 +11,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
+#include <time.h>
 #include <string>
 #include <vector>
 #include "vps_runtime.h"
@@ -176,7 +177,8 @@ extern "C" int vps_write_tables_host(const vps_dataset* ds, const vps_params* pa
     for (int i = 0; i < N; ++i) {
         const int s = ds->d.S > 0 ? ds->popid[i] : 0;
         w.br(); w.s(name_or(ds->ind_names, i, "Ind").c_str()); w.d();
-        w.s(name_or(ds->pop_names, s, "Pop").c_str()); w.d(); w.s("Total");
+        w.s(name_or(ds->pop_names, s, "Pop").c_str()); w.d();
+        w.s((size_t)s < ds->pop_regions.size() && !ds->pop_regions[s].empty() ? ds->pop_regions[s].c_str() : "Total");
         for (int k = 0; k < K; ++k) { w.d(); w.real(st->ancestry[(size_t)i * K + k]); }
     }
 
@@ -262,13 +264,48 @@ extern "C" int vps_chain_write_tables(vps_chain* ch, const char* prefix, const v
     return vps_write_tables_host(ch->ds, &ch->par, (int)K, ch->id, &st, prefix, opt, summary);
 }
 
-extern "C" int vps_write_summary(const char* path, const vps_run_summary* runs, int n, const vps_output_options* opt)
+// LwrLineCmp(prefix, arg) == 0 (src/string2.cpp:247-258): arg starts with
+// prefix, letters compared in lower case
+static bool starts_lower(const char* arg, const char* prefix)
+{
+    for (; *prefix; ++prefix, ++arg) {
+        const int a = (*prefix >= 'A' && *prefix <= 'Z') ? *prefix + 32 : *prefix;
+        const int b = (*arg >= 'A' && *arg <= 'Z') ? *arg + 32 : *arg;
+        if (a != b) return false;
+    }
+    return true;
+}
+
+extern "C" int vps_write_summary(const char* path, const vps_run_summary* runs, int n, const vps_output_options* opt,
+                                 const vps_result_header* hdr)
 {
     if (!path || (!runs && n) || !opt) return fail(VPS_ERR_INVALID, "null argument");
+    if (hdr && hdr->argc && !hdr->argv) return fail(VPS_ERR_INVALID, "null argv");
     Writer w(*opt);
     w.f = fopen(path, "wb");
     if (!w.f) return fail(VPS_ERR_INVALID, "cannot write %s", path);
-    w.s("#Bayesian clustering");
+    // OpenResFile("-structure", "#Bayesian clustering"): a title starting with
+    // '#' prefixes every head line with '#'
+    w.s("##Bayesian clustering calculated by vcfpop v" VPS_VCFPOP_VERSION);
+    w.br();
+    if (hdr) {
+        fprintf(w.f, "#  Input: %s", hdr->input ? hdr->input : ""); w.br();
+        fprintf(w.f, "#  Output: %s", hdr->output ? hdr->output : ""); w.br();
+        const time_t t = hdr->time ? (time_t)hdr->time : time(nullptr);
+        struct tm tmv;
+        localtime_r(&t, &tmv);
+        fprintf(w.f, "#  Time: %04d-%02d-%02d %02d:%02d:%02d", tmv.tm_year + 1900, tmv.tm_mon + 1, tmv.tm_mday,
+                tmv.tm_hour, tmv.tm_min, tmv.tm_sec);
+        w.br();
+        fprintf(w.f, "#  Parameter: %s", hdr->argc > 0 && hdr->argv[0] ? hdr->argv[0] : ""); w.br();
+        for (int i = 1; i < hdr->argc; ++i) {
+            const char* a = hdr->argv[i] ? hdr->argv[i] : "";
+            const bool g = starts_lower(a, "-g_") && !starts_lower(a, "-g_indtext") && !starts_lower(a, "-g_indtab");
+            if ((hdr->filters && starts_lower(a, "-f_")) || g || starts_lower(a, "-structure")) {
+                fprintf(w.f, "#    %s", a); w.br();
+            }
+        }
+    }
     w.br(); w.br();
     w.s("ID"); w.d(); w.s("k"); w.d(); w.s("rep"); w.d(); w.s("Mean lnL"); w.d(); w.s("Var lnL"); w.d(); w.s("lnP(D)");
     for (int i = 0; i < n; ++i) {
@@ -276,7 +313,6 @@ extern "C" int vps_write_summary(const char* path, const vps_run_summary* runs, 
         fprintf(w.f, "%d", runs[i].rep); w.d();
         w.real(runs[i].mean_lnl); w.d(); w.real(runs[i].var_lnl); w.d(); w.real(runs[i].lnpd);
     }
-    w.br();
     fclose(w.f);
     return VPS_OK;
 }

@@ -2,17 +2,21 @@
 
     python -m vcfpop_amd.par par.polygene.txt [--device 0] [--sweeps N] [--topology reference|parallel]
 
-Reads the options the clustering uses and ignores the rest (other analyses
-are not part of this engine):
+The file is split as vcfpop splits it (SetParameters,
+src/parameters.cpp:1345-1362): blank-separated tokens, blanks inside double
+quotes kept, quotes trimmed.  Reads the options the clustering uses and
+ignores the rest (other analyses are not part of this engine):
   -g_input, -g_format (polygene), -g_output, -g_seed, -g_decimal
+  -g_indtext: populations and regions (vcfpop's default without it: one
+    population; the file's population column is not read, as in vcfpop)
   -structure_* (src/parameters.cpp:1230-1315): models, K range, runs,
   burn-in / sampling / thinning lengths, lambda/alpha/r/F settings.
 The runs are the reference's task list (src/structure.cpp:2402-2413): for
 K = krange_min..krange_max and rep = 1..nruns, run id = 1, 2, ... in that
 order and run seed = HashULong(g_seed + id).  All runs are one batch on
 one device; each finished run writes its WriteStructure table and the
-runs' WriteStructureSummary table goes to <g_output>.structure.txt.
-Populations come from the file's population column.
+runs' WriteStructureSummary table, with the result-file head of OpenResFile
+(src/file.cpp:269-287), goes to <g_output>.structure.txt.
 
 -structure_eval=yes switches to vcfpop's evaluation mode instead
 (src/function.cpp:4743-4766): K = 5, one run, one sweep, both models, each
@@ -36,16 +40,56 @@ _DBL_OPTS = {"lambda", "stdlambda", "maxlambda", "alpha", "stdalpha", "maxalpha"
 _IGNORED = {"plot", "nstream"}
 
 
+def trim_quote(a):
+    """TrimQuote (src/string2.cpp:148-163): a leading and a trailing double
+    quote dropped, and one right after the first '='"""
+    if not a:
+        return a
+    if a[0] == '"':
+        a = a[1:]
+    if a and a[-1] == '"':
+        a = a[:-1]
+    i = a.find("=")
+    if i != -1 and i + 1 < len(a) and a[i + 1] == '"':
+        a = a[:i + 1] + a[i + 2:]
+    return a
+
+
+def par_tokens(text):
+    """a parameter file's arguments as vcfpop reads them: split at line
+    breaks, tabs and spaces outside double quotes, quotes trimmed"""
+    text = text.replace("\r\n", "\n")
+    toks, cur, quoted = [], [], False
+    for ch in text:
+        if ch == '"':
+            quoted = not quoted
+        if ch in "\n\t " and not quoted:
+            if cur:
+                toks.append("".join(cur))
+            cur = []
+        else:
+            cur.append(ch)
+    if cur:
+        toks.append("".join(cur))
+    return [trim_quote(t) for t in toks]
+
+
+def path_value(v):
+    """ReplacePathDelim (src/parameters.cpp:364-375): backslashes as '/'"""
+    return v.replace("\\", "/")
+
+
 def read_par(path):
-    """(global options, structure options) of a parameter file"""
+    """(global options, structure options, arguments) of a parameter file"""
     g, st = {}, {}
-    for raw in open(path, encoding="utf-8", errors="replace"):
-        line = raw.strip()
-        m = re.match(r"^-(g|structure)_([A-Za-z0-9]+)=(.*)$", line)
+    with open(path, encoding="utf-8", errors="replace", newline="") as f:
+        argv = par_tokens(f.read())
+    for a in argv:
+        m = re.match(r"^-(g|structure)_([A-Za-z0-9]+)=(.*)$", a, re.S)
         if not m:
             continue
-        (g if m.group(1) == "g" else st)[m.group(2).lower()] = m.group(3).strip().strip('"')
-    return g, st
+        (g if m.group(1).lower() == "g" else st)[m.group(2).lower()] = m.group(3)
+    return g, st, argv
 
 
 def structure_params(st, g_seed):
@@ -92,58 +136,80 @@ class _LogFile:
         pass
 
 
+class Job:
+    """A parameter file's clustering job: the genotype image with vcfpop's
+    populations and regions, the run parameters, the run list and the
+    result-file head."""
+
+    def __init__(self, par_path, sweeps=None, topology="reference"):
+        g, st, argv = read_par(par_path)
+        base = os.path.dirname(os.path.abspath(par_path))
+        if g.get("format", "polygene").lower() != "polygene":
+            raise ValueError(f"-g_format={g['format']}: this engine reads the polygene format")
+        if "indtab" in g:
+            raise ValueError("-g_indtab is not supported: give the populations with -g_indtext")
+        inp = g.get("input")
+        if not inp:
+            raise ValueError("-g_input is missing")
+        self.header = {"input": path_value(inp), "output": path_value(g.get("output", "vcfpop")), "argv": argv,
+                       "filters": any(a.lower() == "-f" for a in argv), "time": 0}
+        self.input = os.path.join(base, path_value(inp).replace("/", os.sep))
+        self.out = os.path.join(base, path_value(g.get("output", "vcfpop")).replace("/", os.sep))
+        self.decimal = int(g.get("decimal", 5))
+        self.seed = int(g.get("seed", 0))
+        kw, (kmin, kmax) = structure_params(st, self.seed)
+        if sweeps is not None:   # a shorter run: burn-in a fifth, sampling the rest
+            kw["nburnin"], kw["nreps"] = sweeps // 5, sweeps - sweeps // 5
+        kw["rng_topology"] = RNG_PARALLEL if topology == "parallel" else RNG_REFERENCE
+        self.kw = kw
+        self.params = p = params_default(**kw)
+        nadm = p.nadmburnin if (p.admix_val == 2 and p.locpriori_val == 2) else 0
+        self.total = nadm + p.nburnin + p.nreps
+        # run id i + 1 for (K, rep) in the reference's task order
+        self.runs = [(k, rep) for k in range(kmin, kmax + 1) for rep in range(1, p.nruns + 1)]
+        self.eval = st.get("eval", "no").lower() == "yes"
+        self.image = Image(self.input)
+        indtext = g.get("indtext")
+        self.image.assign_indtext(indtext.replace("#n", "\n") if indtext else None)
+
+    def close(self):
+        self.image.close()
+
+
 def run(par_path, device=0, sweeps=None, topology="reference", log=print):
     """Run a parameter file's clustering; returns the runs' summaries (or,
     with -structure_eval=yes, the evaluation rows)"""
-    g, st = read_par(par_path)
-    base = os.path.dirname(os.path.abspath(par_path))
-    if g.get("format", "polygene").lower() != "polygene":
-        raise ValueError(f"-g_format={g['format']}: this engine reads the polygene format")
-    inp = g.get("input")
-    if not inp:
-        raise ValueError("-g_input is missing")
-    inp = os.path.join(base, inp.replace("\\", os.sep))
-    out = os.path.join(base, g.get("output", "vcfpop").replace("\\", os.sep))
-    os.makedirs(os.path.dirname(out) or ".", exist_ok=True)
-    decimal = int(g.get("decimal", 5))
-    kw, (kmin, kmax) = structure_params(st, int(g.get("seed", 0)))
-    if sweeps is not None:   # a shorter run: burn-in a fifth, sampling the rest
-        kw["nburnin"], kw["nreps"] = sweeps // 5, sweeps - sweeps // 5
-    kw["rng_topology"] = RNG_PARALLEL if topology == "parallel" else RNG_REFERENCE
-    p = params_default(**kw)
-    nruns = p.nruns
-    nadm = p.nadmburnin if (p.admix_val == 2 and p.locpriori_val == 2) else 0
-    total = nadm + p.nburnin + p.nreps
-
-    im = Image(inp)
-    log(f"{inp}: {im.N} individuals, {im.L} of {im.loci_read} loci (fewer than 2 alleles dropped)")
+    job = Job(par_path, sweeps, topology)
+    im, p, out = job.image, job.params, job.out
+    log(f"{job.input}: {im.N} individuals in {im.desc.n_pop} populations, {im.L} of {im.loci_read} loci "
+        "(fewer than 2 alleles dropped)")
     ds = Dataset(im, device)
-    if st.get("eval", "no").lower() == "yes":
+    if job.eval:
         from . import evalmode
-        base = {k: v for k, v in kw.items() if k not in ("krange_min", "krange_max", "nruns", "rng_topology")}
-        rows = evalmode.run_eval(ds, K=5, seed=int(g.get("seed", 0)), base=base)
+        base = {k: v for k, v in job.kw.items() if k not in ("krange_min", "krange_max", "nruns", "rng_topology")}
+        rows = evalmode.run_eval(ds, K=5, seed=job.seed, base=base)
         evalmode.print_table(rows, file=_LogFile(log))
         ds.close()
-        im.close()
+        job.close()
         return rows
-    runs = [(k, rep) for k in range(kmin, kmax + 1) for rep in range(1, nruns + 1)]
-    chains = [Chain(ds, p, k, i + 1) for i, (k, rep) in enumerate(runs)]
+    os.makedirs(os.path.dirname(out) or ".", exist_ok=True)
+    chains = [Chain(ds, p, k, i + 1) for i, (k, rep) in enumerate(job.runs)]
     t0 = time.perf_counter()
     b = Batch(chains)
-    b.step(total)
+    b.step(job.total)
     b.sync()
     dt = time.perf_counter() - t0
-    log(f"{len(runs)} runs x {total} sweeps in {dt:.2f} s")
+    log(f"{len(job.runs)} runs x {job.total} sweeps in {dt:.2f} s")
     summaries = []
-    for c, (k, rep) in zip(chains, runs):
+    for c, (k, rep) in zip(chains, job.runs):
         c.finish()
-        summaries.append(c.write_tables(out, rep=rep, decimal=decimal, diversity=p.diversity_val))
-    write_summary(out + ".structure.txt", summaries, decimal=decimal)
+        summaries.append(c.write_tables(out, rep=rep, decimal=job.decimal, diversity=p.diversity_val))
+    write_summary(out + ".structure.txt", summaries, header=job.header, decimal=job.decimal)
     b.close()
     for c in chains:
         c.close()
     ds.close()
-    im.close()
+    job.close()
     return summaries
 
 
