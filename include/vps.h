@@ -120,7 +120,7 @@ typedef struct vps_params {
     uint64_t g_seed;          /* -g_seed; run seed = HashULong(g_seed + id) */
     /* engine options (no counterpart in the reference) */
     int32_t rng_topology;     /* VPS_RNG_REFERENCE or VPS_RNG_PARALLEL           */
-    int32_t precision;        /* VPS_PREC_FP64 or VPS_PREC_MIXED                 */
+    int32_t precision;        /* VPS_PREC_FP64, _MIXED, _REAL32 or _FAST32       */
     int32_t z_chunk;          /* individuals per work chunk of the genotype passes:
                                  > 0 fixed; 0 (default) = timed candidates at chain
                                  creation; -1 = occupancy model only.  Results do
@@ -139,6 +139,18 @@ typedef struct vps_params {
 
 #define VPS_PREC_FP64  0     /* every quantity in FP64 (reference REAL=double)                     */
 #define VPS_PREC_MIXED 1     /* FP32 per-genotype weights; FP64 lnL, Dirichlet, Metropolis          */
+/* The reference's BAYESIAN<float> (-g_float=single): P, Q, the Metropolis
+   proposals and the mean frequencies are single-precision values (each
+   Dirichlet row is v_i rounded to float times (float)(1/sum v) in float;
+   priors and initial values rounded to float; FreqM accumulated in float);
+   the genotype passes compute in FP64 from them (-g_fastsingle=no). */
+#define VPS_PREC_REAL32 2
+/* REAL32 with -g_fastsingle=yes (vcfpop's default for -g_float=single):
+   the Z pass draws from RNG<float> (xorshift96) per-locus streams with
+   single-precision weights, and the Record mixture and Metropolis ratios are
+   formed in single precision (src/structureAVX.cpp:725-760, 942-947,
+   1148-1168).  The F model is not available with REAL32 or FAST32. */
+#define VPS_PREC_FAST32 3
 
 /* fills *p with the reference defaults (src/parameters.cpp:584-627, :432-445) */
 void vps_params_default(vps_params* p);

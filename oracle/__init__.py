@@ -51,6 +51,9 @@ def load():
         "or_rngf_init": ([vp, u64, u64], None),
         "or_rngf_xorshift": ([vp], u32),
         "or_rngf_uniform": ([vp, C.c_float, C.c_float], C.c_float),
+        "or_rngf_poly_avx": ([vp, vp, C.c_int], C.c_int),
+        "or_rng_dirichlet_ab32": ([vp, vp, vp, vp, C.c_int, C.c_int], None),
+        "or_rng_dirichlet_af32": ([vp, vp, vp, C.c_int, dbl], None),
         "or_mylog": ([dbl], dbl),
         "or_mysin": ([dbl], dbl),
         "or_mycos": ([dbl], dbl),
@@ -142,6 +145,19 @@ class Rng(C.Structure):
         load().or_rng_dirichlet_af(C.byref(self), res.ctypes.data, a.ctypes.data, len(a), 1.0)
         return res
 
+    def dirichlet_ab32(self, a, b):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.int64)
+        res = np.empty(len(a))
+        load().or_rng_dirichlet_ab32(C.byref(self), res.ctypes.data, a.ctypes.data, b.ctypes.data, 1, len(a))
+        return res
+
+    def dirichlet_a32(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        res = np.empty(len(a))
+        load().or_rng_dirichlet_af32(C.byref(self), res.ctypes.data, a.ctypes.data, len(a), 1.0)
+        return res
+
 
 class RngF(C.Structure):
     _fields_ = [("x", C.c_uint32), ("y", C.c_uint32), ("z", C.c_uint32)]
@@ -152,6 +168,10 @@ class RngF(C.Structure):
 
     def xorshift(self):
         return load().or_rngf_xorshift(C.byref(self))
+
+    def poly_avx(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        return load().or_rngf_poly_avx(C.byref(self), w.ctypes.data, len(w))
 
 
 def charge_log(values):
@@ -218,6 +238,8 @@ class Chain:
         self.S = data._desc.n_pop if data._desc.n_pop > 0 else 1
         self.params = params
         self.h = load().or_chain_create(data.h, C.addressof(params), int(K), int(run_id))
+        if not self.h:
+            raise ValueError("the oracle keeps REAL = double for the F model")
         self._finished = False
 
     def _shape(self, name):

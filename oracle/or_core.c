@@ -432,6 +432,36 @@ void or_rng_dirichlet_af(or_rng* r, double* res, const double* a, int n, double 
     for (int i = 0; i < n; ++i) res[i] *= inv;
 }
 
+/* The same Dirichlet draws stored as REAL = float (BAYESIAN<float>):
+   res[i] = (float)v while s adds the unrounded v, then res[i] *= (float)(1/s)
+   in single precision (Mul(float*, float, n), src/math2.cpp:2088-2102). */
+void or_rng_dirichlet_ab32(or_rng* r, double* res, const double* a, const void* b, int b_is_int64, int n)
+{
+    double s = 0;
+    float v32[VPS_MAX_K * 8];
+    for (int i = 0; i < n; ++i) {
+        double bi = b_is_int64 ? (double)((const int64_t*)b)[i] : (double)((const int32_t*)b)[i];
+        double v = or_rng_gamma(r, a[i] + bi, 1);
+        v32[i] = (float)v;
+        s += v;
+    }
+    float inv = (float)(1.0 / s);
+    for (int i = 0; i < n; ++i) res[i] = (double)(v32[i] * inv);
+}
+
+void or_rng_dirichlet_af32(or_rng* r, double* res, const double* a, int n, double f)
+{
+    double s = 0;
+    float v32[VPS_MAX_K * 8];
+    for (int i = 0; i < n; ++i) {
+        double v = or_rng_gamma(r, a[i] * f, 1);
+        v32[i] = (float)v;
+        s += v;
+    }
+    float inv = (float)(1.0 / s);
+    for (int i = 0; i < n; ++i) res[i] = (double)(v32[i] * inv);
+}
+
 /* ------------------------------------------------------------------ */
 /* RNG<float> (xorshift96), src/statistics.cpp:330-391                  */
 /* ------------------------------------------------------------------ */
@@ -462,4 +492,27 @@ float or_rngf_uniform(or_rngf* r, float mn, float mx)
     b |= u & 0x007FFFFFu;
     float re; memcpy(&re, &b, 4);
     return (re - 1.0f) * (mx - mn) + mn;
+}
+
+/* RNGAVX<float>::Poly, src/mathAVX.cpp:340-386: w[k] are the fast
+   single-precision weights q*p + (float)MIN_FREQ (src/structureAVX.cpp:
+   942-947); s = 0 + w0 + w1 + ... in float; U from the low 23 bits of the
+   next xorshift96 output; t = (U - 1) * s; the first k with t < w[k] after
+   subtracting the previous weights, else n - 1. */
+int or_rngf_poly_avx(or_rngf* r, const float* w, int n)
+{
+    float s = 0.0f;
+    for (int k = 0; k < n; ++k) s = s + w[k];
+    uint32_t u = or_rngf_xorshift(r), b = 0x3F800000u | (u & 0x007FFFFFu);
+    float t;
+    memcpy(&t, &b, 4);
+    t = t - 1.0f;
+    t = t * s;
+    int idx = n - 1, found = 0;
+    for (int k = 0; k < n; ++k) {
+        int lt = t < w[k];
+        t = t - w[k];
+        if (lt && !found) { idx = k; found = 1; }
+    }
+    return idx;
 }

@@ -213,7 +213,14 @@ struct or_chain {
     uint16_t* Z;
     double *bufNK1, *bufN1, *lnL, *rout, lastlnl;
     int misum_is_double;
+    /* REAL = float (-g_float=single): every REAL array of BAYESIAN<float>
+       holds single-precision values; fast32 adds -g_fastsingle=yes (the
+       single-precision genotype passes and RNG<float> Z streams) */
+    int real32, fast32;
 };
+
+/* a value stored into a REAL array */
+static inline double rstore(const or_chain* c, double v) { return c->real32 ? (double)(float)v : v; }
 
 uint64_t or_chain_seed(or_chain* c) { return c->seed; }
 
@@ -232,7 +239,7 @@ static void init_fix(or_chain* c)      /* InitFix, src/structure.cpp:468-553 */
     c->Ni = (int32_t*)xcalloc(K * KT, 4);
     c->Mi = (int64_t*)xcalloc((size_t)c->N * K, 8);
     c->Q = (double*)xcalloc((size_t)c->N * K, 8);
-    for (int64_t j = 0; j < (int64_t)c->N * K; ++j) c->Q[j] = 1.0 / K;
+    for (int64_t j = 0; j < (int64_t)c->N * K; ++j) c->Q[j] = rstore(c, 1.0 / K);
     c->Alpha = (double*)xcalloc(K, 8); c->Lambda = (double*)xcalloc(K, 8);
     for (int k = 0; k < K; ++k) { c->Alpha[k] = c->alpha; c->Lambda[k] = c->lambda; }
     int nf = c->fmodel ? (c->fsame ? 1 : K) : 0;
@@ -248,7 +255,7 @@ static void init_fix(or_chain* c)      /* InitFix, src/structure.cpp:468-553 */
         int k2 = d->k2[l];
         if (k2 != 2)
             for (int k = 0; k < K; ++k)
-                for (int a = 0; a < k2; ++a) c->Freq[k * KT + d->afo[l] + a] = 1.0 / k2;
+                for (int a = 0; a < k2; ++a) c->Freq[k * KT + d->afo[l] + a] = rstore(c, 1.0 / k2);
     }
 }
 
@@ -339,6 +346,9 @@ or_chain* or_chain_create(or_data* d, const vps_params* p, int K, int run_id)
     c->pmeanf = p->pmeanf; c->pstdf = p->pstdf; c->stdf = p->stdf; c->fsame = p->singlef_val == 1;
     c->writelnl = p->writelnl_val == 1;
     c->topology = p->rng_topology;
+    c->real32 = p->precision == VPS_PREC_REAL32 || p->precision == VPS_PREC_FAST32;
+    c->fast32 = p->precision == VPS_PREC_FAST32;
+    if (c->real32 && c->fmodel) { free(c); return NULL; }   /* the F model keeps REAL = double here */
     c->kdis = (int64_t*)xcalloc(d->maxK + 1, 8);
     for (int64_t l = 0; l < c->L; ++l) c->kdis[d->k2[l]]++;
     c->nadmburnin = (!c->locpriori && !c->admix) ? p->nadmburnin : 0;
@@ -457,7 +467,7 @@ static void set_prior_freq(or_chain* c)
                 for (int64_t a = 0; a < KT; ++a) c->Freq[k * KT + a] = c->FreqA[a] * c->f[k];
     } else
         for (int k = 0; k < K; ++k)
-            for (int64_t a = 0; a < KT; ++a) c->Freq[k * KT + a] = c->Lambda[k];
+            for (int64_t a = 0; a < KT; ++a) c->Freq[k * KT + a] = rstore(c, c->Lambda[k]);
 }
 
 void or_update_p(or_chain* c)
@@ -475,7 +485,8 @@ void or_update_p(or_chain* c)
                 int64_t k = kl / L, l = kl % L;
                 int32_t* ni = c->Ni + k * KT + d->afo[l];
                 double* p = c->Freq + k * KT + d->afo[l];
-                or_rng_dirichlet_ab(&rng, p, p, ni, 0, d->k2[l]);
+                if (c->real32) or_rng_dirichlet_ab32(&rng, p, p, ni, 0, d->k2[l]);
+                else or_rng_dirichlet_ab(&rng, p, p, ni, 0, d->k2[l]);
             }
         }
     } else {
@@ -486,7 +497,8 @@ void or_update_p(or_chain* c)
             or_rng rng; or_rng_init(&rng, c->seed + ((uint64_t)c->m * K + k) * L + l, OR_SALT_UPDATEP);
             int32_t* ni = c->Ni + k * KT + d->afo[l];
             double* p = c->Freq + k * KT + d->afo[l];
-            or_rng_dirichlet_ab(&rng, p, p, ni, 0, d->k2[l]);
+            if (c->real32) or_rng_dirichlet_ab32(&rng, p, p, ni, 0, d->k2[l]);
+            else or_rng_dirichlet_ab(&rng, p, p, ni, 0, d->k2[l]);
         }
     }
 }
@@ -555,13 +567,13 @@ void or_update_q_admix(or_chain* c)
     if (c->topology == VPS_RNG_REFERENCE) {
         or_rng rng; or_rng_init(&rng, c->seed + (uint64_t)c->m, OR_SALT_UPDATEQ);
         for (int i = 0; i < N; ++i)
-            if (c->d->vt[i]) or_rng_dirichlet_ab(&rng, c->Q + (int64_t)i * K, alpha_row(c, i), c->Mi + (int64_t)i * K, 1, K);
+            if (c->d->vt[i]) (c->real32 ? or_rng_dirichlet_ab32 : or_rng_dirichlet_ab)(&rng, c->Q + (int64_t)i * K, alpha_row(c, i), c->Mi + (int64_t)i * K, 1, K);
     } else {
         #pragma omp parallel for schedule(static) num_threads(g_threads)
         for (int i = 0; i < N; ++i) {
             if (!c->d->vt[i]) continue;
             or_rng rng; or_rng_init(&rng, c->seed + (uint64_t)c->m * N + i, OR_SALT_UPDATEQ);
-            or_rng_dirichlet_ab(&rng, c->Q + (int64_t)i * K, alpha_row(c, i), c->Mi + (int64_t)i * K, 1, K);
+            (c->real32 ? or_rng_dirichlet_ab32 : or_rng_dirichlet_ab)(&rng, c->Q + (int64_t)i * K, alpha_row(c, i), c->Mi + (int64_t)i * K, 1, K);
         }
     }
 }
@@ -596,12 +608,22 @@ void or_q_metro_logratio(or_chain* c, const double* qprop, double* out)
                     if (j < tl[t].n && a < vv[j]) {
                         int64_t l = tl[t].l0 + j;
                         const double* p = c->Freq + d->afo[l] + als[j][a];
-                        double f1 = 0, f2 = 0;
-                        for (int k = 0; k < K; ++k) {
-                            f1 = f1 + p[k * KT] * qi[k];
-                            f2 = f2 + p[k * KT] * q[k];
+                        if (c->fast32) {
+                            /* the fast single-precision branch, src/structureAVX.cpp:725-760 */
+                            float f1 = 0, f2 = 0;
+                            for (int k = 0; k < K; ++k) {
+                                f1 = f1 + (float)p[k * KT] * (float)qi[k];
+                                f2 = f2 + (float)p[k * KT] * (float)q[k];
+                            }
+                            f[j] = (double)(f1 / f2);
+                        } else {
+                            double f1 = 0, f2 = 0;
+                            for (int k = 0; k < K; ++k) {
+                                f1 = f1 + p[k * KT] * qi[k];
+                                f2 = f2 + p[k * KT] * q[k];
+                            }
+                            f[j] = f1 / f2;
                         }
-                        f[j] = f1 / f2;
                     } else f[j] = 1.0;
                 }
                 or_charge_log(&slog, &prod, tree64(f));
@@ -621,7 +643,7 @@ void or_update_q_metro(or_chain* c)
     if (c->topology == VPS_RNG_REFERENCE) {
         or_rng rng; or_rng_init(&rng, c->seed + (uint64_t)c->m, OR_SALT_UPDATEQ);
         for (int i = 0; i < N; ++i)
-            if (c->d->vt[i]) or_rng_dirichlet_af(&rng, bufi + (int64_t)i * K, alpha_row(c, i), K, 1.0);
+            if (c->d->vt[i]) (c->real32 ? or_rng_dirichlet_af32 : or_rng_dirichlet_af)(&rng, bufi + (int64_t)i * K, alpha_row(c, i), K, 1.0);
         or_q_metro_logratio(c, bufi, c->bufN1);
         for (int i = 0; i < N; ++i) {
             if (!c->d->vt[i]) continue;
@@ -632,7 +654,7 @@ void or_update_q_metro(or_chain* c)
         or_rng* rs = (or_rng*)xcalloc(N, sizeof(or_rng));
         for (int i = 0; i < N; ++i) {
             or_rng_init(&rs[i], c->seed + (uint64_t)c->m * N + i, OR_SALT_UPDATEQ);
-            if (c->d->vt[i]) or_rng_dirichlet_af(&rs[i], bufi + (int64_t)i * K, alpha_row(c, i), K, 1.0);
+            if (c->d->vt[i]) (c->real32 ? or_rng_dirichlet_af32 : or_rng_dirichlet_af)(&rs[i], bufi + (int64_t)i * K, alpha_row(c, i), K, 1.0);
         }
         or_q_metro_logratio(c, bufi, c->bufN1);
         for (int i = 0; i < N; ++i) {
@@ -830,9 +852,11 @@ void or_update_z_admix(or_chain* c)
 #endif
         int64_t* mi = mi_all + (size_t)tid * N * K;
         double w[VPS_MAX_K * 4];
+        float w32[VPS_MAX_K * 4];
         #pragma omp for schedule(dynamic, 64)
         for (int64_t l = 0; l < L; ++l) {
             or_rng rng; or_rng_init(&rng, c->seed + (uint64_t)c->m * L + l, OR_SALT_UPDATEZ);
+            or_rngf rngf; or_rngf_init(&rngf, c->seed + (uint64_t)c->m * L + l, OR_SALT_UPDATEZ);
             const double* p = c->Freq + d->afo[l];
             int32_t* ni = c->Ni + d->afo[l];
             for (int i = 0; i < N; ++i) {
@@ -841,9 +865,21 @@ void or_update_z_admix(or_chain* c)
                 const uint16_t* als = d->alleles + d->gtaoff[g];
                 const double* q = c->Q + (int64_t)i * K;
                 for (int a = 0; a < mp; ++a) {
-                    if (a >= v) { or_rng_xorshift(&rng); continue; }
-                    for (int k = 0; k < K; ++k) w[k] = p[k * KT + als[a]] * q[k] + OR_MIN_FREQ;
-                    int k2 = or_rng_poly_avx(&rng, w, K);
+                    if (a >= v) {
+                        if (c->fast32) or_rngf_xorshift(&rngf);
+                        else or_rng_xorshift(&rng);
+                        continue;
+                    }
+                    int k2;
+                    if (c->fast32) {
+                        /* REAL = float with -g_fastsingle: weights and draw in single
+                           precision from RNG<float> (src/structureAVX.cpp:942-947) */
+                        for (int k = 0; k < K; ++k) w32[k] = (float)p[k * KT + als[a]] * (float)q[k] + (float)OR_MIN_FREQ;
+                        k2 = or_rngf_poly_avx(&rngf, w32, K);
+                    } else {
+                        for (int k = 0; k < K; ++k) w[k] = p[k * KT + als[a]] * q[k] + OR_MIN_FREQ;
+                        k2 = or_rng_poly_avx(&rng, w, K);
+                    }
                     mi[(int64_t)i * K + k2]++;
                     ni[k2 * KT + als[a]]++;
                 }
@@ -1103,7 +1139,12 @@ double or_record_pass(or_chain* c)
                 for (int j = 0; j < 64; ++j) {
                     if (j < tl[t].n && a < vv[j]) {
                         const double* p = c->Freq + d->afo[tl[t].l0 + j] + als[j][a];
-                        if (isadmix) {
+                        if (isadmix && c->fast32) {
+                            /* src/structureAVX.cpp:1148-1168: the mixture in float */
+                            float s = 0;
+                            for (int k = 0; k < K; ++k) s = s + (float)p[k * KT] * (float)q[k];
+                            f[j] = (double)s;
+                        } else if (isadmix) {
                             double s = 0;
                             for (int k = 0; k < K; ++k) s = s + p[k * KT] * q[k];
                             f[j] = s;
@@ -1148,7 +1189,10 @@ void or_record(or_chain* c)
             } else if (c->admix)
                 for (int j = 0; j < na; ++j) c->rout[4 + nl + j] += c->Alpha[j];
             for (int j = 0; j < nf; ++j) c->rout[c->rlen - nf + j] += c->F[j];
-            for (int64_t j = 0; j < K * KT; ++j) c->FreqM[j] += c->Freq[j];
+            if (c->real32)   /* Add(FreqM, Freq) on float arrays */
+                for (int64_t j = 0; j < K * KT; ++j) c->FreqM[j] = (double)((float)c->FreqM[j] + (float)c->Freq[j]);
+            else
+                for (int64_t j = 0; j < K * KT; ++j) c->FreqM[j] += c->Freq[j];
         }
     }
 }
@@ -1165,8 +1209,13 @@ void or_arrange(or_chain* c)
     for (int j = 0; j < c->rlen; ++j) c->rout[j] *= inv;
     c->rout[2] = c->rout[1] - c->rout[0] * c->rout[0];
     c->rout[3] = c->rout[0] - c->rout[2] / 2;
-    double invf = 1.0 / c->nr;
-    for (int64_t j = 0; j < K * KT; ++j) c->FreqM[j] *= invf;
+    if (c->real32) {   /* Mul(FreqM, (REAL)(1.0 / nr)) on a float array */
+        float invf = (float)(1.0 / c->nr);
+        for (int64_t j = 0; j < K * KT; ++j) c->FreqM[j] = (double)((float)c->FreqM[j] * invf);
+    } else {
+        double invf = 1.0 / c->nr;
+        for (int64_t j = 0; j < K * KT; ++j) c->FreqM[j] *= invf;
+    }
 }
 
 /* MCMC body, src/structure.cpp:2122-2158 */

@@ -73,6 +73,9 @@ typedef struct or_rngf { uint32_t x, y, z; } or_rngf;
 void     or_rngf_init(or_rngf* r, uint64_t s, uint64_t salt);
 uint32_t or_rngf_xorshift(or_rngf* r);
 float    or_rngf_uniform(or_rngf* r, float mn, float mx);
+int      or_rngf_poly_avx(or_rngf* r, const float* w, int n);
+void     or_rng_dirichlet_ab32(or_rng* r, double* res, const double* a, const void* b, int b_is_int64, int n);
+void     or_rng_dirichlet_af32(or_rng* r, double* res, const double* a, int n, double f);
 
 /* ---- math ---- */
 double or_mylog(double v);

@@ -32,6 +32,8 @@ from ._lib import (  # noqa: F401
     RNG_PARALLEL,
     PREC_FP64,
     PREC_MIXED,
+    PREC_REAL32,
+    PREC_FAST32,
     device_count,
     debug_jump,
     debug_copy,

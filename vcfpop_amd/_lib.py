@@ -10,6 +10,8 @@ RNG_REFERENCE = 0
 RNG_PARALLEL = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_REAL32 = 2
+PREC_FAST32 = 3
 
 _lib = None
 
