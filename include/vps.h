@@ -452,6 +452,8 @@ int vps_chain_kernel_time(vps_chain* ch, int kernel, double* total_ms, int64_t* 
 int vps_debug_decode(const vps_dataset* ds, int64_t l, int32_t i, int32_t* ploidy, uint16_t* alleles /*[VPS_MAX_PLOIDY]*/);
 /* advance a xorshift128+ state (x, y) by D steps with the GF(2) jump table */
 int vps_debug_jump(uint64_t D, uint64_t* x, uint64_t* y);
+/* advance a xorshift96 state (x, y, z) (RNG<float>) by D steps with its table */
+int vps_debug_jump96(uint64_t D, uint32_t* x, uint32_t* y, uint32_t* z);
 /* a chain's individual chunking of the genotype passes: G chunks of Nc individuals */
 int vps_debug_chunking(const vps_chain* ch, int32_t* G, int32_t* Nc);
 /* wavefronts per block of the multiallelic (GEN) Z-pass kernel for the chain's

@@ -20,6 +20,7 @@ extern "C" void CreateStreamCUDA(int devID);
 extern "C" void WaitCUDA();
 
 int g_seed_val;
+int g_fastsingle_val = 1;
 int structure_admix_val, structure_locpriori_val, structure_f_val;
 int structure_krange_min, structure_krange_max;
 int structure_nburnin_val, structure_nreps_val, structure_nthinning_val, structure_nruns_val, structure_nadmburnin_val;
@@ -143,73 +144,70 @@ void mock_set_options(const vps_params* p)
     g_seed_val = (int)p->g_seed;
 }
 
+}  // extern "C"
+
 // a run as ReadPar leaves it (src/structure.cpp:405-464), with zeroed
-// host buffers the tests fill
-struct MockRun {
-    BAYESIAN<double> b;
-    std::vector<double> Freq, FreqM, FreqA, Q, Alpha, Lambda, f, F, Eta, Gamma, Di, AlphaLocal, SumAlpha, bufNK1,
-        bufN1, lnL, rout;
+// host buffers the tests fill; REAL arrays are REAL, the rest as in vcfpop
+struct MockRunBase {
+    virtual ~MockRunBase() {}
+    virtual void* field(const char* name) = 0;
+    virtual void call(const char* member) = 0;
+};
+
+template <typename REAL>
+struct MockRun : MockRunBase {
+    BAYESIAN<REAL> b;
+    std::vector<REAL> Freq, FreqM, FreqA, Q, Di;
+    std::vector<double> Alpha, Lambda, f, F, Eta, Gamma, AlphaLocal, SumAlpha, bufNK1, bufN1, lnL, rout;
     std::vector<int64> MiSum, Mi;
     std::vector<int> Ni;
     std::vector<ushort> Z;
-};
 
-void* mock_run_new(int K, int id, int rlen, int nlnl)
-{
-    MockRun* r = new MockRun();
-    BAYESIAN<double>& b = r->b;
-    memset(&b, 0, sizeof(b));
-    const int N = nind, S = npop;
-    b.id = id; b.S = S; b.K = K; b.N = N; b.L = nloc;
-    b.admix = structure_admix_val == 1;
-    b.locpriori = structure_locpriori_val == 1;
-    b.fmodel = structure_f_val == 1;
-    b.nreps = structure_nreps_val; b.nthinning = structure_nthinning_val; b.nruns = structure_nruns_val;
-    b.nadmburnin = (!b.locpriori && !b.admix) ? structure_nadmburnin_val : 0;
-    b.nburnin = structure_nburnin_val + b.nadmburnin;
-    b.r = structure_r_val;
-    b.rlen = rlen;
-    r->Freq.assign((size_t)K * KT, 0); r->FreqM.assign((size_t)K * KT, 0); r->FreqA.assign(3 * (size_t)KT, 0);
-    r->Q.assign((size_t)N * K, 0); r->Alpha.assign(K, 0); r->Lambda.assign(K, 0); r->f.assign(K, 0); r->F.assign(K, 0);
-    r->Eta.assign(K, 0); r->Gamma.assign((size_t)S * K, 0); r->Di.assign((size_t)S * K, 0);
-    r->AlphaLocal.assign((size_t)S * K, 0); r->SumAlpha.assign(S, 0);
-    r->bufNK1.assign((size_t)N * K, 0); r->bufN1.assign(N, 0); r->lnL.assign(std::max(nlnl, 1), 0);
-    r->rout.assign(std::max(rlen, 1), 0);
-    r->MiSum.assign((size_t)N * K, 0); r->Mi.assign((size_t)N * K, 0); r->Ni.assign((size_t)K * KT, 0);
-    r->Z.assign(N, 0);
-    b.Freq = r->Freq.data(); b.FreqM = r->FreqM.data(); b.FreqA = r->FreqA.data(); b.Q = r->Q.data();
-    b.Alpha = r->Alpha.data(); b.Lambda = r->Lambda.data(); b.f = r->f.data(); b.F = r->F.data();
-    b.Eta = r->Eta.data(); b.Gamma = r->Gamma.data(); b.Di = r->Di.data(); b.AlphaLocal = r->AlphaLocal.data();
-    b.SumAlpha = r->SumAlpha.data(); b.bufNK1 = r->bufNK1.data(); b.bufN1 = r->bufN1.data(); b.lnL = r->lnL.data();
-    b.rout = r->rout.data(); b.MiSum = r->MiSum.data(); b.Mi = r->Mi.data(); b.Ni = r->Ni.data(); b.Z = r->Z.data();
-    return r;
-}
+    MockRun(int K, int id, int rlen, int nlnl)
+    {
+        memset(&b, 0, sizeof(b));
+        const int N = nind, S = npop;
+        b.id = id; b.S = S; b.K = K; b.N = N; b.L = nloc;
+        b.admix = structure_admix_val == 1;
+        b.locpriori = structure_locpriori_val == 1;
+        b.fmodel = structure_f_val == 1;
+        b.nreps = structure_nreps_val; b.nthinning = structure_nthinning_val; b.nruns = structure_nruns_val;
+        b.nadmburnin = (!b.locpriori && !b.admix) ? structure_nadmburnin_val : 0;
+        b.nburnin = structure_nburnin_val + b.nadmburnin;
+        b.r = structure_r_val;
+        b.rlen = rlen;
+        Freq.assign((size_t)K * KT, 0); FreqM.assign((size_t)K * KT, 0); FreqA.assign(3 * (size_t)KT, 0);
+        Q.assign((size_t)N * K, 0); Alpha.assign(K, 0); Lambda.assign(K, 0); f.assign(K, 0); F.assign(K, 0);
+        Eta.assign(K, 0); Gamma.assign((size_t)S * K, 0); Di.assign((size_t)S * K, 0);
+        AlphaLocal.assign((size_t)S * K, 0); SumAlpha.assign(S, 0);
+        bufNK1.assign((size_t)N * K, 0); bufN1.assign(N, 0); lnL.assign(std::max(nlnl, 1), 0);
+        rout.assign(std::max(rlen, 1), 0);
+        MiSum.assign((size_t)N * K, 0); Mi.assign((size_t)N * K, 0); Ni.assign((size_t)K * KT, 0);
+        Z.assign(N, 0);
+        b.Freq = Freq.data(); b.FreqM = FreqM.data(); b.FreqA = FreqA.data(); b.Q = Q.data();
+        b.Alpha = Alpha.data(); b.Lambda = Lambda.data(); b.f = f.data(); b.F = F.data();
+        b.Eta = Eta.data(); b.Gamma = Gamma.data(); b.Di = Di.data(); b.AlphaLocal = AlphaLocal.data();
+        b.SumAlpha = SumAlpha.data(); b.bufNK1 = bufNK1.data(); b.bufN1 = bufN1.data(); b.lnL = lnL.data();
+        b.rout = rout.data(); b.MiSum = MiSum.data(); b.Mi = Mi.data(); b.Ni = Ni.data(); b.Z = Z.data();
+    }
 
-void mock_run_free(void* p) { delete static_cast<MockRun*>(p); }
+    void* field(const char* name) override
+    {
+        struct { const char* n; void* v; } tab[] = {
+            {"Freq", Freq.data()}, {"FreqM", FreqM.data()}, {"FreqA", FreqA.data()}, {"Q", Q.data()},
+            {"Alpha", Alpha.data()}, {"Lambda", Lambda.data()}, {"f", f.data()}, {"F", F.data()},
+            {"Eta", Eta.data()}, {"Gamma", Gamma.data()}, {"Di", Di.data()}, {"AlphaLocal", AlphaLocal.data()},
+            {"SumAlpha", SumAlpha.data()}, {"bufNK1", bufNK1.data()}, {"bufN1", bufN1.data()},
+            {"lnL", lnL.data()}, {"rout", rout.data()}, {"MiSum", MiSum.data()}, {"Mi", Mi.data()},
+            {"Ni", Ni.data()}, {"Z", Z.data()}, {"r", &b.r}, {"m", &b.m}, {"nr", &b.nr},
+            {"nlnL", &b.nlnL}, {"binaryq", &b.binaryq}, {"seed", &b.seed}};
+        for (auto& e : tab)
+            if (!strcmp(e.n, name)) return e.v;
+        return nullptr;
+    }
 
-// host buffer of a field, for the tests to read and write
-void* mock_run_field(void* p, const char* name)
-{
-    MockRun* r = static_cast<MockRun*>(p);
-    struct { const char* n; void* v; } tab[] = {
-        {"Freq", r->Freq.data()}, {"FreqM", r->FreqM.data()}, {"FreqA", r->FreqA.data()}, {"Q", r->Q.data()},
-        {"Alpha", r->Alpha.data()}, {"Lambda", r->Lambda.data()}, {"f", r->f.data()}, {"F", r->F.data()},
-        {"Eta", r->Eta.data()}, {"Gamma", r->Gamma.data()}, {"Di", r->Di.data()}, {"AlphaLocal", r->AlphaLocal.data()},
-        {"SumAlpha", r->SumAlpha.data()}, {"bufNK1", r->bufNK1.data()}, {"bufN1", r->bufN1.data()},
-        {"lnL", r->lnL.data()}, {"rout", r->rout.data()}, {"MiSum", r->MiSum.data()}, {"Mi", r->Mi.data()},
-        {"Ni", r->Ni.data()}, {"Z", r->Z.data()}, {"r", &r->b.r}, {"m", &r->b.m}, {"nr", &r->b.nr},
-        {"nlnL", &r->b.nlnL}, {"binaryq", &r->b.binaryq}, {"seed", &r->b.seed}};
-    for (auto& e : tab)
-        if (!strcmp(e.n, name)) return e.v;
-    return nullptr;
-}
-
-// one member of the run, or the whole-run hand-off; 0, or 1 with the Exit
-// message in mock_last_exit()
-int mock_run_call(void* p, const char* member)
-{
-    BAYESIAN<double>& b = static_cast<MockRun*>(p)->b;
-    try {
+    void call(const char* member) override
+    {
         if (!strcmp(member, "InitCUDA")) b.InitCUDA();
         else if (!strcmp(member, "UninitCUDA")) b.UninitCUDA();
         else if (!strcmp(member, "UpdatePCUDA")) b.UpdatePCUDA();
@@ -220,6 +218,29 @@ int mock_run_call(void* p, const char* member)
         else if (!strcmp(member, "RecordCUDA")) b.RecordCUDA();
         else if (!strcmp(member, "MCMC")) VpsMCMC(b);
         else throw std::runtime_error(std::string("unknown member ") + member);
+    }
+};
+
+extern "C" {
+
+// real = 1: a BAYESIAN<float> run (-g_float=single), else BAYESIAN<double>
+void* mock_run_new(int K, int id, int rlen, int nlnl, int real)
+{
+    if (real == 1) return static_cast<MockRunBase*>(new MockRun<float>(K, id, rlen, nlnl));
+    return static_cast<MockRunBase*>(new MockRun<double>(K, id, rlen, nlnl));
+}
+
+void mock_run_free(void* p) { delete static_cast<MockRunBase*>(p); }
+
+// host buffer of a field, for the tests to read and write
+void* mock_run_field(void* p, const char* name) { return static_cast<MockRunBase*>(p)->field(name); }
+
+// one member of the run, or the whole-run hand-off; 0, or 1 with the Exit
+// message in mock_last_exit()
+int mock_run_call(void* p, const char* member)
+{
+    try {
+        static_cast<MockRunBase*>(p)->call(member);
     } catch (const std::exception& e) {
         g_exit = e.what();
         return 1;

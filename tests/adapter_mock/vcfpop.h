@@ -5,7 +5,7 @@
  * src/vcfpop.h; these declarations carry the same names and types:
  *   - STRUCTURE_RUNINFO and the BAYESIAN<REAL> fields the five *CUDA members,
  *     InitCUDA and UninitCUDA read or write (src/structure.h:24-306);
- *   - the -structure option globals and g_seed_val (src/parameters.h:20,
+ *   - the -structure option globals, g_seed_val and g_fastsingle_val (src/parameters.h:20, 55,
  *     209-257), the data-model globals nind, nloc, npop, KT, ainds, slocus and
  *     geno_bucket (src/global.h:170-189, src/function.h:621-629), and the
  *     accessors the adapter calls on a locus and a genotype
@@ -32,6 +32,7 @@ typedef int64_t LOCN;
 
 /* options (src/parameters.h) */
 extern int g_seed_val;
+extern int g_fastsingle_val;   /* -g_fastsingle: 1 = yes (default), 2 = no */
 extern int structure_admix_val, structure_locpriori_val, structure_f_val;
 extern int structure_krange_min, structure_krange_max;
 extern int structure_nburnin_val, structure_nreps_val, structure_nthinning_val, structure_nruns_val,

@@ -54,7 +54,7 @@ class Mock:
         L = self.lib
         L.mock_set_image.argtypes = [C.c_void_p]
         L.mock_set_options.argtypes = [C.c_void_p]
-        L.mock_run_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+        L.mock_run_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.mock_run_new.restype = C.c_void_p
         L.mock_run_free.argtypes = [C.c_void_p]
         L.mock_run_field.argtypes = [C.c_void_p, C.c_char_p]
@@ -68,7 +68,8 @@ class Mock:
 
 
 class Run:
-    """a BAYESIAN<double> of the fixture with host buffers as numpy views"""
+    """a BAYESIAN<REAL> of the fixture (REAL = float with real32) with host
+    buffers as numpy views"""
     DT = {"Ni": (np.int32, "KKT"), "Mi": (np.int64, "NK"), "MiSum": (np.int64, "NK"), "Z": (np.uint16, "N"),
           "Freq": (np.float64, "KKT"), "FreqM": (np.float64, "KKT"), "FreqA": (np.float64, "3KT"),
           "Q": (np.float64, "NK"), "Alpha": (np.float64, "K"), "Lambda": (np.float64, "K"), "f": (np.float64, "K"),
@@ -77,13 +78,18 @@ class Run:
           "bufN1": (np.float64, "N"), "rout": (np.float64, "R"), "m": (np.int32, "1"), "nr": (np.int32, "1"),
           "r": (np.float64, "1")}
 
-    def __init__(self, mock, K, run_id, N, KT, S, rlen):
-        self.m, self.h = mock, mock.lib.mock_run_new(K, run_id, rlen, 0)
+    REAL = ("Freq", "FreqM", "FreqA", "Q", "Di")   # REAL arrays of BAYESIAN<REAL>
+
+    def __init__(self, mock, K, run_id, N, KT, S, rlen, real32=False):
+        self.m, self.h = mock, mock.lib.mock_run_new(K, run_id, rlen, 0, 1 if real32 else 0)
+        self.real32 = real32
         self.n = {"KKT": K * KT, "NK": N * K, "N": N, "3KT": 3 * KT, "K": K, "SK": S * K, "S": S, "R": rlen, "1": 1}
         self.K, self.S = K, S
 
     def __getitem__(self, name):
         dt, n = self.DT[name]
+        if self.real32 and name in self.REAL:
+            dt = np.float32
         p = self.m.lib.mock_run_field(self.h, name.encode())
         return np.ctypeslib.as_array(C.cast(p, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), shape=(self.n[n],))
 
@@ -198,6 +204,74 @@ def test_members_through_the_seam_equal_the_oracle(vps, orc):
     tol[2] = tol[3] = RTOL * abs(ro[1])
     assert (np.abs(run["rout"] - ro) <= tol).all()
     say("the whole-run hand-off equals the oracle's run")
+    run.call("UninitCUDA")
+    run.close()
+    mock.seam("FreeStructureMemory", 0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fastsingle", [1, 2])
+def test_float_members_equal_the_oracle(vps, orc, fastsingle):
+    """BAYESIAN<float> (-g_float=single): -g_fastsingle=yes runs the engine's
+    FAST32 mode, =no its REAL32 mode; members and the whole run equal the
+    oracle's single-precision path"""
+    mock = Mock()
+    C.c_int.in_dll(mock.lib, "g_fastsingle_val").value = fastsingle
+    prec = vps.PREC_FAST32 if fastsingle == 1 else vps.PREC_REAL32
+    s = vps.Synth(n_ind=70, n_loc=400, k_true=3, alpha_q=0.5, missing=0.04, frac_multi=0.2, min_alleles=3,
+                  max_alleles=5, ploidy=[(2, 0.6), (4, 0.4)], n_pop=3, seed=61)
+    od = orc.Data(s.desc)
+    N, KT, S = od.N, od.KT, 3
+    assert mock.lib.mock_set_image(C.byref(s.desc)) == 0
+    mock.seam("CopyStructureMemory", 0)
+    mock.seam("CreateStreamCUDA", 0)
+    K = 3
+    p = vps.params_default(admix_val=1, g_seed=5, nburnin=2, nreps=6, precision=prec)
+    mock.lib.mock_set_options(C.byref(p))
+    o = orc.Chain(od, p, K, 2)
+    o.sweep(3)
+    run = Run(mock, K, 2, N, KT, S, o.get("rout").size, real32=True)
+    run.load_from(o)
+    assert (run["Freq"].astype(np.float64) == o.get("Freq").ravel()).all()   # single-precision state
+    run.call("InitCUDA")
+    run.call("UpdateZAdmixCUDA")
+    o.call("update_z_admix")
+    for n in ("Ni", "Mi"):
+        assert (run[n] == o.get(n).ravel()).all(), n
+    run.call("UpdatePCUDA")
+    o.call("update_p")
+    assert (run["Freq"].astype(np.float64) == o.get("Freq").ravel()).all()
+    o.set("m", [9])
+    run["m"][0] = 9
+    run.call("UpdateQMetroCUDA")
+    o.call("update_q_metro")
+    assert (run["Q"].astype(np.float64) == o.get("Q").ravel()).all()
+    run.call("RecordCUDA")
+    ref = o.record_pass()
+    assert abs(run["bufNK1"][0] - ref) <= RTOL * abs(ref)
+    say("BAYESIAN<float> members equal the oracle, fastsingle", fastsingle)
+    run.call("UninitCUDA")
+    run.close()
+    o.close()
+
+    p = vps.params_default(admix_val=1, g_seed=7, nburnin=20, nreps=40, nthinning=4, precision=prec)
+    mock.lib.mock_set_options(C.byref(p))
+    o = orc.Chain(od, p, K, 4)
+    run = Run(mock, K, 4, N, KT, S, o.get("rout").size, real32=True)
+    run.load_from(o)
+    run.call("InitCUDA")
+    run.call("MCMC")
+    o.sweep(60)
+    o.finish()
+    assert run["m"][0] == 60 and run["nr"][0] == 10
+    assert (run["MiSum"].view(np.float64) == o.get("MiSum").ravel()).all()
+    assert close(run["Q"], o.get("Q").ravel())
+    assert close(run["FreqM"], o.get("FreqM").ravel())
+    ro = o.get("rout")
+    tol = RTOL * np.abs(ro)
+    tol[2] = tol[3] = RTOL * abs(ro[1])
+    assert (np.abs(run["rout"] - ro) <= tol).all()
+    say("BAYESIAN<float> whole run equals the oracle's")
     run.call("UninitCUDA")
     run.close()
     mock.seam("FreeStructureMemory", 0)

@@ -192,6 +192,16 @@ def test_jump_table_equals_stepping(vps, orc):
         assert vps.debug_jump(D, x0, y0) == (r.x, r.y)
 
 
+def test_jump96_table_equals_stepping(vps, orc):
+    """the xorshift96 (RNG<float>) chunk jumps of the fast single-precision Z pass"""
+    for D in (1, 2, 7, 64, 1000, 123457):
+        r = orc.RngF(99, SALT_Z)
+        x0, y0, z0 = r.x, r.y, r.z
+        for _ in range(D):
+            r.xorshift()
+        assert vps.debug_jump96(D, x0, y0, z0) == (r.x, r.y, r.z)
+
+
 def test_uniform_bits(orc):
     r1, r2 = orc.Rng(5, 7), orc.Rng(5, 7)
     for _ in range(100):

@@ -67,3 +67,15 @@ def test_example_job_populations(vps):
     assert job.runs[:5] == [(2, 1), (2, 2), (2, 3), (2, 4), (3, 1)] and job.total == 10
     assert job.header["input"] == "data.polygene.txt" and job.header["output"] == "polygene/out"
     job.close()
+
+
+def test_float_options(vps):
+    """-g_float=single runs BAYESIAN<float>: with vcfpop's default
+    -g_fastsingle=yes the fast single-precision passes"""
+    from vcfpop_amd import par
+    assert par.real_precision({}) == vps.PREC_FP64
+    assert par.real_precision({"float": "single"}) == vps.PREC_FAST32
+    assert par.real_precision({"float": "single", "fastsingle": "no"}) == vps.PREC_REAL32
+    assert par.real_precision({"float": "double", "fastsingle": "no"}) == vps.PREC_FP64
+    with pytest.raises(ValueError, match="single or double"):
+        par.real_precision({"float": "half"})

@@ -36,6 +36,7 @@ from ._lib import (  # noqa: F401
     PREC_FAST32,
     device_count,
     debug_jump,
+    debug_jump96,
     debug_copy,
 )
 from .workloads import WORKLOADS, make_synth  # noqa: F401

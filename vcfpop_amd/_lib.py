@@ -194,6 +194,7 @@ _SIGS = {
     "vps_batch_stats": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "vps_debug_decode": ([_P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), _P], C.c_int),
     "vps_debug_jump": ([C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
+    "vps_debug_jump96": ([C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)], C.c_int),
     "vps_debug_chunking": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
     "vps_debug_gen_waves": ([_P, C.POINTER(C.c_int32)], C.c_int),
     "vps_debug_copy": ([C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
@@ -250,6 +251,12 @@ def debug_jump(D, x, y):
     xx, yy = C.c_uint64(x), C.c_uint64(y)
     _check(load().vps_debug_jump(C.c_uint64(D), C.byref(xx), C.byref(yy)))
     return xx.value, yy.value
+
+
+def debug_jump96(D, x, y, z):
+    xx, yy, zz = C.c_uint32(x), C.c_uint32(y), C.c_uint32(z)
+    _check(load().vps_debug_jump96(C.c_uint64(D), C.byref(xx), C.byref(yy), C.byref(zz)))
+    return xx.value, yy.value, zz.value
 
 
 def debug_copy(nbytes, iters=10, device=0):

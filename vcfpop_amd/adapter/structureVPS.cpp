@@ -72,7 +72,10 @@ void get_raw(int which, void* p, size_t bytes) { check(vps_chain_get_state(t_cha
 
 // the -structure options of this process (ReadPar, src/structure.cpp:405-464,
 // reads the same globals) in the engine's parameter block
-vps_params params_from_options()
+// real32: the run is a BAYESIAN<float> (-g_float=single); -g_fastsingle
+// then selects the single-precision genotype passes (src/parameters.cpp:
+// 443-444, 795-798)
+vps_params params_from_options(bool real32)
 {
     vps_params p;
     vps_params_default(&p);
@@ -113,7 +116,7 @@ vps_params params_from_options()
     p.krange_max = structure_krange_max;
     p.g_seed = (uint64_t)(int64_t)g_seed_val;
     p.rng_topology = VPS_RNG_REFERENCE;   // the reference's draw streams
-    p.precision = VPS_PREC_FP64;
+    p.precision = !real32 ? VPS_PREC_FP64 : g_fastsingle_val == 1 ? VPS_PREC_FAST32 : VPS_PREC_REAL32;
     return p;
 }
 
@@ -311,7 +314,7 @@ void BAYESIAN<REAL>::InitCUDA()
     if (t_chain) check(vps_chain_destroy(t_chain), "UninitCUDA");
     t_chain = nullptr;
     if (!g_dataset[t_device]) Exit("\nError: no genotype image on device %d (CopyStructureMemory).\n", t_device);
-    const vps_params p = params_from_options();
+    const vps_params p = params_from_options(std::is_same<REAL, float>::value);
     check(vps_chain_create(g_dataset[t_device], &p, K, id, &t_chain), "InitCUDA");
     const int64 NK = (int64)N * K, KKT = (int64)K * KT;
     put(VPS_STATE_FREQ, Freq, KKT);

@@ -44,6 +44,24 @@ __device__ __forceinline__ void jump_apply(const uint32_t* __restrict__ tab, Xs1
     r.y = (uint64_t)a2 | ((uint64_t)a3 << 32);
 }
 
+// the same for an xorshift96 stream (tables of jump_tables96: words 0..2)
+__device__ __forceinline__ void jump_apply(const uint32_t* __restrict__ tab, Xs96& r)
+{
+    uint32_t w0 = r.x, w1 = r.y, w2 = r.z;
+    uint32_t a0 = 0, a1 = 0, a2 = 0;
+    const uint4* t4 = reinterpret_cast<const uint4*>(tab);
+#pragma unroll 1
+    for (int q = 0; q < 3; ++q, t4 += 8 * 16) {
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+            const uint4 e = t4[p * 16 + ((w0 >> (p * 4)) & 15u)];
+            a0 ^= e.x; a1 ^= e.y; a2 ^= e.z;
+        }
+        w0 = w1; w1 = w2;
+    }
+    r.x = a0; r.y = a1; r.z = a2;
+}
+
 // Pointers and scalars of one chain, passed by value to every kernel.
 struct ChainDev {
     int32_t K, N, S, maxploidy;
@@ -101,6 +119,10 @@ struct ChainDev {
     const uint32_t* jump;
     int32_t   lik_tpr[2];   // tiles per locus range of the likelihood passes (BI, GEN tiles)
 };
+
+// REAL = float (BAYESIAN<float>): P, Q, the proposals and FreqM hold
+// single-precision values
+__host__ __device__ inline bool real32(const ChainDev& c) { return c.precision >= VPS_PREC_REAL32; }
 
 // partials of the grid-wide LogProds of UpdateAlpha and UpdateLambda (more
 // when S*K columns exceed it: one block per column then)

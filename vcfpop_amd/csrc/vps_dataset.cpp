@@ -490,20 +490,39 @@ extern "C" int vps_debug_jump(uint64_t D, uint64_t* x, uint64_t* y)
     return VPS_OK;
 }
 
+extern "C" int vps_debug_jump96(uint64_t D, uint32_t* x, uint32_t* y, uint32_t* z)
+{
+    if (!x || !y || !z) return fail(VPS_ERR_INVALID, "null argument");
+    // tables for chunks 0 (identity) and 1 (D steps); apply the second
+    std::vector<uint32_t> tab(2 * (size_t)kJumpWords);
+    jump_tables96(D, 2, tab.data());
+    const uint32_t st[3] = {*x, *y, *z};
+    uint32_t r[3] = {0, 0, 0};
+    const uint32_t* t = tab.data() + kJumpWords;
+    for (int p = 0; p < 24; ++p) {
+        const uint32_t nib = (st[p >> 3] >> ((p & 7) * 4)) & 15u;
+        for (int w = 0; w < 3; ++w) r[w] ^= t[(p * 16 + nib) * 4 + w];
+    }
+    *x = r[0]; *y = r[1]; *z = r[2];
+    return VPS_OK;
+}
+
 namespace vps {
 
-int dataset_jump(vps_dataset* ds, int Nc, const uint32_t** out)
+int dataset_jump(vps_dataset* ds, int Nc, bool x96, const uint32_t** out)
 {
     std::lock_guard<std::mutex> lock(ds->jump_mu);
-    auto it = ds->jump_cache.find(Nc);
+    const int key = 2 * Nc + (x96 ? 1 : 0);
+    auto it = ds->jump_cache.find(key);
     if (it != ds->jump_cache.end()) { *out = it->second; return VPS_OK; }
     const int G = (ds->d.N + Nc - 1) / Nc;
     std::vector<uint32_t> tab((size_t)G * kJumpWords);
-    jump_tables((uint64_t)Nc * ds->d.maxploidy, G, tab.data());
+    if (x96) jump_tables96((uint64_t)Nc * ds->d.maxploidy, G, tab.data());
+    else jump_tables((uint64_t)Nc * ds->d.maxploidy, G, tab.data());
     uint32_t* dev = nullptr;
     int rc = upload(&dev, tab, ds);
     if (rc) return rc;
-    ds->jump_cache[Nc] = dev;
+    ds->jump_cache[key] = dev;
     *out = dev;
     return VPS_OK;
 }

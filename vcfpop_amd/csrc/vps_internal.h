@@ -153,11 +153,15 @@ int fail(int code, const char* fmt, ...);
 void jump_table(uint64_t D, uint32_t* out /* kJumpWords */);
 // tables for 0, D, 2D, ..., (count-1)D steps
 void jump_tables(uint64_t D, int count, uint32_t* out /* count * kJumpWords */);
+// the same for xorshift96 (RNG<float>), state words x, y, z in entries 0..2
+void jump_tables96(uint64_t D, int count, uint32_t* out /* count * kJumpWords */);
 // largest individual chunk of the genotype passes (a block keeps
 // kMaxChunk x VPS_MAX_K cluster counts in 32 KB of LDS)
 constexpr int kMaxChunk = 512;
 // device jump tables of a chunk size, built once per dataset (vps_dataset.cpp)
-int dataset_jump(vps_dataset* ds, int Nc, const uint32_t** out);
+// the chunk jump tables of the Z pass streams for chunks of Nc individuals
+// (x96: the xorshift96 streams of REAL = float with -g_fastsingle), cached
+int dataset_jump(vps_dataset* ds, int Nc, bool x96, const uint32_t** out);
 // apply a jump table to a state on the host (tests)
 void jump_apply_host(const uint32_t* tab, uint64_t& x, uint64_t& y);
 

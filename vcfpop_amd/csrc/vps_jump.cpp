@@ -1,4 +1,4 @@
-// vps_jump.cpp — jump-ahead for the xorshift128+ stream.
+// vps_jump.cpp — jump-ahead for the xorshift128+ and xorshift96 streams.
 //
 // The per-locus stream of the Z pass draws one number per (individual,
 // slot) in individual order (src/structure.cpp:1386-1413).  To split one
@@ -98,6 +98,72 @@ void jump_tables(uint64_t D, int count, uint32_t* out)
                 e[2] = (uint32_t)acc.hi; e[3] = (uint32_t)(acc.hi >> 32);
             }
         R = mulm(M, R);
+    }
+}
+
+// xorshift96 (RNG<float>): the 96-bit state (x, y, z) in the same table
+// layout, state words 0..2 (nibbles 0..23; the entries of nibbles 24..31 are
+// zero)
+namespace {
+
+struct M96 { uint32_t col[96][3]; };
+
+M96 identity96()
+{
+    M96 I;
+    memset(&I, 0, sizeof I);
+    for (int j = 0; j < 96; ++j) I.col[j][j >> 5] = 1u << (j & 31);
+    return I;
+}
+
+void mulv96(const M96& A, const uint32_t* v, uint32_t* r)
+{
+    r[0] = r[1] = r[2] = 0;
+    for (int b = 0; b < 96; ++b)
+        if ((v[b >> 5] >> (b & 31)) & 1) { r[0] ^= A.col[b][0]; r[1] ^= A.col[b][1]; r[2] ^= A.col[b][2]; }
+}
+
+M96 mulm96(const M96& A, const M96& B)
+{
+    M96 C;
+    for (int j = 0; j < 96; ++j) mulv96(A, B.col[j], C.col[j]);
+    return C;
+}
+
+M96 step_matrix96()
+{
+    M96 T = identity96();
+    for (int j = 0; j < 96; ++j) {
+        Xs96 r{T.col[j][0], T.col[j][1], T.col[j][2]};
+        r.next();
+        T.col[j][0] = r.x; T.col[j][1] = r.y; T.col[j][2] = r.z;
+    }
+    return T;
+}
+
+}  // namespace
+
+void jump_tables96(uint64_t D, int count, uint32_t* out)
+{
+    M96 P = step_matrix96(), M = identity96();
+    for (uint64_t d = D; d; d >>= 1) {
+        if (d & 1) M = mulm96(P, M);
+        P = mulm96(P, P);
+    }
+    M96 R = identity96();
+    for (int c = 0; c < count; ++c) {
+        uint32_t* o = out + (size_t)c * kJumpWords;
+        memset(o, 0, kJumpWords * sizeof(uint32_t));
+        for (int p = 0; p < 24; ++p)
+            for (int v = 0; v < 16; ++v) {
+                uint32_t acc[3] = {0, 0, 0};
+                for (int b = 0; b < 4; ++b)
+                    if ((v >> b) & 1)
+                        for (int w = 0; w < 3; ++w) acc[w] ^= R.col[4 * p + b][w];
+                uint32_t* e = o + (p * 16 + v) * 4;
+                e[0] = acc[0]; e[1] = acc[1]; e[2] = acc[2]; e[3] = 0;
+            }
+        R = mulm96(M, R);
     }
 }
 

@@ -12,6 +12,10 @@
 //           half the FP64 instructions, no division)
 //   mode 3  no-admix log likelihood  b_ik = sum log P[k][l][x]
 //           (UpdateQNoAdmixAVX, src/structureAVX.cpp:254-428)
+//   mode 5  mode 0 with the mixture formed in single precision, and
+//   mode 6  mode 2 with both mixtures and their ratio in single precision:
+//           REAL = float with -g_fastsingle (src/structureAVX.cpp:725-760,
+//           1148-1168); the factors are then charged as doubles
 // sums run over typed allele copies x of individual i at locus l.
 //
 // These passes draw no random numbers, so lanes map to individuals: a lane
@@ -58,7 +62,7 @@ __device__ void lik_load_q(const ChainDev& c, int i, double* q, double* qp)
         if (MODE == 1) q[k] = c.Z[i] == k ? 1.0 : 0.0;   // one-hot: sum_k q_k p_k == p_Z exactly
         else if (MODE != 3) q[k] = c.Q[(int64_t)i * K + k];
         else q[k] = 0.0;
-        qp[k] = (MODE == 2 || MODE == 4) ? c.bufNK1[(int64_t)i * K + k] : 0.0;
+        qp[k] = (MODE == 2 || MODE == 4 || MODE == 6) ? c.bufNK1[(int64_t)i * K + k] : 0.0;
     }
 }
 
@@ -66,12 +70,12 @@ template <int K, int MODE>
 __device__ void lik_write(const DeviceDataset& ds, const ChainDev& c, FrexpAcc* acc, bool ok, int i, int64_t range,
                           int64_t block_slot)
 {
-    if (MODE <= 1) {
+    if (MODE <= 1 || MODE == 5) {
         block_store_sum(ok ? acc[0].close() : 0.0, c.part + block_slot);
-    } else if (MODE == 2 || MODE == 4) {
+    } else if (MODE == 2 || MODE == 4 || MODE == 6) {
         if (ok) {
             const double cur = acc[1].close();
-            c.part[range * ds.N + i] = MODE == 2 ? acc[0].close() : acc[0].close() - cur;
+            c.part[range * ds.N + i] = MODE != 4 ? acc[0].close() : acc[0].close() - cur;
             c.part_cur[range * ds.N + i] = cur;
         }
     } else {
@@ -85,7 +89,7 @@ __device__ void lik_write(const DeviceDataset& ds, const ChainDev& c, FrexpAcc* 
 
 // number of accumulators per lane
 template <int K, int MODE>
-struct LikNA { static constexpr int value = MODE == 3 ? K : ((MODE == 2 || MODE == 4) ? 2 : 1); };
+struct LikNA { static constexpr int value = MODE == 3 ? K : ((MODE == 2 || MODE == 4 || MODE == 6) ? 2 : 1); };
 
 // biallelic tiles (at most two alleles; genotype bytes in ds.bitab, or in
 // the tile's ds.btab columns above 3 bits per id).  MPT is
@@ -179,6 +183,30 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, 
 #pragma unroll
                             for (int k = 0; k < K; ++k)
                                 acc[k].charge((on0 ? (hi0 ? p1[k] : p0[k]) : 1.0) * (on1 ? (hi1 ? p1[k] : p0[k]) : 1.0));
+                        }
+                    } else if (MODE == 5 || MODE == 6) {
+                        // single-precision mixtures of single-precision P and
+                        // Q, and (mode 6) their ratio in single precision
+                        float f0 = (float)p0[0] * (float)q[0], f1 = (float)p1[0] * (float)q[0];
+                        float g0 = MODE == 6 ? (float)p0[0] * (float)qp[0] : 0.f;
+                        float g1 = MODE == 6 ? (float)p1[0] * (float)qp[0] : 0.f;
+#pragma unroll
+                        for (int k = 1; k < K; ++k) {
+                            f0 = f0 + (float)p0[k] * (float)q[k];
+                            f1 = f1 + (float)p1[k] * (float)q[k];
+                            if (MODE == 6) {
+                                g0 = g0 + (float)p0[k] * (float)qp[k];
+                                g1 = g1 + (float)p1[k] * (float)qp[k];
+                            }
+                        }
+                        const double d0 = (double)f0, d1 = (double)f1;
+                        const double r0 = MODE == 6 ? (double)(g0 / f0) : 0.0, r1 = MODE == 6 ? (double)(g1 / f1) : 0.0;
+#pragma unroll kPairs
+                        for (int a = 0; a < ncopies; a += 2) {
+                            const bool on0 = a < v, hi0 = a >= h, on1 = a + 1 < v, hi1 = a + 1 >= h;
+                            if (MODE == 6)
+                                acc[0].charge((on0 ? (hi0 ? r1 : r0) : 1.0) * (on1 ? (hi1 ? r1 : r0) : 1.0));
+                            acc[MODE == 6 ? 1 : 0].charge((on0 ? (hi0 ? d1 : d0) : 1.0) * (on1 ? (hi1 ? d1 : d0) : 1.0));
                         }
                     } else {
                         // sums start from the k = 0 term (0 + x == x for the
@@ -282,6 +310,20 @@ __global__ __launch_bounds__(kLikThreads) void k_lik_gen(const DeviceDataset ds,
                     if (MODE == 3) {
 #pragma unroll
                         for (int k = 0; k < K; ++k) acc[k].charge(px[k * KT]);
+                    } else if (MODE == 5 || MODE == 6) {
+                        float f = 0.f, g = 0.f;
+#pragma unroll
+                        for (int k = 0; k < K; ++k) {
+                            const float p = (float)px[k * KT];
+                            f = k ? f + p * (float)q[k] : p * (float)q[k];
+                            if (MODE == 6) g = k ? g + p * (float)qp[k] : p * (float)qp[k];
+                        }
+                        if (MODE == 6) {
+                            acc[0].charge((double)(g / f));
+                            acc[1].charge((double)f);
+                        } else {
+                            acc[0].charge((double)f);
+                        }
                     } else {
                         // (rounding as on BI tiles: separate products and sums
                         // and a per-copy ratio in mode 2, fused in mode 4)
@@ -499,8 +541,8 @@ int lik_part_count(const DeviceDataset& ds, int mode, int K, const int32_t* tpr)
 {
     const LikGrid g = lik_grid(ds, tpr);
     const int64_t nr = (int64_t)g.nr_bi + g.nr_gen;
-    if (mode <= 1) return (int)(nr * g.gy);
-    return (int)(nr * ds.N * (mode == 3 ? K : 2));   // mode 2: ratios, then current-Q partials
+    if (mode <= 1 || mode == 5) return (int)(nr * g.gy);
+    return (int)(nr * ds.N * (mode == 3 ? K : 2));   // modes 2, 4, 6: ratios, then current-Q partials
 }
 
 int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c0, int mode, hipStream_t st, double* lnl_out)
@@ -512,9 +554,13 @@ int launch_lik_pass(const DeviceDataset& ds, const ChainDev& c0, int mode, hipSt
     c.part_cur = c0.part + (int64_t)nranges * ds.N;
     int rc;
     switch (mode) {
-    case 0: rc = launch_lik_mode<0>(ds, c, st, g); break;
+    case 0: rc = c.precision == VPS_PREC_FAST32 ? launch_lik_mode<5>(ds, c, st, g) : launch_lik_mode<0>(ds, c, st, g); break;
     case 1: rc = launch_lik_mode<1>(ds, c, st, g); break;
-    case 2: rc = c.precision == VPS_PREC_MIXED ? launch_lik_mode<4>(ds, c, st, g) : launch_lik_mode<2>(ds, c, st, g); break;
+    case 2:
+        rc = c.precision == VPS_PREC_MIXED    ? launch_lik_mode<4>(ds, c, st, g)
+             : c.precision == VPS_PREC_FAST32 ? launch_lik_mode<6>(ds, c, st, g)
+                                              : launch_lik_mode<2>(ds, c, st, g);
+        break;
     case 3: rc = launch_lik_mode<3>(ds, c, st, g); break;
     default: return fail(VPS_ERR_INVALID, "likelihood mode %d", mode);
     }

@@ -112,6 +112,39 @@ struct Xs128 {
     }
 };
 
+// MurmurHash32, src/hash.cpp:462-469
+VPS_HD uint32_t murmur32(uint64_t data, uint64_t seed)
+{
+    const uint32_t a = (~(uint32_t)data) ^ (uint32_t)(data >> 32);
+    const uint32_t b = (~(uint32_t)seed) ^ (uint32_t)(seed >> 32);
+    return murmur2(a, b);
+}
+
+// RNG<float>: xorshift96 with shifts (16, 5, 1), seeded by MurmurHash32
+// (src/statistics.cpp:330-354); the Z pass's per-locus streams when
+// REAL = float with -g_fastsingle
+struct Xs96 {
+    uint32_t x, y, z;
+    VPS_HD void seed(uint64_t s, uint64_t salt)
+    {
+        const uint32_t h = murmur32(s, salt);
+        x = 0x075BCD15u ^ h;
+        y = 0x159A55E5u ^ (h << 3);
+        z = 0x1F123BB5u ^ (h << 6);
+    }
+    VPS_HD uint32_t next()
+    {
+        uint32_t t = x;
+        t ^= t << 16;
+        t ^= t >> 5;
+        t ^= t << 1;
+        x = y;
+        y = z;
+        z = t ^ x ^ y;
+        return z;
+    }
+};
+
 // U in [1, 2) from the 52 low bits of a draw
 VPS_HD double unit12(uint64_t r) { return as_f64((r & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull); }
 
@@ -235,8 +268,27 @@ struct Rng {
 
 // Dirichlet(res, a, b, n): Gamma(a[i] + b[i]) then scale by 1/sum,
 // src/statistics.cpp:284-296
+// Scales a row of n Gamma draws v (sum s, added in row order) into its
+// Dirichlet draw.  real32: the row is a REAL = float array of BAYESIAN<float>:
+// v is stored as float and multiplied by (float)(1/s) in single precision
+// (Mul(float*, float, n), src/math2.cpp:2088-2102); the values stay doubles
+// holding single-precision numbers.
+VPS_HD void dirichlet_scale(double* res, double s, int n, bool real32)
+{
+    if (real32) {
+        const float inv = (float)(1.0 / s);
+        for (int i = 0; i < n; ++i) res[i] = (double)((float)res[i] * inv);
+    } else {
+        const double inv = 1.0 / s;
+        for (int i = 0; i < n; ++i) res[i] *= inv;
+    }
+}
+
+// a value stored into a REAL array (real32: rounded to float)
+VPS_HD double real_store(double v, bool real32) { return real32 ? (double)(float)v : v; }
+
 template <typename TB>
-VPS_HD void dirichlet_ab(Rng& r, double* res, const double* a, const TB* b, int n)
+VPS_HD void dirichlet_ab(Rng& r, double* res, const double* a, const TB* b, int n, bool real32)
 {
     double s = 0;
     for (int i = 0; i < n; ++i) {
@@ -244,12 +296,11 @@ VPS_HD void dirichlet_ab(Rng& r, double* res, const double* a, const TB* b, int 
         res[i] = v;
         s += v;
     }
-    double inv = 1.0 / s;
-    for (int i = 0; i < n; ++i) res[i] *= inv;
+    dirichlet_scale(res, s, n, real32);
 }
 
 // Dirichlet(res, a, n, f = 1): Gamma(a[i] * f), src/statistics.cpp:269-281
-VPS_HD void dirichlet_a(Rng& r, double* res, const double* a, int n)
+VPS_HD void dirichlet_a(Rng& r, double* res, const double* a, int n, bool real32)
 {
     double s = 0;
     for (int i = 0; i < n; ++i) {
@@ -257,8 +308,7 @@ VPS_HD void dirichlet_a(Rng& r, double* res, const double* a, int n)
         res[i] = v;
         s += v;
     }
-    double inv = 1.0 / s;
-    for (int i = 0; i < n; ++i) res[i] *= inv;
+    dirichlet_scale(res, s, n, real32);
 }
 
 // PolyLog(a, n), src/statistics.cpp:116-155 (overwrites a), with its one

@@ -145,7 +145,7 @@ int tune_chunks(vps_chain* ch)
         ChainDev t = c;
         t.Nc = (N + cand[x] - 1) / cand[x];
         t.G = (N + t.Nc - 1) / t.Nc;
-        rc = dataset_jump(ch->ds, t.Nc, &t.jump);
+        rc = dataset_jump(ch->ds, t.Nc, c.precision == VPS_PREC_FAST32, &t.jump);
         if (rc) break;
         t.Ni_part = tmp;
         t.Ni = tmp + (size_t)Gmax * c.K * c.KT;
@@ -187,7 +187,7 @@ int tune_chunks(vps_chain* ch)
     if (rc) return rc;
     c.Nc = (N + bestG - 1) / bestG;
     c.G = (N + c.Nc - 1) / c.Nc;
-    rc = dataset_jump(ch->ds, c.Nc, &c.jump);
+    rc = dataset_jump(ch->ds, c.Nc, c.precision == VPS_PREC_FAST32, &c.jump);
     if (!rc) rc = chain_alloc(ch, &c.Ni_part, (int64_t)c.G * c.K * c.KT);
     if (!rc) rc = chain_alloc(ch, &c.zlik, z_lik_parts(d, c.G));
     return rc;
@@ -203,8 +203,10 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     if (p->nthinning < 1) return fail(VPS_ERR_INVALID, "nthinning must be >= 1");
     if (p->rng_topology != VPS_RNG_REFERENCE && p->rng_topology != VPS_RNG_PARALLEL)
         return fail(VPS_ERR_INVALID, "rng_topology %d", p->rng_topology);
-    if (p->precision != VPS_PREC_FP64 && p->precision != VPS_PREC_MIXED)
+    if (p->precision < VPS_PREC_FP64 || p->precision > VPS_PREC_FAST32)
         return fail(VPS_ERR_INVALID, "precision %d", p->precision);
+    if (p->precision >= VPS_PREC_REAL32 && p->f_val == 1)
+        return fail(VPS_ERR_UNSUPPORTED, "the F model runs with REAL = double only (precision FP64 or MIXED)");
     const DeviceDataset& d = ds->d;
     VPS_HIP(hipSetDevice(ds->device));
 
@@ -262,7 +264,7 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
         if (!rc) {
             c.Nc = (int32_t)((N + G - 1) / G);
             c.G = (int32_t)((N + c.Nc - 1) / c.Nc);
-            rc = dataset_jump(ds, c.Nc, &c.jump);
+            rc = dataset_jump(ds, c.Nc, p->precision == VPS_PREC_FAST32, &c.jump);
         }
     }
     if (!rc) rc = chain_alloc(ch, &c.Ni_part, (int64_t)c.G * K * KT);
@@ -314,13 +316,14 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
         for (int64_t l = 0; l < d.L; ++l)
             if (ds->k2[l] != 2)
                 for (int k = 0; k < K; ++k)
-                    for (int a = 0; a < ds->k2[l]; ++a) fr[(size_t)k * KT + ds->afo[l] + a] = 1.0 / ds->k2[l];
+                    for (int a = 0; a < ds->k2[l]; ++a)
+                        fr[(size_t)k * KT + ds->afo[l] + a] = real_store(1.0 / ds->k2[l], p->precision >= VPS_PREC_REAL32);
         // InitFmodel (src/structure.cpp:679-708) sets P = Lambda; UpdateP
         // overwrites P before any use, so the value is immaterial
         e = hipMemcpy(c.Freq, fr.data(), fr.size() * 8, hipMemcpyHostToDevice);
         if (e != hipSuccess) { chain_free(ch); return fail(VPS_ERR_DEVICE, "upload P: %s", hipGetErrorString(e)); }
     }
-    rc = fill_f64(ch, c.Q, N * K, 1.0 / K);
+    rc = fill_f64(ch, c.Q, N * K, real_store(1.0 / K, p->precision >= VPS_PREC_REAL32));
     if (!rc) rc = fill_f64(ch, c.Alpha, K, p->alpha);
     if (!rc) rc = fill_f64(ch, c.Lambda, K, p->lambda);
     if (!rc) rc = fill_f64(ch, c.FreqM, K * KT, 0.0);
@@ -514,9 +517,12 @@ int launch_sweep(vps_chain* ch, const SweepPlan& p)
     // Record's admixture lnL: after a Metropolis UpdateQ, the sum of the
     // per-individual log likelihoods that pass left under the kept Q (the
     // same P and Q Record reads); otherwise from the FP64 Z pass itself, whose
-    // weights start from the same products P[k][l][x] Q[i][k]
-    const bool mrec = p.record && p.q == kQMetro;
-    const bool zrec = p.record && !mrec && p.q != kQNoAdmix && c.precision != VPS_PREC_MIXED;
+    // weights start from the same products P[k][l][x] Q[i][k].  With
+    // single-precision ratios (FAST32) the kept sum would carry their
+    // rounding, so Record takes its own pass.
+    const bool mrec = p.record && p.q == kQMetro && c.precision != VPS_PREC_FAST32;
+    const bool zrec = p.record && !mrec && p.q != kQNoAdmix && c.precision != VPS_PREC_MIXED &&
+                      c.precision != VPS_PREC_FAST32;
     // without Record, the Z pass's last kernel advances the sweep index when
     // nothing beside it still reads the index
     const bool zadv = !p.record && p.q != kQNoAdmix && (fast || side == ch->stream);

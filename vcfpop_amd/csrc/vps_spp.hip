@@ -121,7 +121,7 @@ __device__ __forceinline__ double* draw_target(const DeviceDataset& ds, const Ch
 {
     if (kind == kDrawP) {
         const int64_t k = g / ds.KT, x = g - k * ds.KT;
-        const double prior = c.fmodel ? c.FreqA[x] * c.Fst[c.K + (c.fsame ? 0 : k)] : c.Lambda[k];
+        const double prior = c.fmodel ? c.FreqA[x] * c.Fst[c.K + (c.fsame ? 0 : k)] : real_store(c.Lambda[k], real32(c));
         *alpha = prior + (double)c.Ni[g];
         return c.Freq + g;
     }
@@ -458,8 +458,7 @@ __global__ void k_spp_normalize(const DeviceDataset ds, const ChainDev c, int do
         const int k2 = ds.k2[l];
         double s = 0;
         for (int a = 0; a < k2; ++a) s += p[a];
-        const double inv = 1.0 / s;
-        for (int a = 0; a < k2; ++a) p[a] *= inv;
+        dirichlet_scale(p, s, k2, real32(c));
         return;
     }
     const int64_t i = x - np;
@@ -469,8 +468,7 @@ __global__ void k_spp_normalize(const DeviceDataset ds, const ChainDev c, int do
         double* r = (qkind == kDrawQ ? c.Q : c.bufNK1) + i * K;
         double s = 0;
         for (int k = 0; k < K; ++k) s += r[k];
-        const double inv = 1.0 / s;
-        for (int k = 0; k < K; ++k) r[k] *= inv;
+        dirichlet_scale(r, s, K, real32(c));
     }
     if (clear_mi)
         for (int k = 0; k < K; ++k) c.Mi[i * K + k] = 0;

@@ -109,7 +109,7 @@ extern "C" int vps_synth_create(const vps_synth_config* cfg, vps_synth** out)
     s->qtrue.resize((size_t)N * Kt);
     {
         std::vector<double> a(Kt, cfg->alpha_q);
-        for (int i = 0; i < N; ++i) dirichlet_a(ri, s->qtrue.data() + (size_t)i * Kt, a.data(), Kt);
+        for (int i = 0; i < N; ++i) dirichlet_a(ri, s->qtrue.data() + (size_t)i * Kt, a.data(), Kt, false);
     }
 
     // loci in parallel, each from its own stream
@@ -132,7 +132,7 @@ extern "C" int vps_synth_create(const vps_synth_config* cfg, vps_synth** out)
                 P.assign((size_t)Kt * k2, 0);
                 if (multi) {
                     a.assign(k2, cfg->multi_dir);
-                    for (int k = 0; k < Kt; ++k) dirichlet_a(r, P.data() + (size_t)k * k2, a.data(), k2);
+                    for (int k = 0; k < Kt; ++k) dirichlet_a(r, P.data() + (size_t)k * k2, a.data(), k2, false);
                 } else if (cfg->fst > 0) {
                     const double F = cfg->fst, p0 = r.uniform(0.05, 0.95);
                     for (int k = 0; k < Kt; ++k) {
@@ -141,7 +141,7 @@ extern "C" int vps_synth_create(const vps_synth_config* cfg, vps_synth** out)
                     }
                 } else {
                     a.assign(2, 1.0);
-                    for (int k = 0; k < Kt; ++k) dirichlet_a(r, P.data() + (size_t)k * 2, a.data(), 2);
+                    for (int k = 0; k < Kt; ++k) dirichlet_a(r, P.data() + (size_t)k * 2, a.data(), 2, false);
                 }
                 // cumulative allele distributions per cluster
                 cum.assign((size_t)Kt * k2, 0);

@@ -34,7 +34,8 @@ __device__ __forceinline__ void draw_p_one(const DeviceDataset& ds, const ChainD
     double* p = c.Freq + k * KT + afo;
     const int32_t* ni = c.Ni + k * KT + afo;
     const double fk = c.fmodel ? c.Fst[c.K + (c.fsame ? 0 : k)] : 0.0;
-    const double lam = c.Lambda[k];
+    // the prior is first stored in P (SetVal(ClusterFreq(k), (REAL)Lambda[k]))
+    const double lam = real_store(c.Lambda[k], real32(c));
     double s = 0;
     for (int a = 0; a < k2; ++a) {
         const double prior = c.fmodel ? c.FreqA[afo + a] * fk : lam;
@@ -42,8 +43,7 @@ __device__ __forceinline__ void draw_p_one(const DeviceDataset& ds, const ChainD
         p[a] = v;
         s += v;
     }
-    const double inv = 1.0 / s;
-    for (int a = 0; a < k2; ++a) p[a] *= inv;
+    dirichlet_scale(p, s, k2, real32(c));
 }
 
 __global__ void k_update_p_par(const DeviceDataset ds, const ChainDev c)
@@ -84,7 +84,7 @@ __global__ void k_q_admix(const DeviceDataset ds, const ChainDev c)
     if (i >= ds.N || !ds.vt[i]) return;
     Rng r;
     r.init(c.seed + (uint64_t)m * ds.N + i, kSaltUpdateQ);
-    dirichlet_ab(r, c.Q + (int64_t)i * c.K, alpha_row(ds, c, i), c.Mi + (int64_t)i * c.K, c.K);
+    dirichlet_ab(r, c.Q + (int64_t)i * c.K, alpha_row(ds, c, i), c.Mi + (int64_t)i * c.K, c.K, real32(c));
 }
 
 int launch_update_q_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
@@ -112,7 +112,7 @@ __global__ void k_update_pq(const DeviceDataset ds, const ChainDev c, unsigned n
         if (ds.vt[i]) {
             Rng r;
             r.init(c.seed + (uint64_t)m * ds.N + i, kSaltUpdateQ);
-            dirichlet_ab(r, c.Q + (int64_t)i * c.K, alpha_row(ds, c, i), mi, c.K);
+            dirichlet_ab(r, c.Q + (int64_t)i * c.K, alpha_row(ds, c, i), mi, c.K, real32(c));
         }
         for (int k = 0; k < c.K; ++k) mi[k] = 0;
         return;
@@ -147,7 +147,7 @@ __global__ void k_q_metro_propose(const DeviceDataset ds, const ChainDev c)
     if (i >= ds.N) return;
     Rng r;
     r.init(c.seed + (uint64_t)m * ds.N + i, kSaltUpdateQ);
-    if (ds.vt[i]) dirichlet_a(r, c.bufNK1 + (int64_t)i * c.K, alpha_row(ds, c, i), c.K);
+    if (ds.vt[i]) dirichlet_a(r, c.bufNK1 + (int64_t)i * c.K, alpha_row(ds, c, i), c.K, real32(c));
     c.rngbuf[i] = r;
 }
 
@@ -594,7 +594,7 @@ __device__ void update_a1_locus(const DeviceDataset& ds, const ChainDev& c, Rng&
     for (int a = 0; a < k2; ++a) pb[a] = lam;
     for (int k = 0; k < K; ++k)
         for (int a = 0; a < k2; ++a) pb[a] += c.Freq[k * KT + afo + a] * c.Fst[K + k];
-    dirichlet_a(r, pc, pb, k2);
+    dirichlet_a(r, pc, pb, k2, false);
     double dlnL = 0;
     for (int a = 0; a < k2; ++a) dlnL += (pb[a] - lam) * (my_log(pa[a] / pc[a]));
     for (int k = 0; k < K; ++k) {
@@ -781,7 +781,11 @@ __global__ void k_record_acc(const DeviceDataset ds, const ChainDev c, int post,
     const int K = c.K;
     const int64_t nmi = (int64_t)ds.N * K, nfr = (int64_t)K * ds.KT;
     if (acc_misum && x < nmi) c.MiSum[x] += c.Mi[x];
-    if (post && x < nfr) c.FreqM[x] += c.Freq[x];
+    if (post && x < nfr) {
+        // Add(FreqM, Freq): on float arrays when REAL = float
+        if (real32(c)) c.FreqM[x] = (double)((float)c.FreqM[x] + (float)c.Freq[x]);
+        else c.FreqM[x] += c.Freq[x];
+    }
     if (x != 0) return;
     const double lik = c.scal[1];
     if (trace_slot >= 0) c.lnltrace[trace_slot] = lik;
@@ -939,7 +943,11 @@ __global__ void k_arrange(const DeviceDataset ds, const ChainDev c, int nr)
         for (int k = 0; k < K; ++k) rd[k] = tmp[k];
     }
     const double inv = 1.0 / nr;
-    if (x < (int64_t)K * ds.KT) c.FreqM[x] *= inv;
+    if (x < (int64_t)K * ds.KT) {
+        // Mul(FreqM, (REAL)(1.0 / nr))
+        if (real32(c)) c.FreqM[x] = (double)((float)c.FreqM[x] * (float)inv);
+        else c.FreqM[x] *= inv;
+    }
     if (x == 0) {
         for (int j = 0; j < c.rlen; ++j) c.rout[j] *= inv;
         c.rout[2] = c.rout[1] - c.rout[0] * c.rout[0];

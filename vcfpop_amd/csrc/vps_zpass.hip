@@ -63,6 +63,18 @@ template <> __device__ __forceinline__ float unit_minus1<float>(uint64_t r, cons
 {
     return __builtin_bit_cast(float, bfi(ub.m23, (uint32_t)(r >> 29), ub.one32)) - 1.0f;
 }
+// REAL = float with -g_fastsingle: U from the low 23 bits of an xorshift96
+// output (RNGAVX<float>::Poly, src/mathAVX.cpp:357-365)
+template <typename Real> __device__ __forceinline__ Real unit_minus1(uint32_t r, const UnitBits& ub);
+template <> __device__ __forceinline__ float unit_minus1<float>(uint32_t r, const UnitBits& ub)
+{
+    return __builtin_bit_cast(float, bfi(ub.m23, r, ub.one32)) - 1.0f;
+}
+
+// the Z pass's per-locus streams: RNG<double> (xorshift128+), or RNG<float>
+// (xorshift96) for REAL = float with -g_fastsingle
+template <bool X96> struct ZStream { using T = Xs128; };
+template <> struct ZStream<true> { using T = Xs96; };
 
 __device__ __forceinline__ uint64_t ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
 
@@ -157,7 +169,7 @@ __device__ __forceinline__ void flush_bi_rows(const uint32_t* mi_lds, int nci, i
 
 // one wavefront's chunk of one BI tile; BIG: ids of more than 3 bits, the
 // genotype bytes come from the lane's LDS column `tab_lds`
-template <int K, typename Real, int MPT, bool BIG, bool REC>
+template <int K, typename Real, int MPT, bool BIG, bool REC, bool X96>
 __device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileDesc& t, const double* __restrict__ Q,
                                           uint64_t seed, int m, int64_t l, bool valid, int afo, int k2, int lane,
                                           int c, int i0, int i1, const Real* pl0, uint8_t* tab_lds,
@@ -184,7 +196,7 @@ __device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileD
             tab = st;
         }
     }
-    Xs128 rng;
+    typename ZStream<X96>::T rng;
     rng.seed(seed + (uint64_t)m * (uint64_t)ds.L + (uint64_t)l, kSaltUpdateZ);
     if (c > 0) jump_apply(jump + (size_t)c * kJumpWords, rng);
 
@@ -244,7 +256,7 @@ __device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileD
             constexpr int kSlotUnroll = MPT > 0 ? MPT : 1;
 #pragma unroll kSlotUnroll
             for (int a = 0; a < MP; ++a) {
-                const uint64_t r = rng.next();
+                const auto r = rng.next();
                 const bool typed = (MPT == 2 || MPT == 4) ? db >= (uint32_t)(a + 1) * 16u : a < v;
                 const uint64_t tmask = ballot(typed);
                 // a slot no lane types (an individual below maxploidy) only
@@ -347,7 +359,7 @@ inline int bi_row_words(const DeviceDataset& ds) { return ds.maxploidy == 2 ? mi
 // XCD takes a contiguous run of items, and the chunks of one tile group run
 // back to back on it and read the group's P from its L2 rather than HBM.
 // REC: each block also writes the Record lnL partial of its lanes to zlik[w].
-template <int K, typename Real, int MPT, bool REC>
+template <int K, typename Real, int MPT, bool REC, bool X96>
 __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, const double* __restrict__ Freq,
                                                    const double* __restrict__ Q, uint64_t seed,
                                                    const int32_t* __restrict__ mp,
@@ -402,10 +414,10 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
     double lnl = 0.0;
     if (has_tile) {
         if (t.bits > kBiRegBits)
-            lnl = z_bi_tile<K, Real, MPT, true, REC>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane,
+            lnl = z_bi_tile<K, Real, MPT, true, REC, X96>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane,
                                                     tab_lds, mi_lds, Ni_part, jump);
         else
-            lnl = z_bi_tile<K, Real, MPT, false, REC>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane,
+            lnl = z_bi_tile<K, Real, MPT, false, REC, X96>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane,
                                                      tab_lds, mi_lds, Ni_part, jump);
     }
     __shared__ double wsum[kWavesMax];
@@ -441,7 +453,7 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
 //
 // REC: each block also writes the Record lnL partial of its lanes to
 // zlik[zbase + blockIdx.y * gridDim.x + blockIdx.x].
-template <int K, typename Real, bool PLDS, bool REC>
+template <int K, typename Real, bool PLDS, bool REC, bool X96>
 __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceDataset ds, const double* __restrict__ Freq,
                                                     const double* __restrict__ Q, uint64_t seed,
                                                     const int32_t* __restrict__ mp,
@@ -490,7 +502,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceData
     double lnl = 0.0;   // REC: the lane's Record lnL over its chunk
     if (i0 < i1) {
         const uint64_t* gd = ds.gdesc + ds.gbase[l];
-        Xs128 rng;
+        typename ZStream<X96>::T rng;
         rng.seed(seed + (uint64_t)m * (uint64_t)ds.L + (uint64_t)l, kSaltUpdateZ);
         if (c > 0) jump_apply(jump + (size_t)c * kJumpWords, rng);
 
@@ -535,7 +547,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceData
             for (int k = 0; k < K; ++k) cum[k] = 0;
 
             for (int a = 0; a < MP; ++a) {
-                const uint64_t r = rng.next();
+                const auto r = rng.next();
                 const bool typed = a < v;
                 const uint64_t tmask = ballot(typed);
                 if (tmask == 0) continue;   // no lane types this slot: the draw only
@@ -782,7 +794,7 @@ GenPlan gen_plan(const DeviceDataset& ds, int Nc, int G)
 }
 
 // nparts (may be null): the number of Record partials written
-template <int K, typename Real, bool REC>
+template <int K, typename Real, bool REC, bool X96>
 int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, int64_t* nparts)
 {
     const int nthr = 64 * kWavesPerBlock;
@@ -806,23 +818,23 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st,
         zbase = items;
         const size_t shm = bi_p_bytes<K, Real>(kWavesPerBlock) + bi_tab_bytes(kWavesPerBlock, ds.bi_tab_rows) + mi_packed;
         if (ds.maxploidy == 2)
-            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 2, REC>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 2, REC, X96>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
                                c.Ni_part, c.Mi, c.Nc, c.G, c.jump, c.zlik);
         else if (ds.maxploidy == 4)
-            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 4, REC>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 4, REC, X96>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
                                c.Ni_part, c.Mi, c.Nc, c.G, c.jump, c.zlik);
         else
-            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 0, REC>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 0, REC, X96>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
                                c.Ni_part, c.Mi, c.Nc, c.G, c.jump, c.zlik);
         VPS_HIP(hipGetLastError());
     }
     if (ds.ngen > 0) {
         dim3 grid((unsigned)ds.ngen, (unsigned)((c.G + g.waves - 1) / g.waves));
         if (g.plds)
-            hipLaunchKernelGGL((k_z_admix_gen<K, Real, true, REC>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q,
+            hipLaunchKernelGGL((k_z_admix_gen<K, Real, true, REC, X96>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q,
                                c.seed, c.mdev, c.Ni_part, c.Mi, A, 1, c.Nc, c.G, c.jump, c.zlik, zbase);
         else
-            hipLaunchKernelGGL((k_z_admix_gen<K, Real, false, REC>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q,
+            hipLaunchKernelGGL((k_z_admix_gen<K, Real, false, REC, X96>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q,
                                c.seed, c.mdev, c.Ni_part, c.Mi, A, g.ni_in_lds ? 1 : 0, c.Nc, c.G, c.jump, c.zlik, zbase);
         VPS_HIP(hipGetLastError());
         zbase += (int64_t)grid.x * grid.y;
@@ -833,7 +845,7 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st,
 
 // resident blocks per CU of the BI kernel, and resident GEN wavefronts per
 // SIMD (GEN blocks may hold fewer than 4 wavefronts), for chunks of Nc
-template <int K, typename Real>
+template <int K, typename Real, bool X96>
 int z_occupancy(const DeviceDataset& ds, int Nc, int* bi_blocks, int* gen_waves, int* gen_block_waves)
 {
     const size_t mi_packed = (size_t)Nc * bi_row_words<K>(ds) * sizeof(uint32_t);
@@ -841,16 +853,16 @@ int z_occupancy(const DeviceDataset& ds, int Nc, int* bi_blocks, int* gen_waves,
     *bi_blocks = 1;
     hipError_t e = hipSuccess;
     if (ds.maxploidy == 2)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 2, false>, 256, p_bytes + mi_packed);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 2, false, X96>, 256, p_bytes + mi_packed);
     else if (ds.maxploidy == 4)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 4, false>, 256, p_bytes + mi_packed);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 4, false, X96>, 256, p_bytes + mi_packed);
     else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 0, false>, 256, p_bytes + mi_packed);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(bi_blocks, (const void*)k_z_admix_bi<K, Real, 0, false, X96>, 256, p_bytes + mi_packed);
     if (e != hipSuccess) return fail(VPS_ERR_DEVICE, "occupancy query: %s", hipGetErrorString(e));
     const GenPlan g = gen_plan<K, Real>(ds, Nc, (ds.N + Nc - 1) / Nc);
     int gb = 1;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&gb, g.plds ? (const void*)k_z_admix_gen<K, Real, true, false>
-                                                                 : (const void*)k_z_admix_gen<K, Real, false, false>,
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&gb, g.plds ? (const void*)k_z_admix_gen<K, Real, true, false, X96>
+                                                                 : (const void*)k_z_admix_gen<K, Real, false, false, X96>,
                                                      64 * g.waves, g.shm);
     if (e != hipSuccess) return fail(VPS_ERR_DEVICE, "occupancy query: %s", hipGetErrorString(e));
     *bi_blocks = std::max(*bi_blocks, 1);
@@ -859,11 +871,11 @@ int z_occupancy(const DeviceDataset& ds, int Nc, int* bi_blocks, int* gen_waves,
     return VPS_OK;
 }
 
-template <typename Real>
+template <typename Real, bool X96>
 int z_occupancy_k(const DeviceDataset& ds, int K, int Nc, int* b, int* g, int* gw)
 {
     switch (K) {
-#define VPS_CASE(k) case k: return z_occupancy<k, Real>(ds, Nc, b, g, gw);
+#define VPS_CASE(k) case k: return z_occupancy<k, Real, X96>(ds, Nc, b, g, gw);
         VPS_CASE(1) VPS_CASE(2) VPS_CASE(3) VPS_CASE(4) VPS_CASE(5) VPS_CASE(6) VPS_CASE(7) VPS_CASE(8)
         VPS_CASE(9) VPS_CASE(10) VPS_CASE(11) VPS_CASE(12) VPS_CASE(13) VPS_CASE(14) VPS_CASE(15) VPS_CASE(16)
 #undef VPS_CASE
@@ -871,16 +883,34 @@ int z_occupancy_k(const DeviceDataset& ds, int K, int Nc, int* b, int* g, int* g
     }
 }
 
-template <typename Real, bool REC>
+template <typename Real, bool REC, bool X96 = false>
 int dispatch_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, int64_t* nparts = nullptr)
 {
     switch (c.K) {
-#define VPS_CASE(k) case k: return launch_z_admix_k<k, Real, REC>(ds, c, st, nparts);
+#define VPS_CASE(k) case k: return launch_z_admix_k<k, Real, REC, X96>(ds, c, st, nparts);
         VPS_CASE(1) VPS_CASE(2) VPS_CASE(3) VPS_CASE(4) VPS_CASE(5) VPS_CASE(6) VPS_CASE(7) VPS_CASE(8)
         VPS_CASE(9) VPS_CASE(10) VPS_CASE(11) VPS_CASE(12) VPS_CASE(13) VPS_CASE(14) VPS_CASE(15) VPS_CASE(16)
 #undef VPS_CASE
     default: return fail(VPS_ERR_UNSUPPORTED, "K=%d outside 1..%d", c.K, VPS_MAX_K);
     }
+}
+
+// occupancy of the Z kernels a precision mode launches
+int z_occupancy_prec(const DeviceDataset& ds, int K, int precision, int Nc, int* b, int* g, int* gw)
+{
+    if (precision == VPS_PREC_MIXED) return z_occupancy_k<float, false>(ds, K, Nc, b, g, gw);
+    if (precision == VPS_PREC_FAST32) return z_occupancy_k<float, true>(ds, K, Nc, b, g, gw);
+    return z_occupancy_k<double, false>(ds, K, Nc, b, g, gw);
+}
+
+// the count-only Z pass of a precision mode: FP64 weights (FP64, REAL32),
+// FP32 weights from the xorshift128+ draws (MIXED) or from xorshift96
+// streams (FAST32)
+int dispatch_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
+{
+    if (c.precision == VPS_PREC_MIXED) return dispatch_k<float, false>(ds, c, st);
+    if (c.precision == VPS_PREC_FAST32) return dispatch_k<float, false, true>(ds, c, st);
+    return dispatch_k<double, false>(ds, c, st);
 }
 
 }  // namespace
@@ -914,8 +944,7 @@ int choose_chunks(const DeviceDataset& ds, int K, int precision, int device, int
         const int Gr = (N + Nc - 1) / Nc;
         // occupancy as wavefronts per SIMD (a 4-wave BI block puts one on each)
         int bb = 1, gb = 1, gw = kWavesPerBlock;
-        int rc = precision == VPS_PREC_MIXED ? z_occupancy_k<float>(ds, K, Nc, &bb, &gb, &gw)
-                                             : z_occupancy_k<double>(ds, K, Nc, &bb, &gb, &gw);
+        int rc = z_occupancy_prec(ds, K, precision, Nc, &bb, &gb, &gw);
         if (rc) return rc;
         const double work = ((double)Nc * ds.maxploidy + 8.0) * slot_us;
         double t = (double)Gr * K * ds.KT * 8.0 / hbm_bytes_per_us;
@@ -946,7 +975,7 @@ int launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st)
 int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     int rc = launch_fill_i32(c.Mi, (int64_t)ds.N * c.K, 0, st);
-    if (!rc) rc = c.precision == VPS_PREC_MIXED ? dispatch_k<float, false>(ds, c, st) : dispatch_k<double, false>(ds, c, st);
+    if (!rc) rc = dispatch_counts(ds, c, st);
     if (!rc) rc = launch_reduce_ni(ds, c, st);
     return rc;
 }
@@ -955,8 +984,7 @@ int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_
 int z_gen_block_waves(const DeviceDataset& ds, int K, int precision, int Nc, int* waves)
 {
     int bb = 1, gb = 1;
-    return precision == VPS_PREC_MIXED ? z_occupancy_k<float>(ds, K, Nc, &bb, &gb, waves)
-                                       : z_occupancy_k<double>(ds, K, Nc, &bb, &gb, waves);
+    return z_occupancy_prec(ds, K, precision, Nc, &bb, &gb, waves);
 }
 
 int64_t z_lik_parts(const DeviceDataset& ds, int G)
@@ -969,14 +997,13 @@ int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, h
                    bool rec, bool advance)
 {
     if (rec && advance) return fail(VPS_ERR_INVALID, "Record reads the sweep index after the Z pass");
-    if (rec && c.precision == VPS_PREC_MIXED) return fail(VPS_ERR_INVALID, "the Record lnL of the Z pass needs FP64 weights");
+    if (rec && (c.precision == VPS_PREC_MIXED || c.precision == VPS_PREC_FAST32))
+        return fail(VPS_ERR_INVALID, "the Record lnL of the Z pass needs FP64 weights");
     int rc = clear_mi ? launch_fill_i32(c.Mi, (int64_t)ds.N * c.K, 0, st) : VPS_OK;
     if (rc) return rc;
     if (ev0) VPS_HIP(hipEventRecord(ev0, st));
     int64_t nparts = 0;
-    rc = c.precision == VPS_PREC_MIXED ? dispatch_k<float, false>(ds, c, st)
-         : rec                         ? dispatch_k<double, true>(ds, c, st, &nparts)
-                                       : dispatch_k<double, false>(ds, c, st);
+    rc = rec ? dispatch_k<double, true>(ds, c, st, &nparts) : dispatch_counts(ds, c, st);
     if (ev1) VPS_HIP(hipEventRecord(ev1, st));
     if (rc) return rc;
     rc = launch_reduce_ni(ds, c, st, advance);

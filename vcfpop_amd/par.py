@@ -7,6 +7,7 @@ src/parameters.cpp:1345-1362): blank-separated tokens, blanks inside double
 quotes kept, quotes trimmed.  Reads the options the clustering uses and
 ignores the rest (other analyses are not part of this engine):
   -g_input, -g_format (polygene), -g_output, -g_seed, -g_decimal
+  -g_float, -g_fastsingle: REAL = float runs (PREC_FAST32 / PREC_REAL32)
   -g_indtext: populations and regions (vcfpop's default without it: one
     population; the file's population column is not read, as in vcfpop)
   -structure_* (src/parameters.cpp:1230-1315): models, K range, runs,
@@ -29,7 +30,8 @@ import re
 import sys
 import time
 
-from ._lib import RNG_PARALLEL, RNG_REFERENCE, Batch, Chain, Dataset, Image, params_default, write_summary
+from ._lib import (PREC_FAST32, PREC_FP64, PREC_REAL32, RNG_PARALLEL, RNG_REFERENCE, Batch, Chain, Dataset, Image,
+                   params_default, write_summary)
 
 _YESNO = {"yes": 1, "no": 2}
 _BOOL_OPTS = {"admix", "locpriori", "f", "inferlambda", "difflambda", "diversity", "inferalpha", "diffalpha",
@@ -72,6 +74,21 @@ def par_tokens(text):
     if cur:
         toks.append("".join(cur))
     return [trim_quote(t) for t in toks]
+
+
+def real_precision(g):
+    """-g_float (single|double, default double) and -g_fastsingle (yes|no,
+    default yes; src/parameters.cpp:443-444, 795-798): BAYESIAN<float> runs
+    as PREC_FAST32 (fast single) or PREC_REAL32, BAYESIAN<double> as FP64"""
+    fl = g.get("float", "double").lower()
+    fs = g.get("fastsingle", "yes").lower()
+    if fl not in ("single", "double"):
+        raise ValueError(f"-g_float={fl}: expected single or double")
+    if fs not in _YESNO:
+        raise ValueError(f"-g_fastsingle={fs}: expected yes or no")
+    if fl == "double":
+        return PREC_FP64
+    return PREC_FAST32 if fs == "yes" else PREC_REAL32
 
 
 def path_value(v):
@@ -161,6 +178,7 @@ class Job:
         if sweeps is not None:   # a shorter run: burn-in a fifth, sampling the rest
             kw["nburnin"], kw["nreps"] = sweeps // 5, sweeps - sweeps // 5
         kw["rng_topology"] = RNG_PARALLEL if topology == "parallel" else RNG_REFERENCE
+        kw["precision"] = real_precision(g)
         self.kw = kw
         self.params = p = params_default(**kw)
         nadm = p.nadmburnin if (p.admix_val == 2 and p.locpriori_val == 2) else 0
@@ -186,7 +204,7 @@ def run(par_path, device=0, sweeps=None, topology="reference", log=print):
     ds = Dataset(im, device)
     if job.eval:
         from . import evalmode
-        base = {k: v for k, v in job.kw.items() if k not in ("krange_min", "krange_max", "nruns", "rng_topology")}
+        base = {k: v for k, v in job.kw.items() if k not in ("krange_min", "krange_max", "nruns", "rng_topology", "precision")}
         rows = evalmode.run_eval(ds, K=5, seed=job.seed, base=base)
         evalmode.print_table(rows, file=_LogFile(log))
         ds.close()
