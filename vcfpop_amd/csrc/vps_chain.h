@@ -118,6 +118,14 @@ struct ChainDev {
     int32_t   G, Nc;
     const uint32_t* jump;
     int32_t   lik_tpr[2];   // tiles per locus range of the likelihood passes (BI, GEN tiles)
+    // Metropolis proposals parsed as a stream graph (reference draw streams,
+    // global alpha): the stream's first pg_W words, the generator at every
+    // 8th word, two successor tables of pg_S states, the path of draw states
+    uint64_t* pg_words;
+    Xs128*    pg_st8;
+    int32_t*  pg_next;
+    int32_t*  pg_path;
+    int64_t   pg_W, pg_S;
 };
 
 // REAL = float (BAYESIAN<float>): P, Q, the proposals and FreqM hold
@@ -144,6 +152,8 @@ int launch_reduce_lnl(const double* part, int64_t n, double* out, hipStream_t st
 int launch_z_noadmix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, bool advance = false);
+// the last cluster's Ni row and Mi column after the admixture Z pass
+int launch_z_finish(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, bool advance = false);
 // likelihood passes: mode 0 = Record lnL (admix), 1 = Record lnL (no admix),
 // 2 = Metropolis log ratio per individual, 3 = no-admix per-(i,k) log likelihood.
 // Modes 0/1 write the lnL to lnl_out, or to c.scal[1] when it is null.
@@ -161,6 +171,12 @@ constexpr int kSppQNone = -1, kSppQAdmix = 1, kSppQProposals = 2;
 int launch_spp(const DeviceDataset& ds, const ChainDev& c, bool do_p, int qkind, bool clear_mi, hipStream_t st);
 int launch_spp_metro_accept(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 int launch_spp_polylog(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+// the stream-graph parse of the Metropolis proposals (then their Dirichlet
+// rows), when the chain has its buffers (pg_words); see vps_spp.hip
+bool pg_applies(const ChainDev& c);
+int launch_pg_proposals(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
+// the stream graph's sizes for a chain: words W and states S (0 = not used)
+void pg_sizes(int ntyped, int K, int64_t* W, int64_t* S);
 
 int launch_update_p(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 // UpdateP + Dirichlet UpdateQ + Mi clear in one launch (parallel topology)

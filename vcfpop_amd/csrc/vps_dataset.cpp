@@ -99,6 +99,8 @@ const std::vector<uint32_t>& ring_jump_tables()
         for (int l = 0; l < kRingLaneTabs; ++l) jump_table((uint64_t)8 * l, tabs.data() + (size_t)l * kJumpWords);
         for (int w = 0; w < kRingWaveTabs; ++w)
             jump_table((uint64_t)512 * w, tabs.data() + (size_t)(kRingLaneTabs + w) * kJumpWords);
+        for (int j = 0; j < kRingPow2Tabs; ++j)
+            jump_table((uint64_t)4096 << j, tabs.data() + (size_t)(kRingPow2Base + j) * kJumpWords);
     });
     return tabs;
 }

@@ -194,9 +194,11 @@ VPS_HD double my_sin(double x)
 VPS_HD double my_cos(double x) { return my_sin(x + (kPi / 2)); }
 
 // RNG<double> with the Box–Muller cache (U1, U2), src/statistics.h:15-87.
-// The cache is empty when U1's low 32 bits are all ones.
-struct Rng {
-    Xs128 s;
+// The cache is empty when U1's low 32 bits are all ones.  Src supplies the
+// raw 64-bit draws: the xorshift128+ generator, or words already drawn.
+template <typename Src>
+struct RngT {
+    Src s;
     double U1, U2;
     VPS_HD void init(uint64_t seed, uint64_t salt)
     {
@@ -265,6 +267,7 @@ struct Rng {
         return a;
     }
 };
+using Rng = RngT<Xs128>;
 
 // Dirichlet(res, a, b, n): Gamma(a[i] + b[i]) then scale by 1/sum,
 // src/statistics.cpp:284-296

@@ -292,6 +292,20 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
         VPS_HIP(hipMemcpyAsync(c.lcnt, &zero, sizeof(zero), hipMemcpyHostToDevice, ch->stream));
         VPS_HIP(hipStreamSynchronize(ch->stream));
     }
+    // the stream-graph parse of the Metropolis proposals (reference draw
+    // streams, admixture with a global alpha)
+    if (!rc && c.admix && !c.locpriori && c.topology == VPS_RNG_REFERENCE && p->metrofreq > 0) {
+        int64_t W = 0, Sg = 0;
+        pg_sizes(d.ntyped, K, &W, &Sg);
+        if (W > 0) {
+            c.pg_W = W;
+            c.pg_S = Sg;
+            rc = chain_alloc(ch, &c.pg_words, W);
+            if (!rc) rc = chain_alloc(ch, &c.pg_st8, W / 8);
+            if (!rc) rc = chain_alloc(ch, &c.pg_next, 2 * Sg);
+            if (!rc) rc = chain_alloc(ch, &c.pg_path, (int64_t)d.ntyped * K);
+        }
+    }
     if (!rc) rc = chain_alloc(ch, &c.Eta, K);
     if (!rc) rc = chain_alloc(ch, &c.Gamma, S * K);
     if (!rc) rc = chain_alloc(ch, &c.Di, S * K);
@@ -489,8 +503,14 @@ int launch_sweep(vps_chain* ch, const SweepPlan& p)
         rc = c.topology == VPS_RNG_PARALLEL ? launch_update_pq(d, c, ch->stream)
                                             : launch_spp(d, c, true, kSppQAdmix, true, ch->stream);
     } else if (p.q == kQMetro && c.topology == VPS_RNG_REFERENCE) {
-        // UpdateP's streams and the proposals' stream in one parse
-        rc = launch_spp(d, c, true, kSppQProposals, false, ch->stream);
+        // UpdateP's streams, and the proposals' stream as a stream graph (or
+        // in the same window parse)
+        if (pg_applies(c)) {
+            rc = launch_spp(d, c, true, kSppQNone, false, ch->stream);
+            if (!rc) rc = launch_pg_proposals(d, c, ch->stream);
+        } else {
+            rc = launch_spp(d, c, true, kSppQProposals, false, ch->stream);
+        }
         if (!rc) rc = launch_lik_pass(d, c, 2, ch->stream);
         if (!rc) rc = launch_q_metro_accept(d, c, ch->stream);
     } else {

@@ -94,7 +94,7 @@ int copy_image(const vps_dataset_desc* in, HostImage& h)
 // The reservation is corrected to the measured bytes once the job is loaded.
 constexpr int64_t kEstChunkParts = 16;
 
-int64_t estimate_bytes(const HostImage& h, int K)
+int64_t estimate_bytes(const HostImage& h, int K, const vps_params& par)
 {
     const int64_t N = h.N, L = h.L;
     int64_t KT = 0;
@@ -119,7 +119,14 @@ int64_t estimate_bytes(const HostImage& h, int K)
     const int64_t chain = K * KT * (8 + 4 + 8) + parts * (K * KT * 4 + (ntiles / 4 + ntiles) * 8) + 3 * KT * 8 +
                           N * K * (8 + 4 + 8 + 8 + 8) +
                           N * 64 + (ntiles / 4 + 2) * N * std::max<int64_t>(K, 2) * 8 + (64 << 10);
-    return dataset + chain;
+    // the stream graph of the Metropolis proposals (at most N typed)
+    int64_t pg = 0;
+    if (par.admix_val == 1 && par.locpriori_val != 1 && par.rng_topology == VPS_RNG_REFERENCE && par.metrofreq > 0) {
+        int64_t W = 0, S = 0;
+        pg_sizes((int)N, K, &W, &S);
+        pg = W * 8 + (W / 8) * 16 + 2 * S * 4 + N * K * 4;
+    }
+    return dataset + chain + pg;
 }
 
 struct Job {
@@ -382,7 +389,7 @@ extern "C" int vps_server_submit(vps_server* s, const vps_dataset_desc* desc, co
     j->par.sweep_graphs = 0;   // loader threads make HIP calls while jobs run
     if (j->par.z_chunk == 0) j->par.z_chunk = -1;   // no timing trials beside running jobs
     j->sweeps = n_sweeps;
-    j->est = estimate_bytes(j->img, K);
+    j->est = estimate_bytes(j->img, K, j->par);
     if (j->est > s->cfg.hbm_budget_bytes)
         return fail(VPS_ERR_BUDGET, "job needs ~%lld bytes, budget %lld", (long long)j->est,
                     (long long)s->cfg.hbm_budget_bytes);

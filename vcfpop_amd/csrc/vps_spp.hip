@@ -36,6 +36,7 @@
 // finite shape; a NaN shape spins) ends the parse's wait: it is stored as NaN
 // and its stream moves to the end of the ring.
 #include <climits>
+#include <utility>
 #include "vps_chain.h"
 
 namespace vps {
@@ -54,7 +55,12 @@ struct SppFront {
     double U1, U2;
     int32_t cached;
     int32_t pad;
+    int64_t qa, qb;   // d as (cluster, allele entry) k*KT + x for P, (typed individual, cluster) t*K + k for Q
 };
+
+// the draw counts a block keeps in LDS: P's Ni entries of its slice, or Q's
+// Mi entries of the typed individuals in draw order and then their indices
+constexpr int kStageInts = 20480;
 
 // this thread's generator: the state `8 * threadIdx.x` draws into the ring
 // (two jumps: 512 per wave, then 8 per lane)
@@ -116,24 +122,47 @@ constexpr int kDrawP = 0;       // Freq[g] <- Gamma(prior + Ni[g]), g = k*KT + a
 constexpr int kDrawQ = 1;       // Q[i][k] <- Gamma(alpha_row(i)[k] + Mi[i][k]), typed i
 constexpr int kDrawProp = 2;    // bufNK1[i][k] <- Gamma(alpha_row(i)[k]), typed i
 
-__device__ __forceinline__ double* draw_target(const DeviceDataset& ds, const ChainDev& c, int kind, int64_t g,
-                                               double* alpha)
+// (qa, qb) of draw d (see SppFront)
+__device__ __forceinline__ void draw_coords(int kind, int64_t d, int64_t KT, int K, int64_t* qa, int64_t* qb)
 {
+    const int64_t n = kind == kDrawP ? KT : K;
+    *qa = d / n;
+    *qb = d - *qa * n;
+}
+
+// draw j of a window whose first draw is the front f: its shape and where its
+// value goes.  The coordinates follow from the front's without a division
+// (the window holds at most 1024 draws); counts come from the LDS stage
+// (indexed from the block's first draw d0) when the block has one.
+__device__ __forceinline__ double* draw_target(const DeviceDataset& ds, const ChainDev& c, int kind, const SppFront& f,
+                                               int j, const int32_t* stage, int64_t d0, int64_t nd, double* alpha)
+{
+    const int64_t g = f.d + j;
     if (kind == kDrawP) {
-        const int64_t k = g / ds.KT, x = g - k * ds.KT;
+        int64_t k = f.qa, x = f.qb + j;
+        if (x >= ds.KT) {
+            k += x / ds.KT;
+            x %= ds.KT;
+        }
         const double prior = c.fmodel ? c.FreqA[x] * c.Fst[c.K + (c.fsame ? 0 : k)] : real_store(c.Lambda[k], real32(c));
-        *alpha = prior + (double)c.Ni[g];
+        *alpha = prior + (double)(stage ? stage[g - d0] : c.Ni[g]);
         return c.Freq + g;
     }
-    const int64_t t = g / c.K, k = g - t * c.K;
-    const int i = ds.typed[t];
-    const double* ar = c.locpriori ? c.AlphaLocal + (int64_t)ds.popid[i] * c.K : c.Alpha;
+    const int K = c.K;
+    int k = (int)f.qb + j % K;
+    int64_t t = f.qa + j / K;
+    if (k >= K) {
+        k -= K;
+        ++t;
+    }
+    const int i = stage ? stage[nd + t] : ds.typed[t];
+    const double* ar = c.locpriori ? c.AlphaLocal + (int64_t)ds.popid[i] * K : c.Alpha;
     if (kind == kDrawQ) {
-        *alpha = ar[k] + (double)c.Mi[(int64_t)i * c.K + k];
-        return c.Q + (int64_t)i * c.K + k;
+        *alpha = ar[k] + (double)(stage ? stage[g] : c.Mi[(int64_t)i * K + k]);
+        return c.Q + (int64_t)i * K + k;
     }
     *alpha = ar[k] * 1.0;
-    return c.bufNK1 + (int64_t)i * c.K + k;
+    return c.bufNK1 + (int64_t)i * K + k;
 }
 
 // Slots.  Seen from the front of a window, the stream is a sequence of
@@ -178,13 +207,14 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
     __shared__ int s_nseg, s_commit, s_stop_ovf, s_H;
     __shared__ float s_rate;
     __shared__ SppFront s_front;
+    __shared__ int32_t s_stage[kStageInts];
     constexpr uint8_t kNextStop = 0xFF, kNextOvf = 0xFE;
 
     const int m = *c.mdev;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int kind;
     uint64_t sseed, salt;
-    int64_t d1;
+    int64_t d0 = 0, d1;
     if ((int)blockIdx.x < np) {
         kind = kDrawP;
         const int b = blockIdx.x;
@@ -196,16 +226,27 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
             const int64_t k = kl / ds.L;
             return k * ds.KT + ds.afo[kl - k * ds.L];
         };
-        if (tid == 0) s_front.d = first_g(kl0);
+        d0 = first_g(kl0);
         d1 = first_g(kl1);
         sseed = c.seed + (uint64_t)m * kPBlocks + b;
         salt = kSaltUpdateP;
     } else {
         kind = qkind;
-        if (tid == 0) s_front.d = 0;
         d1 = (int64_t)ds.ntyped * c.K;
         sseed = c.seed + (uint64_t)m;
         salt = kSaltUpdateQ;
+    }
+    // the counts the draws read, staged in LDS when they fit
+    const int64_t nd = d1 - d0;
+    const int32_t* stage = nullptr;
+    if (kind == kDrawP && nd <= kStageInts) {
+        for (int64_t x = tid; x < nd; x += kSppThreads) s_stage[x] = c.Ni[d0 + x];
+        stage = s_stage;
+    } else if (kind == kDrawQ && nd + ds.ntyped <= kStageInts) {
+        const int K = c.K;
+        for (int64_t x = tid; x < nd; x += kSppThreads) s_stage[x] = c.Mi[(int64_t)ds.typed[x / K] * K + x % K];
+        for (int64_t x = tid; x < ds.ntyped; x += kSppThreads) s_stage[nd + x] = ds.typed[x];
+        stage = s_stage;
     }
     const uint32_t* J = ds.ring_jump;
     ring_load_advance(J, s_adv);
@@ -215,6 +256,8 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
     int64_t ws = 0;
     ring_fill(st, ring);
     if (tid == 0) {
+        s_front.d = d0;
+        draw_coords(kind, d0, ds.KT, c.K, &s_front.qa, &s_front.qb);
         s_front.pos = 0;
         s_front.bm = 0;
         s_front.U1 = as_f64(0xFFFFFFFFull);
@@ -236,7 +279,7 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
         const int64_t nact = min((int64_t)(64 * G), d1 - f.d);
         const bool act = jd < nact;
         double alpha = 2.0;
-        double* out = act ? draw_target(ds, c, kind, f.d + jd, &alpha) : nullptr;
+        double* out = act ? draw_target(ds, c, kind, f, jd, stage, d0, nd, &alpha) : nullptr;
         const bool lt1 = act && alpha < 1;
         // prefix count of shapes below 1 over the window's draws (their
         // ballot is the same in every hypothesis's wave of a group)
@@ -406,6 +449,7 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
                 nf.U1 = as_f64(0xFFFFFFFFull);
                 nf.U2 = 0;
                 nf.pad = 0;
+                draw_coords(kind, nf.d, ds.KT, c.K, &nf.qa, &nf.qb);
                 s_front = nf;
             }
         } else if (mine && jd < commit) {
@@ -419,6 +463,7 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
                 nf.U1 = cache ? U1 : as_f64(0xFFFFFFFFull);
                 nf.U2 = U2;
                 nf.pad = 0;
+                draw_coords(kind, nf.d, ds.KT, c.K, &nf.qa, &nf.qb);
                 s_front = nf;
             }
         }
@@ -546,6 +591,216 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_polylog(const DeviceDataset
         used += total;
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------
+// The Metropolis proposals as a stream graph.
+//
+// With a global alpha (no location prior) the proposal draws of UpdateQMetro
+// (src/structure.cpp:1012-1020) are Gamma(Alpha[k]) for k = 0..K-1 of each
+// typed individual in turn: the shape depends only on the draw's position
+// modulo K.  A draw's start is then fully described by a state
+// (p, c, k): the next word p of the stream, whether a Box-Muller normal is
+// cached, and k.  A cached normal always comes from the pair 3 words back (4
+// when the previous draw's shape was below 1, for its boost uniform), so
+// the state fixes its value.  Each state has one successor, the state after
+// its draw.  Rather than parsing the stream in order, every state of the
+// stream's first W words is evaluated at once (one thread each), and the
+// path from the start (0, empty, 0) is found by pointer doubling: with
+// path[0..n) known and T = succ^n, path[n + i] = T[path[i]], then T = T o T.
+// The values of the D draws on the path are then evaluated again, in
+// parallel.  W = 4D + 8192 words covers the path (a draw takes 2 to 3 words
+// on average, one more per rejection); a path that would run past W makes
+// the draws from there NaN, as the window parse does for a draw longer than
+// its ring.  The draws equal the serial parse's exactly: each is the
+// reference's Gamma on the words the serial parse gives it.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct WordSrc {
+    const uint64_t* w;
+    int64_t pos, lim;
+    bool ovf;
+    __device__ uint64_t next()
+    {
+        if (pos >= lim) {
+            ovf = true;
+            return 0x3FF8000000000000ull;   // U = 1.5: ends any loop
+        }
+        return w[pos++];
+    }
+};
+
+struct PgStep {
+    double val;
+    int64_t p;
+    bool cached, ok;
+};
+
+// the proposal draw starting at state (p, cf, k)
+__device__ PgStep pg_draw(const uint64_t* w, int64_t W, const double* alpha, int K, int64_t p, int cf, int k)
+{
+    PgStep out{0.0, 0, false, false};
+    RngT<WordSrc> r;
+    r.s = WordSrc{w, p, W, false};
+    r.U1 = as_f64(0xFFFFFFFFull);
+    r.U2 = 0;
+    if (cf) {
+        const int kp = k == 0 ? K - 1 : k - 1;
+        const int64_t bm = p - 3 - (alpha[kp] * 1.0 < 1 ? 1 : 0);
+        if (bm < 0) return out;
+        r.U1 = my_sqrt(-2.0 * log(unit12(w[bm]) - 1.0));
+        r.U2 = 2.0 * kPi * (unit12(w[bm + 1]) - 1.0);
+        if (!r.cached()) return out;   // (no draw leaves such a state)
+    }
+    const double a = alpha[k] * 1.0;
+    out.val = r.gamma(a);
+    out.p = r.s.pos;
+    out.cached = r.cached();
+    out.ok = !r.s.ovf;
+    if (out.ok && out.cached) {
+        // the cache must be the pair 3 (+1) words back, as the states assume
+        const int64_t bm = out.p - 3 - (a < 1 ? 1 : 0);
+        const bool same = bm >= 0 && as_u64(r.U1) == as_u64(my_sqrt(-2.0 * log(unit12(w[bm]) - 1.0))) &&
+                          as_u64(r.U2) == as_u64(2.0 * kPi * (unit12(w[bm + 1]) - 1.0));
+        if (!same) out.ok = false;
+    }
+    return out;
+}
+
+__device__ __forceinline__ int64_t pg_state(int64_t p, int k, int K, bool cf) { return ((p * K + k) << 1) | (cf ? 1 : 0); }
+
+// the stream's words, 4,096 per block (each thread 8), the generator at each
+// 8th word, and path[0] = the start state
+__global__ __launch_bounds__(512) void k_pg_words(const DeviceDataset ds, const ChainDev c)
+{
+    const int m = *c.mdev;
+    const int64_t t = (int64_t)blockIdx.x * 512 + threadIdx.x;
+    Xs128 s;
+    s.seed(c.seed + (uint64_t)m, kSaltUpdateQ);
+    const uint32_t* J = ds.ring_jump;
+    const uint64_t b = blockIdx.x;
+    for (int j = 0; j < kRingPow2Tabs && (b >> j); ++j)
+        if ((b >> j) & 1) jump_apply(J + (size_t)(kRingPow2Base + j) * kJumpWords, s);
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (wv) jump_apply(J + (size_t)(kRingLaneTabs + wv) * kJumpWords, s);
+    if (l) jump_apply(J + (size_t)l * kJumpWords, s);
+    c.pg_st8[t] = s;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) c.pg_words[t * 8 + u] = s.next();
+    if (t == 0) c.pg_path[0] = 0;
+}
+
+// successor of every state
+__global__ void k_pg_next(const ChainDev c)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= c.pg_S) return;
+    const int K = c.K;
+    const int cf = (int)(x & 1);
+    const int64_t pk = x >> 1;
+    const int k = (int)(pk % K);
+    const int64_t p = pk / K;
+    const PgStep st = pg_draw(c.pg_words, c.pg_W, c.Alpha, K, p, cf, k);
+    c.pg_next[x] = (st.ok && st.p < c.pg_W) ? (int32_t)pg_state(st.p, (k + 1) % K, K, st.cached) : -1;
+}
+
+__global__ void k_pg_extend(int32_t* path, const int32_t* T, int64_t n, int64_t D)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || n + i >= D) return;
+    const int32_t s = path[i];
+    path[n + i] = s < 0 ? -1 : T[s];
+}
+
+__global__ void k_pg_square(const int32_t* T, int32_t* T2, int64_t S)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= S) return;
+    const int32_t t = T[x];
+    T2[x] = t < 0 ? -1 : T[t];
+}
+
+// the draws on the path into bufNK1 (typed individual d / K, cluster d % K);
+// the last one also leaves the stream's state for the acceptance uniforms
+__global__ void k_pg_values(const DeviceDataset ds, const ChainDev c, int64_t D)
+{
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= D) return;
+    const int K = c.K;
+    const int k = (int)(d % K);
+    const int i = ds.typed[d / K];
+    const int32_t s = c.pg_path[d];
+    double* out = c.bufNK1 + (int64_t)i * K + k;
+    const PgStep st = s < 0 ? PgStep{0.0, 0, false, false}
+                            : pg_draw(c.pg_words, c.pg_W, c.Alpha, K, (int64_t)(s >> 1) / K, s & 1, k);
+    *out = st.ok ? st.val : __builtin_nan("");
+    if (d != D - 1) return;
+    if (!st.ok || st.p >= c.pg_W) {
+        // (a path past the words: its draws are NaN, so no proposal is
+        // accepted whatever the acceptance uniforms)
+        Rng r;
+        r.init(c.seed + (uint64_t)*c.mdev, kSaltUpdateQ);
+        c.rngbuf[0] = r;
+    } else {
+        Rng r;
+        r.s = c.pg_st8[st.p >> 3];
+        for (int x = 0; x < (int)(st.p & 7); ++x) r.s.next();
+        if (st.cached) {
+            const int64_t bm = st.p - 3 - (c.Alpha[k] * 1.0 < 1 ? 1 : 0);
+            r.U1 = my_sqrt(-2.0 * log(unit12(c.pg_words[bm]) - 1.0));
+            r.U2 = 2.0 * kPi * (unit12(c.pg_words[bm + 1]) - 1.0);
+        } else {
+            r.U1 = as_f64(0xFFFFFFFFull);
+            r.U2 = 0;
+        }
+        c.rngbuf[0] = r;
+    }
+}
+
+}  // namespace
+
+void pg_sizes(int ntyped, int K, int64_t* W, int64_t* S)
+{
+    const int64_t D = (int64_t)ntyped * K;
+    int64_t w = 4 * D + 8192;
+    w = (w + 4095) / 4096 * 4096;
+    const int64_t s = 2 * (int64_t)K * w;
+    // state indices are int32; the pow2 jump tables reach 4096 * 2^24 words
+    if (D <= 0 || s >= ((int64_t)1 << 30) || w / 4096 >= ((int64_t)1 << kRingPow2Tabs)) {
+        *W = *S = 0;
+        return;
+    }
+    *W = w;
+    *S = s;
+}
+
+bool pg_applies(const ChainDev& c)
+{
+    return c.pg_words != nullptr && c.topology == VPS_RNG_REFERENCE && !c.locpriori;
+}
+
+int launch_pg_proposals(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
+{
+    const int64_t D = (int64_t)ds.ntyped * c.K;
+    const int64_t S = c.pg_S;
+    hipLaunchKernelGGL(k_pg_words, dim3((unsigned)(c.pg_W / 4096)), dim3(512), 0, st, ds, c);
+    hipLaunchKernelGGL(k_pg_next, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, st, c);
+    int32_t* T = c.pg_next;
+    int32_t* T2 = c.pg_next + S;
+    for (int64_t n = 1; n < D; n *= 2) {
+        hipLaunchKernelGGL(k_pg_extend, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c.pg_path, T, n, D);
+        if (2 * n < D) {
+            hipLaunchKernelGGL(k_pg_square, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, st, T, T2, S);
+            std::swap(T, T2);
+        }
+    }
+    hipLaunchKernelGGL(k_pg_values, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, st, ds, c, D);
+    VPS_HIP(hipGetLastError());
+    // the proposals' Dirichlet rows
+    hipLaunchKernelGGL(k_spp_normalize, dim3((unsigned)((ds.N + 255) / 256)), dim3(256), 0, st, ds, c, 0, kDrawProp, 0);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
 }
 
 int launch_spp(const DeviceDataset& ds, const ChainDev& c, bool do_p, int qkind, bool clear_mi, hipStream_t st)

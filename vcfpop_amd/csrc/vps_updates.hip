@@ -170,7 +170,8 @@ __global__ void k_q_metro_accept(const DeviceDataset ds, const ChainDev c)
 
 int launch_q_metro_propose(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
-    if (c.topology == VPS_RNG_REFERENCE) return launch_spp(ds, c, false, kSppQProposals, false, st);
+    if (c.topology == VPS_RNG_REFERENCE)
+        return pg_applies(c) ? launch_pg_proposals(ds, c, st) : launch_spp(ds, c, false, kSppQProposals, false, st);
     const unsigned nb = (unsigned)((ds.N + 255) / 256);
     hipLaunchKernelGGL(k_q_metro_propose, dim3(nb), dim3(256), 0, st, ds, c);
     VPS_HIP(hipGetLastError());

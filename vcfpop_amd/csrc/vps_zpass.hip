@@ -19,12 +19,16 @@
 //     remainder: t_k < w_k  <=>  fl(t_k - w_k) < 0 for finite doubles, and
 //     once negative the remainder stays negative, so
 //     C_k = [t_{k+1} < 0] = [k* <= k]  (cumulative indicator, k < K-1).
-//   * Ni: each lane accumulates C_k per allele in registers;
-//     Ni[k] = C_k - C_{k-1} after the chunk (linear in the counts).
+//   * Ni: each lane accumulates C_k per allele in registers and adds
+//     Ni[k] = C_k - C_{k-1} into Ni by atomics after the chunk (linear in
+//     the counts; no per-chunk partials).
 //   * Mi: per k the wavefront ballots C_k over typed lanes and popcounts
-//     it on the scalar unit; per individual the K cluster counts go to an
-//     LDS row shared by the block's wavefronts (same individuals, other
-//     tiles), and the block adds its rows to Mi once at the end.
+//     it on the scalar unit; per individual the counts go to the
+//     wavefront's LDS row, and the block adds its wavefronts' rows (same
+//     individuals, other tiles) to Mi once at the end.
+//   * Only clusters k < K-1 are counted: every typed copy lands in one
+//     cluster, so after the pass Ni[K-1] = copies of the allele - the rest
+//     and Mi[i][K-1] = vt_i - the rest (k_z_finish).
 #include <algorithm>
 #include "vps_chain.h"
 
@@ -142,28 +146,36 @@ __device__ __forceinline__ uint32_t slot_flags(uint32_t e)
     return f | ((uint32_t)v << 4);
 }
 
-// Per-individual Mi rows of the BI kernel: the K cluster counts of one
-// individual summed over the block's wavefronts, FB-bit fields, FPW per
-// 32-bit word.  A block adds at most 4 waves x 64 loci x MPT slots per
-// count: 512 < 2^10 for diploid tables, < 2^16 otherwise (<= 10 slots).
+// Per-individual Mi rows of the BI kernel: one row per (wavefront,
+// individual) holding the counts of clusters k < K-1 (the last cluster's
+// count is vt_i minus the others, set after the pass), FB-bit fields, FPW
+// per 32-bit word.  A wavefront adds at most 64 loci x MPT slots per count:
+// 128 < 2^8 for diploid tables, 256 < 2^9 for tetraploid ones, 640 < 2^10
+// otherwise.
 template <int K, int MPT>
 struct MiPack {
-    static constexpr int FB = MPT == 2 ? 10 : 16;
+    static constexpr int KC = K > 1 ? K - 1 : 1;     // counted clusters
+    static constexpr int FB = MPT == 2 ? 8 : MPT == 4 ? 9 : 10;
     static constexpr int FPW = 32 / FB;
-    static constexpr int MW = (K + FPW - 1) / FPW;   // words per row
+    static constexpr int MW = (KC + FPW - 1) / FPW;   // words per row
 };
 template <int K, int MPT>
 __host__ __device__ constexpr int mi_row_words() { return MiPack<K, MPT>::MW; }
 
-// add a BI block's packed per-individual rows to Mi
+// add a BI block's packed per-individual rows (nwv wavefronts' rows of nci
+// individuals each) to Mi
 template <int K, int MPT>
-__device__ __forceinline__ void flush_bi_rows(const uint32_t* mi_lds, int nci, int32_t* __restrict__ Mi_blk)
+__device__ __forceinline__ void flush_bi_rows(const uint32_t* mi_lds, int nci, int nwv, int32_t* __restrict__ Mi_blk)
 {
     using MP = MiPack<K, MPT>;
-    for (int x = threadIdx.x; x < nci * K; x += blockDim.x) {
-        const int i = x / K, k = x - i * K;
-        const int v = (int)((mi_lds[i * MP::MW + k / MP::FPW] >> ((k % MP::FPW) * MP::FB)) & ((1u << MP::FB) - 1u));
-        if (v) atomicAdd(Mi_blk + x, v);
+    if (K < 2) return;
+    for (int x = threadIdx.x; x < nci * (K - 1); x += blockDim.x) {
+        const int i = x / (K - 1), k = x - i * (K - 1);
+        int v = 0;
+        for (int w = 0; w < nwv; ++w)
+            v += (int)((mi_lds[((size_t)w * nci + i) * MP::MW + k / MP::FPW] >> ((k % MP::FPW) * MP::FB)) &
+                       ((1u << MP::FB) - 1u));
+        if (v) atomicAdd(Mi_blk + i * K + k, v);
     }
 }
 
@@ -173,7 +185,7 @@ template <int K, typename Real, int MPT, bool BIG, bool REC, bool X96>
 __device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileDesc& t, const double* __restrict__ Q,
                                           uint64_t seed, int m, int64_t l, bool valid, int afo, int k2, int lane,
                                           int c, int i0, int i1, const Real* pl0, uint8_t* tab_lds,
-                                          uint32_t* mi_lds, int32_t* __restrict__ Ni_part,
+                                          uint32_t* mi_lds, int32_t* __restrict__ Ni,
                                           const uint32_t* __restrict__ jump)
 {
     const int64_t KT = ds.KT;
@@ -290,10 +302,10 @@ __device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileD
                     cum[k] += __builtin_popcountll(ballot(neg) & tmask);
                     nc[k] += (neg && typed) ? inc : 0u;
                 }
-                cum[K - 1] += __builtin_popcountll(tmask);
             }
             if (REC && MPT == 2) acc.charge(pr);
-            // the individual's K counts, packed, into lane (i - i0) % 64 of mrow
+            // the individual's counts of clusters k < K-1, packed, into lane
+            // (i - i0) % 64 of mrow
             const int rr = i - i0;
 #pragma unroll
             for (int x = 0; x < MPK::MW; ++x) {
@@ -301,13 +313,13 @@ __device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileD
 #pragma unroll
                 for (int f = 0; f < MPK::FPW; ++f) {
                     const int k = x * MPK::FPW + f;
-                    if (k < K) pk |= (uint32_t)(cum[k] - (k ? cum[k - 1] : 0)) << (f * MPK::FB);
+                    if (k < K - 1) pk |= (uint32_t)(cum[k] - (k ? cum[k - 1] : 0)) << (f * MPK::FB);
                 }
                 mrow[x] = (uint32_t)writelane_i32((int)pk, rr & 63, (int)mrow[x]);
             }
             if ((rr & 63) == 63) {
 #pragma unroll
-                for (int x = 0; x < MPK::MW; ++x) atomicAdd(mi_lds + ((rr & ~63) + lane) * MPK::MW + x, mrow[x]);
+                for (int x = 0; x < MPK::MW; ++x) mi_lds[((rr & ~63) + lane) * MPK::MW + x] = mrow[x];
             }
         }
     }
@@ -316,20 +328,23 @@ __device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileD
         const int rr = i1 - 1 - i0;
         if ((rr & 63) != 63 && lane <= (rr & 63)) {
 #pragma unroll
-            for (int x = 0; x < MPK::MW; ++x) atomicAdd(mi_lds + ((rr & ~63) + lane) * MPK::MW + x, mrow[x]);
+            for (int x = 0; x < MPK::MW; ++x) mi_lds[((rr & ~63) + lane) * MPK::MW + x] = mrow[x];
         }
     }
 
+    // the lane's counts of clusters k < K-1 go straight into Ni (the chunks
+    // of a tile add into the same entries); the last cluster's follow from
+    // the allele totals after the pass
     if (valid) {
-        int32_t* dst = Ni_part + (int64_t)c * K * KT + afo;
+        int32_t* dst = Ni + afo;
         uint32_t prev = 0;
 #pragma unroll
         for (int k = 0; k < K - 1; ++k) {
             const uint32_t d = nc[k] - prev;   // per-half differences never borrow (cumulative counts)
             prev = nc[k];
             const int all = (int)(d & 0xFFFFu), one = (int)(d >> 16);
-            dst[k * KT] = all - one;
-            if (k2 > 1) dst[k * KT + 1] = one;
+            if (all - one) atomicAdd(dst + k * KT, all - one);
+            if (one) atomicAdd(dst + k * KT + 1, one);
         }
     }
     return REC ? acc.close() : 0.0;
@@ -345,13 +360,16 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // LDS of the BI kernel: [wave][k][allele][lane] P values, then
 // [wave][genotype][lane] table bytes (bi_tab_rows rows; none when every BI
-// tile decodes from registers), then the block's packed Mi rows
+// tile decodes from registers), then the packed Mi rows [wave][individual]
 template <int K, typename Real>
 __host__ __device__ constexpr size_t bi_p_bytes(int nwv) { return (size_t)nwv * K * 2 * kTileLoci * sizeof(Real); }
 __host__ __device__ inline size_t bi_tab_bytes(int nwv, int rows) { return (size_t)nwv * rows * kTileLoci; }
 // words per Mi row of the BI kernel launched for this dataset (its MPT)
 template <int K>
-inline int bi_row_words(const DeviceDataset& ds) { return ds.maxploidy == 2 ? mi_row_words<K, 2>() : mi_row_words<K, 0>(); }
+inline int bi_row_words(const DeviceDataset& ds)
+{
+    return ds.maxploidy == 2 ? mi_row_words<K, 2>() : ds.maxploidy == 4 ? mi_row_words<K, 4>() : mi_row_words<K, 0>();
+}
 
 // Work items are (tile group of nwv tiles, chunk) pairs, w = group * G +
 // chunk.  Blocks are dealt to the XCDs round-robin by block index, so block
@@ -363,7 +381,7 @@ template <int K, typename Real, int MPT, bool REC, bool X96>
 __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, const double* __restrict__ Freq,
                                                    const double* __restrict__ Q, uint64_t seed,
                                                    const int32_t* __restrict__ mp,
-                                                   int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
+                                                   int32_t* __restrict__ Ni, int32_t* __restrict__ Mi,
                                                    int Nc, int G, const uint32_t* __restrict__ jump,
                                                    double* __restrict__ zlik)
 {
@@ -386,8 +404,9 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
     uint8_t* lds_u8 = reinterpret_cast<uint8_t*>(lds_i32);
     Real* p_lds = reinterpret_cast<Real*>(lds_u8) + (size_t)wv * (K * 2 * kTileLoci);
     uint8_t* tab_lds = lds_u8 + bi_p_bytes<K, Real>(nwv) + (size_t)wv * ds.bi_tab_rows * kTileLoci;
-    uint32_t* mi_lds = reinterpret_cast<uint32_t*>(lds_u8 + bi_p_bytes<K, Real>(nwv) + bi_tab_bytes(nwv, ds.bi_tab_rows));
-    for (int x = threadIdx.x; x < nci * MW; x += blockDim.x) mi_lds[x] = 0;
+    uint32_t* mi_all = reinterpret_cast<uint32_t*>(lds_u8 + bi_p_bytes<K, Real>(nwv) + bi_tab_bytes(nwv, ds.bi_tab_rows));
+    for (int x = threadIdx.x; x < nwv * nci * MW; x += blockDim.x) mi_all[x] = 0;
+    uint32_t* mi_lds = mi_all + (size_t)wv * nci * MW;   // this wavefront's rows
 
     const int64_t tix = (w / G) * nwv + wv;
     const bool has_tile = tix < ds.nbi;
@@ -415,10 +434,10 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
     if (has_tile) {
         if (t.bits > kBiRegBits)
             lnl = z_bi_tile<K, Real, MPT, true, REC, X96>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane,
-                                                    tab_lds, mi_lds, Ni_part, jump);
+                                                    tab_lds, mi_lds, Ni, jump);
         else
             lnl = z_bi_tile<K, Real, MPT, false, REC, X96>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane,
-                                                     tab_lds, mi_lds, Ni_part, jump);
+                                                     tab_lds, mi_lds, Ni, jump);
     }
     __shared__ double wsum[kWavesMax];
     if (REC) {
@@ -431,7 +450,7 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
         for (int x = 0; x < nwv; ++x) s += wsum[x];
         zlik[w] = s;
     }
-    flush_bi_rows<K, MPT>(mi_lds, nci, Mi + (int64_t)i0 * K);
+    flush_bi_rows<K, MPT>(mi_all, nci, nwv, Mi + (int64_t)i0 * K);
 }
 
 // ---------------------------------------------------------------------------
@@ -457,7 +476,7 @@ template <int K, typename Real, bool PLDS, bool REC, bool X96>
 __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceDataset ds, const double* __restrict__ Freq,
                                                     const double* __restrict__ Q, uint64_t seed,
                                                     const int32_t* __restrict__ mp,
-                                                    int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
+                                                    int32_t* __restrict__ Ni, int32_t* __restrict__ Mi,
                                                     int A, int ni_in_lds, int Nc, int G,
                                                     const uint32_t* __restrict__ jump, double* __restrict__ zlik,
                                                     int64_t zbase)
@@ -524,7 +543,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceData
             return g;
         };
         const int MP = ds.maxploidy;
-        int32_t* nig = Ni_part + (int64_t)c * K * KT + afo;
+        int32_t* nig = Ni + afo;
         uint32_t mval = 0;
 
         UnitBits ub;
@@ -600,10 +619,11 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceData
     }
     if (ni_in_lds && valid) {
         for (int k = wv; k < K - 1; k += nwv) {
-            int32_t* dst = Ni_part + ((int64_t)c0 * K + k) * KT + afo;
-            for (int x = 0; x < k2; ++x) dst[x] = ni_lds[(k * A + x) * kTileLoci + lane];
-            for (int cc = 1; cc < nwv && c0 + cc < G; ++cc)
-                for (int x = 0; x < k2; ++x) dst[(int64_t)cc * K * KT + x] = 0;
+            int32_t* dst = Ni + (int64_t)k * KT + afo;
+            for (int x = 0; x < k2; ++x) {
+                const int32_t v = ni_lds[(k * A + x) * kTileLoci + lane];
+                if (v) atomicAdd(dst + x, v);
+            }
         }
     }
     flush_mi_rows<K>(mi_lds, gi1 - gi0, Mi + (int64_t)gi0 * K);
@@ -724,6 +744,30 @@ __global__ __launch_bounds__(64 * kReduceRows) void k_reduce_ni(const int32_t* _
     }
 }
 
+// After the admixture Z pass: the last cluster's counts, which the pass
+// does not count: Ni[K-1][a] = copies of allele a - sum of the other rows,
+// and Mi[i][K-1] = vt_i - the individual's other counts (every typed copy is
+// in exactly one cluster).  One thread per allele entry and per individual.
+// (advance, when not null: the sweep index, advanced once by the first thread)
+__global__ void k_z_finish(const int32_t* __restrict__ ncopies, const int32_t* __restrict__ vt,
+                           int32_t* __restrict__ Ni, int32_t* __restrict__ Mi, int64_t KT, int K, int N,
+                           int32_t* __restrict__ advance)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (advance && x == 0) advance[0] += 1;
+    if (x < KT) {
+        int32_t t = 0;
+        for (int k = 0; k < K - 1; ++k) t += Ni[k * KT + x];
+        Ni[(int64_t)(K - 1) * KT + x] = ncopies[x] - t;
+    }
+    if (x < N) {
+        int32_t* row = Mi + x * K;
+        int32_t t = 0;
+        for (int k = 0; k < K - 1; ++k) t += row[k];
+        row[K - 1] = vt[x] - t;
+    }
+}
+
 __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v)
 {
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -799,12 +843,11 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st,
 {
     const int nthr = 64 * kWavesPerBlock;
     const int A = std::max(ds.maxK, 1);
-    const size_t mi_packed = (size_t)c.Nc * bi_row_words<K>(ds) * sizeof(uint32_t);
+    const size_t mi_packed = (size_t)kWavesPerBlock * c.Nc * bi_row_words<K>(ds) * sizeof(uint32_t);
     const GenPlan g = gen_plan<K, Real>(ds, c.Nc, c.G);
-    // GEN counts in global atomics add into the chunk partials, which must
-    // then be cleared before any tile (of either kind) writes its counts
-    if (ds.ngen > 0 && !g.ni_in_lds) {
-        int rc = launch_fill_i32(c.Ni_part, (int64_t)c.G * K * ds.KT, 0, st);
+    // the tiles add their counts of clusters k < K-1 into Ni
+    if (K > 1) {
+        int rc = launch_fill_i32(c.Ni, (int64_t)(K - 1) * ds.KT, 0, st);
         if (rc) return rc;
     }
     int64_t zbase = 0;   // Record partials: the BI work items', then the GEN blocks'
@@ -819,23 +862,23 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st,
         const size_t shm = bi_p_bytes<K, Real>(kWavesPerBlock) + bi_tab_bytes(kWavesPerBlock, ds.bi_tab_rows) + mi_packed;
         if (ds.maxploidy == 2)
             hipLaunchKernelGGL((k_z_admix_bi<K, Real, 2, REC, X96>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
-                               c.Ni_part, c.Mi, c.Nc, c.G, c.jump, c.zlik);
+                               c.Ni, c.Mi, c.Nc, c.G, c.jump, c.zlik);
         else if (ds.maxploidy == 4)
             hipLaunchKernelGGL((k_z_admix_bi<K, Real, 4, REC, X96>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
-                               c.Ni_part, c.Mi, c.Nc, c.G, c.jump, c.zlik);
+                               c.Ni, c.Mi, c.Nc, c.G, c.jump, c.zlik);
         else
             hipLaunchKernelGGL((k_z_admix_bi<K, Real, 0, REC, X96>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
-                               c.Ni_part, c.Mi, c.Nc, c.G, c.jump, c.zlik);
+                               c.Ni, c.Mi, c.Nc, c.G, c.jump, c.zlik);
         VPS_HIP(hipGetLastError());
     }
     if (ds.ngen > 0) {
         dim3 grid((unsigned)ds.ngen, (unsigned)((c.G + g.waves - 1) / g.waves));
         if (g.plds)
             hipLaunchKernelGGL((k_z_admix_gen<K, Real, true, REC, X96>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q,
-                               c.seed, c.mdev, c.Ni_part, c.Mi, A, 1, c.Nc, c.G, c.jump, c.zlik, zbase);
+                               c.seed, c.mdev, c.Ni, c.Mi, A, 1, c.Nc, c.G, c.jump, c.zlik, zbase);
         else
             hipLaunchKernelGGL((k_z_admix_gen<K, Real, false, REC, X96>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q,
-                               c.seed, c.mdev, c.Ni_part, c.Mi, A, g.ni_in_lds ? 1 : 0, c.Nc, c.G, c.jump, c.zlik, zbase);
+                               c.seed, c.mdev, c.Ni, c.Mi, A, g.ni_in_lds ? 1 : 0, c.Nc, c.G, c.jump, c.zlik, zbase);
         VPS_HIP(hipGetLastError());
         zbase += (int64_t)grid.x * grid.y;
     }
@@ -848,7 +891,7 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st,
 template <int K, typename Real, bool X96>
 int z_occupancy(const DeviceDataset& ds, int Nc, int* bi_blocks, int* gen_waves, int* gen_block_waves)
 {
-    const size_t mi_packed = (size_t)Nc * bi_row_words<K>(ds) * sizeof(uint32_t);
+    const size_t mi_packed = (size_t)kWavesPerBlock * Nc * bi_row_words<K>(ds) * sizeof(uint32_t);
     const size_t p_bytes = bi_p_bytes<K, Real>(kWavesPerBlock) + bi_tab_bytes(kWavesPerBlock, ds.bi_tab_rows);
     *bi_blocks = 1;
     hipError_t e = hipSuccess;
@@ -976,7 +1019,7 @@ int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_
 {
     int rc = launch_fill_i32(c.Mi, (int64_t)ds.N * c.K, 0, st);
     if (!rc) rc = dispatch_counts(ds, c, st);
-    if (!rc) rc = launch_reduce_ni(ds, c, st);
+    if (!rc) rc = launch_z_finish(ds, c, st);
     return rc;
 }
 
@@ -1006,9 +1049,18 @@ int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, h
     rc = rec ? dispatch_k<double, true>(ds, c, st, &nparts) : dispatch_counts(ds, c, st);
     if (ev1) VPS_HIP(hipEventRecord(ev1, st));
     if (rc) return rc;
-    rc = launch_reduce_ni(ds, c, st, advance);
+    rc = launch_z_finish(ds, c, st, advance);
     if (!rc && rec) rc = launch_reduce_lnl(c.zlik, nparts, c.scal + 1, st);
     return rc;
+}
+
+int launch_z_finish(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, bool advance)
+{
+    const int64_t n = std::max<int64_t>(ds.KT, ds.N);
+    hipLaunchKernelGGL(k_z_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ds.ncopies, ds.vt, c.Ni, c.Mi,
+                       ds.KT, c.K, ds.N, advance ? c.mdev : nullptr);
+    VPS_HIP(hipGetLastError());
+    return VPS_OK;
 }
 
 int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, bool advance)
