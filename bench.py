@@ -2,17 +2,23 @@
 """MCMC sweeps/s of vcfpop's Bayesian clustering on one MI355X, plus the
 batched, served and mixed-precision forms.
 
-Headline (`value`): the plain drop-in at N=2,000 individuals, L=100,000
-biallelic diploid SNPs packed at 2 bits, K=5, admixture model with alpha
-Metropolis, FP64 (synthetic data: Fst 0.05 drift, 2% missing).  A step is one
-full sweep (UpdateP, UpdateQ [Metropolis every 10th], UpdateZ, UpdateAlpha,
-Record every 10th) with inputs resident in HBM.
+Headline (`value`): the drop-in at N=2,000 individuals, L=100,000 biallelic
+diploid SNPs packed at 2 bits, K=5, admixture model with alpha Metropolis,
+FP64 (synthetic data: Fst 0.05 drift, 2% missing), drawing the reference's
+own random streams (rng_topology "reference": every draw, and so every
+result, equals vcfpop's).  A step is one full sweep (UpdateP, UpdateQ
+[Metropolis every 10th], UpdateZ, UpdateAlpha, Record every 10th) with
+inputs resident in HBM.
 
 Extra objects on the same JSON line:
   roofline      the Z pass, timed with HIP events on the chain's stream
                 (every 10th pass of the timed region)
   cpu_baseline  the CPU oracle (C, OpenMP) on a bounded number of sweeps of
                 the same workload on the host cores
+  parallel_topology  the same workload with one stream per (k, l) and per
+                individual for P and Q (same distributions, other draws)
+  real_float    the same workload as BAYESIAN<float> (-g_float=single):
+                REAL32 (-g_fastsingle=no) and FAST32 (vcfpop's default)
   batched       64 chains (K=2..9 x 8 seeds) on N=1,000 x L=50,000: chains/hr
                 one chain at a time vs one graph launch per sweep of all 64
   served        a Poisson job stream (N ~ U[100,5000], L ~ logU[1k,200k],
@@ -55,8 +61,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="admix_n2000_l100k_k5")
-    ap.add_argument("--topology", default="parallel", choices=["parallel", "reference"])
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--topology", default="reference", choices=["parallel", "reference"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "real32", "fast32"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
@@ -87,7 +93,7 @@ def main():
     s = V.make_synth(args.workload)
     ds = V.Dataset(s, 0)
     topo = V.RNG_PARALLEL if args.topology == "parallel" else V.RNG_REFERENCE
-    prec = V.PREC_MIXED if args.precision == "mixed" else V.PREC_FP64
+    prec = {"fp64": V.PREC_FP64, "mixed": V.PREC_MIXED, "real32": V.PREC_REAL32, "fast32": V.PREC_FAST32}[args.precision]
     p = V.params_default(admix_val=1, locpriori_val=1 if spec.get("locpriori") else 2,
                          nburnin=spec["nburnin"], nreps=spec["nreps"], g_seed=1,
                          rng_topology=topo, precision=prec)
@@ -131,6 +137,25 @@ def main():
     other_leg = {"rng_topology": other, "sweeps_per_s": round(args.steps / dto, 3),
                  "ms_per_step": round(dto * 1e3 / args.steps, 4), "steps": args.steps, "warmup": args.warmup}
 
+    # the same workload as BAYESIAN<float>, same topology and steps
+    real_float = {}
+    for name, pr in (("real32", V.PREC_REAL32), ("fast32", V.PREC_FAST32)):
+        if pr == prec:
+            continue
+        log("precision", name)
+        pf = V.params_default(**{f: getattr(p, f) for f, _ in p._fields_})
+        pf.precision = pr
+        cf = V.Chain(ds, pf, K, 1)
+        cf.step(args.warmup)
+        cf.sync()
+        t0 = time.perf_counter()
+        cf.step(args.steps)
+        cf.sync()
+        dtf = time.perf_counter() - t0
+        cf.close()
+        real_float[name] = {"sweeps_per_s": round(args.steps / dtf, 3), "ms_per_step": round(dtf * 1e3 / args.steps, 4),
+                            "rng_topology": args.topology}
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -148,7 +173,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == V.PREC_FP64 else "f32-weights/f64",
+        "dtype": {V.PREC_FP64: "f64", V.PREC_MIXED: "f32-weights/f64", V.PREC_REAL32: "f32-state/f64",
+                  V.PREC_FAST32: "f32"}[prec],
         "data": "synthetic (seeded generator: Fst 0.05 drift, 2% missing)",
         "config": {"workload": args.workload, "N": ds.N, "L": ds.L, "K": K, "model": "admixture+alpha Metropolis" + ("+locprior" if spec.get("locpriori") else ""),
                    "rng_topology": args.topology, "precision": args.precision,
@@ -167,6 +193,7 @@ def main():
         },
         "cpu_baseline": cpu,
         f"{other}_topology": other_leg,
+        "real_float": real_float,
     }
     ds.close()
     if not args.no_extra:

@@ -151,9 +151,9 @@ int64_t z_lik_parts(const DeviceDataset& ds, int G);
 int launch_reduce_lnl(const double* part, int64_t n, double* out, hipStream_t st);
 int launch_z_noadmix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
 int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);
-int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, bool advance = false);
-// the last cluster's Ni row and Mi column after the admixture Z pass
-int launch_z_finish(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, bool advance = false);
+// Ni from the chunk partials (and, mi_last: the Mi column of the last
+// cluster, after the admixture Z pass)
+int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, bool advance = false, bool mi_last = false);
 // likelihood passes: mode 0 = Record lnL (admix), 1 = Record lnL (no admix),
 // 2 = Metropolis log ratio per individual, 3 = no-admix per-(i,k) log likelihood.
 // Modes 0/1 write the lnL to lnl_out, or to c.scal[1] when it is null.

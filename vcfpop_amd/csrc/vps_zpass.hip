@@ -19,16 +19,16 @@
 //     remainder: t_k < w_k  <=>  fl(t_k - w_k) < 0 for finite doubles, and
 //     once negative the remainder stays negative, so
 //     C_k = [t_{k+1} < 0] = [k* <= k]  (cumulative indicator, k < K-1).
-//   * Ni: each lane accumulates C_k per allele in registers and adds
-//     Ni[k] = C_k - C_{k-1} into Ni by atomics after the chunk (linear in
-//     the counts; no per-chunk partials).
+//   * Ni: each lane accumulates C_k per allele in registers;
+//     Ni[k] = C_k - C_{k-1} after the chunk (linear in the counts), written
+//     to the chunk's partial table, which k_reduce_ni sums.
 //   * Mi: per k the wavefront ballots C_k over typed lanes and popcounts
 //     it on the scalar unit; per individual the counts go to the
 //     wavefront's LDS row, and the block adds its wavefronts' rows (same
 //     individuals, other tiles) to Mi once at the end.
 //   * Only clusters k < K-1 are counted: every typed copy lands in one
 //     cluster, so after the pass Ni[K-1] = copies of the allele - the rest
-//     and Mi[i][K-1] = vt_i - the rest (k_z_finish).
+//     and Mi[i][K-1] = vt_i - the rest (k_reduce_ni).
 #include <algorithm>
 #include "vps_chain.h"
 
@@ -185,7 +185,7 @@ template <int K, typename Real, int MPT, bool BIG, bool REC, bool X96>
 __device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileDesc& t, const double* __restrict__ Q,
                                           uint64_t seed, int m, int64_t l, bool valid, int afo, int k2, int lane,
                                           int c, int i0, int i1, const Real* pl0, uint8_t* tab_lds,
-                                          uint32_t* mi_lds, int32_t* __restrict__ Ni,
+                                          uint32_t* mi_lds, int32_t* __restrict__ Ni_part,
                                           const uint32_t* __restrict__ jump)
 {
     const int64_t KT = ds.KT;
@@ -332,19 +332,19 @@ __device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileD
         }
     }
 
-    // the lane's counts of clusters k < K-1 go straight into Ni (the chunks
-    // of a tile add into the same entries); the last cluster's follow from
-    // the allele totals after the pass
+    // the lane's counts of clusters k < K-1, into the chunk's partial
+    // (plain stores: device-wide atomics would each reach memory); the last
+    // cluster's follow from the allele totals after the pass
     if (valid) {
-        int32_t* dst = Ni + afo;
+        int32_t* dst = Ni_part + (int64_t)c * K * KT + afo;
         uint32_t prev = 0;
 #pragma unroll
         for (int k = 0; k < K - 1; ++k) {
             const uint32_t d = nc[k] - prev;   // per-half differences never borrow (cumulative counts)
             prev = nc[k];
             const int all = (int)(d & 0xFFFFu), one = (int)(d >> 16);
-            if (all - one) atomicAdd(dst + k * KT, all - one);
-            if (one) atomicAdd(dst + k * KT + 1, one);
+            dst[k * KT] = all - one;
+            if (k2 > 1) dst[k * KT + 1] = one;
         }
     }
     return REC ? acc.close() : 0.0;
@@ -381,7 +381,7 @@ template <int K, typename Real, int MPT, bool REC, bool X96>
 __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, const double* __restrict__ Freq,
                                                    const double* __restrict__ Q, uint64_t seed,
                                                    const int32_t* __restrict__ mp,
-                                                   int32_t* __restrict__ Ni, int32_t* __restrict__ Mi,
+                                                   int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
                                                    int Nc, int G, const uint32_t* __restrict__ jump,
                                                    double* __restrict__ zlik)
 {
@@ -434,10 +434,10 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
     if (has_tile) {
         if (t.bits > kBiRegBits)
             lnl = z_bi_tile<K, Real, MPT, true, REC, X96>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane,
-                                                    tab_lds, mi_lds, Ni, jump);
+                                                    tab_lds, mi_lds, Ni_part, jump);
         else
             lnl = z_bi_tile<K, Real, MPT, false, REC, X96>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane,
-                                                     tab_lds, mi_lds, Ni, jump);
+                                                     tab_lds, mi_lds, Ni_part, jump);
     }
     __shared__ double wsum[kWavesMax];
     if (REC) {
@@ -476,7 +476,7 @@ template <int K, typename Real, bool PLDS, bool REC, bool X96>
 __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceDataset ds, const double* __restrict__ Freq,
                                                     const double* __restrict__ Q, uint64_t seed,
                                                     const int32_t* __restrict__ mp,
-                                                    int32_t* __restrict__ Ni, int32_t* __restrict__ Mi,
+                                                    int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
                                                     int A, int ni_in_lds, int Nc, int G,
                                                     const uint32_t* __restrict__ jump, double* __restrict__ zlik,
                                                     int64_t zbase)
@@ -543,7 +543,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceData
             return g;
         };
         const int MP = ds.maxploidy;
-        int32_t* nig = Ni + afo;
+        int32_t* nig = Ni_part + (int64_t)c * K * KT + afo;
         uint32_t mval = 0;
 
         UnitBits ub;
@@ -619,11 +619,10 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceData
     }
     if (ni_in_lds && valid) {
         for (int k = wv; k < K - 1; k += nwv) {
-            int32_t* dst = Ni + (int64_t)k * KT + afo;
-            for (int x = 0; x < k2; ++x) {
-                const int32_t v = ni_lds[(k * A + x) * kTileLoci + lane];
-                if (v) atomicAdd(dst + x, v);
-            }
+            int32_t* dst = Ni_part + ((int64_t)c0 * K + k) * KT + afo;
+            for (int x = 0; x < k2; ++x) dst[x] = ni_lds[(k * A + x) * kTileLoci + lane];
+            for (int cc = 1; cc < nwv && c0 + cc < G; ++cc)
+                for (int x = 0; x < k2; ++x) dst[(int64_t)cc * K * KT + x] = 0;
         }
     }
     flush_mi_rows<K>(mi_lds, gi1 - gi0, Mi + (int64_t)gi0 * K);
@@ -706,13 +705,27 @@ __global__ __launch_bounds__(64) void k_z_noadmix(const DeviceDataset ds, const 
 // (the Z passes leave the last cluster's partials unwritten).  A block covers
 // 64 alleles x 4 cluster rows; the row totals meet in LDS in a fixed order.
 constexpr int kReduceRows = 4;
+// With vt (after the admixture Z pass, which counts clusters k < K-1 only)
+// it also sets Mi[i][K-1] = vt_i - the individual's other counts, one thread
+// per individual.
 // (advance, when not null: the sweep index, advanced once by the first thread)
 __global__ __launch_bounds__(64 * kReduceRows) void k_reduce_ni(const int32_t* __restrict__ Ni_part,
                                                                const int32_t* __restrict__ ncopies,
                                                                int32_t* __restrict__ Ni, int64_t KT, int K, int G,
-                                                               int32_t* __restrict__ advance)
+                                                               int32_t* __restrict__ advance,
+                                                               const int32_t* __restrict__ vt, int32_t* __restrict__ Mi,
+                                                               int N)
 {
     if (advance && blockIdx.x == 0 && threadIdx.x == 0) advance[0] += 1;
+    if (vt) {
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i < N) {
+            int32_t* row = Mi + i * K;
+            int32_t t = 0;
+            for (int k = 0; k < K - 1; ++k) t += row[k];
+            row[K - 1] = vt[i] - t;
+        }
+    }
     __shared__ int32_t tot[kReduceRows][64];
     const int al = threadIdx.x & 63, kk = threadIdx.x >> 6;
     const int64_t a = (int64_t)blockIdx.x * 64 + al;
@@ -741,30 +754,6 @@ __global__ __launch_bounds__(64 * kReduceRows) void k_reduce_ni(const int32_t* _
         int32_t t = 0;
         for (int r = 0; r < kReduceRows; ++r) t += tot[r][al];
         Ni[(int64_t)(K - 1) * KT + a] = ncopies[a] - t;
-    }
-}
-
-// After the admixture Z pass: the last cluster's counts, which the pass
-// does not count: Ni[K-1][a] = copies of allele a - sum of the other rows,
-// and Mi[i][K-1] = vt_i - the individual's other counts (every typed copy is
-// in exactly one cluster).  One thread per allele entry and per individual.
-// (advance, when not null: the sweep index, advanced once by the first thread)
-__global__ void k_z_finish(const int32_t* __restrict__ ncopies, const int32_t* __restrict__ vt,
-                           int32_t* __restrict__ Ni, int32_t* __restrict__ Mi, int64_t KT, int K, int N,
-                           int32_t* __restrict__ advance)
-{
-    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (advance && x == 0) advance[0] += 1;
-    if (x < KT) {
-        int32_t t = 0;
-        for (int k = 0; k < K - 1; ++k) t += Ni[k * KT + x];
-        Ni[(int64_t)(K - 1) * KT + x] = ncopies[x] - t;
-    }
-    if (x < N) {
-        int32_t* row = Mi + x * K;
-        int32_t t = 0;
-        for (int k = 0; k < K - 1; ++k) t += row[k];
-        row[K - 1] = vt[x] - t;
     }
 }
 
@@ -845,9 +834,10 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st,
     const int A = std::max(ds.maxK, 1);
     const size_t mi_packed = (size_t)kWavesPerBlock * c.Nc * bi_row_words<K>(ds) * sizeof(uint32_t);
     const GenPlan g = gen_plan<K, Real>(ds, c.Nc, c.G);
-    // the tiles add their counts of clusters k < K-1 into Ni
-    if (K > 1) {
-        int rc = launch_fill_i32(c.Ni, (int64_t)(K - 1) * ds.KT, 0, st);
+    // GEN counts in global atomics add into the chunk partials, which must
+    // then be cleared before any tile (of either kind) writes its counts
+    if (ds.ngen > 0 && !g.ni_in_lds) {
+        int rc = launch_fill_i32(c.Ni_part, (int64_t)c.G * K * ds.KT, 0, st);
         if (rc) return rc;
     }
     int64_t zbase = 0;   // Record partials: the BI work items', then the GEN blocks'
@@ -862,23 +852,23 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st,
         const size_t shm = bi_p_bytes<K, Real>(kWavesPerBlock) + bi_tab_bytes(kWavesPerBlock, ds.bi_tab_rows) + mi_packed;
         if (ds.maxploidy == 2)
             hipLaunchKernelGGL((k_z_admix_bi<K, Real, 2, REC, X96>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
-                               c.Ni, c.Mi, c.Nc, c.G, c.jump, c.zlik);
+                               c.Ni_part, c.Mi, c.Nc, c.G, c.jump, c.zlik);
         else if (ds.maxploidy == 4)
             hipLaunchKernelGGL((k_z_admix_bi<K, Real, 4, REC, X96>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
-                               c.Ni, c.Mi, c.Nc, c.G, c.jump, c.zlik);
+                               c.Ni_part, c.Mi, c.Nc, c.G, c.jump, c.zlik);
         else
             hipLaunchKernelGGL((k_z_admix_bi<K, Real, 0, REC, X96>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
-                               c.Ni, c.Mi, c.Nc, c.G, c.jump, c.zlik);
+                               c.Ni_part, c.Mi, c.Nc, c.G, c.jump, c.zlik);
         VPS_HIP(hipGetLastError());
     }
     if (ds.ngen > 0) {
         dim3 grid((unsigned)ds.ngen, (unsigned)((c.G + g.waves - 1) / g.waves));
         if (g.plds)
             hipLaunchKernelGGL((k_z_admix_gen<K, Real, true, REC, X96>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q,
-                               c.seed, c.mdev, c.Ni, c.Mi, A, 1, c.Nc, c.G, c.jump, c.zlik, zbase);
+                               c.seed, c.mdev, c.Ni_part, c.Mi, A, 1, c.Nc, c.G, c.jump, c.zlik, zbase);
         else
             hipLaunchKernelGGL((k_z_admix_gen<K, Real, false, REC, X96>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q,
-                               c.seed, c.mdev, c.Ni, c.Mi, A, g.ni_in_lds ? 1 : 0, c.Nc, c.G, c.jump, c.zlik, zbase);
+                               c.seed, c.mdev, c.Ni_part, c.Mi, A, g.ni_in_lds ? 1 : 0, c.Nc, c.G, c.jump, c.zlik, zbase);
         VPS_HIP(hipGetLastError());
         zbase += (int64_t)grid.x * grid.y;
     }
@@ -1019,7 +1009,7 @@ int launch_z_admix_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_
 {
     int rc = launch_fill_i32(c.Mi, (int64_t)ds.N * c.K, 0, st);
     if (!rc) rc = dispatch_counts(ds, c, st);
-    if (!rc) rc = launch_z_finish(ds, c, st);
+    if (!rc) rc = launch_reduce_ni(ds, c, st, false, true);
     return rc;
 }
 
@@ -1049,24 +1039,16 @@ int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, h
     rc = rec ? dispatch_k<double, true>(ds, c, st, &nparts) : dispatch_counts(ds, c, st);
     if (ev1) VPS_HIP(hipEventRecord(ev1, st));
     if (rc) return rc;
-    rc = launch_z_finish(ds, c, st, advance);
+    rc = launch_reduce_ni(ds, c, st, advance, true);
     if (!rc && rec) rc = launch_reduce_lnl(c.zlik, nparts, c.scal + 1, st);
     return rc;
 }
 
-int launch_z_finish(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, bool advance)
+int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, bool advance, bool mi_last)
 {
-    const int64_t n = std::max<int64_t>(ds.KT, ds.N);
-    hipLaunchKernelGGL(k_z_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ds.ncopies, ds.vt, c.Ni, c.Mi,
-                       ds.KT, c.K, ds.N, advance ? c.mdev : nullptr);
-    VPS_HIP(hipGetLastError());
-    return VPS_OK;
-}
-
-int launch_reduce_ni(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, bool advance)
-{
-    hipLaunchKernelGGL(k_reduce_ni, dim3((unsigned)((ds.KT + 63) / 64)), dim3(64 * kReduceRows), 0, st, c.Ni_part,
-                       ds.ncopies, c.Ni, ds.KT, c.K, c.G, advance ? c.mdev : nullptr);
+    const int64_t nb = std::max<int64_t>((ds.KT + 63) / 64, mi_last ? (ds.N + 64 * kReduceRows - 1) / (64 * kReduceRows) : 0);
+    hipLaunchKernelGGL(k_reduce_ni, dim3((unsigned)nb), dim3(64 * kReduceRows), 0, st, c.Ni_part, ds.ncopies, c.Ni, ds.KT,
+                       c.K, c.G, advance ? c.mdev : nullptr, mi_last ? ds.vt : nullptr, c.Mi, ds.N);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
