@@ -65,7 +65,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "real32", "fast32"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="only the headline workload (no other-topology, float, batched, served or precision legs)")
     ap.add_argument("--batch-sweeps", type=int, default=40)
     ap.add_argument("--serve-jobs", type=int, default=24)
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
@@ -123,24 +124,26 @@ def main():
 
     # the same workload with the other RNG topology, same steps
     other = "reference" if args.topology == "parallel" else "parallel"
-    log("topology", other)
-    po = V.params_default(**{f: getattr(p, f) for f, _ in p._fields_})
-    po.rng_topology = V.RNG_REFERENCE if other == "reference" else V.RNG_PARALLEL
-    co = V.Chain(ds, po, K, 1)
-    co.step(args.warmup)
-    co.sync()
-    t0 = time.perf_counter()
-    co.step(args.steps)
-    co.sync()
-    dto = time.perf_counter() - t0
-    co.close()
-    other_leg = {"rng_topology": other, "sweeps_per_s": round(args.steps / dto, 3),
-                 "ms_per_step": round(dto * 1e3 / args.steps, 4), "steps": args.steps, "warmup": args.warmup}
+    other_leg = None
+    if not args.no_extra:
+        log("topology", other)
+        po = V.params_default(**{f: getattr(p, f) for f, _ in p._fields_})
+        po.rng_topology = V.RNG_REFERENCE if other == "reference" else V.RNG_PARALLEL
+        co = V.Chain(ds, po, K, 1)
+        co.step(args.warmup)
+        co.sync()
+        t0 = time.perf_counter()
+        co.step(args.steps)
+        co.sync()
+        dto = time.perf_counter() - t0
+        co.close()
+        other_leg = {"rng_topology": other, "sweeps_per_s": round(args.steps / dto, 3),
+                     "ms_per_step": round(dto * 1e3 / args.steps, 4), "steps": args.steps, "warmup": args.warmup}
 
     # the same workload as BAYESIAN<float>, same topology and steps
     real_float = {}
     for name, pr in (("real32", V.PREC_REAL32), ("fast32", V.PREC_FAST32)):
-        if pr == prec:
+        if pr == prec or args.no_extra:
             continue
         log("precision", name)
         pf = V.params_default(**{f: getattr(p, f) for f, _ in p._fields_})
