@@ -459,6 +459,12 @@ int vps_debug_chunking(const vps_chain* ch, int32_t* G, int32_t* Nc);
 /* wavefronts per block of the multiallelic (GEN) Z-pass kernel for the chain's
    chunking: 4, or 8 when LDS limits the resident blocks and G >= 8 */
 int vps_debug_gen_waves(const vps_chain* ch, int32_t* waves);
+/* Marsaglia–Tsang's second acceptance test (src/statistics.cpp:239) as the
+   device's Gamma draws decide it, from single-precision estimates with an
+   error bound and the exact double form inside the bound: accept = 1/0 for n
+   values, and *n_exact = how many needed the exact form */
+int vps_debug_mt_accept(const double* u, const double* t, const double* v, const double* d, int64_t n,
+                        uint8_t* accept, int64_t* n_exact);
 /* one warm-up plus `iters` launches of a plain device copy of `bytes` bytes
    (scales hardware traffic counters); average ms per copy */
 int vps_debug_copy(int device, size_t bytes, int iters, double* ms_per_copy);

@@ -37,6 +37,7 @@ from ._lib import (  # noqa: F401
     device_count,
     debug_jump,
     debug_jump96,
+    debug_mt_accept,
     debug_copy,
 )
 from .workloads import WORKLOADS, make_synth  # noqa: F401

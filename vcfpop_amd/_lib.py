@@ -195,6 +195,7 @@ _SIGS = {
     "vps_debug_decode": ([_P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), _P], C.c_int),
     "vps_debug_jump": ([C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
     "vps_debug_jump96": ([C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)], C.c_int),
+    "vps_debug_mt_accept": ([C.POINTER(C.c_double)] * 4 + [C.c_int64, C.POINTER(C.c_uint8), C.POINTER(C.c_int64)], C.c_int),
     "vps_debug_chunking": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
     "vps_debug_gen_waves": ([_P, C.POINTER(C.c_int32)], C.c_int),
     "vps_debug_copy": ([C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
@@ -257,6 +258,19 @@ def debug_jump96(D, x, y, z):
     xx, yy, zz = C.c_uint32(x), C.c_uint32(y), C.c_uint32(z)
     _check(load().vps_debug_jump96(C.c_uint64(D), C.byref(xx), C.byref(yy), C.byref(zz)))
     return xx.value, yy.value, zz.value
+
+
+def debug_mt_accept(u, t, v, d):
+    """the device Gamma draws' acceptance decision (host build of the same
+    code): (accept flags, number decided by the exact double form)"""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, t, v, d)]
+    n = len(arrs[0])
+    out = np.zeros(n, dtype=np.uint8)
+    ex = C.c_int64()
+    ptr = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    _check(load().vps_debug_mt_accept(*[ptr(a) for a in arrs], C.c_int64(n),
+                                       out.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(ex)))
+    return out.astype(bool), ex.value
 
 
 def debug_copy(nbytes, iters=10, device=0):

@@ -126,7 +126,14 @@ struct ChainDev {
     int32_t*  pg_next;
     int32_t*  pg_path;
     int64_t   pg_W, pg_S;
+    // reference draw streams: the generator of every thread of the parallel
+    // stream parse at its ring offset, prepared one sweep ahead for UpdateP's
+    // 128 streams and UpdateQ's stream, [2][kSppStreams][1024] by sweep
+    // parity, with the sweep each stream's entry was prepared for
+    Xs128*    spp_prep;
+    int32_t*  spp_tag;   // [2][kSppStreams]
 };
+constexpr int kSppStreams = 129;    // UpdateP's 128 streams, then UpdateQ's
 
 // REAL = float (BAYESIAN<float>): P, Q, the proposals and FreqM hold
 // single-precision values

@@ -492,6 +492,20 @@ extern "C" int vps_debug_jump(uint64_t D, uint64_t* x, uint64_t* y)
     return VPS_OK;
 }
 
+extern "C" int vps_debug_mt_accept(const double* u, const double* t, const double* v, const double* d, int64_t n,
+                                   uint8_t* accept, int64_t* n_exact)
+{
+    if (!u || !t || !v || !d || !accept || n < 0) return fail(VPS_ERR_INVALID, "null argument");
+    int64_t ex = 0;
+    for (int64_t x = 0; x < n; ++x) {
+        const MtDecision r = mt_log_decide(u[x], t[x], v[x], d[x]);
+        accept[x] = r == kMtAccept ? 1 : r == kMtReject ? 0 : (mt_log_exact(u[x], t[x], v[x], d[x]) ? 1 : 0);
+        ex += r == kMtExact;
+    }
+    if (n_exact) *n_exact = ex;
+    return VPS_OK;
+}
+
 extern "C" int vps_debug_jump96(uint64_t D, uint32_t* x, uint32_t* y, uint32_t* z)
 {
     if (!x || !y || !z) return fail(VPS_ERR_INVALID, "null argument");

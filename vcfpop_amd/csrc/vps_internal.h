@@ -61,13 +61,12 @@ struct TileDesc {
 // p holding v (4 dwords: x_lo, x_hi, y_lo, y_hi).
 constexpr int kJumpWords = 32 * 16 * 4;
 // jump tables of the parallel stream parse: [0, 64) advance 8*l draws (l =
-// lane), [64, 80) advance 512*w draws (w = wave); table 72 is 4,096 draws
+// lane), [64, 80) advance 512*w draws (w = wave)
 constexpr int kRingLaneTabs = 64, kRingWaveTabs = 16;
 // then tables by 4096 * 2^j draws, j < kRingPow2Tabs (any multiple of 4096
-// in at most kRingPow2Tabs jumps)
+// in at most kRingPow2Tabs jumps; table j = 1 is one 8,192-word ring cycle)
 constexpr int kRingPow2Base = kRingLaneTabs + kRingWaveTabs, kRingPow2Tabs = 24;
 constexpr int kRingJumps = kRingPow2Base + kRingPow2Tabs;
-constexpr int kRingAdvanceTab = kRingLaneTabs + 8;
 
 struct DeviceDataset {
     // sizes

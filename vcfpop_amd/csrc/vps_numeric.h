@@ -193,6 +193,63 @@ VPS_HD double my_sin(double x)
 }
 VPS_HD double my_cos(double x) { return my_sin(x + (kPi / 2)); }
 
+// Marsaglia–Tsang's second acceptance test as the reference evaluates it,
+//     log(u) < 0.5 * t * t + d * (1.0 - v + log(v))      (src/statistics.cpp:239)
+// in double precision.  Two double-precision logarithms cost about 200
+// instructions, so the two sides are first estimated cheaply:
+//   * log(u) from the single-precision logarithm of (float)u: input
+//     rounding moves it by at most 2^-24 and the single-precision result is
+//     within a few ulps, so it is within 3e-7 * (1 + |log u|);
+//   * g = 1 - v + log(v) = log1p(w) - w with w = v - 1 (exact for v in
+//     [0.5, 2]): for |w| <= 1/8 the alternating series -w^2/2 + w^3/3 - ...
+//     to w^17, whose tail is below |w|^18 / 18 / (1 - |w|) < 4e-18; otherwise
+//     1 - v plus the single-precision log(v), within 3e-7 * (1 + |log v|);
+//   * the reference's own rounding of the double expression is a few ulps
+//     of its largest intermediate, bounded here by 1e-15 of their sum.
+// When the estimates differ by more than the bound the exact comparison has
+// the same outcome; otherwise it is evaluated as the reference writes it.
+// Inputs outside the ranges the bounds cover (u below 2^-100, v outside
+// [2^-100, 2^100], NaN) take the exact form too.
+enum MtDecision { kMtReject = 0, kMtAccept = 1, kMtExact = 2 };
+VPS_HD bool mt_log_exact(double u, double t, double v, double d)
+{
+    return log(u) < 0.5 * t * t + d * (1.0 - v + log(v));
+}
+VPS_HD MtDecision mt_log_decide(double u, double t, double v, double d)
+{
+    const double tt = 0.5 * t * t;
+    if (u >= 0x1p-100 && v >= 0x1p-100 && v <= 0x1p100 && d >= 0) {
+        const double lu = (double)logf((float)u);
+        const double w = v - 1.0;
+        double g, eg;
+        if (w >= -0.125 && w <= 0.125) {
+            // log1p(w) - w = w^2 * S, S = sum_{n>=2} (-1)^(n+1) w^(n-2) / n,
+            // by Horner from n = 17
+            double s = 1.0 / 17;
+#pragma unroll
+            for (int n = 16; n >= 2; --n) s = ((n & 1) ? 1.0 : -1.0) / n + w * s;
+            g = (w * w) * s;
+            eg = 4e-18 + 4e-15 * (w * w);
+        } else {
+            const double lv = (double)logf((float)v);
+            g = -w + lv;
+            eg = 3e-7 * (1.0 + fabs(lv)) + 1e-15 * fabs(w);
+        }
+        const double rhs = tt + d * g;
+        const double bound = 3e-7 * (1.0 + fabs(lu)) + d * eg +
+                             1e-15 * (tt + d * (fabs(w) + fabs(g) + 1.0) + fabs(lu) + 1.0);
+        const double diff = lu - rhs;
+        if (diff < -bound) return kMtAccept;
+        if (diff > bound) return kMtReject;
+    }
+    return kMtExact;
+}
+VPS_HD bool mt_log_accept(double u, double t, double v, double d)
+{
+    const MtDecision r = mt_log_decide(u, t, v, d);
+    return r == kMtExact ? mt_log_exact(u, t, v, d) : r == kMtAccept;
+}
+
 // RNG<double> with the Box–Muller cache (U1, U2), src/statistics.h:15-87.
 // The cache is empty when U1's low 32 bits are all ones.  Src supplies the
 // raw 64-bit draws: the xorshift128+ generator, or words already drawn.
@@ -254,7 +311,7 @@ struct RngT {
             v = v * v * v;
             u = uniform();
             if (u < 1.0 - 0.0331 * t * t * t * t) break;
-            if (log(u) < 0.5 * t * t + d * (1.0 - v + log(v))) break;
+            if (mt_log_accept(u, t, v, d)) break;
         }
         return beta * d * v;
     }

@@ -306,6 +306,12 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
             if (!rc) rc = chain_alloc(ch, &c.pg_path, (int64_t)d.ntyped * K);
         }
     }
+    if (!rc && c.topology == VPS_RNG_REFERENCE) {
+        rc = chain_alloc(ch, &c.spp_prep, 2 * (int64_t)kSppStreams * 1024);
+        if (!rc) rc = chain_alloc(ch, &c.spp_tag, 2 * kSppStreams);
+        if (!rc && hipMemsetAsync(c.spp_tag, 0xFF, 2 * kSppStreams * sizeof(int32_t), ch->stream) != hipSuccess)
+            rc = fail(VPS_ERR_DEVICE, "clear stream preparation tags");
+    }
     if (!rc) rc = chain_alloc(ch, &c.Eta, K);
     if (!rc) rc = chain_alloc(ch, &c.Gamma, S * K);
     if (!rc) rc = chain_alloc(ch, &c.Di, S * K);

@@ -126,7 +126,9 @@ int64_t estimate_bytes(const HostImage& h, int K, const vps_params& par)
         pg_sizes((int)N, K, &W, &S);
         pg = W * 8 + (W / 8) * 16 + 2 * S * 4 + N * K * 4;
     }
-    return dataset + chain + pg;
+    // the reference streams' generators prepared one sweep ahead
+    const int64_t prep = par.rng_topology == VPS_RNG_REFERENCE ? 2 * kSppStreams * (1024 * 16 + 4) : 0;
+    return dataset + chain + pg + prep;
 }
 
 struct Job {

@@ -12,7 +12,8 @@
 // every rejection.  A stream therefore has to be parsed in order.
 //
 // Parsing in parallel.  One workgroup of 1024 lanes owns a stream; lane j
-// takes draw front + j.  Almost every draw takes its first candidate, so the
+// takes draw front + j (and, in long windows, front + 1024 + j; see the
+// thread layout below).  Almost every draw takes its first candidate, so the
 // lane assumes every earlier draw of the window did: draws then alternate
 // between "pair + uniform" (3 words, leaves a normal cached) and "cached
 // normal + uniform" (1 word), plus one word per shape below 1 (a prefix
@@ -24,11 +25,15 @@
 // lane f's true end state.  Rejections are rare (about 0.3% of draws at shape
 // 10, under 0.03% at shape 1000), so a window commits ~1000 draws.
 //
-// Words come from an LDS ring of 8,192 consecutive stream outputs: thread t
-// holds the generator state 8t draws past the ring's start and writes 8
-// words; the ring moves on in steps of 4,096 with one GF(2) jump per thread.
-// Every lane reads words at known positions, so no lane steps a generator
-// serially.
+// Words come from a circular LDS ring of 8,192 consecutive stream outputs:
+// thread t writes the 8 words at indices [8t, 8t + 8); the ring moves on in
+// steps of 4,096 words, the 512 threads of the stale half each taking one
+// GF(2) jump of a ring cycle.  Every lane reads words at known positions, so
+// no lane steps a generator serially.
+//
+// The second acceptance test of each Gamma draw is decided from
+// single-precision estimates with an error bound (mt_log_accept), which keeps
+// the outcome of the reference's double expression.
 //
 // The values equal the serial parse exactly: each committed draw is the
 // reference's arithmetic (vps_numeric.h) on the words the serial parse would
@@ -71,6 +76,19 @@ __device__ __forceinline__ void ring_seed(Xs128& st, const uint32_t* __restrict_
     if (l) jump_apply(J + (size_t)l * kJumpWords, st);
 }
 
+// the seed and salt of stream s of sweep m: UpdateP's streams s < 128
+// (src/structure.cpp:843-848), UpdateQ's stream s = 128 (src/structure.cpp:990)
+__device__ __forceinline__ void spp_stream_seed(const ChainDev& c, int s, int m, uint64_t* seed, uint64_t* salt)
+{
+    if (s < kPBlocks) {
+        *seed = c.seed + (uint64_t)m * kPBlocks + s;
+        *salt = kSaltUpdateP;
+    } else {
+        *seed = c.seed + (uint64_t)m;
+        *salt = kSaltUpdateQ;
+    }
+}
+
 __device__ __forceinline__ void ring_fill(const Xs128& st, uint64_t* ring)
 {
     Xs128 s = st;
@@ -79,25 +97,45 @@ __device__ __forceinline__ void ring_fill(const Xs128& st, uint64_t* ring)
     for (int u = 0; u < 8; ++u) o[u] = s.next();
 }
 
-// the 4,096-draw jump table, copied to LDS once per block
+// A workgroup barrier that orders LDS only: the parse's global stores are
+// each written once and read by later kernels, so no wave waits for them.
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// the ring's word at stream position p (held while ws <= p < ws + kRingWords)
+__device__ __forceinline__ uint64_t ring_word(const uint64_t* ring, int64_t p) { return ring[p & (kRingWords - 1)]; }
+
+// the 8,192-draw jump table (one ring cycle), copied to LDS once per block
 __device__ __forceinline__ void ring_load_advance(const uint32_t* __restrict__ J, uint32_t* tab)
 {
-    const uint32_t* src = J + (size_t)kRingAdvanceTab * kJumpWords;
+    const uint32_t* src = J + (size_t)(kRingPow2Base + 1) * kJumpWords;
     for (int x = threadIdx.x; x < kJumpWords; x += kSppThreads) tab[x] = src[x];
 }
 
-// moves the ring's window forward by whole steps until `pos` lies in its
-// first half (a block-uniform condition; every thread calls it)
+// The ring is circular: position p sits at index p mod kRingWords, and
+// thread t owns indices [8t, 8t + 8), i.e. positions 8t + kRingWords * r.
+// `st` is the thread's generator at the start of its current block.  While
+// `pos` lies past the ring's first half, the half holding [ws, ws + kRingStep)
+// is refilled with the next cycle's words by the 512 threads that own it (one
+// cycle jump each) and the window moves on by kRingStep (a block-uniform
+// condition; every thread calls it).
 __device__ __forceinline__ void ring_follow(int64_t pos, int64_t& ws, Xs128& st, uint64_t* ring, const uint32_t* tab)
 {
     if (pos - ws < kRingStep) return;
     const int64_t n = (pos - ws) / kRingStep;
+    lds_barrier();   // (the ring's last readers are done)
 #pragma unroll 1
-    for (int64_t x = 0; x < n; ++x) jump_apply(tab, st);
-    ws += n * kRingStep;
-    __syncthreads();   // (the ring's last readers are done)
-    ring_fill(st, ring);
-    __syncthreads();
+    for (int64_t x = 0; x < n; ++x) {
+        const int half = (int)((ws / kRingStep) & 1);
+        if ((int)(threadIdx.x >> 9) == half) {
+            jump_apply(tab, st);
+            ring_fill(st, ring);
+        }
+        ws += kRingStep;
+    }
+    lds_barrier();
 }
 
 // exclusive prefix count of `flag` over the block's lanes; *total = all
@@ -132,10 +170,11 @@ __device__ __forceinline__ void draw_coords(int kind, int64_t d, int64_t KT, int
 
 // draw j of a window whose first draw is the front f: its shape and where its
 // value goes.  The coordinates follow from the front's without a division
-// (the window holds at most 1024 draws); counts come from the LDS stage
+// (the window holds at most kSppD * 1024 draws); counts come from the LDS stage
 // (indexed from the block's first draw d0) when the block has one.
 __device__ __forceinline__ double* draw_target(const DeviceDataset& ds, const ChainDev& c, int kind, const SppFront& f,
-                                               int j, const int32_t* stage, int64_t d0, int64_t nd, double* alpha)
+                                               int j, const int32_t* stage, int64_t d0, int64_t nd,
+                                               const double* prior_k, double* alpha)
 {
     const int64_t g = f.d + j;
     if (kind == kDrawP) {
@@ -144,7 +183,8 @@ __device__ __forceinline__ double* draw_target(const DeviceDataset& ds, const Ch
             k += x / ds.KT;
             x %= ds.KT;
         }
-        const double prior = c.fmodel ? c.FreqA[x] * c.Fst[c.K + (c.fsame ? 0 : k)] : real_store(c.Lambda[k], real32(c));
+        const double prior = c.fmodel ? c.FreqA[x] * c.Fst[c.K + (c.fsame ? 0 : k)]
+                                      : prior_k ? prior_k[k] : real_store(c.Lambda[k], real32(c));
         *alpha = prior + (double)(stage ? stage[g - d0] : c.Ni[g]);
         return c.Freq + g;
     }
@@ -156,7 +196,7 @@ __device__ __forceinline__ double* draw_target(const DeviceDataset& ds, const Ch
         ++t;
     }
     const int i = stage ? stage[nd + t] : ds.typed[t];
-    const double* ar = c.locpriori ? c.AlphaLocal + (int64_t)ds.popid[i] * K : c.Alpha;
+    const double* ar = c.locpriori ? c.AlphaLocal + (int64_t)ds.popid[i] * K : prior_k ? prior_k : c.Alpha;
     if (kind == kDrawQ) {
         *alpha = ar[k] + (double)(stage ? stage[g] : c.Mi[(int64_t)i * K + k]);
         return c.Q + (int64_t)i * K + k;
@@ -190,23 +230,42 @@ __device__ __forceinline__ int64_t slot_words(int64_t s, bool front_cached)
 // turns each row into its Dirichlet draw.  The Metropolis proposals' stream
 // state after the parse goes to rngbuf[0] for the acceptance kernel.
 //
-// Thread layout of a window with H hypotheses (H in 1, 2, 4, 8, 16): wave w
-// takes hypothesis w % H for draws 64 * (w / H) + lane, so a window holds
-// 1024 / H draws.  H adapts to the deviation rate seen so far: a window
-// commits about min(1024 / H, H / rate) draws.
+// Thread layout of a window with H hypotheses (H in 1, 2, 4, 8, 16) and D
+// draws per thread (D in 1, 2): thread evaluates D draws, as the virtual
+// waves w and w + 16 (w its wave); virtual wave v takes hypothesis v % H for
+// draws 64 * (v / H) + lane, so a window holds D * 1024 / H draws.  H and D
+// adapt to the deviation rate seen so far: a window commits about
+// min(D * 1024 / H, H / rate) draws at a cost of about D + kSppWindowCost
+// 1024-draw evaluations.  With one hypothesis the window ends at its first
+// deviating draw, which one wave finds by ballots; with more, one thread
+// follows the path through the hypotheses over the deviating draws.
+constexpr int kSppD = 2;                        // most draws per thread per window
+constexpr int kSppVT = kSppD * kSppThreads;     // draws (x hypotheses) per window
+constexpr int kSppVW = kSppVT / 64;             // virtual waves
+constexpr float kSppWindowCost = 0.5f;          // a window's fixed cost, in 1024-draw evaluations
+
+// one draw of a window, evaluated from its speculated start
+struct SppDraw {
+    double* out;
+    double val, U1, U2;
+    int64_t pos, bm;
+    bool cache, live;
+};
+
 __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset ds, const ChainDev c, int np, int qkind)
 {
     __shared__ uint64_t ring[kRingWords];
     __shared__ uint32_t s_adv[kJumpWords];
-    __shared__ int s_cnt[kSppThreads / 64];
-    __shared__ uint8_t s_lt1[kSppThreads];       // per draw of the window
-    __shared__ uint8_t s_next[kSppThreads];      // per thread: next hypothesis, or kNextStop / kNextOvf
-    __shared__ uint64_t s_dev[kSppThreads / 64]; // per wave: draws not typical under its hypothesis
+    __shared__ int s_cnt[kSppVW];                // per draw group: shapes below 1
+    __shared__ uint8_t s_lt1[kSppVT];            // per draw of the window
+    __shared__ uint8_t s_next[kSppVT];           // per virtual thread: next hypothesis, or kNextStop / kNextOvf
+    __shared__ uint64_t s_dev[kSppVW];           // per virtual wave: draws not typical under its hypothesis
     __shared__ int s_seg_lane[17];
     __shared__ uint8_t s_seg_h[17];
-    __shared__ int s_nseg, s_commit, s_stop_ovf, s_H;
+    __shared__ int s_nseg, s_commit, s_H, s_D, s_lt1_tot;
     __shared__ float s_rate;
     __shared__ SppFront s_front;
+    __shared__ double s_prior[VPS_MAX_K];        // per-cluster prior of the draws, when it has no other index
     __shared__ int32_t s_stage[kStageInts];
     constexpr uint8_t kNextStop = 0xFF, kNextOvf = 0xFE;
 
@@ -228,14 +287,31 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
         };
         d0 = first_g(kl0);
         d1 = first_g(kl1);
-        sseed = c.seed + (uint64_t)m * kPBlocks + b;
-        salt = kSaltUpdateP;
-    } else {
+    } else if ((int)blockIdx.x == np && qkind >= 0) {
         kind = qkind;
         d1 = (int64_t)ds.ntyped * c.K;
-        sseed = c.seed + (uint64_t)m;
-        salt = kSaltUpdateQ;
+    } else {
+        // the blocks past the streams prepare the next sweep's stream
+        // generators at every thread's ring offset (two jumps each), on CUs
+        // the streams leave free
+        const int pb = (int)blockIdx.x - np - (qkind >= 0 ? 1 : 0);
+        for (int s = 2 * pb; s < min(2 * pb + 2, kSppStreams); ++s) {
+            uint64_t sd, sa;
+            spp_stream_seed(c, s, m + 1, &sd, &sa);
+            Xs128 g;
+            g.seed(sd, sa);
+            ring_seed(g, ds.ring_jump);
+            c.spp_prep[((int64_t)((m + 1) & 1) * kSppStreams + s) * kSppThreads + tid] = g;
+            __syncthreads();
+            if (tid == 0) {
+                __threadfence();
+                c.spp_tag[((m + 1) & 1) * kSppStreams + s] = m + 1;
+            }
+        }
+        return;
     }
+    const int stream = kind == kDrawP ? (int)blockIdx.x : kPBlocks;
+    spp_stream_seed(c, stream, m, &sseed, &salt);
     // the counts the draws read, staged in LDS when they fit
     const int64_t nd = d1 - d0;
     const int32_t* stage = nullptr;
@@ -244,15 +320,29 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
         stage = s_stage;
     } else if (kind == kDrawQ && nd + ds.ntyped <= kStageInts) {
         const int K = c.K;
-        for (int64_t x = tid; x < nd; x += kSppThreads) s_stage[x] = c.Mi[(int64_t)ds.typed[x / K] * K + x % K];
-        for (int64_t x = tid; x < ds.ntyped; x += kSppThreads) s_stage[nd + x] = ds.typed[x];
+        for (int t = tid; t < ds.ntyped; t += kSppThreads) {
+            const int i = ds.typed[t];
+            for (int k = 0; k < K; ++k) s_stage[t * K + k] = c.Mi[(int64_t)i * K + k];
+            s_stage[nd + t] = i;
+        }
         stage = s_stage;
     }
+    // the per-cluster priors: lambda for P (no F model), the global alpha
+    // for Q and the proposals (no location prior)
+    const bool prior_lds = kind == kDrawP ? !c.fmodel : !c.locpriori;
+    if (tid < c.K && prior_lds) s_prior[tid] = kind == kDrawP ? real_store(c.Lambda[tid], real32(c)) : c.Alpha[tid];
     const uint32_t* J = ds.ring_jump;
     ring_load_advance(J, s_adv);
     Xs128 st;
-    st.seed(sseed, salt);
-    ring_seed(st, J);
+    // the generator at this thread's ring offset: prepared by the previous
+    // sweep's launch when it ran one, else two jumps here
+    const int64_t prep = ((int64_t)(m & 1) * kSppStreams + stream);
+    if (c.spp_prep && c.spp_tag[prep] == m) {
+        st = c.spp_prep[prep * kSppThreads + tid];
+    } else {
+        st.seed(sseed, salt);
+        ring_seed(st, J);
+    }
     int64_t ws = 0;
     ring_fill(st, ring);
     if (tid == 0) {
@@ -264,7 +354,9 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
         s_front.U2 = 0;
         s_front.cached = 0;
         s_H = 1;
+        s_D = kSppD;
         s_rate = 0.f;
+        s_lt1_tot = 0;
     }
     __syncthreads();
 
@@ -272,212 +364,282 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
         if (s_front.d >= d1) break;
         ring_follow(s_front.pos, ws, st, ring, s_adv);
         const SppFront f = s_front;
-        const int H = s_H;
-        const int G = kSppThreads / 64 / H;            // draw groups of 64
-        const int h = wave % H, grp = wave / H;
-        const int jd = grp * 64 + lane;                // this thread's draw in the window
+        const int H = s_H, Dw = s_D;
+        const int G = 16 * Dw / H;                     // draw groups of 64
+        const int h = wave % H;                        // the same in both virtual waves (16 % H == 0)
         const int64_t nact = min((int64_t)(64 * G), d1 - f.d);
-        const bool act = jd < nact;
-        double alpha = 2.0;
-        double* out = act ? draw_target(ds, c, kind, f, jd, stage, d0, nd, &alpha) : nullptr;
-        const bool lt1 = act && alpha < 1;
-        // prefix count of shapes below 1 over the window's draws (their
-        // ballot is the same in every hypothesis's wave of a group)
-        const uint64_t bal = __ballot(lt1);
-        if (h == 0) {
-            s_lt1[jd] = lt1 ? 1 : 0;
-            if (lane == 0) s_cnt[grp] = __popcll(bal);
-        }
-        __syncthreads();
-        int A = __popcll(bal & ((1ull << lane) - 1ull));
-        for (int x = 0; x < grp; ++x) A += s_cnt[x];
-
-        // start of this draw under hypothesis h
-        const int64_t sl = jd + h;
-        const bool fc = f.cached != 0;
-        const bool cj = fc ^ (sl & 1);
-        const int64_t pj = f.pos + slot_words(sl, fc) + A;
-        const bool live = act && (jd > 0 || h == 0);   // draw 0 starts at the front
-        const int64_t bmj = cj ? (sl == 0 ? f.bm : pj - 3 - (jd > 0 ? s_lt1[jd - 1] : 0)) : pj;
-
-        bool ovf = false;
-        auto u01 = [&](int64_t p) -> double {
-            const int64_t o = p - ws;
-            if (o < 0 || o >= kRingWords) { ovf = true; return 0.5; }
-            return unit12(ring[o]) - 1.0;
-        };
-        // the draw's first normal: the cached pair's sine, or a new pair's
-        // cosine (one form for both, so the wave stays convergent).  At the
-        // front a cached pair comes from the front's values (its words may
-        // have left the ring); the ring is read in range only to keep the form.
-        const int64_t bsrc = (sl == 0 && cj) || !live ? f.pos : bmj;
-        double U1 = my_sqrt(-2.0 * log(u01(bsrc)));
-        double U2 = 2.0 * kPi * u01(bsrc + 1);
-        if (sl == 0 && cj) { U1 = f.U1; U2 = f.U2; }
-        double t = U1 * my_sin(cj ? U2 : U2 + (kPi / 2));
-        int64_t pos = pj, bm = f.bm;
-        bool cache;
-        if (cj) {
-            cache = false;
-        } else {
-            pos += 2;
-            bm = pj;
-            cache = (uint32_t)as_u64(U1) != 0xFFFFFFFFu;
-        }
-        bool off_slot = !cj && !cache;   // a pair whose U1 reads as the empty marker
-        auto normal = [&]() -> double {
-            if (cache) {
-                cache = false;
-                return U1 * my_sin(U2);
-            }
-            const double v1 = u01(pos), v2 = u01(pos + 1);
-            bm = pos;
-            pos += 2;
-            U1 = my_sqrt(-2.0 * log(v1));
-            U2 = 2.0 * kPi * v2;
-            cache = (uint32_t)as_u64(U1) != 0xFFFFFFFFu;
-            if (!cache) off_slot = true;
-            return U1 * my_cos(U2);
-        };
-        // Gamma(alpha) of RNG<double>, src/statistics.cpp:212-242: shapes
-        // below 1 draw Gamma(1 + alpha) and a boost
-        const double ag = alpha < 1 ? 1.0 + alpha : alpha;
-        const double dd = ag - 0.333333333333333;
-        const double cc = 1.0 / (sqrt(dd) * 3.0);
-        double v = 1.0, u;
-        bool first = true;
-        int slots = 0;
-        for (int guard = 0; guard < 4096 && !ovf; ++guard) {
-            int normals = 0;
-            do {
-                if (!first) t = normal();
-                first = false;
-                ++normals;
-                const double v1 = cc * t;
-                v = 1.0 + v1;
-            } while (v <= 0 && !ovf);
-            if (normals > 1) off_slot = true;
-            v = v * v * v;
-            u = u01(pos++);
-            ++slots;
-            if (u < 1.0 - 0.0331 * t * t * t * t) break;
-            if (log(u) < 0.5 * t * t + dd * (1.0 - v + log(v))) break;
-        }
-        double val = dd * v;
-        if (alpha < 1) {
-            const double boost = pow(u01(pos++), 1.0 / alpha);
-            val = val * boost;
-        }
-        // where the next draw starts if this one filled `slots` slots: slot
-        // sl + slots, behind this draw's own uniform for a shape below 1
-        const int64_t nsl = sl + slots;
-        const bool ncached = fc ^ (nsl & 1);
-        const int64_t npos = f.pos + slot_words(nsl, fc) + A + (lt1 ? 1 : 0);
-        const bool aligned = !ovf && !off_slot && pos == npos && cache == ncached && (!cache || bm == npos - 3 - (lt1 ? 1 : 0));
-        const int nh = h + slots - 1;
-        uint8_t nx;
-        if (!live) nx = kNextStop;
-        else if (ovf) nx = kNextOvf;
-        else if (!aligned || nh >= H) nx = kNextStop;
-        else nx = (uint8_t)nh;
-        s_next[tid] = nx;
-        const uint64_t devb = __ballot(live && nx != (uint8_t)h);
-        if (lane == 0) s_dev[wave] = devb;
-        __syncthreads();
-
-        // the path through the hypotheses: one thread follows it over the
-        // deviating draws only
-        if (tid == 0) {
-            int hh = 0, nseg = 1, commit = (int)nact, stop_ovf = 0, events = 0;
-            s_seg_lane[0] = 0;
-            s_seg_h[0] = 0;
-            for (int g = 0; g < G && commit == (int)nact; ++g) {
-                int j = 0;
-                while (j < 64) {
-                    const uint64_t msk = s_dev[g * H + hh] & (~0ull << j);
-                    if (!msk) break;
-                    const int jj = __ffsll((unsigned long long)msk) - 1;
-                    const int draw = g * 64 + jj;
-                    if (draw >= (int)nact) break;
-                    const uint8_t e = s_next[(g * H + hh) * 64 + jj];
-                    if (e == kNextStop || e == kNextOvf) {
-                        commit = e == kNextOvf ? draw : draw + 1;
-                        stop_ovf = e == kNextOvf;
-                        ++events;
-                        break;
-                    }
-                    events += e - hh;
-                    hh = e;
-                    if (nseg < 17) {
-                        s_seg_lane[nseg] = draw + 1;
-                        s_seg_h[nseg] = (uint8_t)hh;
-                        ++nseg;
-                    }
-                    j = jj + 1;
+        int jd[kSppD];
+        double alpha[kSppD];
+        SppDraw r[kSppD];
+#pragma unroll
+        for (int x = 0; x < kSppD; ++x) {
+            const int grp = (wave + 16 * x) / H;
+            jd[x] = grp * 64 + lane;                   // this draw's place in the window
+            const bool act = x < Dw && jd[x] < nact;
+            alpha[x] = 2.0;
+            r[x].out = act ? draw_target(ds, c, kind, f, jd[x], stage, d0, nd, prior_lds ? s_prior : nullptr, &alpha[x])
+                           : nullptr;
+            r[x].live = act && (jd[x] > 0 || h == 0);  // draw 0 starts at the front
+            // count of shapes below 1 per group of 64 draws (their ballot is
+            // the same in every hypothesis's wave of a group)
+            const bool lt1 = act && alpha[x] < 1;
+            const uint64_t bal = __ballot(lt1);
+            if (h == 0 && x < Dw) {
+                s_lt1[jd[x]] = lt1 ? 1 : 0;
+                if (lane == 0) {
+                    s_cnt[grp] = __popcll(bal);
+                    if (bal) atomicAdd(&s_lt1_tot, __popcll(bal));
                 }
             }
-            s_nseg = nseg;
-            s_commit = commit;
-            s_stop_ovf = stop_ovf;
-            // the next window's hypothesis count, from the deviation rate
-            const int done = max(commit, 1);
-            const float rate = 0.5f * s_rate + 0.5f * (float)events / (float)done;
-            s_rate = rate;
-            int bestH = 1;
-            float best = 0.f;
-            for (int HH = 1; HH <= 16; HH *= 2) {
-                const float expect = fminf((float)(kSppThreads / HH), (float)HH / fmaxf(rate, 1e-5f));
-                if (expect > best * 1.05f) { best = expect; bestH = HH; }
-            }
-            s_H = bestH;
         }
-        __syncthreads();
+        lds_barrier();
+        const bool any_lt1 = s_lt1_tot != 0;
+
+        bool dev[kSppD];
+#pragma unroll
+        for (int x = 0; x < kSppD; ++x) {
+            dev[x] = false;
+            if (x >= Dw) continue;
+            const int grp = jd[x] >> 6;
+            const bool lt1 = r[x].out && alpha[x] < 1;
+            // shapes below 1 before this draw in the window
+            int A = 0;
+            if (any_lt1) {
+                A = __popcll(__ballot(lt1) & ((1ull << lane) - 1ull));
+                for (int g = 0; g < grp; ++g) A += s_cnt[g];
+            }
+            // start of this draw under hypothesis h
+            const int64_t sl = jd[x] + h;
+            const bool fc = f.cached != 0;
+            const bool cj = fc ^ (sl & 1);
+            const int64_t pj = f.pos + slot_words(sl, fc) + A;
+            const bool live = r[x].live;
+            const int64_t bmj = cj ? (sl == 0 ? f.bm : pj - 3 - (jd[x] > 0 ? s_lt1[jd[x] - 1] : 0)) : pj;
+
+            bool ovf = false;
+            auto u01 = [&](int64_t p) -> double {
+                const int64_t o = p - ws;
+                if (o < 0 || o >= kRingWords) { ovf = true; return 0.5; }
+                return unit12(ring_word(ring, p)) - 1.0;
+            };
+            // the draw's first normal: the cached pair's sine, or a new pair's
+            // cosine (one form for both, so the wave stays convergent).  At the
+            // front a cached pair comes from the front's values (its words may
+            // have left the ring); the ring is read in range only to keep the form.
+            const int64_t bsrc = (sl == 0 && cj) || !live ? f.pos : bmj;
+            double U1 = my_sqrt(-2.0 * log(u01(bsrc)));
+            double U2 = 2.0 * kPi * u01(bsrc + 1);
+            if (sl == 0 && cj) { U1 = f.U1; U2 = f.U2; }
+            double t = U1 * my_sin(cj ? U2 : U2 + (kPi / 2));
+            int64_t pos = pj, bm = f.bm;
+            bool cache;
+            if (cj) {
+                cache = false;
+            } else {
+                pos += 2;
+                bm = pj;
+                cache = (uint32_t)as_u64(U1) != 0xFFFFFFFFu;
+            }
+            bool off_slot = !cj && !cache;   // a pair whose U1 reads as the empty marker
+            auto normal = [&]() -> double {
+                if (cache) {
+                    cache = false;
+                    return U1 * my_sin(U2);
+                }
+                const double v1 = u01(pos), v2 = u01(pos + 1);
+                bm = pos;
+                pos += 2;
+                U1 = my_sqrt(-2.0 * log(v1));
+                U2 = 2.0 * kPi * v2;
+                cache = (uint32_t)as_u64(U1) != 0xFFFFFFFFu;
+                if (!cache) off_slot = true;
+                return U1 * my_cos(U2);
+            };
+            // Gamma(alpha) of RNG<double>, src/statistics.cpp:212-242: shapes
+            // below 1 draw Gamma(1 + alpha) and a boost
+            const double a = alpha[x];
+            const double ag = a < 1 ? 1.0 + a : a;
+            const double dd = ag - 0.333333333333333;
+            const double cc = 1.0 / (sqrt(dd) * 3.0);
+            double v = 1.0, u;
+            bool first = true;
+            int slots = 0;
+            for (int guard = 0; guard < 4096 && !ovf; ++guard) {
+                int normals = 0;
+                do {
+                    if (!first) t = normal();
+                    first = false;
+                    ++normals;
+                    const double v1 = cc * t;
+                    v = 1.0 + v1;
+                } while (v <= 0 && !ovf);
+                if (normals > 1) off_slot = true;
+                v = v * v * v;
+                u = u01(pos++);
+                ++slots;
+                if (u < 1.0 - 0.0331 * t * t * t * t) break;
+                if (mt_log_accept(u, t, v, dd)) break;
+            }
+            double val = dd * v;
+            if (a < 1) {
+                const double boost = pow(u01(pos++), 1.0 / a);
+                val = val * boost;
+            }
+            // where the next draw starts if this one filled `slots` slots: slot
+            // sl + slots, behind this draw's own uniform for a shape below 1
+            const int64_t nsl = sl + slots;
+            const bool ncached = fc ^ (nsl & 1);
+            const int64_t npos = f.pos + slot_words(nsl, fc) + A + (lt1 ? 1 : 0);
+            const bool aligned = !ovf && !off_slot && pos == npos && cache == ncached &&
+                                 (!cache || bm == npos - 3 - (lt1 ? 1 : 0));
+            const int nh = h + slots - 1;
+            uint8_t nx;
+            if (!live) nx = kNextStop;
+            else if (ovf) nx = kNextOvf;
+            else if (!aligned || nh >= H) nx = kNextStop;
+            else nx = (uint8_t)nh;
+            s_next[(wave + 16 * x) * 64 + lane] = nx;
+            dev[x] = live && nx != (uint8_t)h;
+            r[x].val = val;
+            r[x].U1 = U1;
+            r[x].U2 = U2;
+            r[x].pos = pos;
+            r[x].bm = bm;
+            r[x].cache = cache;
+        }
+#pragma unroll
+        for (int x = 0; x < kSppD; ++x) {
+            const uint64_t devb = __ballot(dev[x]);
+            if (lane == 0) s_dev[wave + 16 * x] = devb;
+        }
+        lds_barrier();
+
+        // the window's end and the path through the hypotheses
+        if (wave == 0) {
+            int commit = (int)nact, events = 0;
+            if (H == 1) {
+                // the first deviating draw ends the window (its next
+                // hypothesis would be 1 or more, or it ran out of ring)
+                const uint64_t dv = lane < G ? s_dev[lane] : 0ull;
+                const uint64_t nz = __ballot(dv != 0ull);
+                if (nz) {
+                    const int g0 = __ffsll((unsigned long long)nz) - 1;
+                    const uint64_t dg = __shfl(dv, g0, 64);
+                    const int draw = g0 * 64 + __ffsll((unsigned long long)dg) - 1;
+                    if (draw < (int)nact) {
+                        const uint8_t e = s_next[draw];
+                        commit = e == kNextOvf ? draw : draw + 1;
+                        events = 1;
+                    }
+                }
+                if (lane == 0) {
+                    s_seg_lane[0] = 0;
+                    s_seg_h[0] = 0;
+                    s_nseg = 1;
+                }
+            } else if (lane == 0) {
+                int hh = 0, nseg = 1;
+                s_seg_lane[0] = 0;
+                s_seg_h[0] = 0;
+                for (int g = 0; g < G && commit == (int)nact; ++g) {
+                    int j = 0;
+                    while (j < 64) {
+                        const uint64_t msk = s_dev[g * H + hh] & (~0ull << j);
+                        if (!msk) break;
+                        const int jj = __ffsll((unsigned long long)msk) - 1;
+                        const int draw = g * 64 + jj;
+                        if (draw >= (int)nact) break;
+                        const uint8_t e = s_next[(g * H + hh) * 64 + jj];
+                        if (e == kNextStop || e == kNextOvf) {
+                            commit = e == kNextOvf ? draw : draw + 1;
+                            ++events;
+                            break;
+                        }
+                        events += e - hh;
+                        hh = e;
+                        if (nseg < 17) {
+                            s_seg_lane[nseg] = draw + 1;
+                            s_seg_h[nseg] = (uint8_t)hh;
+                            ++nseg;
+                        }
+                        j = jj + 1;
+                    }
+                }
+                s_nseg = nseg;
+            }
+            if (lane == 0) {
+                s_commit = commit;
+                s_lt1_tot = 0;
+                // the deviation rate so far
+                const int done = max(commit, 1);
+                const float rate = 0.5f * s_rate + 0.5f * (float)events / (float)done;
+                s_rate = rate;
+                // the next window's hypotheses H and draws per thread D: the
+                // most draws expected per unit of time, a window of D * 1024
+                // evaluations costing D plus a fixed kSppWindowCost
+                int bestH = 1, bestD = kSppD;
+                float best = 0.f;
+                for (int DD = 1; DD <= kSppD; ++DD)
+                    for (int HH = 1; HH <= 16; HH *= 2) {
+                        const float expect = fminf((float)(DD * kSppThreads / HH), (float)HH / fmaxf(rate, 1e-5f)) /
+                                             ((float)DD + kSppWindowCost);
+                        if (expect > best * 1.05f) { best = expect; bestH = HH; bestD = DD; }
+                    }
+                s_H = bestH;
+                s_D = bestD;
+            }
+        }
+        lds_barrier();
         const int commit = s_commit;
-        int hj = 0;
-        for (int x = 0; x < s_nseg; ++x)
-            if (s_seg_lane[x] <= jd) hj = s_seg_h[x];
-        const bool mine = live && h == hj;
-        if (commit == 0) {
-            // the front draw would need more words than the ring holds
-            if (jd == 0 && h == 0) {
-                *out = __builtin_nan("");
-                SppFront nf;
-                nf.d = f.d + 1;
-                nf.pos = ws + kRingWords;
-                nf.cached = 0;
-                nf.bm = 0;
-                nf.U1 = as_f64(0xFFFFFFFFull);
-                nf.U2 = 0;
-                nf.pad = 0;
-                draw_coords(kind, nf.d, ds.KT, c.K, &nf.qa, &nf.qb);
-                s_front = nf;
-            }
-        } else if (mine && jd < commit) {
-            *out = val;
-            if (jd == commit - 1) {
-                SppFront nf;
-                nf.d = f.d + commit;
-                nf.pos = pos;
-                nf.cached = cache ? 1 : 0;
-                nf.bm = bm;
-                nf.U1 = cache ? U1 : as_f64(0xFFFFFFFFull);
-                nf.U2 = U2;
-                nf.pad = 0;
-                draw_coords(kind, nf.d, ds.KT, c.K, &nf.qa, &nf.qb);
-                s_front = nf;
+        const int nseg = s_nseg;
+#pragma unroll
+        for (int x = 0; x < kSppD; ++x) {
+            int hj = 0;
+            for (int y = 0; y < nseg; ++y)
+                if (s_seg_lane[y] <= jd[x]) hj = s_seg_h[y];
+            const bool mine = r[x].live && h == hj;
+            if (commit == 0) {
+                // the front draw would need more words than the ring holds
+                if (jd[x] == 0 && h == 0) {
+                    *r[x].out = __builtin_nan("");
+                    SppFront nf;
+                    nf.d = f.d + 1;
+                    nf.pos = ws + kRingWords;
+                    nf.cached = 0;
+                    nf.bm = 0;
+                    nf.U1 = as_f64(0xFFFFFFFFull);
+                    nf.U2 = 0;
+                    nf.pad = 0;
+                    draw_coords(kind, nf.d, ds.KT, c.K, &nf.qa, &nf.qb);
+                    s_front = nf;
+                }
+            } else if (mine && jd[x] < commit) {
+                *r[x].out = r[x].val;
+                if (jd[x] == commit - 1) {
+                    SppFront nf;
+                    nf.d = f.d + commit;
+                    nf.pos = r[x].pos;
+                    nf.cached = r[x].cache ? 1 : 0;
+                    nf.bm = r[x].bm;
+                    nf.U1 = r[x].cache ? r[x].U1 : as_f64(0xFFFFFFFFull);
+                    nf.U2 = r[x].U2;
+                    nf.pad = 0;
+                    draw_coords(kind, nf.d, ds.KT, c.K, &nf.qa, &nf.qb);
+                    s_front = nf;
+                }
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
 
     // the proposals' stream continues in the acceptance kernel: its state
     // after the last draw (generator at `pos`, and the cache)
     if (kind == kDrawProp) {
         ring_follow(s_front.pos, ws, st, ring, s_adv);
-        const int64_t off = s_front.pos - ws;
-        if (tid == (int)(off >> 3)) {
+        const int64_t p = s_front.pos;
+        if (tid == (int)((p & (kRingWords - 1)) >> 3)) {
             Xs128 s = st;
-            for (int x = 0; x < (int)(off & 7); ++x) s.next();
+            for (int x = 0; x < (int)(p & 7); ++x) s.next();
             Rng r;
             r.s = s;
             r.U1 = s_front.cached ? s_front.U1 : as_f64(0xFFFFFFFFull);
@@ -546,7 +708,7 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_metro_accept(const DeviceDa
         int total;
         const int off = block_prefix(need, s_cnt, &total);
         bool acc = typed && r >= kNZero;
-        if (need) acc = unit12(ring[used + off - ws]) - 1.0 < exp(r);
+        if (need) acc = unit12(ring_word(ring, used + off)) - 1.0 < exp(r);
         if (acc) {
             for (int k = 0; k < K; ++k) c.Q[(int64_t)i * K + k] = c.bufNK1[(int64_t)i * K + k];
             c.bufN2[i] += r;
@@ -583,7 +745,7 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_polylog(const DeviceDataset
         if (i < ds.N) {
             for (int k = 0; k < K; ++k) c.Q[(int64_t)i * K + k] = 0;
             if (typed) {
-                const int k2 = polylog_word(ring[used + off - ws], c.bufNK1 + (int64_t)i * K, K);
+                const int k2 = polylog_word(ring_word(ring, used + off), c.bufNK1 + (int64_t)i * K, K);
                 c.Q[(int64_t)i * K + k2] = 1;
                 c.Z[i] = (uint16_t)k2;
             }
@@ -806,7 +968,9 @@ int launch_pg_proposals(const DeviceDataset& ds, const ChainDev& c, hipStream_t 
 int launch_spp(const DeviceDataset& ds, const ChainDev& c, bool do_p, int qkind, bool clear_mi, hipStream_t st)
 {
     const int np = do_p ? kPBlocks : 0;
-    const int nb = np + (qkind >= 0 ? 1 : 0);
+    // with UpdateP's streams, blocks that prepare the next sweep's streams
+    const int nprep = do_p && c.spp_prep ? (kSppStreams + 1) / 2 : 0;
+    const int nb = np + (qkind >= 0 ? 1 : 0) + nprep;
     if (nb > 0) {
         hipLaunchKernelGGL(k_spp_gamma, dim3(nb), dim3(kSppThreads), 0, st, ds, c, np, qkind);
         VPS_HIP(hipGetLastError());
