@@ -1,5 +1,6 @@
 // vps_chain.h — device view of one MCMC chain and kernel entry points.
 #pragma once
+#include <algorithm>
 #include "vps_internal.h"
 
 namespace vps {
@@ -153,7 +154,25 @@ constexpr int kLikLociPerWave = 1024;
 int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1,
                    bool clear_mi = true, bool rec = false, bool advance = false);
 // number of Record partials the Z pass writes with G chunks
-int64_t z_lik_parts(const DeviceDataset& ds, int G);
+// Record's lnL summed inside the FP64 Z pass: one partial per (tile, group of
+// kLnlGroup individuals), each the wavefront sum of its lanes' closed
+// accumulators, so the total does not depend on the individual chunking as
+// long as chunk boundaries fall on group boundaries (every automatic chunking
+// does: chunk_size); then kLnlStage1 fixed ranges and their sum
+constexpr int kLnlGroup = 32;
+constexpr int kLnlStage1 = 64;
+int64_t z_lik_parts(const DeviceDataset& ds);
+// individuals per chunk for G chunks of N: balanced, rounded up to whole
+// lnL groups when there is more than one chunk
+inline int chunk_size(int N, int G)
+{
+    int nc = (N + G - 1) / G;
+    if (G > 1) nc = (nc + kLnlGroup - 1) / kLnlGroup * kLnlGroup;
+    return std::min(nc, std::max(N, 1));
+}
+// whether the chain's chunking keeps the Z pass's Record lnL partials whole
+inline bool z_lnl_aligned(const ChainDev& c) { return c.G == 1 || c.Nc % kLnlGroup == 0; }
+int launch_reduce_lnl_parts(const double* part, int64_t n, double* stage, double* out, hipStream_t st);
 // out = the fixed-order sum of n partials (one block)
 int launch_reduce_lnl(const double* part, int64_t n, double* out, hipStream_t st);
 int launch_z_noadmix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st);

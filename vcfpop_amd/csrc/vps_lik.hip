@@ -477,6 +477,30 @@ int launch_lik_mode(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, 
 
 }  // namespace
 
+// stage 1 of a fixed-order sum of n partials: block b adds the range
+// [b n / nb, (b+1) n / nb) (the ranges depend only on n and nb)
+__global__ __launch_bounds__(256) void k_reduce_lnl_stage(const double* __restrict__ part, int64_t n, double* __restrict__ out)
+{
+    __shared__ double sh[256];
+    const int64_t b0 = (int64_t)blockIdx.x * n / gridDim.x, b1 = (int64_t)(blockIdx.x + 1) * n / gridDim.x;
+    double s = 0;
+    for (int64_t x = b0 + threadIdx.x; x < b1; x += 256) s += part[x];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+int launch_reduce_lnl_parts(const double* part, int64_t n, double* stage, double* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce_lnl_stage, dim3(kLnlStage1), dim3(256), 0, st, part, n, stage);
+    VPS_HIP(hipGetLastError());
+    return launch_reduce_lnl(stage, kLnlStage1, out, st);
+}
+
 int launch_reduce_lnl(const double* part, int64_t n, double* out, hipStream_t st)
 {
     hipLaunchKernelGGL(k_reduce_lnl, dim3(1), dim3(256), 0, st, part, n, out);

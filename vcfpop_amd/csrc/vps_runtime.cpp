@@ -123,14 +123,16 @@ int tune_chunks(vps_chain* ch)
     const DeviceDataset& d = ch->ds->d;
     const int N = c.N;
     std::vector<int> cand;
+    // candidates by chunk count; chunk sizes are whole lnL groups (chunk_size)
     auto add = [&](int nc) {
         nc = std::max(1, std::min(nc, std::min(N, (int)kMaxChunk)));
-        const int G = (N + nc - 1) / nc;
+        int G = (N + nc - 1) / nc;
+        G = (N + chunk_size(N, G) - 1) / chunk_size(N, G);
         if (G > kTuneMaxChunks && G != c.G) return;
         if (std::find(cand.begin(), cand.end(), G) == cand.end()) cand.push_back(G);
     };
     add(c.Nc);
-    for (int nc : {32, 48, 64, 96, 128, 192, 256, 384, 512}) add(nc);
+    for (int nc : {32, 64, 96, 128, 160, 192, 256, 320, 384, 512}) add(nc);
     if (cand.size() < 2) return VPS_OK;
     const int Gmax = *std::max_element(cand.begin(), cand.end());
     const size_t part_bytes = (size_t)(Gmax + 1) * c.K * c.KT * sizeof(int32_t);   // partials + a scratch Ni
@@ -143,7 +145,7 @@ int tune_chunks(vps_chain* ch)
     int bestG = c.G;
     for (size_t x = 0; x < cand.size() && !rc; ++x) {
         ChainDev t = c;
-        t.Nc = (N + cand[x] - 1) / cand[x];
+        t.Nc = chunk_size(N, cand[x]);
         t.G = (N + t.Nc - 1) / t.Nc;
         rc = dataset_jump(ch->ds, t.Nc, c.precision == VPS_PREC_FAST32, &t.jump);
         if (rc) break;
@@ -183,13 +185,11 @@ int tune_chunks(vps_chain* ch)
         return VPS_OK;
     };
     rc = drop(c.Ni_part, (int64_t)c.G * c.K * c.KT * sizeof(int32_t));
-    if (!rc) rc = drop(c.zlik, std::max<int64_t>(z_lik_parts(d, c.G), 1) * sizeof(double));
     if (rc) return rc;
-    c.Nc = (N + bestG - 1) / bestG;
+    c.Nc = chunk_size(N, bestG);
     c.G = (N + c.Nc - 1) / c.Nc;
     rc = dataset_jump(ch->ds, c.Nc, c.precision == VPS_PREC_FAST32, &c.jump);
     if (!rc) rc = chain_alloc(ch, &c.Ni_part, (int64_t)c.G * c.K * c.KT);
-    if (!rc) rc = chain_alloc(ch, &c.zlik, z_lik_parts(d, c.G));
     return rc;
 }
 
@@ -259,16 +259,18 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
         rc = fail(VPS_ERR_INVALID, "z_chunk %d outside -1..%d", p->z_chunk, (int)kMaxChunk);
     if (!rc) {
         int G = 0;
+        // an explicit z_chunk fixes the chunk count, with balanced chunks;
+        // otherwise chunks are whole lnL groups (chunk_size)
         if (p->z_chunk > 0) G = (int)((N + p->z_chunk - 1) / p->z_chunk);
         else rc = choose_chunks(d, K, p->precision, ds->device, &G);
         if (!rc) {
-            c.Nc = (int32_t)((N + G - 1) / G);
+            c.Nc = p->z_chunk > 0 ? (int32_t)((N + G - 1) / G) : (int32_t)chunk_size(N, G);
             c.G = (int32_t)((N + c.Nc - 1) / c.Nc);
             rc = dataset_jump(ds, c.Nc, p->precision == VPS_PREC_FAST32, &c.jump);
         }
     }
     if (!rc) rc = chain_alloc(ch, &c.Ni_part, (int64_t)c.G * K * KT);
-    if (!rc) rc = chain_alloc(ch, &c.zlik, z_lik_parts(d, c.G));
+    if (!rc) rc = chain_alloc(ch, &c.zlik, z_lik_parts(d) + kLnlStage1);
     if (!rc) rc = chain_alloc(ch, &c.Mi, N * K);
     if (!rc) rc = chain_alloc(ch, &c.Alpha, K);
     if (!rc) rc = chain_alloc(ch, &c.Lambda, K);
@@ -548,7 +550,7 @@ int launch_sweep(vps_chain* ch, const SweepPlan& p)
     // rounding, so Record takes its own pass.
     const bool mrec = p.record && p.q == kQMetro && c.precision != VPS_PREC_FAST32;
     const bool zrec = p.record && !mrec && p.q != kQNoAdmix && c.precision != VPS_PREC_MIXED &&
-                      c.precision != VPS_PREC_FAST32;
+                      c.precision != VPS_PREC_FAST32 && z_lnl_aligned(c);
     // without Record, the Z pass's last kernel advances the sweep index when
     // nothing beside it still reads the index
     const bool zadv = !p.record && p.q != kQNoAdmix && (fast || side == ch->stream);

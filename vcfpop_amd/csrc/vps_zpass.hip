@@ -181,12 +181,33 @@ __device__ __forceinline__ void flush_bi_rows(const uint32_t* mi_lds, int nci, i
 
 // one wavefront's chunk of one BI tile; BIG: ids of more than 3 bits, the
 // genotype bytes come from the lane's LDS column `tab_lds`
+// fixed-order sum over the wavefront (every lane ends with the sum)
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// REC: after individual i, close the lanes' lnL accumulators at the end of
+// each group of kLnlGroup individuals (or of the chunk) into the partial of
+// (tile, group)
+__device__ __forceinline__ void lnl_group_end(FrexpAcc& acc, int i, int i1, int N, int64_t tile, int lane,
+                                              double* __restrict__ zlik)
+{
+    if ((i + 1) % kLnlGroup == 0 || i + 1 == i1) {
+        const double v = wave_sum(acc.close());
+        if (lane == 0) zlik[tile * ((N + kLnlGroup - 1) / kLnlGroup) + i / kLnlGroup] = v;
+        acc.open();
+    }
+}
+
 template <int K, typename Real, int MPT, bool BIG, bool REC, bool X96>
-__device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileDesc& t, const double* __restrict__ Q,
+__device__ __forceinline__ void z_bi_tile(const DeviceDataset& ds, const TileDesc& t, const double* __restrict__ Q,
                                           uint64_t seed, int m, int64_t l, bool valid, int afo, int k2, int lane,
                                           int c, int i0, int i1, const Real* pl0, uint8_t* tab_lds,
                                           uint32_t* mi_lds, int32_t* __restrict__ Ni_part,
-                                          const uint32_t* __restrict__ jump)
+                                          const uint32_t* __restrict__ jump, int64_t tile, double* __restrict__ zlik)
 {
     const int64_t KT = ds.KT;
     uint64_t tab = 0;
@@ -304,6 +325,7 @@ __device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileD
                 }
             }
             if (REC && MPT == 2) acc.charge(pr);
+            if (REC) lnl_group_end(acc, i, i1, ds.N, tile, lane, zlik);
             // the individual's counts of clusters k < K-1, packed, into lane
             // (i - i0) % 64 of mrow
             const int rr = i - i0;
@@ -347,15 +369,6 @@ __device__ __forceinline__ double z_bi_tile(const DeviceDataset& ds, const TileD
             if (k2 > 1) dst[k * KT + 1] = one;
         }
     }
-    return REC ? acc.close() : 0.0;
-}
-
-// fixed-order sum over the wavefront (every lane ends with the sum)
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
 }
 
 // LDS of the BI kernel: [wave][k][allele][lane] P values, then
@@ -376,7 +389,8 @@ inline int bi_row_words(const DeviceDataset& ds)
 // b runs item (b % kXcds) * per + b / kXcds (gridDim.x = kXcds * per): each
 // XCD takes a contiguous run of items, and the chunks of one tile group run
 // back to back on it and read the group's P from its L2 rather than HBM.
-// REC: each block also writes the Record lnL partial of its lanes to zlik[w].
+// REC: each wavefront also writes the Record lnL partials of its tile's
+// groups of individuals in its chunk (lnl_group_end).
 template <int K, typename Real, int MPT, bool REC, bool X96>
 __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, const double* __restrict__ Freq,
                                                    const double* __restrict__ Q, uint64_t seed,
@@ -430,26 +444,16 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
     }
     __syncthreads();
 
-    double lnl = 0.0;
     if (has_tile) {
+        const int64_t tile = ds.bi_list[tix];
         if (t.bits > kBiRegBits)
-            lnl = z_bi_tile<K, Real, MPT, true, REC, X96>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane,
-                                                    tab_lds, mi_lds, Ni_part, jump);
+            z_bi_tile<K, Real, MPT, true, REC, X96>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane,
+                                                    tab_lds, mi_lds, Ni_part, jump, tile, zlik);
         else
-            lnl = z_bi_tile<K, Real, MPT, false, REC, X96>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane,
-                                                     tab_lds, mi_lds, Ni_part, jump);
-    }
-    __shared__ double wsum[kWavesMax];
-    if (REC) {
-        lnl = wave_sum(lnl);
-        if (lane == 0) wsum[wv] = lnl;
+            z_bi_tile<K, Real, MPT, false, REC, X96>(ds, t, Q, seed, m, l, valid, afo, k2, lane, c, i0, i1, p_lds + lane,
+                                                     tab_lds, mi_lds, Ni_part, jump, tile, zlik);
     }
     __syncthreads();
-    if (REC && threadIdx.x == 0) {
-        double s = 0.0;
-        for (int x = 0; x < nwv; ++x) s += wsum[x];
-        zlik[w] = s;
-    }
     flush_bi_rows<K, MPT>(mi_all, nci, nwv, Mi + (int64_t)i0 * K);
 }
 
@@ -470,16 +474,15 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
 // chunk; the partials of its other chunks are written as zeros.
 // ---------------------------------------------------------------------------
 //
-// REC: each block also writes the Record lnL partial of its lanes to
-// zlik[zbase + blockIdx.y * gridDim.x + blockIdx.x].
+// REC: each wavefront also writes the Record lnL partials of the tile's
+// groups of individuals in its chunk (lnl_group_end).
 template <int K, typename Real, bool PLDS, bool REC, bool X96>
 __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceDataset ds, const double* __restrict__ Freq,
                                                     const double* __restrict__ Q, uint64_t seed,
                                                     const int32_t* __restrict__ mp,
                                                     int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
                                                     int A, int ni_in_lds, int Nc, int G,
-                                                    const uint32_t* __restrict__ jump, double* __restrict__ zlik,
-                                                    int64_t zbase)
+                                                    const uint32_t* __restrict__ jump, double* __restrict__ zlik)
 {
     const int m = *mp;
     extern __shared__ int32_t lds[];
@@ -518,7 +521,6 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceData
     }
     __syncthreads();
 
-    double lnl = 0.0;   // REC: the lane's Record lnL over its chunk
     if (i0 < i1) {
         const uint64_t* gd = ds.gdesc + ds.gbase[l];
         typename ZStream<X96>::T rng;
@@ -603,20 +605,10 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceData
                 }
             }
             flush_mi<K>(mi_lds + (i - gi0) * KH, cum, lane, mval);
+            if (REC) lnl_group_end(acc, i, i1, ds.N, ds.gen_list[blockIdx.x], lane, zlik);
         }
-        if (REC) lnl = acc.close();
-    }
-    __shared__ double wsum[kWavesMax];
-    if (REC) {
-        lnl = wave_sum(lnl);
-        if (lane == 0) wsum[wv] = lnl;
     }
     __syncthreads();
-    if (REC && threadIdx.x == 0) {
-        double s = 0.0;
-        for (int w = 0; w < nwv; ++w) s += wsum[w];
-        zlik[zbase + (int64_t)blockIdx.y * gridDim.x + blockIdx.x] = s;
-    }
     if (ni_in_lds && valid) {
         for (int k = wv; k < K - 1; k += nwv) {
             int32_t* dst = Ni_part + ((int64_t)c0 * K + k) * KT + afo;
@@ -840,7 +832,6 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st,
         int rc = launch_fill_i32(c.Ni_part, (int64_t)c.G * K * ds.KT, 0, st);
         if (rc) return rc;
     }
-    int64_t zbase = 0;   // Record partials: the BI work items', then the GEN blocks'
     if (ds.nbi > 0) {
         // one block per (group of 4 tiles, chunk) item, dealt round-robin to
         // the XCDs; padded to a multiple of kXcds (padding blocks exit)
@@ -848,7 +839,6 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st,
         const int64_t per = (items + kXcds - 1) / kXcds;
         if (per * kXcds > 0x7FFFFFFF) return fail(VPS_ERR_UNSUPPORTED, "Z pass grid of %lld blocks", (long long)(per * kXcds));
         const dim3 grid((unsigned)(per * kXcds));
-        zbase = items;
         const size_t shm = bi_p_bytes<K, Real>(kWavesPerBlock) + bi_tab_bytes(kWavesPerBlock, ds.bi_tab_rows) + mi_packed;
         if (ds.maxploidy == 2)
             hipLaunchKernelGGL((k_z_admix_bi<K, Real, 2, REC, X96>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
@@ -865,14 +855,13 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st,
         dim3 grid((unsigned)ds.ngen, (unsigned)((c.G + g.waves - 1) / g.waves));
         if (g.plds)
             hipLaunchKernelGGL((k_z_admix_gen<K, Real, true, REC, X96>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q,
-                               c.seed, c.mdev, c.Ni_part, c.Mi, A, 1, c.Nc, c.G, c.jump, c.zlik, zbase);
+                               c.seed, c.mdev, c.Ni_part, c.Mi, A, 1, c.Nc, c.G, c.jump, c.zlik);
         else
             hipLaunchKernelGGL((k_z_admix_gen<K, Real, false, REC, X96>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q,
-                               c.seed, c.mdev, c.Ni_part, c.Mi, A, g.ni_in_lds ? 1 : 0, c.Nc, c.G, c.jump, c.zlik, zbase);
+                               c.seed, c.mdev, c.Ni_part, c.Mi, A, g.ni_in_lds ? 1 : 0, c.Nc, c.G, c.jump, c.zlik);
         VPS_HIP(hipGetLastError());
-        zbase += (int64_t)grid.x * grid.y;
     }
-    if (nparts) *nparts = zbase;
+    if (nparts) *nparts = z_lik_parts(ds);
     return VPS_OK;
 }
 
@@ -971,7 +960,7 @@ int choose_chunks(const DeviceDataset& ds, int K, int precision, int device, int
     int lastNc = -1;
     auto eff = [](int64_t waves_per_simd) { return waves_per_simd >= 3 ? 1.0 : waves_per_simd == 2 ? 0.8 : 0.55; };
     for (int G = gmin; G <= gmax; ++G) {
-        const int Nc = (N + G - 1) / G;
+        const int Nc = chunk_size(N, G);
         if (Nc == lastNc) continue;
         lastNc = Nc;
         const int Gr = (N + Nc - 1) / Nc;
@@ -1020,10 +1009,9 @@ int z_gen_block_waves(const DeviceDataset& ds, int K, int precision, int Nc, int
     return z_occupancy_prec(ds, K, precision, Nc, &bb, &gb, waves);
 }
 
-int64_t z_lik_parts(const DeviceDataset& ds, int G)
+int64_t z_lik_parts(const DeviceDataset& ds)
 {
-    // BI items: 4 tiles x 1 chunk; GEN blocks: 1 tile x 1..4 chunks
-    return (ds.nbi + kWavesPerBlock - 1) / kWavesPerBlock * G + ds.ngen * (int64_t)G;
+    return ds.ntiles * (((int64_t)ds.N + kLnlGroup - 1) / kLnlGroup);
 }
 
 int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, bool clear_mi,
@@ -1032,6 +1020,8 @@ int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, h
     if (rec && advance) return fail(VPS_ERR_INVALID, "Record reads the sweep index after the Z pass");
     if (rec && (c.precision == VPS_PREC_MIXED || c.precision == VPS_PREC_FAST32))
         return fail(VPS_ERR_INVALID, "the Record lnL of the Z pass needs FP64 weights");
+    if (rec && !z_lnl_aligned(c))
+        return fail(VPS_ERR_INVALID, "the Record lnL of the Z pass needs chunks of whole lnL groups");
     int rc = clear_mi ? launch_fill_i32(c.Mi, (int64_t)ds.N * c.K, 0, st) : VPS_OK;
     if (rc) return rc;
     if (ev0) VPS_HIP(hipEventRecord(ev0, st));
@@ -1040,7 +1030,7 @@ int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, h
     if (ev1) VPS_HIP(hipEventRecord(ev1, st));
     if (rc) return rc;
     rc = launch_reduce_ni(ds, c, st, advance, true);
-    if (!rc && rec) rc = launch_reduce_lnl(c.zlik, nparts, c.scal + 1, st);
+    if (!rc && rec) rc = launch_reduce_lnl_parts(c.zlik, nparts, c.zlik + nparts, c.scal + 1, st);
     return rc;
 }
 
