@@ -72,6 +72,14 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-sweep-scale", type=int, default=20,
                     help="job sweep counts U[2000,10000] are divided by this")
+    ap.add_argument("--leg", choices=["small", "batched", "served", "precision"], default=None,
+                    help="only this extra leg (its JSON object on one line), for profiling")
+    ap.add_argument("--served-window", type=float, default=0.0, metavar="SECONDS",
+                    help="only the served job stream at full size (sweeps U[2000,10000]), arrivals over "
+                         "the first SECONDS of the Poisson stream, against the same jobs one at a time; "
+                         "prints one JSON line and exits")
+    ap.add_argument("--serve-max-active", type=int, nargs="+", default=[32],
+                    help="jobs the server keeps on the device at once (--served-window: one served run each)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB megabytes (scales traffic counters) and exit")
     args = ap.parse_args()
@@ -81,6 +89,18 @@ def main():
         ms = V.debug_copy(nbytes, iters=5)
         print(json.dumps({"copy_bytes": nbytes, "launches": 6, "ms_per_copy": ms,
                           "GB/s (read+write)": 2 * nbytes / ms / 1e6}), flush=True)
+        return
+    if args.leg:
+        import vcfpop_amd as V
+        fn = {"small": lambda: bench_small(V, V.RNG_REFERENCE if args.topology == "reference" else V.RNG_PARALLEL),
+              "batched": lambda: bench_batched(V, args.batch_sweeps),
+              "served": lambda: bench_served(V, args.serve_jobs, args.serve_rate, args.serve_sweep_scale),
+              "precision": lambda: bench_precision(V)}[args.leg]
+        print(json.dumps({args.leg: fn()}), flush=True)
+        return
+    if args.served_window > 0:
+        import vcfpop_amd as V
+        print(json.dumps(bench_served_window(V, args.served_window, args.serve_rate, args.serve_max_active)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (chains shard across GPUs "
@@ -352,12 +372,18 @@ def bench_batched(V, S):
             "graphs": st}
 
 
-def make_jobs(V, n, rate, scale, seed=7):
+def make_jobs(V, n, rate, scale, seed=7, window=None):
+    """n jobs of the served stream (or every job arriving before `window`
+    seconds): Poisson arrivals at `rate`, N ~ U[100,5000], L ~ logU[1k,200k],
+    ploidy 2 or 4, K ~ U[1,10], admixture / no-admixture / F model,
+    U[2000,10000] sweeps divided by `scale`"""
     import numpy as np
     rng = np.random.default_rng(seed)
     jobs, t = [], 0.0
-    for j in range(n):
+    for j in range(n if window is None else 1 << 30):
         t += rng.exponential(1.0 / rate)
+        if window is not None and t > window:
+            break
         N = int(rng.integers(100, 5001))
         L = int(np.exp(rng.uniform(np.log(1000), np.log(200000))))
         pl = [(2, 1.0)] if rng.random() < 0.5 else [(4, 1.0)]
@@ -373,6 +399,60 @@ def make_jobs(V, n, rate, scale, seed=7):
                              rng_topology=V.RNG_PARALLEL, g_seed=j, z_chunk=-1, **kw)
         jobs.append(dict(arrival=t, synth=s, params=p, K=K, N=N, L=L, sweeps=sweeps, model=model))
     return jobs
+
+
+def bench_served_window(V, window, rate, max_active=(32,)):
+    """The served stream as specified (full job sizes and sweep counts) for
+    the jobs arriving in the first `window` seconds: open-loop latency
+    through the server (queueing included, the stream drained) against the
+    same jobs run one at a time in arrival order on plain chains"""
+    import numpy as np
+    jobs = make_jobs(V, 0, rate, 1, window=window)
+    log(f"served window {window} s: {len(jobs)} jobs")
+    total = lambda j: j["params"].nburnin + j["params"].nreps + (j["params"].nadmburnin if j["params"].admix_val != 1 else 0)
+    dur = []
+    for x, j in enumerate(jobs):
+        a = time.perf_counter()
+        d = V.Dataset(j["synth"], 0)
+        c = V.Chain(d, j["params"], j["K"], 1)
+        c.step(total(j))
+        c.finish()
+        c.close()
+        d.close()
+        dur.append(time.perf_counter() - a)
+        log(f"plain job {x + 1}/{len(jobs)}: N={j['N']} L={j['L']} K={j['K']} {j['model']} {j['sweeps']} sweeps {dur[-1]:.2f} s")
+    end, lat_plain = 0.0, []
+    for j, d_ in zip(jobs, dur):
+        end = max(end, j["arrival"]) + d_
+        lat_plain.append(end - j["arrival"])
+    pct = lambda a, q: round(float(np.percentile(a, q)), 3)
+    out = {"arrival_window_s": window, "arrival_rate_per_s": rate, "jobs": len(jobs),
+           "sweeps_per_job": "U[2000,10000]", "model_mix": "admix/no-admix/F",
+           "plain": {"device_seconds": round(sum(dur), 2), "p50_s": pct(lat_plain, 50), "p99_s": pct(lat_plain, 99),
+                     "makespan_s": round(end, 2), "chains_per_hr": round(len(jobs) * 3600.0 / sum(dur), 1)},
+           "served": {},
+           "policy": "FIFO admission under a 128 GB budget, at most max_active jobs on the device; "
+                     "latency counts queueing; the stream is drained"}
+    for ma in max_active:
+        srv = V.Server(hbm_budget_bytes=128 << 30, max_active=ma, slice_sweeps=16, loaders=2)
+        ids = []
+        t0 = time.perf_counter()
+        for j in jobs:
+            now = time.perf_counter() - t0
+            if now < j["arrival"]:
+                time.sleep(j["arrival"] - now)
+            ids.append(srv.submit(j["synth"], j["params"], j["K"], 1))
+        for x, i in enumerate(ids):
+            srv.wait(i, 3600.0)
+            log(f"max_active {ma}: served job {x + 1}/{len(ids)} done at {time.perf_counter() - t0:.1f} s")
+        span = time.perf_counter() - t0
+        lat = [x["t_end"] - x["t_submit"] for x in (srv.info(i) for i in ids)]
+        st = srv.stats()
+        srv.close()
+        out["served"][f"max_active_{ma}"] = {"p50_s": pct(lat, 50), "p99_s": pct(lat, 99), "makespan_s": round(span, 2),
+                                             "chains_per_hr": round(len(jobs) * 3600.0 / span, 1),
+                                             "peak_reserved_GB": round(st["peak_reserved_bytes"] / 1e9, 3)}
+    return {"served_window": out}
 
 
 def bench_served(V, n, rate, scale):

@@ -27,6 +27,23 @@ typedef struct vps_image vps_image;
    allele identifiers outside 0..65534) return VPS_ERR_INVALID with a
    message naming the line. */
 int vps_image_read_polygene(const char* path, vps_image** out);
+/* Read a VCF file, plain or gzip/BGZF-compressed (LoadVCF, src/load.cpp:
+   2111-2330; IND::AddVCFGenotype, :3495-3560; the VCF LOCUS,
+   src/function.cpp:3166-3240).  Samples are the #CHROM line's columns after
+   FORMAT; a variant has k = REF + ALT entries alleles (an ALT of "." counts
+   as one) and its GT numbers are the allele indices; the locus name is ID,
+   or POS_REF_ALT without one.  A GT of ploidy v (1 + its '/' and '|') that
+   starts with '.' or holds a '.' allele is the missing genotype of ploidy v.
+   Genotype ids follow first appearance over samples (each sample's missing
+   genotype of its ploidy first); loci whose typed genotypes carry fewer than
+   two distinct alleles are dropped (src/filter.cpp:100-133).  Every sample
+   is in one population, "Total", until vps_image_assign_indtext.  Errors (no
+   #CHROM line, a FORMAT without GT, a line with another number of fields, an
+   empty or malformed GT, an allele index >= k, ploidy outside 1..10) return
+   VPS_ERR_INVALID with a message naming the line and sample; more than
+   VPS_MAX_ALLELES alleles at a kept locus is VPS_ERR_UNSUPPORTED.  The
+   -f_* filters, -ad and phased (-haplotype) reading are not applied. */
+int vps_image_read_vcf(const char* path, vps_image** out);
 /* Populations and regions from vcfpop's -g_indtext option, replacing those
    of the file's population column (which vcfpop does not read).  NULL or an
    empty text puts every individual in one population, "Total", as vcfpop

@@ -67,3 +67,19 @@ def test_record_lnl_bits_equal_across_chunkings(vps, orc, data, name):
     # the oracle and a group-splitting chunking: to 1e-12
     assert np.allclose(traces[a], ot, rtol=1e-12, atol=0)
     assert np.allclose(traces[split], ot, rtol=1e-12, atol=0)
+
+
+def test_automatic_chunks_keep_groups_only_when_needed(vps, data):
+    """a chain whose Record sweeps can sum in the Z pass (thinning not a
+    multiple of metrofreq) gets chunks of whole groups from the tuner; with
+    the default schedule every Record sweep is a Metropolis sweep and the
+    chunks stay balanced"""
+    s, ds, od = data["bi"]
+    c = vps.Chain(ds, vps.params_default(admix_val=1, nburnin=0, nreps=40, nthinning=3, g_seed=5), 3, 1)
+    G, Nc = c.chunking()
+    c.close()
+    assert G == 1 or Nc % 32 == 0, (G, Nc)
+    c = vps.Chain(ds, vps.params_default(admix_val=1, nburnin=20, nreps=40, g_seed=5), 3, 1)
+    G, Nc = c.chunking()
+    c.close()
+    assert Nc == -(-ds.N // G), (G, Nc)

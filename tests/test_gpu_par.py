@@ -104,3 +104,69 @@ def test_example_par_eval_mode(vps, tmp_path):
     for model in ("admix", "noadmix"):    # the same seed: initialisation agrees on both paths
         assert by[(model, "init", "fp64")]["Mi"] == by[(model, "init", "mixed")]["Mi"]
     assert any("update_z_admix" in ln for ln in lines)
+
+
+def _write_vcf(path, n_ind=90, n_loc=400, seed=3):
+    """a seeded VCF: diploid and tetraploid samples, 2-4 alleles per variant
+    with cluster-dependent frequencies, some missing genotypes, some phased"""
+    rng = np.random.default_rng(seed)
+    ploidy = np.where(rng.random(n_ind) < 0.8, 2, 4)
+    q = rng.dirichlet([0.5, 0.5, 0.5], n_ind)
+    rows = ["##fileformat=VCFv4.2",
+            "#CHROM\tPOS\tID\tREF\tALT\tQUAL\tFILTER\tINFO\tFORMAT\t" + "\t".join(f"S{i + 1}" for i in range(n_ind))]
+    for l in range(n_loc):
+        k = int(rng.choice([2, 2, 2, 3, 4])) if l % 50 else 2
+        p = rng.dirichlet(np.full(k, 0.7), 3)
+        alt = ",".join("ACGT"[x] for x in range(1, k)) if l % 50 else "."   # an ALT of '.' is one allele
+        f = []
+        for i in range(n_ind):
+            v = int(ploidy[i])
+            if rng.random() < 0.03:
+                f.append("/".join(["."] * v))
+                continue
+            mix = q[i] @ p
+            al = rng.choice(k, size=v, p=mix / mix.sum())
+            sep = "|" if rng.random() < 0.2 else "/"
+            f.append(sep.join(str(int(a)) for a in al) + ":7")
+        rows.append(f"{1 + l // 200}\t{100 + l}\t{'.' if l % 7 == 0 else f'v{l}'}\tA\t{alt}\t.\tPASS\t.\tGT:DP\t" + "\t".join(f))
+    path.write_text("\n".join(rows) + "\n")
+
+
+def test_vcf_par_runs_equal_oracle(vps, orc, tmp_path):
+    """a VCF through a vcfpop parameter file (-g_format=vcf, populations from
+    -g_indtext): the image read by the VCF reader, runs 1-3 (K = 2 and 3)
+    batched on the device for 30 sweeps, against the oracle on the same
+    image: Z, Ni, Mi and the ancestry bit for bit; Q and P to 1e-12"""
+    from vcfpop_amd import par
+    _write_vcf(tmp_path / "data.vcf")
+    (tmp_path / "par.txt").write_text(
+        "-g_input=data.vcf\n-g_format=vcf\n-g_output=res/out\n-g_seed=7\n"
+        "-g_indtext=\"popA:#1-#30 popB:#31-#90\"\n"
+        "-structure\n-structure_admix=yes\n-structure_krange=[2,3]\n-structure_nruns=2\n")
+    job = par.Job(str(tmp_path / "par.txt"), sweeps=30)
+    assert job.image.loci_read == 400 and job.image.names("pop") == ["popA", "popB"]
+    ds = vps.Dataset(job.image, 0)
+    od = orc.Data(job.image.desc)
+    sel = list(enumerate(job.runs))[:3]
+    say("vcf par: runs", [i + 1 for i, _ in sel], "x", job.total, "sweeps; loci kept", job.image.loci_kept)
+    chains = [vps.Chain(ds, job.params, K, i + 1) for i, (K, rep) in sel]
+    b = vps.Batch(chains)
+    b.step(job.total)
+    b.sync()
+    for c, (i, (K, rep)) in zip(chains, sel):
+        o = orc.Chain(od, job.params, K, i + 1)
+        o.sweep(job.total)
+        for n in ("Z", "Ni", "Mi"):
+            assert (c.get(n) == o.get(n)).all(), (i + 1, n)
+        assert np.allclose(c.get("Q"), o.get("Q"), rtol=RTOL, atol=0), i + 1
+        assert np.allclose(c.get("Freq"), o.get("Freq"), rtol=RTOL, atol=0), i + 1
+        c.finish()
+        o.finish()
+        assert (c.get("MiSum") == o.get("MiSum")).all(), i + 1
+        say("vcf run", i + 1, f"(K={K}, rep={rep}) equals the oracle")
+        o.close()
+    b.close()
+    for c in chains:
+        c.close()
+    ds.close()
+    job.close()

@@ -206,6 +206,7 @@ _SIGS = {
     "vps_synth_sizes": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "vps_synth_destroy": ([_P], C.c_int),
     "vps_image_read_polygene": ([C.c_char_p, C.POINTER(C.c_void_p)], C.c_int),
+    "vps_image_read_vcf": ([C.c_char_p, C.POINTER(C.c_void_p)], C.c_int),
     "vps_image_desc": ([_P, C.POINTER(VpsDatasetDesc)], C.c_int),
     "vps_image_assign_indtext": ([_P, C.c_char_p], C.c_int),
     "vps_image_sizes": ([_P] + [C.POINTER(C.c_int64)] * 5, C.c_int),
@@ -367,11 +368,13 @@ class Image(_PackedArrays):
     """A genotype file read into vcfpop's packed layout (include/vps_ingest.h)."""
 
     def __init__(self, path, fmt="polygene"):
-        if fmt != "polygene":
+        """fmt: "polygene", or "vcf" (plain or gzip/BGZF-compressed)"""
+        readers = {"polygene": "vps_image_read_polygene", "vcf": "vps_image_read_vcf"}
+        if fmt not in readers:
             raise ValueError(f"unsupported genotype format {fmt!r}")
         lib = load()
         h = C.c_void_p()
-        _check(lib.vps_image_read_polygene(os.fsencode(str(path)), C.byref(h)))
+        _check(getattr(lib, readers[fmt])(os.fsencode(str(path)), C.byref(h)))
         self.h = h
         self.desc = VpsDatasetDesc()
         _check(lib.vps_image_desc(h, C.byref(self.desc)))

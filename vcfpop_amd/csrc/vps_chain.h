@@ -158,16 +158,17 @@ int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, h
 // kLnlGroup individuals), each the wavefront sum of its lanes' closed
 // accumulators, so the total does not depend on the individual chunking as
 // long as chunk boundaries fall on group boundaries (every automatic chunking
-// does: chunk_size); then kLnlStage1 fixed ranges and their sum
+// of a chain whose Record sweeps can take this path does: chunk_size); then
+// kLnlStage1 fixed ranges and their sum
 constexpr int kLnlGroup = 32;
 constexpr int kLnlStage1 = 64;
 int64_t z_lik_parts(const DeviceDataset& ds);
-// individuals per chunk for G chunks of N: balanced, rounded up to whole
-// lnL groups when there is more than one chunk
-inline int chunk_size(int N, int G)
+// individuals per chunk for G chunks of N: balanced, and with `groups`
+// rounded up to whole lnL groups when there is more than one chunk
+inline int chunk_size(int N, int G, bool groups)
 {
     int nc = (N + G - 1) / G;
-    if (G > 1) nc = (nc + kLnlGroup - 1) / kLnlGroup * kLnlGroup;
+    if (groups && G > 1) nc = (nc + kLnlGroup - 1) / kLnlGroup * kLnlGroup;
     return std::min(nc, std::max(N, 1));
 }
 // whether the chain's chunking keeps the Z pass's Record lnL partials whole
@@ -226,7 +227,7 @@ int launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
 // wavefronts per block of the GEN Z kernel (4 or 8) for chunks of Nc
 int z_gen_block_waves(const DeviceDataset& ds, int K, int precision, int Nc, int* waves);
 // chunk count minimising (grid rounds x per-block work) at the kernel's occupancy
-int choose_chunks(const DeviceDataset& ds, int K, int precision, int device, int* G);
+int choose_chunks(const DeviceDataset& ds, int K, int precision, int device, bool groups, int* G);
 int launch_init_admix(const DeviceDataset& ds, const ChainDev& c, int singlez, hipStream_t st);
 
 }  // namespace vps

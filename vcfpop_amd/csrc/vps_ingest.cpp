@@ -17,6 +17,28 @@
 //     start (src/math2.h:503-512, src/misc.cpp:1624-1636).
 // One deviation: a single allele in a field of ploidy > 1 is an error here;
 // the reference pads such a field with missing copies.
+//
+// The VCF format (plain or gzip/BGZF-compressed text), restated from the
+// reference's loader (src/load.cpp:2111-2330, LoadVCF; :3495-3560,
+// IND::AddVCFGenotype; src/function.cpp:3166-3240, the LOCUS of a VCF line;
+// src/string2.cpp:41-60, ReadVCFGenoString; src/load.h:86-93, CountPloidy):
+//   Samples are the #CHROM line's columns after FORMAT.  A variant's alleles
+//     are REF and the comma-separated ALT entries: k = commas in ALT + 2
+//     (an ALT of "." counts as one allele), and allele indices are the GT
+//     numbers themselves.  The locus name is ID, or "POS_REF_ALT" when ID is
+//     "." or empty.
+//   A sample's genotype is its GT subfield: ploidy = 1 + the '/' and '|'
+//     after the first character (1..10, else an error); a genotype that
+//     starts with '.' or has any '.' allele is the missing genotype of that
+//     ploidy; alleles are sorted (unphased); an allele index >= k is an
+//     error.  Genotype ids follow first appearance over samples, each sample
+//     registering the missing genotype of its ploidy before its own.
+//   ApplyFilter (src/filter.cpp:100-133, run on every input) then drops the
+//     loci whose typed genotypes carry fewer than two distinct alleles; k
+//     stays the REF/ALT count, so unobserved alleles keep their P entries.
+//   The options that filter or rewrite genotypes (-f_*, -ad, -haplotype
+//     phasing) are not applied.
+#include <zlib.h>
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -260,6 +282,211 @@ extern "C" int vps_image_read_polygene(const char* path, vps_image** out)
     for (auto& s : im->ind_names) im->ind_ptr.push_back(s.c_str());
     for (auto& s : im->pop_names) im->pop_ptr.push_back(s.c_str());
     for (auto& s : im->loc_names) im->loc_ptr.push_back(s.c_str());
+    *out = im;
+    return VPS_OK;
+}
+
+namespace {
+
+// the image's packed ids, tables and names from per-locus genotype tables:
+// locus l keeps its alleles' indices (k alleles) and ids[l][i] per individual
+int pack_loci(vps_image* im, const char* path, std::vector<std::vector<Geno>>& gtab,
+              std::vector<std::vector<uint32_t>>& ids, const std::vector<int>& kk, const std::vector<std::string>& names,
+              const std::vector<bool>& keep)
+{
+    const int64_t N = im->N;
+    im->gtoff.assign(1, 0);
+    uint64_t bytes = 0;
+    for (size_t l = 0; l < gtab.size(); ++l) {
+        if (!keep[l]) continue;
+        const int k = kk[l];
+        if (k > VPS_MAX_ALLELES)
+            return fail(VPS_ERR_UNSUPPORTED, "%s locus %s: %d alleles, the device path takes at most %d", path,
+                        names[l].c_str(), k, VPS_MAX_ALLELES);
+        const int64_t ng = (int64_t)gtab[l].size();
+        const int b = ceil_log2(ng);
+        if (b > 24) return fail(VPS_ERR_UNSUPPORTED, "%s locus %s: %lld genotypes", path, names[l].c_str(), (long long)ng);
+        im->loc_names.push_back(names[l]);
+        im->nallele.push_back(k);
+        im->bitsize.push_back((uint8_t)b);
+        im->byteoff.push_back(bytes);
+        for (const Geno& g : gtab[l]) {
+            im->gtploidy.push_back(g.missing ? 0 : (uint8_t)g.v);
+            im->gtaoff.push_back((int64_t)im->pool.size());
+            im->pool.insert(im->pool.end(), g.al.begin(), g.al.end());
+        }
+        im->gtoff.push_back((int64_t)im->gtploidy.size());
+        const uint64_t nb = ((uint64_t)b * N + 7) / 8;
+        im->bits.resize(bytes + nb, 0);
+        for (int64_t i = 0; i < N; ++i) {
+            const uint64_t v = ids[l][i];
+            const uint64_t bit = (uint64_t)i * b;
+            for (int x = 0; x < b; ++x)
+                if (v >> x & 1) im->bits[bytes + ((bit + x) >> 3)] |= (uint8_t)(1u << ((bit + x) & 7));
+        }
+        bytes += nb;
+    }
+    im->L = (int64_t)im->nallele.size();
+    if (im->L == 0) return fail(VPS_ERR_INVALID, "%s: no locus carries two alleles", path);
+    if (im->pool.empty()) im->pool.push_back(0);
+    return VPS_OK;
+}
+
+// one VCF genotype (the GT subfield) of ploidy v into g; false on a malformed
+// allele or an index >= k
+bool vcf_genotype(const std::string& gt, int k, Geno* g)
+{
+    int v = 1;
+    for (size_t x = 1; x < gt.size(); ++x) v += (gt[x] == '/' || gt[x] == '|') ? 1 : 0;
+    g->v = v;
+    g->al.clear();
+    g->missing = gt.empty() || gt[0] == '.';
+    if (g->missing) return true;
+    size_t p = 0;
+    for (int a = 0; a < v; ++a) {
+        if (p < gt.size() && gt[p] == '.') {
+            g->missing = true;
+            g->al.clear();
+            return true;
+        }
+        int64_t x = 0;
+        size_t q = p;
+        while (q < gt.size() && gt[q] >= '0' && gt[q] <= '9') x = x * 10 + (gt[q++] - '0');
+        if (q == p || x >= k || x > 0xFFFE) return false;
+        g->al.push_back((uint16_t)x);
+        if (q < gt.size() && gt[q] != '/' && gt[q] != '|') return false;
+        p = q + 1;
+    }
+    std::sort(g->al.begin(), g->al.end());   // unphased
+    return true;
+}
+
+}  // namespace
+
+extern "C" int vps_image_read_vcf(const char* path, vps_image** out)
+{
+    if (!path || !out) return fail(VPS_ERR_INVALID, "null argument");
+    *out = nullptr;
+    // gzread reads plain text too, and every member of a BGZF file
+    gzFile f = gzopen(path, "rb");
+    if (!f) return fail(VPS_ERR_INVALID, "cannot open %s", path);
+    std::string text;
+    std::vector<char> buf(1 << 20);
+    int n;
+    while ((n = gzread(f, buf.data(), (unsigned)buf.size())) > 0) text.append(buf.data(), (size_t)n);
+    const bool err = n < 0;
+    gzclose(f);
+    if (err) return fail(VPS_ERR_INVALID, "%s: read or decompression error", path);
+
+    std::vector<std::string> lines = split(text, '\n');
+    std::string().swap(text);
+    vps_image* im = new vps_image();
+    auto bad = [&](int code, const char* fmt, auto... a) {
+        delete im;
+        return fail(code, fmt, a...);
+    };
+    size_t li = 0;
+    std::vector<std::string> samples;
+    for (; li < lines.size(); ++li) {
+        std::string& line = lines[li];
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        if (line.compare(0, 2, "##") == 0 || line.empty()) continue;
+        if (line.compare(0, 6, "#CHROM") != 0) return bad(VPS_ERR_INVALID, "%s line %zu: expected the #CHROM header line", path, li + 1);
+        const std::vector<std::string> head = split(line, '\t');
+        if (head.size() < 10) return bad(VPS_ERR_INVALID, "%s: the #CHROM line names no samples", path);
+        samples.assign(head.begin() + 9, head.end());
+        ++li;
+        break;
+    }
+    if (samples.empty()) return bad(VPS_ERR_INVALID, "%s: no #CHROM header line", path);
+    const int64_t N = (int64_t)samples.size();
+    if (N > 0x7FFFFFFF) return bad(VPS_ERR_UNSUPPORTED, "%s: too many samples", path);
+    im->N = (int32_t)N;
+    im->ind_names = samples;
+    im->pop.assign((size_t)N, 0);
+    im->pop_names.assign(1, "Total");
+    im->S = 1;
+
+    std::vector<std::vector<Geno>> gtab;
+    std::vector<std::vector<uint32_t>> ids;
+    std::vector<int> kk;
+    std::vector<std::string> names;
+    std::vector<bool> keep;
+    for (; li < lines.size(); ++li) {
+        std::string& line = lines[li];
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        if (line.empty()) continue;
+        const std::vector<std::string> fld = split(line, '\t');
+        const size_t lineno = li + 1;
+        if ((int64_t)fld.size() != 9 + N) return bad(VPS_ERR_INVALID, "%s line %zu: %zu fields, expected %lld", path, lineno, fld.size(), (long long)(9 + N));
+        const std::string &pos = fld[1], &id = fld[2], &ref = fld[3], &alt = fld[4];
+        const int k = (int)std::count(alt.begin(), alt.end(), ',') + 2;
+        const std::vector<std::string> fmt = split(fld[8], ':');
+        int gti = -1;
+        for (size_t x = 0; x < fmt.size(); ++x)
+            if (fmt[x] == "GT") { gti = (int)x; break; }
+        if (gti < 0) return bad(VPS_ERR_INVALID, "%s line %zu: FORMAT has no GT", path, lineno);
+        names.push_back(id.empty() || id == "." ? pos + "_" + ref + "_" + alt : id);
+        kk.push_back(k);
+        gtab.emplace_back();
+        ids.emplace_back();
+        std::unordered_map<std::string, uint32_t> gid;
+        std::vector<uint32_t>& row = ids.back();
+        row.reserve((size_t)N);
+        std::vector<bool> seen(k, false);
+        int distinct = 0;
+        bool multi = false;   // a typed genotype with two or more distinct alleles
+        for (int64_t i = 0; i < N; ++i) {
+            const std::string& sf = fld[9 + i];
+            // the GT subfield (trailing subfields may be left out)
+            size_t a = 0;
+            for (int x = 0; x < gti && a != std::string::npos; ++x) {
+                a = sf.find(':', a);
+                if (a != std::string::npos) ++a;
+            }
+            std::string gt;
+            if (a != std::string::npos) gt = sf.substr(a, sf.find(':', a) == std::string::npos ? std::string::npos : sf.find(':', a) - a);
+            Geno g;
+            if (gt.empty()) return bad(VPS_ERR_INVALID, "%s line %zu sample %s: no GT value", path, lineno, samples[i].c_str());
+            if (!vcf_genotype(gt, k, &g))
+                return bad(VPS_ERR_INVALID, "%s line %zu sample %s: genotype '%s' (alleles 0..%d)", path, lineno,
+                           samples[i].c_str(), gt.c_str(), k - 1);
+            if (g.v > VPS_MAX_PLOIDY)
+                return bad(VPS_ERR_INVALID, "%s line %zu sample %s: ploidy %d outside 1..%d", path, lineno, samples[i].c_str(), g.v,
+                           VPS_MAX_PLOIDY);
+            if (!g.missing) {
+                int nd = 0;
+                for (size_t x = 0; x < g.al.size(); ++x) {
+                    if (x == 0 || g.al[x] != g.al[x - 1]) ++nd;
+                    if (!seen[g.al[x]]) { seen[g.al[x]] = true; ++distinct; }
+                }
+                multi = multi || nd >= 2;
+            }
+            Geno miss;
+            miss.v = g.v;
+            miss.missing = true;
+            uint32_t v = 0;
+            for (const Geno* e : {&miss, &g}) {
+                const std::string key = geno_key(*e);
+                auto it = gid.find(key);
+                if (it == gid.end()) {
+                    it = gid.emplace(key, (uint32_t)gtab.back().size()).first;
+                    gtab.back().push_back(*e);
+                }
+                v = it->second;
+                if (g.missing) break;
+            }
+            row.push_back(v);
+        }
+        keep.push_back(multi || distinct >= 2);
+        std::string().swap(line);
+    }
+    im->L_read = (int64_t)gtab.size();
+    const int rc = pack_loci(im, path, gtab, ids, kk, names, keep);
+    if (rc) { delete im; return rc; }
+    for (auto& s2 : im->ind_names) im->ind_ptr.push_back(s2.c_str());
+    for (auto& s2 : im->pop_names) im->pop_ptr.push_back(s2.c_str());
+    for (auto& s2 : im->loc_names) im->loc_ptr.push_back(s2.c_str());
     *out = im;
     return VPS_OK;
 }

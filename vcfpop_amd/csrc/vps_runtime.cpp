@@ -123,11 +123,13 @@ int tune_chunks(vps_chain* ch)
     const DeviceDataset& d = ch->ds->d;
     const int N = c.N;
     std::vector<int> cand;
-    // candidates by chunk count; chunk sizes are whole lnL groups (chunk_size)
+    // candidates by chunk count (chunk sizes of whole lnL groups when
+    // Record can sum in the Z pass)
+    const bool groups = ch->lnl_groups;
     auto add = [&](int nc) {
         nc = std::max(1, std::min(nc, std::min(N, (int)kMaxChunk)));
         int G = (N + nc - 1) / nc;
-        G = (N + chunk_size(N, G) - 1) / chunk_size(N, G);
+        G = (N + chunk_size(N, G, groups) - 1) / chunk_size(N, G, groups);
         if (G > kTuneMaxChunks && G != c.G) return;
         if (std::find(cand.begin(), cand.end(), G) == cand.end()) cand.push_back(G);
     };
@@ -145,7 +147,7 @@ int tune_chunks(vps_chain* ch)
     int bestG = c.G;
     for (size_t x = 0; x < cand.size() && !rc; ++x) {
         ChainDev t = c;
-        t.Nc = chunk_size(N, cand[x]);
+        t.Nc = chunk_size(N, cand[x], groups);
         t.G = (N + t.Nc - 1) / t.Nc;
         rc = dataset_jump(ch->ds, t.Nc, c.precision == VPS_PREC_FAST32, &t.jump);
         if (rc) break;
@@ -186,7 +188,7 @@ int tune_chunks(vps_chain* ch)
     };
     rc = drop(c.Ni_part, (int64_t)c.G * c.K * c.KT * sizeof(int32_t));
     if (rc) return rc;
-    c.Nc = chunk_size(N, bestG);
+    c.Nc = chunk_size(N, bestG, groups);
     c.G = (N + c.Nc - 1) / c.Nc;
     rc = dataset_jump(ch->ds, c.Nc, c.precision == VPS_PREC_FAST32, &c.jump);
     if (!rc) rc = chain_alloc(ch, &c.Ni_part, (int64_t)c.G * c.K * c.KT);
@@ -236,6 +238,13 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     ch->nburnin = p->nburnin + ch->nadmburnin;
     ch->nreps = p->nreps; ch->nthinning = p->nthinning; ch->metrofreq = p->metrofreq;
     ch->writelnl = p->writelnl_val == 1;
+    // Record sums its lnL inside the Z pass on FP64 admixture sweeps without a
+    // Metropolis UpdateQ.  Every Record sweep is a Metropolis sweep when the
+    // thinning and the burn-in are multiples of metrofreq; otherwise the
+    // automatic chunkings keep the lnL groups whole, so the reported lnL does
+    // not depend on the chunk count chosen.
+    ch->lnl_groups = c.admix && p->precision == VPS_PREC_FP64 &&
+                     !(ch->metrofreq > 0 && ch->nthinning % ch->metrofreq == 0 && ch->nburnin % ch->metrofreq == 0);
     c.seed = hash_ulong_host(p->g_seed + (uint64_t)run_id);
     const int nf = c.fmodel ? (c.fsame ? 1 : K) : 0;
     const int na = c.admix ? (c.diffalpha ? K : 1) : 0;
@@ -260,11 +269,12 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
     if (!rc) {
         int G = 0;
         // an explicit z_chunk fixes the chunk count, with balanced chunks;
-        // otherwise chunks are whole lnL groups (chunk_size)
+        // otherwise chunks are whole lnL groups when Record can sum in the
+        // Z pass (chunk_size)
         if (p->z_chunk > 0) G = (int)((N + p->z_chunk - 1) / p->z_chunk);
-        else rc = choose_chunks(d, K, p->precision, ds->device, &G);
+        else rc = choose_chunks(d, K, p->precision, ds->device, ch->lnl_groups, &G);
         if (!rc) {
-            c.Nc = p->z_chunk > 0 ? (int32_t)((N + G - 1) / G) : (int32_t)chunk_size(N, G);
+            c.Nc = p->z_chunk > 0 ? (int32_t)((N + G - 1) / G) : (int32_t)chunk_size(N, G, ch->lnl_groups);
             c.G = (int32_t)((N + c.Nc - 1) / c.Nc);
             rc = dataset_jump(ds, c.Nc, p->precision == VPS_PREC_FAST32, &c.jump);
         }
@@ -550,7 +560,7 @@ int launch_sweep(vps_chain* ch, const SweepPlan& p)
     // rounding, so Record takes its own pass.
     const bool mrec = p.record && p.q == kQMetro && c.precision != VPS_PREC_FAST32;
     const bool zrec = p.record && !mrec && p.q != kQNoAdmix && c.precision != VPS_PREC_MIXED &&
-                      c.precision != VPS_PREC_FAST32 && z_lnl_aligned(c);
+                      c.precision != VPS_PREC_FAST32 && ch->lnl_groups && z_lnl_aligned(c);
     // without Record, the Z pass's last kernel advances the sweep index when
     // nothing beside it still reads the index
     const bool zadv = !p.record && p.q != kQNoAdmix && (fast || side == ch->stream);

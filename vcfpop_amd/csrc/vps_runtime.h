@@ -29,6 +29,7 @@ struct vps_chain {
     vps::ChainDev c{};
     int K = 0, id = 0, m = 0, nr = 0, nlnl = 0, binaryq = 0, singlez = 0, finished = 0;
     int nburnin = 0, nreps = 0, nthinning = 1, metrofreq = 0, writelnl = 0, nadmburnin = 0;
+    bool lnl_groups = false;   // automatic chunk sizes of whole lnL groups (Record can sum in the Z pass)
     int64_t lnl_cap = 0;
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;                  // parameter updates beside the Z pass

@@ -226,8 +226,8 @@ __device__ __forceinline__ int64_t slot_words(int64_t s, bool front_cached)
 
 // The parse of one stream per block.  Blocks [0, np) are UpdateP's 128
 // streams (np = 0 or 128); block np, when qkind >= 0, is UpdateQ's stream with
-// draws of kind qkind.  Values are unnormalised Gamma draws; k_spp_normalize
-// turns each row into its Dirichlet draw.  The Metropolis proposals' stream
+// draws of kind qkind.  Each block turns its rows of Gamma draws into their
+// Dirichlet draws once its stream is parsed.  The Metropolis proposals' stream
 // state after the parse goes to rngbuf[0] for the acceptance kernel.
 //
 // Thread layout of a window with H hypotheses (H in 1, 2, 4, 8, 16) and D
@@ -252,7 +252,8 @@ struct SppDraw {
     bool cache, live;
 };
 
-__global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset ds, const ChainDev c, int np, int qkind)
+__global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset ds, const ChainDev c, int np, int qkind,
+                                                            int clear_mi)
 {
     __shared__ uint64_t ring[kRingWords];
     __shared__ uint32_t s_adv[kJumpWords];
@@ -273,12 +274,15 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int kind;
     uint64_t sseed, salt;
-    int64_t d0 = 0, d1;
+    int64_t d0 = 0, d1, kl0 = 0, kl1 = 0;
+    // the sweep index for the kernels that run beside the Z pass
+    if (blockIdx.x == 0 && tid == 0) c.mside[0] = m;
     if ((int)blockIdx.x < np) {
         kind = kDrawP;
         const int b = blockIdx.x;
         const int64_t KL = (int64_t)c.K * ds.L;
-        const int64_t kl0 = (int64_t)b * KL / kPBlocks, kl1 = (int64_t)(b + 1) * KL / kPBlocks;
+        kl0 = (int64_t)b * KL / kPBlocks;
+        kl1 = (int64_t)(b + 1) * KL / kPBlocks;
         // a slice of kl is a contiguous range of Freq: g = k*KT + afo[l] + allele
         auto first_g = [&](int64_t kl) -> int64_t {
             if (kl >= KL) return (int64_t)c.K * ds.KT;
@@ -632,6 +636,35 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
         lds_barrier();
     }
 
+    // the stream's Dirichlet rows, each scaled by 1/sum (the reference's
+    // Dirichlet, src/statistics.cpp:269-296): P rows (k, l) of the slice, or
+    // each typed individual's Q (or proposal) row; then the Mi clear for the
+    // Z pass (Mi is read only by UpdateQ's stream)
+    __syncthreads();   // (the block's stores of the values are visible to all its threads)
+    if (kind == kDrawP) {
+        for (int64_t kl = kl0 + tid; kl < kl1; kl += kSppThreads) {
+            const int64_t k = kl / ds.L, l = kl - k * ds.L;
+            double* p = c.Freq + k * ds.KT + ds.afo[l];
+            const int k2 = ds.k2[l];
+            double sum = 0;
+            for (int a = 0; a < k2; ++a) sum += p[a];
+            dirichlet_scale(p, sum, k2, real32(c));
+        }
+    } else {
+        const int K = c.K;
+        double* base = kind == kDrawQ ? c.Q : c.bufNK1;
+        for (int t = tid; t < ds.ntyped; t += kSppThreads) {
+            double* r = base + (int64_t)ds.typed[t] * K;
+            double sum = 0;
+            for (int k = 0; k < K; ++k) sum += r[k];
+            dirichlet_scale(r, sum, K, real32(c));
+        }
+        if (clear_mi) {
+            __syncthreads();   // (the stage may have been read from Mi)
+            for (int64_t x = tid; x < (int64_t)ds.N * K; x += kSppThreads) c.Mi[x] = 0;
+        }
+    }
+
     // the proposals' stream continues in the acceptance kernel: its state
     // after the last draw (generator at `pos`, and the cache)
     if (kind == kDrawProp) {
@@ -649,11 +682,12 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
     }
 }
 
-// Dirichlet rows from the Gamma draws: each P row (k, l) and each typed
-// individual's Q (or proposal) row is scaled by 1/sum (the reference's
-// Dirichlet, src/statistics.cpp:269-296).  clear_mi zeroes every Mi row for
-// the Z pass; thread 0 copies the sweep index for the kernels that run
-// beside the Z pass.
+// Dirichlet rows from Gamma draws, for the launches whose rows no stream
+// block normalises (the proposals' stream graph; a launch with no stream):
+// each P row (k, l) and each typed individual's Q (or proposal) row is scaled
+// by 1/sum (the reference's Dirichlet, src/statistics.cpp:269-296).  clear_mi
+// zeroes every Mi row for the Z pass; thread 0 copies the sweep index for the
+// kernels that run beside the Z pass.
 __global__ void k_spp_normalize(const DeviceDataset ds, const ChainDev c, int do_p, int qkind, int clear_mi)
 {
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -971,14 +1005,19 @@ int launch_spp(const DeviceDataset& ds, const ChainDev& c, bool do_p, int qkind,
     // with UpdateP's streams, blocks that prepare the next sweep's streams
     const int nprep = do_p && c.spp_prep ? (kSppStreams + 1) / 2 : 0;
     const int nb = np + (qkind >= 0 ? 1 : 0) + nprep;
+    // the stream blocks normalise their own Dirichlet rows; UpdateQ's block
+    // clears Mi after reading it
     if (nb > 0) {
-        hipLaunchKernelGGL(k_spp_gamma, dim3(nb), dim3(kSppThreads), 0, st, ds, c, np, qkind);
+        hipLaunchKernelGGL(k_spp_gamma, dim3(nb), dim3(kSppThreads), 0, st, ds, c, np, qkind,
+                           clear_mi && qkind >= 0 ? 1 : 0);
         VPS_HIP(hipGetLastError());
     }
-    const int64_t n = (do_p ? (int64_t)c.K * ds.L : 0) + ds.N;
-    hipLaunchKernelGGL(k_spp_normalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ds, c, do_p ? 1 : 0,
-                       qkind, clear_mi ? 1 : 0);
-    VPS_HIP(hipGetLastError());
+    if (nb == 0 || (clear_mi && qkind < 0)) {
+        const int64_t n = ds.N;
+        hipLaunchKernelGGL(k_spp_normalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ds, c, 0, -1,
+                           clear_mi ? 1 : 0);
+        VPS_HIP(hipGetLastError());
+    }
     return VPS_OK;
 }
 

@@ -945,7 +945,7 @@ int dispatch_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 // allele) of HBM traffic (written, then reduced).  A slot-step is ~(12 + 10K)
 // VALU instructions of 4 cycles at 2.4 GHz.  The chunk count with the
 // smallest modelled time wins, among chunks of 16..kMaxChunk individuals.
-int choose_chunks(const DeviceDataset& ds, int K, int precision, int device, int* Gout)
+int choose_chunks(const DeviceDataset& ds, int K, int precision, int device, bool groups, int* Gout)
 {
     int cus = 256;
     hipDeviceProp_t prop;
@@ -960,7 +960,7 @@ int choose_chunks(const DeviceDataset& ds, int K, int precision, int device, int
     int lastNc = -1;
     auto eff = [](int64_t waves_per_simd) { return waves_per_simd >= 3 ? 1.0 : waves_per_simd == 2 ? 0.8 : 0.55; };
     for (int G = gmin; G <= gmax; ++G) {
-        const int Nc = chunk_size(N, G);
+        const int Nc = chunk_size(N, G, groups);
         if (Nc == lastNc) continue;
         lastNc = Nc;
         const int Gr = (N + Nc - 1) / Nc;

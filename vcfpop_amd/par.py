@@ -6,7 +6,8 @@ The file is split as vcfpop splits it (SetParameters,
 src/parameters.cpp:1345-1362): blank-separated tokens, blanks inside double
 quotes kept, quotes trimmed.  Reads the options the clustering uses and
 ignores the rest (other analyses are not part of this engine):
-  -g_input, -g_format (polygene), -g_output, -g_seed, -g_decimal
+  -g_input, -g_format (polygene or vcf; a .vcf.gz input is read through
+    zlib), -g_output, -g_seed, -g_decimal
   -g_float, -g_fastsingle: REAL = float runs (PREC_FAST32 / PREC_REAL32)
   -g_indtext: populations and regions (vcfpop's default without it: one
     population; the file's population column is not read, as in vcfpop)
@@ -161,8 +162,9 @@ class Job:
     def __init__(self, par_path, sweeps=None, topology="reference"):
         g, st, argv = read_par(par_path)
         base = os.path.dirname(os.path.abspath(par_path))
-        if g.get("format", "polygene").lower() != "polygene":
-            raise ValueError(f"-g_format={g['format']}: this engine reads the polygene format")
+        self.format = g.get("format", "vcf").lower()
+        if self.format not in ("polygene", "vcf"):
+            raise ValueError(f"-g_format={g['format']}: this engine reads the polygene and VCF formats")
         if "indtab" in g:
             raise ValueError("-g_indtab is not supported: give the populations with -g_indtext")
         inp = g.get("input")
@@ -186,7 +188,7 @@ class Job:
         # run id i + 1 for (K, rep) in the reference's task order
         self.runs = [(k, rep) for k in range(kmin, kmax + 1) for rep in range(1, p.nruns + 1)]
         self.eval = st.get("eval", "no").lower() == "yes"
-        self.image = Image(self.input)
+        self.image = Image(self.input, self.format)
         indtext = g.get("indtext")
         self.image.assign_indtext(indtext.replace("#n", "\n") if indtext else None)
 
