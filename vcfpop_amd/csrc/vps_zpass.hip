@@ -30,6 +30,7 @@
 //     cluster, so after the pass Ni[K-1] = copies of the allele - the rest
 //     and Mi[i][K-1] = vt_i - the rest (k_reduce_ni).
 #include <algorithm>
+#include <type_traits>
 #include "vps_chain.h"
 
 namespace vps {
@@ -73,6 +74,17 @@ template <typename Real> __device__ __forceinline__ Real unit_minus1(uint32_t r,
 template <> __device__ __forceinline__ float unit_minus1<float>(uint32_t r, const UnitBits& ub)
 {
     return __builtin_bit_cast(float, bfi(ub.m23, r, ub.one32)) - 1.0f;
+}
+
+// a slot's cluster weight w = P Q + eps: a product and a sum rounded
+// separately, as the reference forms it, except with FP32 weights in the
+// mixed-precision mode (this build's own mode, with no reference rounding to
+// keep), where one fused multiply-add forms it
+template <typename Real, bool X96>
+__device__ __forceinline__ Real z_weight(Real p, Real q, Real eps)
+{
+    if constexpr (std::is_same<Real, float>::value && !X96) return __builtin_fmaf(p, q, eps);
+    else return p * q + eps;
 }
 
 // the Z pass's per-locus streams: RNG<double> (xorshift128+), or RNG<float>
@@ -302,9 +314,13 @@ __device__ __forceinline__ void z_bi_tile(const DeviceDataset& ds, const TileDes
                 Real f = 0;
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const Real pq = pa[k * 2 * kTileLoci] * q[k];
-                    wk[k] = pq + eps;
-                    if (REC) f = k ? f + pq : pq;
+                    if (REC) {
+                        const Real pq = pa[k * 2 * kTileLoci] * q[k];
+                        wk[k] = pq + eps;
+                        f = k ? f + pq : pq;
+                    } else {
+                        wk[k] = z_weight<Real, X96>(pa[k * 2 * kTileLoci], q[k], eps);
+                    }
                 }
                 if (REC) {
                     const double fc = typed ? (double)f : 1.0;
@@ -579,9 +595,14 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceData
                 const Real* px = p_lds + x * kTileLoci + lane;
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const Real pq = (PLDS ? px[k * A * kTileLoci] : (Real)pl[k * KT + x]) * q[k];
-                    wk[k] = pq + eps;
-                    if (REC) f = k ? f + pq : pq;
+                    const Real pk = PLDS ? px[k * A * kTileLoci] : (Real)pl[k * KT + x];
+                    if (REC) {
+                        const Real pq = pk * q[k];
+                        wk[k] = pq + eps;
+                        f = k ? f + pq : pq;
+                    } else {
+                        wk[k] = z_weight<Real, X96>(pk, q[k], eps);
+                    }
                 }
                 if (REC) acc.charge(typed ? (double)f : 1.0);
                 Real s = wk[0];
