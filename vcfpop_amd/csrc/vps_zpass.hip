@@ -76,6 +76,14 @@ template <> __device__ __forceinline__ float unit_minus1<float>(uint32_t r, cons
     return __builtin_bit_cast(float, bfi(ub.m23, r, ub.one32)) - 1.0f;
 }
 
+// the mixed-precision mode (FP32 weights, RNG<double> streams): this build's
+// own mode, free of the reference's rounding; its cluster test compares the
+// scaled draw with the running sums of the weights (t < w_0 + ... + w_k, the
+// reference's "first k with t - w_0 - ... - w_{k-1} < w_k" in exact
+// arithmetic), one compare per cluster instead of a subtraction and a compare
+template <typename Real, bool X96>
+constexpr bool z_mixed() { return std::is_same<Real, float>::value && !X96; }
+
 // a slot's cluster weight w = P Q + eps: a product and a sum rounded
 // separately, as the reference forms it, except with FP32 weights in the
 // mixed-precision mode (this build's own mode, with no reference rounding to
@@ -83,7 +91,7 @@ template <> __device__ __forceinline__ float unit_minus1<float>(uint32_t r, cons
 template <typename Real, bool X96>
 __device__ __forceinline__ Real z_weight(Real p, Real q, Real eps)
 {
-    if constexpr (std::is_same<Real, float>::value && !X96) return __builtin_fmaf(p, q, eps);
+    if constexpr (z_mixed<Real, X96>()) return __builtin_fmaf((float)p, (float)q, (float)eps);
     else return p * q + eps;
 }
 
@@ -327,15 +335,23 @@ __device__ __forceinline__ void z_bi_tile(const DeviceDataset& ds, const TileDes
                     if (MPT == 2) pr = pr * fc;
                     else acc.charge(fc);
                 }
-                Real s = wk[0];
+                // the running sums of the weights; s is the last
+                Real cs[K];
+                cs[0] = wk[0];
 #pragma unroll
-                for (int k = 1; k < K; ++k) s = s + wk[k];
+                for (int k = 1; k < K; ++k) cs[k] = cs[k - 1] + wk[k];
+                const Real s = cs[K - 1];
                 Real tt = unit_minus1<Real>(r, ub) * s;
                 const uint32_t inc = (al1b << 16) | 1u;
 #pragma unroll
                 for (int k = 0; k < K - 1; ++k) {
-                    tt = tt - wk[k];
-                    const bool neg = tt < (Real)0;
+                    bool neg;
+                    if constexpr (z_mixed<Real, X96>() && !REC) {
+                        neg = tt < cs[k];
+                    } else {
+                        tt = tt - wk[k];
+                        neg = tt < (Real)0;
+                    }
                     cum[k] += __builtin_popcountll(ballot(neg) & tmask);
                     nc[k] += (neg && typed) ? inc : 0u;
                 }
@@ -605,15 +621,22 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceData
                     }
                 }
                 if (REC) acc.charge(typed ? (double)f : 1.0);
-                Real s = wk[0];
+                Real cs[K];
+                cs[0] = wk[0];
 #pragma unroll
-                for (int k = 1; k < K; ++k) s = s + wk[k];
+                for (int k = 1; k < K; ++k) cs[k] = cs[k - 1] + wk[k];
+                const Real s = cs[K - 1];
                 Real tt = unit_minus1<Real>(r, ub) * s;
                 int nneg = 0;
 #pragma unroll
                 for (int k = 0; k < K - 1; ++k) {
-                    tt = tt - wk[k];
-                    const bool neg = tt < (Real)0;
+                    bool neg;
+                    if constexpr (z_mixed<Real, X96>() && !REC) {
+                        neg = tt < cs[k];
+                    } else {
+                        tt = tt - wk[k];
+                        neg = tt < (Real)0;
+                    }
                     cum[k] += __builtin_popcountll(ballot(neg) & tmask);
                     nneg += neg ? 1 : 0;
                 }
