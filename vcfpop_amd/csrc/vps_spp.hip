@@ -901,20 +901,19 @@ __global__ void k_pg_next(const ChainDev c)
     c.pg_next[x] = (st.ok && st.p < c.pg_W) ? (int32_t)pg_state(st.p, (k + 1) % K, K, st.cached) : -1;
 }
 
-__global__ void k_pg_extend(int32_t* path, const int32_t* T, int64_t n, int64_t D)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || n + i >= D) return;
-    const int32_t s = path[i];
-    path[n + i] = s < 0 ? -1 : T[s];
-}
-
-__global__ void k_pg_square(const int32_t* T, int32_t* T2, int64_t S)
+// one doubling step: with path[0..n) known and T = succ^n, path[n + i] =
+// T[path[i]] for i < n, and (square) T2 = T o T; both only read T
+__global__ void k_pg_double(int32_t* path, const int32_t* T, int32_t* T2, int64_t n, int64_t D, int64_t S, int square)
 {
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= S) return;
-    const int32_t t = T[x];
-    T2[x] = t < 0 ? -1 : T[t];
+    if (x < n && n + x < D) {
+        const int32_t s = path[x];
+        path[n + x] = s < 0 ? -1 : T[s];
+    }
+    if (square && x < S) {
+        const int32_t t = T[x];
+        T2[x] = t < 0 ? -1 : T[t];
+    }
 }
 
 // the draws on the path into bufNK1 (typed individual d / K, cluster d % K);
@@ -985,11 +984,11 @@ int launch_pg_proposals(const DeviceDataset& ds, const ChainDev& c, hipStream_t 
     int32_t* T = c.pg_next;
     int32_t* T2 = c.pg_next + S;
     for (int64_t n = 1; n < D; n *= 2) {
-        hipLaunchKernelGGL(k_pg_extend, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c.pg_path, T, n, D);
-        if (2 * n < D) {
-            hipLaunchKernelGGL(k_pg_square, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, st, T, T2, S);
-            std::swap(T, T2);
-        }
+        const int square = 2 * n < D ? 1 : 0;
+        const int64_t m = std::max(std::min(n, D - n), square ? S : (int64_t)0);
+        hipLaunchKernelGGL(k_pg_double, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, c.pg_path, T, T2, n, D, S,
+                           square);
+        if (square) std::swap(T, T2);
     }
     hipLaunchKernelGGL(k_pg_values, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, st, ds, c, D);
     VPS_HIP(hipGetLastError());
