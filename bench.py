@@ -323,13 +323,15 @@ def bench_small(V, topology):
 def bench_batched(V, S):
     """64 chains (K=2..9 x 8 replicate seeds) over one N=1,000 x L=50,000
     dataset, 5,000 sweeps each: chains/hr when chains run one at a time
-    (the reference's one-chain-per-device leasing) and as one batch"""
+    (the reference's one-chain-per-device leasing) and as one batch, in the
+    parallel draw topology and in the reference's (the drop-in as wired into
+    vcfpop, whose batched FP64 chains equal the plain ones bit for bit)"""
     s = V.make_synth("batch_n1000_l50k_k2to9x8")
     ds = V.Dataset(s, 0)
 
-    def chains(precision=V.PREC_FP64):
+    def chains(precision=V.PREC_FP64, topology=V.RNG_PARALLEL):
         return [V.Chain(ds, V.params_default(admix_val=1, nburnin=0, nreps=5000, g_seed=1,
-                                             rng_topology=V.RNG_PARALLEL, precision=precision), K, r)
+                                             rng_topology=topology, precision=precision), K, r)
                 for K in range(2, 10) for r in range(1, 9)]
 
     def run_batch(cs):
@@ -346,30 +348,47 @@ def bench_batched(V, S):
             c.close()
         return t, st
 
-    cs = chains()
-    for c in cs:
-        c.step(2)
-        c.sync()
-    t0 = time.perf_counter()
-    for c in cs:
-        c.step(S)
-        c.sync()
-    plain = (time.perf_counter() - t0) / S
-    for c in cs:
-        c.close()
+    def run_plain(cs):
+        for c in cs:
+            c.step(2)
+            c.sync()
+        t0 = time.perf_counter()
+        for c in cs:
+            c.step(S)
+            c.sync()
+        t = (time.perf_counter() - t0) / S
+        for c in cs:
+            c.close()
+        return t
+
+    plain = run_plain(chains())
     batched, st = run_batch(chains())
     # the same batch with FP32 weights (FP64 lnL, draws and Metropolis steps)
     mixed, _ = run_batch(chains(V.PREC_MIXED))
+    # the drop-in's own draws (the reference's serial P and Q streams)
+    log("batched: reference topology")
+    plain_ref = run_plain(chains(topology=V.RNG_REFERENCE))
+    batched_ref, _ = run_batch(chains(topology=V.RNG_REFERENCE))
+    mixed_ref, _ = run_batch(chains(V.PREC_MIXED, V.RNG_REFERENCE))
     ds.close()
     per_hr = lambda t: 64 * 3600.0 / (t * 5000)
     return {"workload": "batch_n1000_l50k_k2to9x8", "chains": 64, "sweeps_per_chain": 5000,
-            "measured_sweeps": S, "plain_chains_per_hr": round(per_hr(plain), 1),
+            "measured_sweeps": S, "rng_topology": "parallel", "plain_chains_per_hr": round(per_hr(plain), 1),
             "batched_chains_per_hr": round(per_hr(batched), 1), "speedup": round(plain / batched, 3),
             "batched_mixed_chains_per_hr": round(per_hr(mixed), 1),
             "speedup_mixed_vs_plain_fp64": round(plain / mixed, 3),
             "ms_per_sweep_of_64": {"plain": round(plain * 1e3, 3), "batched": round(batched * 1e3, 3),
                                    "batched_mixed": round(mixed * 1e3, 3)},
-            "graphs": st}
+            "graphs": st,
+            "speedup_mixed_vs_plain_drop_in": round(plain_ref / mixed, 3),
+            "reference_topology": {
+                "plain_drop_in_chains_per_hr": round(per_hr(plain_ref), 1),
+                "batched_chains_per_hr": round(per_hr(batched_ref), 1),
+                "speedup": round(plain_ref / batched_ref, 3),
+                "batched_mixed_chains_per_hr": round(per_hr(mixed_ref), 1),
+                "speedup_mixed_vs_plain_drop_in": round(plain_ref / mixed_ref, 3),
+                "ms_per_sweep_of_64": {"plain": round(plain_ref * 1e3, 3), "batched": round(batched_ref * 1e3, 3),
+                                       "batched_mixed": round(mixed_ref * 1e3, 3)}}}
 
 
 def make_jobs(V, n, rate, scale, seed=7, window=None):
@@ -467,19 +486,31 @@ def bench_served(V, n, rate, scale):
     log(f"served: generating {n} jobs")
     jobs = make_jobs(V, n, rate, scale)
     total = lambda j: j["params"].nburnin + j["params"].nreps + (j["params"].nadmburnin if j["params"].admix_val != 1 else 0)
+    def with_(p, **kw):
+        q = V.params_default(**{f: getattr(p, f) for f, _ in p._fields_})
+        for k, v in kw.items():
+            setattr(q, k, v)
+        return q
+
     # plain, back to back: per-job durations and the saturated makespan
-    dur = []
-    t0 = time.perf_counter()
-    for j in jobs:
-        a = time.perf_counter()
-        d = V.Dataset(j["synth"], 0)
-        c = V.Chain(d, j["params"], j["K"], 1)
-        c.step(total(j))
-        c.finish()
-        c.close()
-        d.close()
-        dur.append(time.perf_counter() - a)
-    plain_sat = time.perf_counter() - t0
+    def run_plain(topology):
+        dur = []
+        t0 = time.perf_counter()
+        for j in jobs:
+            a = time.perf_counter()
+            d = V.Dataset(j["synth"], 0)
+            c = V.Chain(d, with_(j["params"], rng_topology=topology), j["K"], 1)
+            c.step(total(j))
+            c.finish()
+            c.close()
+            d.close()
+            dur.append(time.perf_counter() - a)
+        return dur, time.perf_counter() - t0
+
+    dur, plain_sat = run_plain(V.RNG_PARALLEL)
+    # the drop-in as wired into vcfpop: the reference's own P and Q streams
+    log("served: plain drop-in (reference topology)")
+    _, plain_ref_sat = run_plain(V.RNG_REFERENCE)
     end, lat_plain = 0.0, []
     for j, d_ in zip(jobs, dur):
         end = max(end, j["arrival"]) + d_
@@ -493,10 +524,7 @@ def bench_served(V, n, rate, scale):
             now = time.perf_counter() - t0
             if at_arrival and now < j["arrival"]:
                 time.sleep(j["arrival"] - now)
-            p = j["params"]
-            if mixed:
-                p = V.params_default(**{f: getattr(p, f) for f, _ in p._fields_})
-                p.precision = V.PREC_MIXED
+            p = with_(j["params"], precision=V.PREC_MIXED) if mixed else j["params"]
             ids.append(srv.submit(j["synth"], p, j["K"], 1))
         for i in ids:
             srv.wait(i, 900.0)
@@ -522,6 +550,9 @@ def bench_served(V, n, rate, scale):
                           "served_mixed_makespan_s": round(mix_span, 3),
                           "served_mixed_chains_per_hr": round(n * 3600.0 / mix_span, 1),
                           "speedup_mixed_vs_plain_fp64": round(plain_sat / mix_span, 3),
+                          "plain_drop_in_makespan_s": round(plain_ref_sat, 3),
+                          "plain_drop_in_chains_per_hr": round(n * 3600.0 / plain_ref_sat, 1),
+                          "speedup_mixed_vs_plain_drop_in": round(plain_ref_sat / mix_span, 3),
                           "peak_reserved_GB": round(st_sat["peak_reserved_bytes"] / 1e9, 3)},
             "policy": "FIFO admission under a 128 GB budget, at most 32 jobs on the device; "
                       "latency counts queueing; the stream is drained"}

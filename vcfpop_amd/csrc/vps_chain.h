@@ -79,6 +79,7 @@ struct ChainDev {
     // state
     double*   Freq;      // [K*KT]
     double*   Q;         // [N*K]
+    float*    Qf;        // [N*K] Q rounded to single precision, the single-precision Z passes' copy
     uint16_t* Z;         // [N]
     int32_t*  Ni;        // [K*KT]
     int32_t*  Ni_part;   // [G*K*KT]

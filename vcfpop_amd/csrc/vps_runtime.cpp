@@ -260,6 +260,7 @@ extern "C" int vps_chain_create(vps_dataset* ds, const vps_params* p, int K, int
                                           lik_part_count(d, 3, K, c.lik_tpr)});
     if (!rc) rc = chain_alloc(ch, &c.Freq, K * KT + 2);   // +2: paired allele loads past the last locus
     if (!rc) rc = chain_alloc(ch, &c.Q, N * K);
+    if (!rc) rc = chain_alloc(ch, &c.Qf, N * K);
     if (!rc) rc = chain_alloc(ch, &c.Z, N + 32);
     if (!rc) rc = chain_alloc(ch, &c.Ni, K * KT);
     // individual chunking of the genotype passes (model choice here; z_chunk

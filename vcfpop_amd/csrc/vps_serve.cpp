@@ -117,7 +117,7 @@ int64_t estimate_bytes(const HostImage& h, int K, const vps_params& par)
                             ntiles * (56 + 64 * 4) + btab + ((N + 15) / 16) * 8192 + (64 << 10);
     const int64_t parts = std::min<int64_t>(kEstChunkParts, (N + 15) / 16);
     const int64_t chain = K * KT * (8 + 4 + 8) + parts * (K * KT * 4 + (ntiles / 4 + ntiles) * 8) + 3 * KT * 8 +
-                          N * K * (8 + 4 + 8 + 8 + 8) +
+                          N * K * (8 + 4 + 4 + 8 + 8 + 8) +
                           N * 64 + (ntiles / 4 + 2) * N * std::max<int64_t>(K, 2) * 8 + (64 << 10);
     // the stream graph of the Metropolis proposals (at most N typed)
     int64_t pg = 0;

@@ -223,7 +223,7 @@ __device__ __forceinline__ void lnl_group_end(FrexpAcc& acc, int i, int i1, int 
 }
 
 template <int K, typename Real, int MPT, bool BIG, bool REC, bool X96>
-__device__ __forceinline__ void z_bi_tile(const DeviceDataset& ds, const TileDesc& t, const double* __restrict__ Q,
+__device__ __forceinline__ void z_bi_tile(const DeviceDataset& ds, const TileDesc& t, const Real* __restrict__ Q,
                                           uint64_t seed, int m, int64_t l, bool valid, int afo, int k2, int lane,
                                           int c, int i0, int i1, const Real* pl0, uint8_t* tab_lds,
                                           uint32_t* mi_lds, int32_t* __restrict__ Ni_part,
@@ -300,7 +300,7 @@ __device__ __forceinline__ void z_bi_tile(const DeviceDataset& ds, const TileDes
             const int h = v - (int)((desc >> 4) & 15u);   // the generic path's dosage form
             Real q[K];
 #pragma unroll
-            for (int k = 0; k < K; ++k) q[k] = (Real)Q[(int64_t)i * K + k];
+            for (int k = 0; k < K; ++k) q[k] = Q[(int64_t)i * K + k];
             int cum[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) cum[k] = 0;
@@ -425,7 +425,7 @@ inline int bi_row_words(const DeviceDataset& ds)
 // groups of individuals in its chunk (lnl_group_end).
 template <int K, typename Real, int MPT, bool REC, bool X96>
 __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, const double* __restrict__ Freq,
-                                                   const double* __restrict__ Q, uint64_t seed,
+                                                   const Real* __restrict__ Q, uint64_t seed,
                                                    const int32_t* __restrict__ mp,
                                                    int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
                                                    int Nc, int G, const uint32_t* __restrict__ jump,
@@ -510,7 +510,7 @@ __global__ __launch_bounds__(256) void k_z_admix_bi(const DeviceDataset ds, cons
 // groups of individuals in its chunk (lnl_group_end).
 template <int K, typename Real, bool PLDS, bool REC, bool X96>
 __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceDataset ds, const double* __restrict__ Freq,
-                                                    const double* __restrict__ Q, uint64_t seed,
+                                                    const Real* __restrict__ Q, uint64_t seed,
                                                     const int32_t* __restrict__ mp,
                                                     int32_t* __restrict__ Ni_part, int32_t* __restrict__ Mi,
                                                     int A, int ni_in_lds, int Nc, int G,
@@ -594,7 +594,7 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceData
             const int v = (int)(cur & 15u);
             Real q[K];
 #pragma unroll
-            for (int k = 0; k < K; ++k) q[k] = (Real)Q[(int64_t)i * K + k];
+            for (int k = 0; k < K; ++k) q[k] = Q[(int64_t)i * K + k];
             int cum[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) cum[k] = 0;
@@ -799,6 +799,15 @@ __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v)
     if (x < n) p[x] = v;
 }
 
+// Q rounded to single precision once per pass, so that the single-precision
+// Z passes read their Q values ready to use (scalar loads, no conversion per
+// individual and tile)
+__global__ void k_q_to_float(const double* __restrict__ q, float* __restrict__ qf, int64_t n)
+{
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < n) qf[x] = (float)q[x];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -862,10 +871,23 @@ GenPlan gen_plan(const DeviceDataset& ds, int Nc, int G)
     return resident(g8) > resident(g4) ? g8 : g4;
 }
 
+// the Q values a Z pass of this precision reads
+template <typename Real>
+const Real* z_q(const ChainDev& c);
+template <>
+const double* z_q<double>(const ChainDev& c) { return c.Q; }
+template <>
+const float* z_q<float>(const ChainDev& c) { return c.Qf; }
+
 // nparts (may be null): the number of Record partials written
 template <int K, typename Real, bool REC, bool X96>
 int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, int64_t* nparts)
 {
+    if (std::is_same<Real, float>::value) {
+        const int64_t n = (int64_t)ds.N * K;
+        hipLaunchKernelGGL(k_q_to_float, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c.Q, c.Qf, n);
+        VPS_HIP(hipGetLastError());
+    }
     const int nthr = 64 * kWavesPerBlock;
     const int A = std::max(ds.maxK, 1);
     const size_t mi_packed = (size_t)kWavesPerBlock * c.Nc * bi_row_words<K>(ds) * sizeof(uint32_t);
@@ -885,23 +907,23 @@ int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st,
         const dim3 grid((unsigned)(per * kXcds));
         const size_t shm = bi_p_bytes<K, Real>(kWavesPerBlock) + bi_tab_bytes(kWavesPerBlock, ds.bi_tab_rows) + mi_packed;
         if (ds.maxploidy == 2)
-            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 2, REC, X96>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 2, REC, X96>), grid, dim3(nthr), shm, st, ds, c.Freq, z_q<Real>(c), c.seed, c.mdev,
                                c.Ni_part, c.Mi, c.Nc, c.G, c.jump, c.zlik);
         else if (ds.maxploidy == 4)
-            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 4, REC, X96>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 4, REC, X96>), grid, dim3(nthr), shm, st, ds, c.Freq, z_q<Real>(c), c.seed, c.mdev,
                                c.Ni_part, c.Mi, c.Nc, c.G, c.jump, c.zlik);
         else
-            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 0, REC, X96>), grid, dim3(nthr), shm, st, ds, c.Freq, c.Q, c.seed, c.mdev,
+            hipLaunchKernelGGL((k_z_admix_bi<K, Real, 0, REC, X96>), grid, dim3(nthr), shm, st, ds, c.Freq, z_q<Real>(c), c.seed, c.mdev,
                                c.Ni_part, c.Mi, c.Nc, c.G, c.jump, c.zlik);
         VPS_HIP(hipGetLastError());
     }
     if (ds.ngen > 0) {
         dim3 grid((unsigned)ds.ngen, (unsigned)((c.G + g.waves - 1) / g.waves));
         if (g.plds)
-            hipLaunchKernelGGL((k_z_admix_gen<K, Real, true, REC, X96>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q,
+            hipLaunchKernelGGL((k_z_admix_gen<K, Real, true, REC, X96>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, z_q<Real>(c),
                                c.seed, c.mdev, c.Ni_part, c.Mi, A, 1, c.Nc, c.G, c.jump, c.zlik);
         else
-            hipLaunchKernelGGL((k_z_admix_gen<K, Real, false, REC, X96>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, c.Q,
+            hipLaunchKernelGGL((k_z_admix_gen<K, Real, false, REC, X96>), grid, dim3(64 * g.waves), g.shm, st, ds, c.Freq, z_q<Real>(c),
                                c.seed, c.mdev, c.Ni_part, c.Mi, A, g.ni_in_lds ? 1 : 0, c.Nc, c.G, c.jump, c.zlik);
         VPS_HIP(hipGetLastError());
     }
