@@ -68,6 +68,14 @@ template <> __device__ __forceinline__ float unit_minus1<float>(uint64_t r, cons
 {
     return __builtin_bit_cast(float, bfi(ub.m23, (uint32_t)(r >> 29), ub.one32)) - 1.0f;
 }
+// FP32 weights, and the slot's lane mask folded into the draw: an untyped
+// slot's U takes the exponent of 2^127 instead of 2^0, so its scaled draw
+// lies above every running sum of the weights (or is +inf) and no cluster
+// test passes on that lane; the counts then need no typed-lane mask
+__device__ __forceinline__ float unit_minus1_typed(uint64_t r, const UnitBits& ub, bool typed)
+{
+    return __builtin_bit_cast(float, bfi(ub.m23, (uint32_t)(r >> 29), typed ? ub.one32 : 0x7f000000u)) - 1.0f;
+}
 // REAL = float with -g_fastsingle: U from the low 23 bits of an xorshift96
 // output (RNGAVX<float>::Poly, src/mathAVX.cpp:357-365)
 template <typename Real> __device__ __forceinline__ Real unit_minus1(uint32_t r, const UnitBits& ub);
@@ -341,19 +349,24 @@ __device__ __forceinline__ void z_bi_tile(const DeviceDataset& ds, const TileDes
 #pragma unroll
                 for (int k = 1; k < K; ++k) cs[k] = cs[k - 1] + wk[k];
                 const Real s = cs[K - 1];
-                Real tt = unit_minus1<Real>(r, ub) * s;
+                // mixed precision: untyped lanes fail every cluster test
+                constexpr bool TYPED_U = z_mixed<Real, X96>() && !REC;
+                Real tt;
+                if constexpr (TYPED_U) tt = unit_minus1_typed(r, ub, typed) * s;
+                else tt = unit_minus1<Real>(r, ub) * s;
+                const uint64_t cmask = TYPED_U ? ~0ull : tmask;
                 const uint32_t inc = (al1b << 16) | 1u;
 #pragma unroll
                 for (int k = 0; k < K - 1; ++k) {
                     bool neg;
-                    if constexpr (z_mixed<Real, X96>() && !REC) {
+                    if constexpr (TYPED_U) {
                         neg = tt < cs[k];
                     } else {
                         tt = tt - wk[k];
                         neg = tt < (Real)0;
                     }
-                    cum[k] += __builtin_popcountll(ballot(neg) & tmask);
-                    nc[k] += (neg && typed) ? inc : 0u;
+                    cum[k] += __builtin_popcountll(ballot(neg) & cmask);
+                    nc[k] += (neg && (TYPED_U || typed)) ? inc : 0u;
                 }
             }
             if (REC && MPT == 2) acc.charge(pr);
@@ -626,18 +639,23 @@ __global__ __launch_bounds__(64 * kWavesMax) void k_z_admix_gen(const DeviceData
 #pragma unroll
                 for (int k = 1; k < K; ++k) cs[k] = cs[k - 1] + wk[k];
                 const Real s = cs[K - 1];
-                Real tt = unit_minus1<Real>(r, ub) * s;
+                // mixed precision: untyped lanes fail every cluster test
+                constexpr bool TYPED_U = z_mixed<Real, X96>() && !REC;
+                Real tt;
+                if constexpr (TYPED_U) tt = unit_minus1_typed(r, ub, typed) * s;
+                else tt = unit_minus1<Real>(r, ub) * s;
+                const uint64_t cmask = TYPED_U ? ~0ull : tmask;
                 int nneg = 0;
 #pragma unroll
                 for (int k = 0; k < K - 1; ++k) {
                     bool neg;
-                    if constexpr (z_mixed<Real, X96>() && !REC) {
+                    if constexpr (TYPED_U) {
                         neg = tt < cs[k];
                     } else {
                         tt = tt - wk[k];
                         neg = tt < (Real)0;
                     }
-                    cum[k] += __builtin_popcountll(ballot(neg) & tmask);
+                    cum[k] += __builtin_popcountll(ballot(neg) & cmask);
                     nneg += neg ? 1 : 0;
                 }
                 cum[K - 1] += __builtin_popcountll(tmask);
