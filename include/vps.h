@@ -245,7 +245,9 @@ enum vps_state {
     VPS_STATE_FST    = 16,  /* double[2*K]   F[k] then f[k] = (1-F)/F          */
     VPS_STATE_LOC    = 17,  /* double[2 + K + 3*S*K]: r, nr-unused, Eta[K], Gamma[S*K], AlphaLocal[S*K], Di[S*K] */
     VPS_STATE_BINARYQ= 18,  /* int32[1]                                        */
-    VPS_STATE_LNLTRACE=19   /* double[nlnL]  per-record lnL trace (writelnl)   */
+    VPS_STATE_LNLTRACE=19,  /* double[nlnL]  per-record lnL trace (writelnl)   */
+    VPS_STATE_QF     = 20   /* float[N*K]    read-only: Q rounded to single precision, the copy the
+                               FP32-weight Z passes read (current after each sweep's Z pass) */
 };
 int vps_chain_get_state(vps_chain* ch, int which, void* host, size_t bytes);
 int vps_chain_set_state(vps_chain* ch, int which, const void* host, size_t bytes);

@@ -16,6 +16,29 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("topology", ["RNG_PARALLEL", "RNG_REFERENCE"])
+@pytest.mark.parametrize("prec", ["PREC_MIXED", "PREC_FAST32"])
+def test_single_precision_q_copy_follows_q(vps, topology, prec):
+    """the FP32-weight Z passes read Q rounded to single precision: written
+    beside Q by the Dirichlet UpdateQ, rounded before the pass otherwise
+    (Metropolis sweeps); after every sweep it equals Q rounded, for every
+    typed individual (an untyped one's slots are never counted)"""
+    import numpy as np
+    print("Q copy:", topology, prec, flush=True)
+    s = vps.Synth(n_ind=150, n_loc=400, k_true=3, alpha_q=0.5, fst=0.05, missing=0.02, seed=41)
+    ds = vps.Dataset(s, 0)
+    vt = ds.order()[2]
+    p = vps.params_default(admix_val=1, nburnin=4, nreps=8, metrofreq=3, g_seed=9,
+                           rng_topology=getattr(vps, topology), precision=getattr(vps, prec))
+    c = vps.Chain(ds, p, 4, 1)
+    for sweep in range(8):
+        c.step(1)
+        q, qf = c.get("Q"), c.get("Qf")
+        assert (qf[vt > 0] == q[vt > 0].astype(np.float32)).all(), sweep
+    c.close()
+    ds.close()
+
+
 def test_mixed_one_sweep_error_full_size(vps):
     from vcfpop_amd.precision import one_sweep_error
     print("mixed-precision error: N=3000 L=200000 ploidy 2/4/6 K=6 locprior", flush=True)

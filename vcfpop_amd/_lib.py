@@ -503,6 +503,7 @@ STATE = {
     "loc": (17, np.float64, lambda c: (1 + c.K + 3 * c.S * c.K + c.S,)),
     "binaryq": (18, np.int32, lambda c: (1,)),
     "lnltrace": (19, np.float64, lambda c: (c.nlnl(),)),
+    "Qf": (20, np.float32, lambda c: (c.N, c.K)),
 }
 
 

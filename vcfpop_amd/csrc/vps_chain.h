@@ -152,8 +152,10 @@ constexpr int kLikLociPerWave = 1024;
 // clear_mi = false: the caller has already zeroed Mi on an ordered stream
 // rec = true: the FP64 Z pass also sums the Record lnL (mode 0) into c.scal[1]
 // advance = true: the pass's last kernel also advances the sweep index
+// qf_fresh = true: Qf already holds Q rounded to single precision (the
+// sweep's UpdateQ wrote both), so the FP32-weight passes skip the rounding
 int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1,
-                   bool clear_mi = true, bool rec = false, bool advance = false);
+                   bool clear_mi = true, bool rec = false, bool advance = false, bool qf_fresh = false);
 // number of Record partials the Z pass writes with G chunks
 // Record's lnL summed inside the FP64 Z pass: one partial per (tile, group of
 // kLnlGroup individuals), each the wavefront sum of its lanes' closed

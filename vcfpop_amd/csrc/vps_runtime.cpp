@@ -443,7 +443,7 @@ int do_q_metro(vps_chain* ch)
     return rc;
 }
 
-int do_z_admix(vps_chain* ch, bool clear_mi = true, bool rec = false, bool advance = false)
+int do_z_admix(vps_chain* ch, bool clear_mi = true, bool rec = false, bool advance = false, bool qf_fresh = false)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     // timing samples the plain pass: a Record sweep's pass (which also sums
@@ -460,7 +460,7 @@ int do_z_admix(vps_chain* ch, bool clear_mi = true, bool rec = false, bool advan
         e1 = ch->evs[ch->ev_used + 1];
         ch->ev_used += 2;
     }
-    return launch_z_admix(ch->ds->d, ch->c, ch->stream, e0, e1, clear_mi, rec, advance);
+    return launch_z_admix(ch->ds->d, ch->c, ch->stream, e0, e1, clear_mi, rec, advance, qf_fresh);
 }
 
 int do_z_noadmix(vps_chain* ch)
@@ -565,7 +565,8 @@ int launch_sweep(vps_chain* ch, const SweepPlan& p)
     // without Record, the Z pass's last kernel advances the sweep index when
     // nothing beside it still reads the index
     const bool zadv = !p.record && p.q != kQNoAdmix && (fast || side == ch->stream);
-    if (!rc) rc = p.q == kQNoAdmix ? do_z_noadmix(ch) : do_z_admix(ch, !joint, zrec, zadv);
+    // (the joint P/Q step writes Q's single-precision copy beside Q)
+    if (!rc) rc = p.q == kQNoAdmix ? do_z_noadmix(ch) : do_z_admix(ch, !joint, zrec, zadv, joint);
     // Record reads alpha, lambda and F: a Record sweep joins here
     if (!rc && side != ch->stream) {
         VPS_HIP(hipEventRecord(ch->ev_join, side));
@@ -809,6 +810,7 @@ extern "C" int vps_chain_get_state(vps_chain* ch, int which, void* host, size_t 
     case VPS_STATE_BINARYQ: if ((rc = need(4))) return rc; *(int32_t*)host = ch->binaryq; return VPS_OK;
     case VPS_STATE_LNLTRACE: if ((rc = need((size_t)ch->nlnl * 8))) return rc;
         return ch->nlnl ? copy_dev(ch, host, c.lnltrace, bytes, hipMemcpyDeviceToHost) : VPS_OK;
+    case VPS_STATE_QF: if ((rc = need(N * K * 4))) return rc; return copy_dev(ch, host, c.Qf, bytes, hipMemcpyDeviceToHost);
     case VPS_STATE_LOC: {
         if ((rc = need((size_t)(1 + K + 3 * S * K + S) * 8))) return rc;
         double* h = (double*)host;

@@ -653,11 +653,15 @@ __global__ __launch_bounds__(kSppThreads) void k_spp_gamma(const DeviceDataset d
     } else {
         const int K = c.K;
         double* base = kind == kDrawQ ? c.Q : c.bufNK1;
+        const bool qf = kind == kDrawQ && (c.precision == VPS_PREC_MIXED || c.precision == VPS_PREC_FAST32);
         for (int t = tid; t < ds.ntyped; t += kSppThreads) {
             double* r = base + (int64_t)ds.typed[t] * K;
             double sum = 0;
             for (int k = 0; k < K; ++k) sum += r[k];
             dirichlet_scale(r, sum, K, real32(c));
+            // the single-precision copy the FP32-weight Z passes read
+            if (qf)
+                for (int k = 0; k < K; ++k) c.Qf[(int64_t)ds.typed[t] * K + k] = (float)r[k];
         }
         if (clear_mi) {
             __syncthreads();   // (the stage may have been read from Mi)

@@ -112,7 +112,11 @@ __global__ void k_update_pq(const DeviceDataset ds, const ChainDev c, unsigned n
         if (ds.vt[i]) {
             Rng r;
             r.init(c.seed + (uint64_t)m * ds.N + i, kSaltUpdateQ);
-            dirichlet_ab(r, c.Q + (int64_t)i * c.K, alpha_row(ds, c, i), mi, c.K, real32(c));
+            double* q = c.Q + (int64_t)i * c.K;
+            dirichlet_ab(r, q, alpha_row(ds, c, i), mi, c.K, real32(c));
+            // the single-precision copy the FP32-weight Z passes read
+            if (c.precision == VPS_PREC_MIXED || c.precision == VPS_PREC_FAST32)
+                for (int k = 0; k < c.K; ++k) c.Qf[(int64_t)i * c.K + k] = (float)q[k];
         }
         for (int k = 0; k < c.K; ++k) mi[k] = 0;
         return;

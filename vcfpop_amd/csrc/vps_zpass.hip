@@ -901,11 +901,6 @@ const float* z_q<float>(const ChainDev& c) { return c.Qf; }
 template <int K, typename Real, bool REC, bool X96>
 int launch_z_admix_k(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, int64_t* nparts)
 {
-    if (std::is_same<Real, float>::value) {
-        const int64_t n = (int64_t)ds.N * K;
-        hipLaunchKernelGGL(k_q_to_float, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c.Q, c.Qf, n);
-        VPS_HIP(hipGetLastError());
-    }
     const int nthr = 64 * kWavesPerBlock;
     const int A = std::max(ds.maxK, 1);
     const size_t mi_packed = (size_t)kWavesPerBlock * c.Nc * bi_row_words<K>(ds) * sizeof(uint32_t);
@@ -1011,11 +1006,18 @@ int z_occupancy_prec(const DeviceDataset& ds, int K, int precision, int Nc, int*
 
 // the count-only Z pass of a precision mode: FP64 weights (FP64, REAL32),
 // FP32 weights from the xorshift128+ draws (MIXED) or from xorshift96
-// streams (FAST32)
-int dispatch_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
+// streams (FAST32).  The FP32-weight passes read Q's single-precision copy,
+// rounded here first unless qf_fresh (the sweep's UpdateQ wrote both)
+int dispatch_counts(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, bool qf_fresh = false)
 {
-    if (c.precision == VPS_PREC_MIXED) return dispatch_k<float, false>(ds, c, st);
-    if (c.precision == VPS_PREC_FAST32) return dispatch_k<float, false, true>(ds, c, st);
+    if (c.precision == VPS_PREC_MIXED || c.precision == VPS_PREC_FAST32) {
+        if (!qf_fresh) {
+            const int64_t n = (int64_t)ds.N * c.K;
+            hipLaunchKernelGGL(k_q_to_float, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c.Q, c.Qf, n);
+            VPS_HIP(hipGetLastError());
+        }
+        return c.precision == VPS_PREC_MIXED ? dispatch_k<float, false>(ds, c, st) : dispatch_k<float, false, true>(ds, c, st);
+    }
     return dispatch_k<double, false>(ds, c, st);
 }
 
@@ -1099,7 +1101,7 @@ int64_t z_lik_parts(const DeviceDataset& ds)
 }
 
 int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, bool clear_mi,
-                   bool rec, bool advance)
+                   bool rec, bool advance, bool qf_fresh)
 {
     if (rec && advance) return fail(VPS_ERR_INVALID, "Record reads the sweep index after the Z pass");
     if (rec && (c.precision == VPS_PREC_MIXED || c.precision == VPS_PREC_FAST32))
@@ -1110,7 +1112,7 @@ int launch_z_admix(const DeviceDataset& ds, const ChainDev& c, hipStream_t st, h
     if (rc) return rc;
     if (ev0) VPS_HIP(hipEventRecord(ev0, st));
     int64_t nparts = 0;
-    rc = rec ? dispatch_k<double, true>(ds, c, st, &nparts) : dispatch_counts(ds, c, st);
+    rc = rec ? dispatch_k<double, true>(ds, c, st, &nparts) : dispatch_counts(ds, c, st, qf_fresh);
     if (ev1) VPS_HIP(hipEventRecord(ev1, st));
     if (rc) return rc;
     rc = launch_reduce_ni(ds, c, st, advance, true);
