@@ -634,20 +634,104 @@ __device__ void update_a2_locus(const DeviceDataset& ds, const ChainDev& c, Rng&
     if (dlnL >= kNZero || r.uniform() < exp(dlnL)) { pa[m1] = pm1; pa[n1] = pn1; }
 }
 
+// The reference's 32 serial streams (one per block of L/32 loci): one
+// wavefront per stream.  Every lane runs the stream's draws (the same
+// values on every lane), so the proposal and the decision are wave-uniform;
+// the lanes evaluate the log ratio's per-cluster (A2) or per-(cluster,
+// allele) (A1) terms side by side, and the terms are then added one by one
+// in the reference's order, so dlnL has the reference's rounding.  Lane 0
+// writes the kept frequencies.
+constexpr int kAStreams = 32;
+
+// dlnL += term of lane t, for t = 0 .. n-1 in order (n <= 64)
+__device__ __forceinline__ double add_lane_terms(double dlnL, double term, int n)
+{
+    for (int t = 0; t < n; ++t) dlnL += __shfl(term, t, 64);
+    return dlnL;
+}
+
+__device__ void update_a1_locus_wave(const DeviceDataset& ds, const ChainDev& c, Rng& r, int64_t l, int lane, double* pc)
+{
+    const int K = c.K;
+    const int64_t KT = ds.KT;
+    const double lam = c.Lambda[0];
+    const int k2 = ds.k2[l];
+    const int afo = ds.afo[l];
+    double* pa = c.FreqA + afo;
+    double pb[VPS_MAX_ALLELES], pcr[VPS_MAX_ALLELES];
+    for (int a = 0; a < k2; ++a) pb[a] = lam;
+    for (int k = 0; k < K; ++k)
+        for (int a = 0; a < k2; ++a) pb[a] += c.Freq[k * KT + afo + a] * c.Fst[K + k];
+    dirichlet_a(r, pcr, pb, k2, false);
+    double dlnL = 0;
+    for (int a = 0; a < k2; ++a) dlnL += (pb[a] - lam) * (my_log(pa[a] / pcr[a]));
+    // the proposal where every lane can index it
+    if (lane == 0)
+        for (int a = 0; a < k2; ++a) pc[a] = pcr[a];
+    __syncthreads();
+    // terms (k, a), k-major as the reference adds them, 64 at a time
+    const int T = K * k2;
+    for (int t0 = 0; t0 < T; t0 += 64) {
+        const int t = t0 + lane;
+        double term = 0;
+        if (t < T) {
+            const int k = t / k2, a = t - k * k2;
+            const double fr = c.Fst[K + k];
+            term = lgamma(fr * pa[a]) - lgamma(fr * pc[a]) + fr * (pc[a] - pa[a]) * my_log(c.Freq[k * KT + afo + a]);
+        }
+        dlnL = add_lane_terms(dlnL, term, min(64, T - t0));
+    }
+    const bool keep = dlnL >= kNZero || r.uniform() < exp(dlnL);
+    __syncthreads();   // (every lane has read pa and pc)
+    if (keep && lane == 0)
+        for (int a = 0; a < k2; ++a) pa[a] = pcr[a];
+}
+
+__device__ void update_a2_locus_wave(const DeviceDataset& ds, const ChainDev& c, Rng& r, int64_t l, int lane)
+{
+    const int K = c.K;
+    const int64_t KT = ds.KT;
+    const double lam = c.Lambda[0];
+    const int k2 = ds.k2[l];
+    const int afo = ds.afo[l];
+    double* pa = c.FreqA + afo;
+    const double* p = c.Freq + afo;
+    const int m1 = (int)r.next(k2), n1 = (int)r.next_avoid(k2, m1);
+    const double delta = r.uniform(0, pow((double)ds.N, -0.5));
+    const double pm0 = pa[m1], pm1 = pm0 + delta, pn0 = pa[n1], pn1 = pn0 - delta;
+    if (pm1 >= 1 || pn1 <= 0) return;
+    double dlnL = (lam - 1) * my_log(pm1 * pn1 / pm0 / pn0);
+    double term = 0;
+    if (lane < K) {
+        const double fr = c.Fst[K + lane];
+        term = lgamma(fr * pm0) + lgamma(fr * pn0) - lgamma(fr * pm1) - lgamma(fr * pn1) +
+               (fr * delta) * my_log(p[lane * KT + m1] / p[lane * KT + n1]);
+    }
+    dlnL = add_lane_terms(dlnL, term, K);
+    const bool keep = dlnL >= kNZero || r.uniform() < exp(dlnL);
+    __syncthreads();   // (every lane has read pa)
+    if (keep && lane == 0) { pa[m1] = pm1; pa[n1] = pn1; }
+}
+
 __global__ void k_update_a(const DeviceDataset ds, const ChainDev c)
 {
     const int m = *c.mdev;
     // UpdateA picks A1 or A2 by HashULong(seed + m) & 1, src/structure.cpp:1639-1647
     const int use_a1 = hash_ulong(c.seed + (uint64_t)m) & 1;
-    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c.topology == VPS_RNG_REFERENCE) {
-        if (x >= 32) return;
-        const int64_t l0 = x * ds.L / 32, l1 = (x + 1) * ds.L / 32;
+        // one block of 64 threads per stream
+        __shared__ double pc[VPS_MAX_ALLELES];
+        const int b = blockIdx.x, lane = threadIdx.x;
+        const int64_t l0 = (int64_t)b * ds.L / kAStreams, l1 = (int64_t)(b + 1) * ds.L / kAStreams;
         Rng r;
-        r.init(c.seed + (uint64_t)m * 32 + x, kSaltUpdateA);
-        for (int64_t l = l0; l < l1; ++l) use_a1 ? update_a1_locus(ds, c, r, l) : update_a2_locus(ds, c, r, l);
+        r.init(c.seed + (uint64_t)m * kAStreams + b, kSaltUpdateA);
+        for (int64_t l = l0; l < l1; ++l) {
+            if (use_a1) update_a1_locus_wave(ds, c, r, l, lane, pc);
+            else update_a2_locus_wave(ds, c, r, l, lane);
+        }
         return;
     }
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= ds.L) return;
     Rng r;
     r.init(c.seed + (uint64_t)m * ds.L + x, kSaltUpdateA);
@@ -657,8 +741,10 @@ __global__ void k_update_a(const DeviceDataset ds, const ChainDev c)
 int launch_update_a(const DeviceDataset& ds, const ChainDev& c, hipStream_t st)
 {
     if (!c.fmodel) return VPS_OK;
-    const unsigned nb = c.topology == VPS_RNG_REFERENCE ? 1 : (unsigned)((ds.L + 255) / 256);
-    hipLaunchKernelGGL(k_update_a, dim3(nb), dim3(c.topology == VPS_RNG_REFERENCE ? 64 : 256), 0, st, ds, c);
+    if (c.topology == VPS_RNG_REFERENCE)
+        hipLaunchKernelGGL(k_update_a, dim3(kAStreams), dim3(64), 0, st, ds, c);
+    else
+        hipLaunchKernelGGL(k_update_a, dim3((unsigned)((ds.L + 255) / 256)), dim3(256), 0, st, ds, c);
     VPS_HIP(hipGetLastError());
     return VPS_OK;
 }
