@@ -636,81 +636,98 @@ __device__ void update_a2_locus(const DeviceDataset& ds, const ChainDev& c, Rng&
 
 // The reference's 32 serial streams (one per block of L/32 loci): one
 // wavefront per stream.  Every lane runs the stream's draws (the same
-// values on every lane), so the proposal and the decision are wave-uniform;
-// the lanes evaluate the log ratio's per-cluster (A2) or per-(cluster,
-// allele) (A1) terms side by side, and the terms are then added one by one
-// in the reference's order, so dlnL has the reference's rounding.  Lane 0
-// writes the kept frequencies.
+// values on every lane), so the proposal and the decision are wave-uniform.
+// The lanes evaluate the log ratio's lgamma calls side by side (one per
+// lane), each term is assembled in the reference's expression order, and the
+// terms are added one by one in the reference's order, so dlnL has the
+// reference's rounding.  A locus's fields come from a batch of 64 loaded one
+// per lane.  Lane 0 writes the kept frequencies.
 constexpr int kAStreams = 32;
 
-// dlnL += term of lane t, for t = 0 .. n-1 in order (n <= 64)
-__device__ __forceinline__ double add_lane_terms(double dlnL, double term, int n)
+__device__ __forceinline__ double lane_f64(double v, int l)
 {
-    for (int t = 0; t < n; ++t) dlnL += __shfl(term, t, 64);
-    return dlnL;
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-__device__ void update_a1_locus_wave(const DeviceDataset& ds, const ChainDev& c, Rng& r, int64_t l, int lane, double* pc)
+// UpdateA1 at one locus (src/structure.cpp:1672-1700): Dirichlet proposal
+// from lam + sum_k f_k P_k, accepted by its log ratio
+__device__ void update_a1_locus_wave(const DeviceDataset& ds, const ChainDev& c, Rng& r, int k2, int afo, int lane)
 {
     const int K = c.K;
     const int64_t KT = ds.KT;
     const double lam = c.Lambda[0];
-    const int k2 = ds.k2[l];
-    const int afo = ds.afo[l];
     double* pa = c.FreqA + afo;
-    double pb[VPS_MAX_ALLELES], pcr[VPS_MAX_ALLELES];
-    for (int a = 0; a < k2; ++a) pb[a] = lam;
-    for (int k = 0; k < K; ++k)
-        for (int a = 0; a < k2; ++a) pb[a] += c.Freq[k * KT + afo + a] * c.Fst[K + k];
-    dirichlet_a(r, pcr, pb, k2, false);
+    // lane a < k2: pb[a] = lam, then + P_k[a] * f_k in cluster order, and pa[a]
+    double pb = lam, pal = 0;
+    if (lane < k2) {
+        for (int k = 0; k < K; ++k) pb += c.Freq[k * KT + afo + lane] * c.Fst[K + k];
+        pal = pa[lane];
+    }
+    // the Dirichlet draw: the stream's k2 Gamma draws in allele order (every
+    // lane), their sum in that order, each lane keeping its allele's draw
+    double g = 0, sum = 0;
+    for (int a = 0; a < k2; ++a) {
+        const double v = r.gamma(lane_f64(pb, a) * 1.0);
+        if (lane == a) g = v;
+        sum += v;
+    }
+    const double pcl = g * (1.0 / sum);
+    // sum_a (pb[a] - lam) log(pa[a] / pc[a]), in allele order
+    const double u = lane < k2 ? (pb - lam) * (my_log(pal / pcl)) : 0.0;
     double dlnL = 0;
-    for (int a = 0; a < k2; ++a) dlnL += (pb[a] - lam) * (my_log(pa[a] / pcr[a]));
-    // the proposal where every lane can index it
-    if (lane == 0)
-        for (int a = 0; a < k2; ++a) pc[a] = pcr[a];
-    __syncthreads();
+    for (int a = 0; a < k2; ++a) dlnL += lane_f64(u, a);
     // terms (k, a), k-major as the reference adds them, 64 at a time
     const int T = K * k2;
     for (int t0 = 0; t0 < T; t0 += 64) {
         const int t = t0 + lane;
+        const int k = t / k2, a = t - k * k2;
+        const double pat = __shfl(pal, a, 64), pct = __shfl(pcl, a, 64);
         double term = 0;
         if (t < T) {
-            const int k = t / k2, a = t - k * k2;
             const double fr = c.Fst[K + k];
-            term = lgamma(fr * pa[a]) - lgamma(fr * pc[a]) + fr * (pc[a] - pa[a]) * my_log(c.Freq[k * KT + afo + a]);
+            term = lgamma(fr * pat) - lgamma(fr * pct) + fr * (pct - pat) * my_log(c.Freq[k * KT + afo + a]);
         }
-        dlnL = add_lane_terms(dlnL, term, min(64, T - t0));
+        const int n = min(64, T - t0);
+        for (int x = 0; x < n; ++x) dlnL += lane_f64(term, x);
     }
-    const bool keep = dlnL >= kNZero || r.uniform() < exp(dlnL);
-    __syncthreads();   // (every lane has read pa and pc)
-    if (keep && lane == 0)
-        for (int a = 0; a < k2; ++a) pa[a] = pcr[a];
+    if (dlnL >= kNZero || r.uniform() < exp(dlnL)) {
+        // every lane has read pa: the block is this one wavefront
+        for (int a = 0; a < k2; ++a) {
+            const double v = lane_f64(pcl, a);
+            if (lane == 0) pa[a] = v;
+        }
+    }
 }
 
-__device__ void update_a2_locus_wave(const DeviceDataset& ds, const ChainDev& c, Rng& r, int64_t l, int lane)
+// UpdateA2 at one locus (src/structure.cpp:1760-1800): a shift between two
+// alleles, accepted by its log ratio; lane 4k + j evaluates the j-th lgamma
+// of cluster k
+__device__ void update_a2_locus_wave(const DeviceDataset& ds, const ChainDev& c, Rng& r, int k2, int afo, int lane,
+                                     double sdn)
 {
     const int K = c.K;
     const int64_t KT = ds.KT;
     const double lam = c.Lambda[0];
-    const int k2 = ds.k2[l];
-    const int afo = ds.afo[l];
     double* pa = c.FreqA + afo;
     const double* p = c.Freq + afo;
     const int m1 = (int)r.next(k2), n1 = (int)r.next_avoid(k2, m1);
-    const double delta = r.uniform(0, pow((double)ds.N, -0.5));
+    const double delta = r.uniform(0, sdn);
     const double pm0 = pa[m1], pm1 = pm0 + delta, pn0 = pa[n1], pn1 = pn0 - delta;
     if (pm1 >= 1 || pn1 <= 0) return;
     double dlnL = (lam - 1) * my_log(pm1 * pn1 / pm0 / pn0);
+    const int k = lane >> 2, j = lane & 3;
+    const double fr = k < K ? c.Fst[K + k] : 1.0;
+    const double lg = lgamma(fr * (j == 0 ? pm0 : j == 1 ? pn0 : j == 2 ? pm1 : pn1));
+    const double lg1 = __shfl(lg, lane + 1, 64), lg2 = __shfl(lg, lane + 2, 64), lg3 = __shfl(lg, lane + 3, 64);
     double term = 0;
-    if (lane < K) {
-        const double fr = c.Fst[K + lane];
-        term = lgamma(fr * pm0) + lgamma(fr * pn0) - lgamma(fr * pm1) - lgamma(fr * pn1) +
-               (fr * delta) * my_log(p[lane * KT + m1] / p[lane * KT + n1]);
+    if (j == 0 && k < K) term = lg + lg1 - lg2 - lg3 + (fr * delta) * my_log(p[k * KT + m1] / p[k * KT + n1]);
+    for (int x = 0; x < K; ++x) dlnL += lane_f64(term, 4 * x);
+    if (dlnL >= kNZero || r.uniform() < exp(dlnL)) {
+        if (lane == 0) { pa[m1] = pm1; pa[n1] = pn1; }
     }
-    dlnL = add_lane_terms(dlnL, term, K);
-    const bool keep = dlnL >= kNZero || r.uniform() < exp(dlnL);
-    __syncthreads();   // (every lane has read pa)
-    if (keep && lane == 0) { pa[m1] = pm1; pa[n1] = pn1; }
 }
 
 __global__ void k_update_a(const DeviceDataset ds, const ChainDev c)
@@ -719,15 +736,22 @@ __global__ void k_update_a(const DeviceDataset ds, const ChainDev c)
     // UpdateA picks A1 or A2 by HashULong(seed + m) & 1, src/structure.cpp:1639-1647
     const int use_a1 = hash_ulong(c.seed + (uint64_t)m) & 1;
     if (c.topology == VPS_RNG_REFERENCE) {
-        // one block of 64 threads per stream
-        __shared__ double pc[VPS_MAX_ALLELES];
+        // one block of 64 threads (one wavefront) per stream
         const int b = blockIdx.x, lane = threadIdx.x;
         const int64_t l0 = (int64_t)b * ds.L / kAStreams, l1 = (int64_t)(b + 1) * ds.L / kAStreams;
         Rng r;
         r.init(c.seed + (uint64_t)m * kAStreams + b, kSaltUpdateA);
-        for (int64_t l = l0; l < l1; ++l) {
-            if (use_a1) update_a1_locus_wave(ds, c, r, l, lane, pc);
-            else update_a2_locus_wave(ds, c, r, l, lane);
+        const double sdn = pow((double)ds.N, -0.5);   // A2's step bound, the same at every locus
+        for (int64_t lb = l0; lb < l1; lb += 64) {
+            // the next 64 loci's allele counts and offsets, one per lane
+            const int64_t ll = min(lb + lane, l1 - 1);
+            const int k2v = ds.k2[ll], afov = ds.afo[ll];
+            const int nl = (int)min<int64_t>(64, l1 - lb);
+            for (int x = 0; x < nl; ++x) {
+                const int k2 = __builtin_amdgcn_readlane(k2v, x), afo = __builtin_amdgcn_readlane(afov, x);
+                if (use_a1) update_a1_locus_wave(ds, c, r, k2, afo, lane);
+                else update_a2_locus_wave(ds, c, r, k2, afo, lane, sdn);
+            }
         }
         return;
     }
