@@ -335,6 +335,26 @@ def test_sweep_tracks_oracle(vps, orc, pairs, model):
     assert close(g.get("rout"), o.get("rout"), rtol=1e-11)
 
 
+@pytest.mark.parametrize("topology", [0, 1])
+def test_fmodel_sweeps_many_alleles(vps, orc, pairs, topology):
+    """the F model's UpdateA at loci of 12-40 alleles and K = 4: on the
+    reference's streams, A1's (cluster, allele) terms span several rounds of
+    64 lanes; A1 and A2 both occur over the sweeps (each sweep picks one)"""
+    say("F model, many alleles, topology", topology)
+    kw = dict(admix_val=1, f_val=1, nburnin=3, nreps=8, nthinning=2, g_seed=17, rng_topology=topology)
+    ds, od, g, o = make(vps, orc, pairs, "many_alleles", 4, run_id=1, **kw)
+    for sweep in range(8):
+        feed(g, o, FED + ["m"])
+        g.step(1)
+        o.sweep(1)
+        for n in ("Z", "Ni", "Mi"):
+            assert (g.get(n) == o.get(n)).all(), (sweep, n)
+        for n in ("Freq", "Q", "Fst", "FreqA"):
+            assert close(g.get(n), o.get(n), rtol=1e-11), (sweep, n)
+    g.close()
+    o.close()
+
+
 def test_mixed_precision_z_agreement(vps, orc, pairs):
     """FP32 weights: same streams, so assignments differ from FP64 only where
     the float rounding crosses a cluster boundary; counts stay consistent"""
