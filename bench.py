@@ -68,6 +68,10 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="only the headline workload (no other-topology, float, batched, served or precision legs)")
     ap.add_argument("--batch-sweeps", type=int, default=40)
+    ap.add_argument("--batch-form", default="all",
+                    choices=["all", "plain", "batched", "batched_mixed", "plain_drop_in", "batched_drop_in",
+                             "batched_mixed_drop_in"],
+                    help="with --leg batched: time only this form of the 64 chains (for a profiler run)")
     ap.add_argument("--serve-jobs", type=int, default=24)
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-sweep-scale", type=int, default=20,
@@ -93,7 +97,7 @@ def main():
     if args.leg:
         import vcfpop_amd as V
         fn = {"small": lambda: bench_small(V, V.RNG_REFERENCE if args.topology == "reference" else V.RNG_PARALLEL),
-              "batched": lambda: bench_batched(V, args.batch_sweeps),
+              "batched": lambda: bench_batched(V, args.batch_sweeps, args.batch_form),
               "served": lambda: bench_served(V, args.serve_jobs, args.serve_rate, args.serve_sweep_scale),
               "precision": lambda: bench_precision(V)}[args.leg]
         print(json.dumps({args.leg: fn()}), flush=True)
@@ -320,7 +324,7 @@ def bench_small(V, topology):
             "chain_create_s": round(t1 - t0, 4)}
 
 
-def bench_batched(V, S):
+def bench_batched(V, S, form="all"):
     """64 chains (K=2..9 x 8 replicate seeds) over one N=1,000 x L=50,000
     dataset, 5,000 sweeps each: chains/hr when chains run one at a time
     (the reference's one-chain-per-device leasing) and as one batch, in the
@@ -361,6 +365,17 @@ def bench_batched(V, S):
             c.close()
         return t
 
+    per_hr = lambda t: 64 * 3600.0 / (t * 5000)
+    if form != "all":
+        # one form alone (for a profiler run)
+        prec = V.PREC_MIXED if "mixed" in form else V.PREC_FP64
+        topo = V.RNG_REFERENCE if form.endswith("drop_in") else V.RNG_PARALLEL
+        cs = chains(prec, topo)
+        t = run_plain(cs) if form.startswith("plain") else run_batch(cs)[0]
+        ds.close()
+        return {"workload": "batch_n1000_l50k_k2to9x8", "form": form, "measured_sweeps": S,
+                "ms_per_sweep_of_64": round(t * 1e3, 3), "chains_per_hr": round(per_hr(t), 1)}
+
     plain = run_plain(chains())
     batched, st = run_batch(chains())
     # the same batch with FP32 weights (FP64 lnL, draws and Metropolis steps)
@@ -371,7 +386,6 @@ def bench_batched(V, S):
     batched_ref, _ = run_batch(chains(topology=V.RNG_REFERENCE))
     mixed_ref, _ = run_batch(chains(V.PREC_MIXED, V.RNG_REFERENCE))
     ds.close()
-    per_hr = lambda t: 64 * 3600.0 / (t * 5000)
     return {"workload": "batch_n1000_l50k_k2to9x8", "chains": 64, "sweeps_per_chain": 5000,
             "measured_sweeps": S, "rng_topology": "parallel", "plain_chains_per_hr": round(per_hr(plain), 1),
             "batched_chains_per_hr": round(per_hr(batched), 1), "speedup": round(plain / batched, 3),
