@@ -524,7 +524,17 @@ int launch_sweep(vps_chain* ch, const SweepPlan& p)
     } else if (p.q == kQMetro && c.topology == VPS_RNG_REFERENCE) {
         // UpdateP's streams, and the proposals' stream as a stream graph (or
         // in the same window parse)
-        if (pg_applies(c)) {
+        if (pg_applies(c) && !ch->capturing) {
+            // the proposals' graph (UpdateQ's stream) and UpdateP's streams
+            // read and write disjoint state: the graph runs on the second
+            // stream beside the P streams' parse, joined before the ratio pass
+            VPS_HIP(hipEventRecord(ch->ev_fork, ch->stream));
+            VPS_HIP(hipStreamWaitEvent(ch->aux, ch->ev_fork, 0));
+            rc = launch_pg_proposals(d, c, ch->aux);
+            if (!rc) VPS_HIP(hipEventRecord(ch->ev_join, ch->aux));
+            if (!rc) rc = launch_spp(d, c, true, kSppQNone, false, ch->stream);
+            if (!rc) VPS_HIP(hipStreamWaitEvent(ch->stream, ch->ev_join, 0));
+        } else if (pg_applies(c)) {
             rc = launch_spp(d, c, true, kSppQNone, false, ch->stream);
             if (!rc) rc = launch_pg_proposals(d, c, ch->stream);
         } else {
