@@ -94,8 +94,17 @@ struct LikNA { static constexpr int value = MODE == 3 ? K : ((MODE == 2 || MODE 
 // biallelic tiles (at most two alleles; genotype bytes in ds.bitab, or in
 // the tile's ds.btab columns above 3 bits per id).  MPT is
 // the maximum ploidy when it is 2 or 4 (copies loop fully unrolled), else 0.
+// Resident wavefronts per SIMD the BI likelihood kernel asks the register
+// allocator for: the FP64 Metropolis ratio (mode 2) at K = 5 takes 106
+// registers on its own (4 wavefronts) and fits 5 in 94 with no spills (its
+// pass 0.69 -> 0.65 ms at N = 2,000, L = 100,000); at K <= 4 every mode
+// already fits 5, and from K = 6 a cap of 5 spills.
+template <int K, int MODE>
+constexpr int lik_bi_waves() { return MODE == 2 && K == 5 ? 5 : 1; }
+
 template <int K, int MODE, int MPT>
-__global__ __launch_bounds__(kLikThreads) void k_lik_bi(const DeviceDataset ds, const ChainDev c, int tiles_per_range)
+__global__ __launch_bounds__(kLikThreads, (lik_bi_waves<K, MODE>())) void k_lik_bi(const DeviceDataset ds, const ChainDev c,
+                                                                               int tiles_per_range)
 {
     const int i = blockIdx.y * kLikThreads + threadIdx.x;
     const bool ok = i < ds.N;
