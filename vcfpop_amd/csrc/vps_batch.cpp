@@ -24,6 +24,7 @@ using namespace vps;
 
 struct vps_batch {
     std::vector<vps_chain*> chains;
+    std::vector<int> own_chunks;   // each chain's chunk count before the batch
     int device = 0;
     hipStream_t root = nullptr;
     hipEvent_t fork = nullptr, done = nullptr;
@@ -38,6 +39,8 @@ constexpr size_t kMaxGraphs = 32;   // beyond this, sweeps launch stream by stre
 
 void destroy_batch(vps_batch* b)
 {
+    // the chains continue alone with their own chunking
+    for (size_t i = 0; i < b->own_chunks.size() && i < b->chains.size(); ++i) (void)set_chunks(b->chains[i], b->own_chunks[i]);
     for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
     b->graphs.clear();
     for (hipEvent_t e : b->join) if (e) (void)hipEventDestroy(e);
@@ -109,6 +112,12 @@ extern "C" int vps_batch_create(vps_chain* const* chains, int n, vps_batch** out
         if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ready[i], hipEventDisableTiming);
     }
     if (e != hipSuccess) { destroy_batch(b); return fail(VPS_ERR_DEVICE, "batch streams: %s", hipGetErrorString(e)); }
+    // the chains' chunking for a batch (results do not depend on it)
+    for (vps_chain* ch : b->chains) {
+        b->own_chunks.push_back(ch->c.G);
+        const int rc = set_chunks(ch, batch_chunk_count(ch));
+        if (rc) { destroy_batch(b); return rc; }
+    }
     *out = b;
     return VPS_OK;
 }

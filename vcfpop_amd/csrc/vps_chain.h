@@ -118,6 +118,7 @@ struct ChainDev {
     // individual chunking of the genotype passes: G chunks of Nc individuals;
     // jump[c] advances a locus stream to individual c*Nc
     int32_t   G, Nc;
+    int32_t   Gcap;      // the chunk count Ni_part holds
     const uint32_t* jump;
     int32_t   lik_tpr[2];   // tiles per locus range of the likelihood passes (BI, GEN tiles)
     // Metropolis proposals parsed as a stream graph (reference draw streams,

@@ -50,6 +50,12 @@ struct vps_chain {
 namespace vps {
 
 SweepPlan plan_sweep(const vps_chain* ch);
+// A chain's chunking of individuals in the Z pass (the draws and counts do
+// not depend on it): set_chunks(ch, G) takes G chunks (at most the chunk
+// count the chain's buffers were sized for; its own single-chain graphs are
+// dropped), batch_chunk_count gives the count a chain takes in a batch
+int set_chunks(vps_chain* ch, int G);
+int batch_chunk_count(const vps_chain* ch);
 int launch_sweep(vps_chain* ch, const SweepPlan& p);   // every launch of the sweep, on ch->stream
 int settle_side(vps_chain* ch);                        // ch->stream waits for the second stream's updates
 void commit_sweep(vps_chain* ch, const SweepPlan& p);  // host bookkeeping after the launch
